@@ -1,0 +1,149 @@
+/*
+ * mpqc_ccsd.h -- C-ABI of the MI355X-native closed-shell CCSD residual library.
+ *
+ * Drop-in boundary for MPQC's CCSD solver loop.  Each entry point replaces one
+ * reference interface (paths relative to the MPQC source tree, src/mpqc/...):
+ *
+ *   mpqc_ccsd_ctx_create          CCSD<Tile,Policy>::compute_ccsd set-up
+ *                                 (chemistry/qc/lcao/cc/ccsd.h:268-342) and the
+ *                                 TiledRange1 of the amplitudes
+ *                                 (chemistry/qc/lcao/expression/trange1_engine.cpp:10-20)
+ *   mpqc_ccsd_set_integral        cc::Integrals<Array> members Fia..Gabcd
+ *                                 (chemistry/qc/lcao/cc/ccsd_r1_r2.h:24-56), filled at
+ *                                 ccsd.h:281-305
+ *   mpqc_ccsd_set_orbital_energies JacobiDIISSolver(kv, f_ii, f_aa)
+ *                                 (chemistry/qc/lcao/cc/solvers.h:1140-1146; values from
+ *                                 make_diagonal_fpq, chemistry/qc/lcao/scf/mo_build.h:20-26)
+ *   mpqc_ccsd_solver_init         DIISSolver(const KeyVal&) keywords diis_start, n_diis,
+ *                                 diis_damp, diis_ngroup, diis_group_nstart
+ *                                 (chemistry/qc/cc/solvers.h:93-95)
+ *   mpqc_ccsd_residual            cc::compute_cs_ccsd_r1 then cc::compute_cs_ccsd_r2
+ *                                 (ccsd_r1_r2.h:70-124, 198-293), incl. the FIJ/FAB
+ *                                 hand-off (ccsd_r1_r2.h:90,96) and the tau rebuild
+ *                                 (ccsd.h:389)
+ *   mpqc_ccsd_energy_error        Solver::error (chemistry/qc/cc/solvers.h:60-63) and the
+ *                                 energy expression at ccsd.h:376-380
+ *   mpqc_ccsd_update              DIISSolver::update = JacobiDIISSolver::update_only
+ *                                 (lcao/cc/solvers.h:1153-1158) + TA::DIIS::extrapolate
+ *                                 (chemistry/qc/cc/solvers.h:112-121), then tau (ccsd.h:389)
+ *   mpqc_ccsd_solve               the whole solver loop CCSD::compute_ccsd (ccsd.h:344-477)
+ *   mpqc_ccsd_get_trace           the iteration table printed by detail::print_ccsd
+ *                                 (ccsd.h:28-36)
+ *   mpqc_ccsd_set/get_amplitudes  CCSD::T1_/T2_ (ccsd.h:121-122, 252-253)
+ *   mpqc_ccsd_pp_ladder           the particle-particle ladder contraction alone,
+ *                                 b("a,b,i,j") = tau("c,d,i,j") * Gabcd("a,b,c,d")
+ *                                 (ccsd_r1_r2.h:275)
+ *   mpqc_ccsd_gemm                one TiledArray contraction "C = alpha A*B + beta C"
+ *                                 (math/external/tiledarray/tiledarray_impl.cpp:36-47,
+ *                                 TA's ContractReduce over TA::Tensor<double>)
+ *
+ * Conventions: row-major FP64 buffers in the reference's index orders: t1 "a,i",
+ * t2/r2 "a,b,i,j", Fia "i,a", Fij "i,j", Fab "a,b", Gijab "i,j,a,b" = <ij|ab>,
+ * Gijkl "i,j,k,l", Giajb "i,a,j,b", Giabc "i,a,b,c", Gijka "i,j,k,a",
+ * Gabcd "a,b,c,d" (physicist notation).  MPQC_LAYOUT_TA_TILES means the buffer is
+ * the concatenation, in row-major tile-ordinal order, of row-major tiles over the
+ * TiledRange1 hashmarks given at create time (occupied modes use the occupied
+ * hashmarks, virtual modes the virtual ones).  MPQC_MEM_HOST / MPQC_MEM_DEVICE
+ * says where a buffer lives.  Every function returns an MPQC_CCSD_* status; on
+ * failure mpqc_ccsd_last_error() describes it.  Calls on one context must be
+ * serialised by the caller (one HIP stream per context).
+ */
+#ifndef MPQC_CCSD_H
+#define MPQC_CCSD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mpqc_ccsd_ctx mpqc_ccsd_ctx;
+
+enum {
+  MPQC_CCSD_OK = 0,
+  MPQC_CCSD_ERR_ARG = 1,       /* invalid argument (InputError in the reference) */
+  MPQC_CCSD_ERR_HIP = 2,       /* HIP runtime error */
+  MPQC_CCSD_ERR_STATE = 3,     /* call out of order (ProgrammingError in the reference) */
+  MPQC_CCSD_ERR_NONFINITE = 4, /* non-finite residual/amplitudes (SIGFPE in the reference) */
+  MPQC_CCSD_ERR_DIIS = 5,      /* DIIS system poorly conditioned (TA::DIIS domain_error) */
+  MPQC_CCSD_ERR_NOMEM = 6      /* device allocation failed */
+};
+
+enum { MPQC_CCSD_PREC_F64 = 0, MPQC_CCSD_PREC_F32 = 1, MPQC_CCSD_PREC_BF16X3 = 2 };
+enum { MPQC_LAYOUT_DENSE = 0, MPQC_LAYOUT_TA_TILES = 1 };
+enum { MPQC_MEM_HOST = 0, MPQC_MEM_DEVICE = 1 };
+enum {
+  MPQC_INT_FIA = 0,
+  MPQC_INT_FIJ = 1,
+  MPQC_INT_FAB = 2,
+  MPQC_INT_GIJAB = 3,
+  MPQC_INT_GIJKL = 4,
+  MPQC_INT_GIAJB = 5,
+  MPQC_INT_GIABC = 6,
+  MPQC_INT_GIJKA = 7,
+  MPQC_INT_GABCD = 8
+};
+
+/* Library version string. */
+const char* mpqc_ccsd_version(void);
+
+/* Creates a context for o active occupied and v virtual orbitals on the current
+ * HIP device.  Hashmarks (n_tiles+1 entries, first 0, last o or v) describe the
+ * TiledArray tiling used by MPQC_LAYOUT_TA_TILES buffers; pass NULL for one tile.
+ * precision: MPQC_CCSD_PREC_* for the contractions (the update, DIIS and energy
+ * are always FP64). */
+int mpqc_ccsd_ctx_create(int o, int v, const int64_t* occ_hashmarks, int n_occ_tiles,
+                         const int64_t* vir_hashmarks, int n_vir_tiles, int precision,
+                         mpqc_ccsd_ctx** out);
+int mpqc_ccsd_ctx_destroy(mpqc_ccsd_ctx* ctx);
+const char* mpqc_ccsd_last_error(const mpqc_ccsd_ctx* ctx);
+
+/* Numeric options: "fov_sign" (+1 reference F_VV as written, -1 exact CCSD for
+ * non-Brillouin references), "precision" (MPQC_CCSD_PREC_*). */
+int mpqc_ccsd_set_option(mpqc_ccsd_ctx* ctx, const char* key, double value);
+
+int mpqc_ccsd_set_integral(mpqc_ccsd_ctx* ctx, int which, const double* data, int layout,
+                           int memspace);
+int mpqc_ccsd_set_orbital_energies(mpqc_ccsd_ctx* ctx, const double* eps_occ_act,
+                                   const double* eps_vir);
+int mpqc_ccsd_solver_init(mpqc_ccsd_ctx* ctx, int diis_start, int n_diis, double diis_damp,
+                          int diis_ngroup, int diis_group_nstart);
+
+int mpqc_ccsd_set_amplitudes(mpqc_ccsd_ctx* ctx, const double* t1, const double* t2, int layout,
+                             int memspace);
+int mpqc_ccsd_get_amplitudes(mpqc_ccsd_ctx* ctx, double* t1, double* t2, int layout,
+                             int memspace);
+int mpqc_ccsd_get_residuals(mpqc_ccsd_ctx* ctx, double* r1, double* r2, int layout,
+                            int memspace);
+/* The effective one-body Hamiltonians FIJ ("k,i") and FAB ("a,c") that
+ * compute_cs_ccsd_r1 writes into cc::Integrals (host buffers). */
+int mpqc_ccsd_get_dressed_fock(mpqc_ccsd_ctx* ctx, double* FIJ, double* FAB);
+
+/* Sets r1 = f_ai, r2 = g_abij, t = 0 (ccsd.h:331-352). */
+int mpqc_ccsd_init_guess(mpqc_ccsd_ctx* ctx);
+int mpqc_ccsd_residual(mpqc_ccsd_ctx* ctx);
+int mpqc_ccsd_energy_error(mpqc_ccsd_ctx* ctx, double* energy, double* error);
+int mpqc_ccsd_update(mpqc_ccsd_ctx* ctx);
+
+/* Full solver loop.  Returns the correlation energy and the number of loop
+ * passes; target_precision as Energy::target_precision (default 1e-9). */
+int mpqc_ccsd_solve(mpqc_ccsd_ctx* ctx, int max_iter, double target_precision, int verbose,
+                    double* e_corr, int* n_iter);
+/* Rows of the iteration table of the last solve: iteration, dE, residual error,
+ * energy, wall seconds of that iteration.  Returns the row count in *n_rows. */
+int mpqc_ccsd_get_trace(mpqc_ccsd_ctx* ctx, int max_rows, int* n_rows, int* iters, double* dE,
+                        double* error, double* energy, double* seconds);
+
+/* Device-pointer kernel entry points (stream may be NULL for the null stream). */
+int mpqc_ccsd_pp_ladder(int o, int v, const double* gabcd_dev, const double* tau_dev,
+                        double* r2_dev, double alpha, double beta, int precision, void* stream);
+int mpqc_ccsd_gemm(int transa, int transb, int m, int n, int k, double alpha, const double* a,
+                   int64_t lda, int64_t stride_a, const double* b, int64_t ldb, int64_t stride_b,
+                   double beta, double* c, int64_t ldc, int64_t stride_c, int batch, int precision,
+                   void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MPQC_CCSD_H */

@@ -1,0 +1,253 @@
+// extern "C" boundary (include/mpqc_ccsd.h).  Every exception thrown below the
+// boundary becomes a status code plus a message for mpqc_ccsd_last_error().
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/mpqc_ccsd.h"
+#include "ccsd_ctx.hpp"
+
+struct mpqc_ccsd_ctx {
+  mpqc_hip::CCSDContext* impl = nullptr;
+  std::string err;
+};
+
+namespace {
+
+thread_local std::string g_last_error;  // for failures without a context
+
+int status_of(const std::exception& e) {
+  if (dynamic_cast<const mpqc_hip::HipError*>(&e)) return MPQC_CCSD_ERR_HIP;
+  if (dynamic_cast<const std::bad_alloc*>(&e)) return MPQC_CCSD_ERR_NOMEM;
+  if (dynamic_cast<const std::domain_error*>(&e)) {
+    const std::string w = e.what();
+    return w.find("DIIS") != std::string::npos ? MPQC_CCSD_ERR_DIIS : MPQC_CCSD_ERR_ARG;
+  }
+  if (dynamic_cast<const std::range_error*>(&e)) return MPQC_CCSD_ERR_NONFINITE;
+  if (dynamic_cast<const std::logic_error*>(&e) && !dynamic_cast<const std::invalid_argument*>(&e))
+    return MPQC_CCSD_ERR_STATE;
+  return MPQC_CCSD_ERR_ARG;
+}
+
+template <typename F>
+int guarded(mpqc_ccsd_ctx* ctx, F&& f) {
+  if (ctx == nullptr || ctx->impl == nullptr) {
+    g_last_error = "null context";
+    return MPQC_CCSD_ERR_ARG;
+  }
+  try {
+    f(*ctx->impl);
+    return MPQC_CCSD_OK;
+  } catch (const std::bad_alloc&) {
+    ctx->err = "device or host allocation failed";
+    return MPQC_CCSD_ERR_NOMEM;
+  } catch (const std::exception& e) {
+    ctx->err = e.what();
+    return status_of(e);
+  }
+}
+
+template <typename F>
+int guarded_free(F&& f) {
+  try {
+    f();
+    return MPQC_CCSD_OK;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "device or host allocation failed";
+    return MPQC_CCSD_ERR_NOMEM;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return status_of(e);
+  }
+}
+
+// Split-K workspace for the context-free GEMM entry points: one per device,
+// grown on demand, kept until process exit.
+struct FreeWorkspace {
+  std::mutex m;
+  double* ptr[64] = {nullptr};
+  size_t bytes[64] = {0};
+};
+FreeWorkspace g_ws;
+
+mpqc_hip::GemmWorkspace workspace_for_current_device(size_t want) {
+  int dev = 0;
+  MPQC_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) throw std::runtime_error("device index out of range");
+  std::lock_guard<std::mutex> lk(g_ws.m);
+  if (g_ws.bytes[dev] < want) {
+    if (g_ws.ptr[dev]) MPQC_HIP_CHECK(hipFree(g_ws.ptr[dev]));
+    g_ws.ptr[dev] = nullptr;
+    g_ws.bytes[dev] = 0;
+    void* p = nullptr;
+    MPQC_HIP_CHECK(hipMalloc(&p, want));
+    g_ws.ptr[dev] = (double*)p;
+    g_ws.bytes[dev] = want;
+  }
+  return {g_ws.ptr[dev], g_ws.bytes[dev]};
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* mpqc_ccsd_version(void) { return "mpqc_amd-ccsd 0.1 (gfx950)"; }
+
+int mpqc_ccsd_ctx_create(int o, int v, const int64_t* occ_hashmarks, int n_occ_tiles,
+                         const int64_t* vir_hashmarks, int n_vir_tiles, int precision,
+                         mpqc_ccsd_ctx** out) {
+  if (out == nullptr) {
+    g_last_error = "null output pointer";
+    return MPQC_CCSD_ERR_ARG;
+  }
+  *out = nullptr;
+  return guarded_free([&] {
+    auto* c = new mpqc_ccsd_ctx;
+    try {
+      c->impl = new mpqc_hip::CCSDContext(o, v, occ_hashmarks, n_occ_tiles, vir_hashmarks,
+                                          n_vir_tiles, precision);
+    } catch (...) {
+      delete c;
+      throw;
+    }
+    *out = c;
+  });
+}
+
+int mpqc_ccsd_ctx_destroy(mpqc_ccsd_ctx* ctx) {
+  if (ctx == nullptr) return MPQC_CCSD_OK;
+  delete ctx->impl;
+  delete ctx;
+  return MPQC_CCSD_OK;
+}
+
+const char* mpqc_ccsd_last_error(const mpqc_ccsd_ctx* ctx) {
+  if (ctx == nullptr) return g_last_error.c_str();
+  return ctx->err.c_str();
+}
+
+int mpqc_ccsd_set_option(mpqc_ccsd_ctx* ctx, const char* key, double value) {
+  return guarded(ctx, [&](mpqc_hip::CCSDContext& c) {
+    if (!key) throw std::invalid_argument("null option key");
+    c.set_option(key, value);
+  });
+}
+
+int mpqc_ccsd_set_integral(mpqc_ccsd_ctx* ctx, int which, const double* data, int layout,
+                           int memspace) {
+  return guarded(ctx, [&](mpqc_hip::CCSDContext& c) { c.set_integral(which, data, layout, memspace); });
+}
+
+int mpqc_ccsd_set_orbital_energies(mpqc_ccsd_ctx* ctx, const double* eps_occ_act,
+                                   const double* eps_vir) {
+  return guarded(ctx, [&](mpqc_hip::CCSDContext& c) { c.set_orbital_energies(eps_occ_act, eps_vir); });
+}
+
+int mpqc_ccsd_solver_init(mpqc_ccsd_ctx* ctx, int diis_start, int n_diis, double diis_damp,
+                          int diis_ngroup, int diis_group_nstart) {
+  return guarded(ctx, [&](mpqc_hip::CCSDContext& c) {
+    c.solver_init(diis_start, n_diis, diis_damp, diis_ngroup, diis_group_nstart);
+  });
+}
+
+int mpqc_ccsd_set_amplitudes(mpqc_ccsd_ctx* ctx, const double* t1, const double* t2, int layout,
+                             int memspace) {
+  return guarded(ctx, [&](mpqc_hip::CCSDContext& c) {
+    if (!t1 || !t2) throw std::invalid_argument("null amplitude buffer");
+    c.set_amplitudes(t1, t2, layout, memspace);
+  });
+}
+
+int mpqc_ccsd_get_amplitudes(mpqc_ccsd_ctx* ctx, double* t1, double* t2, int layout, int memspace) {
+  return guarded(ctx, [&](mpqc_hip::CCSDContext& c) { c.get_amplitudes(t1, t2, layout, memspace); });
+}
+
+int mpqc_ccsd_get_residuals(mpqc_ccsd_ctx* ctx, double* r1, double* r2, int layout, int memspace) {
+  return guarded(ctx, [&](mpqc_hip::CCSDContext& c) { c.get_residuals(r1, r2, layout, memspace); });
+}
+
+int mpqc_ccsd_get_dressed_fock(mpqc_ccsd_ctx* ctx, double* FIJ, double* FAB) {
+  return guarded(ctx, [&](mpqc_hip::CCSDContext& c) { c.get_dressed_fock(FIJ, FAB); });
+}
+
+int mpqc_ccsd_init_guess(mpqc_ccsd_ctx* ctx) {
+  return guarded(ctx, [&](mpqc_hip::CCSDContext& c) { c.init_guess(); });
+}
+
+int mpqc_ccsd_residual(mpqc_ccsd_ctx* ctx) {
+  return guarded(ctx, [&](mpqc_hip::CCSDContext& c) { c.residual(); });
+}
+
+int mpqc_ccsd_energy_error(mpqc_ccsd_ctx* ctx, double* energy, double* error) {
+  return guarded(ctx, [&](mpqc_hip::CCSDContext& c) { c.energy_error(energy, error); });
+}
+
+int mpqc_ccsd_update(mpqc_ccsd_ctx* ctx) {
+  return guarded(ctx, [&](mpqc_hip::CCSDContext& c) { c.update(); });
+}
+
+int mpqc_ccsd_solve(mpqc_ccsd_ctx* ctx, int max_iter, double target_precision, int verbose,
+                    double* e_corr, int* n_iter) {
+  return guarded(ctx, [&](mpqc_hip::CCSDContext& c) {
+    if (max_iter < 1) throw std::invalid_argument("max_iter must be >= 1");
+    if (!(target_precision > 0.0)) throw std::invalid_argument("target_precision must be > 0");
+    const double e = c.solve(max_iter, target_precision, verbose, n_iter);
+    if (e_corr) *e_corr = e;
+  });
+}
+
+int mpqc_ccsd_get_trace(mpqc_ccsd_ctx* ctx, int max_rows, int* n_rows, int* iters, double* dE,
+                        double* error, double* energy, double* seconds) {
+  return guarded(ctx, [&](mpqc_hip::CCSDContext& c) {
+    const auto& t = c.trace();
+    const int n = (int)std::min<size_t>(t.size(), (size_t)std::max(0, max_rows));
+    for (int k = 0; k < n; ++k) {
+      if (iters) iters[k] = t[k].iter;
+      if (dE) dE[k] = t[k].dE;
+      if (error) error[k] = t[k].error;
+      if (energy) energy[k] = t[k].energy;
+      if (seconds) seconds[k] = t[k].seconds;
+    }
+    if (n_rows) *n_rows = (int)t.size();
+  });
+}
+
+int mpqc_ccsd_gemm(int transa, int transb, int m, int n, int k, double alpha, const double* a,
+                   int64_t lda, int64_t stride_a, const double* b, int64_t ldb, int64_t stride_b,
+                   double beta, double* c, int64_t ldc, int64_t stride_c, int batch, int precision,
+                   void* stream) {
+  return guarded_free([&] {
+    if (m < 0 || n < 0 || k <= 0 || batch < 1) throw std::invalid_argument("invalid GEMM sizes");
+    if (!a || !b || !c) throw std::invalid_argument("null GEMM operand");
+    if (precision < 0 || precision > 2) throw std::invalid_argument("unknown precision");
+    mpqc_hip::GemmDesc d = mpqc_hip::make_gemm(transa != 0, transb != 0, m, n, k, alpha, a, lda, b,
+                                               ldb, beta, c, ldc);
+    d.sA = stride_a; d.sB = stride_b; d.sC = stride_c; d.batch = batch;
+    const size_t want = std::max<size_t>((size_t)m * n * batch * 64 * sizeof(double), 1u << 20);
+    auto ws = workspace_for_current_device(std::min<size_t>(want, (size_t)1 << 30));
+    mpqc_hip::gemm(d, (mpqc_hip::GemmPrecision)precision, ws, (hipStream_t)stream);
+  });
+}
+
+int mpqc_ccsd_pp_ladder(int o, int v, const double* gabcd_dev, const double* tau_dev,
+                        double* r2_dev, double alpha, double beta, int precision, void* stream) {
+  return guarded_free([&] {
+    if (o <= 0 || v <= 0) throw std::invalid_argument("o and v must be positive");
+    if (!gabcd_dev || !tau_dev || !r2_dev) throw std::invalid_argument("null operand");
+    if (precision < 0 || precision > 2) throw std::invalid_argument("unknown precision");
+    const long vv = (long)v * v, oo = (long)o * o;
+    if (vv > 0x7fffffffL) throw std::invalid_argument("v^2 too large");
+    // b("a,b,i,j") = tau("c,d,i,j") * Gabcd("a,b,c,d"): M = v^2, N = o^2, K = v^2
+    mpqc_hip::GemmDesc d = mpqc_hip::make_gemm(false, false, (int)vv, (int)oo, (int)vv, alpha,
+                                               gabcd_dev, vv, tau_dev, oo, beta, r2_dev, oo);
+    auto ws = workspace_for_current_device((size_t)1 << 20);
+    mpqc_hip::gemm(d, (mpqc_hip::GemmPrecision)precision, ws, (hipStream_t)stream);
+  });
+}
+
+}  // extern "C"

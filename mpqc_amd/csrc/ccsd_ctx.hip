@@ -1,0 +1,797 @@
+// Device-resident closed-shell CCSD iteration.
+//
+// One residual evaluation is a fixed sequence of GEMMs on the matrix cores and
+// permuted linear combinations streaming through HBM.  The order and the
+// coefficients follow src/mpqc/chemistry/qc/lcao/cc/ccsd_r1_r2.h:70-293 term by
+// term; the comments give the reference line each step implements.  Operands
+// are brought into GEMM layout explicitly (integral layouts once per solve in
+// prepare(), amplitude layouts once per iteration), so every contraction runs as
+// a dense row-major GEMM with a contiguous K dimension.
+#include "ccsd_ctx.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+namespace mpqc_hip {
+
+namespace {
+
+constexpr int kOcc = 0, kVir = 1, kNone = -1;
+
+// Column-pivoted Householder QR solve of a small dense system (row-major A).
+// Returns |det A|; x gets the rank-truncated solution (components of zero
+// pivots are zero).  Mirrors the solve used for the DIIS system.
+double colpiv_qr_solve(int n, std::vector<double> A, std::vector<double> b, std::vector<double>& x) {
+  auto at = [&](int r, int c) -> double& { return A[(size_t)r * n + c]; };
+  std::vector<int> perm(n);
+  std::iota(perm.begin(), perm.end(), 0);
+  std::vector<double> diag(n, 0.0);
+  for (int k = 0; k < n; ++k) {
+    // pivot: column with the largest remaining norm
+    int p = k;
+    double best = -1.0;
+    for (int j = k; j < n; ++j) {
+      double s = 0.0;
+      for (int r = k; r < n; ++r) s += at(r, j) * at(r, j);
+      if (s > best) { best = s; p = j; }
+    }
+    if (p != k) {
+      for (int r = 0; r < n; ++r) std::swap(at(r, k), at(r, p));
+      std::swap(perm[k], perm[p]);
+    }
+    // Householder reflector for column k, rows k..n-1
+    double norm = 0.0;
+    for (int r = k; r < n; ++r) norm += at(r, k) * at(r, k);
+    norm = std::sqrt(norm);
+    if (norm == 0.0) { diag[k] = 0.0; continue; }
+    const double alpha = at(k, k) > 0 ? -norm : norm;
+    std::vector<double> vh(n, 0.0);
+    for (int r = k; r < n; ++r) vh[r] = at(r, k);
+    vh[k] -= alpha;
+    double vn = 0.0;
+    for (int r = k; r < n; ++r) vn += vh[r] * vh[r];
+    if (vn > 0.0) {
+      for (int j = k; j < n; ++j) {
+        double s = 0.0;
+        for (int r = k; r < n; ++r) s += vh[r] * at(r, j);
+        s = 2.0 * s / vn;
+        for (int r = k; r < n; ++r) at(r, j) -= s * vh[r];
+      }
+      double s = 0.0;
+      for (int r = k; r < n; ++r) s += vh[r] * b[r];
+      s = 2.0 * s / vn;
+      for (int r = k; r < n; ++r) b[r] -= s * vh[r];
+    }
+    diag[k] = at(k, k);
+  }
+  double absdet = 1.0, maxpiv = 0.0;
+  for (int k = 0; k < n; ++k) {
+    absdet *= std::fabs(diag[k]);
+    maxpiv = std::max(maxpiv, std::fabs(diag[k]));
+  }
+  const double thresh = maxpiv * 2.220446049250313e-16 * n;
+  int rank = 0;
+  for (int k = 0; k < n; ++k)
+    if (std::fabs(diag[k]) > thresh) ++rank;
+  std::vector<double> z(n, 0.0);
+  for (int k = rank - 1; k >= 0; --k) {
+    double s = b[k];
+    for (int j = k + 1; j < rank; ++j) s -= at(k, j) * z[j];
+    z[k] = s / at(k, k);
+  }
+  x.assign(n, 0.0);
+  for (int k = 0; k < n; ++k) x[perm[k]] = z[k];
+  return absdet;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- lifecycle
+CCSDContext::CCSDContext(int o, int v, const int64_t* occ_hm, int n_occ_tiles, const int64_t* vir_hm,
+                         int n_vir_tiles, int precision)
+    : o_(o), v_(v) {
+  try {
+    construct(occ_hm, n_occ_tiles, vir_hm, n_vir_tiles, precision);
+  } catch (...) {
+    release();
+    throw;
+  }
+}
+
+void CCSDContext::release() {
+  if (stream_) hipStreamSynchronize(stream_);
+  for (double* p : allocations_) hipFree(p);
+  allocations_.clear();
+  alloc_sizes_.clear();
+  if (modeinfo_) hipFree(modeinfo_);
+  modeinfo_ = nullptr;
+  if (staging_) hipFree(staging_);
+  staging_ = nullptr;
+  if (stream_) hipStreamDestroy(stream_);
+  stream_ = nullptr;
+}
+
+void CCSDContext::construct(const int64_t* occ_hm, int n_occ_tiles, const int64_t* vir_hm,
+                            int n_vir_tiles, int precision) {
+  const int o = o_, v = v_;
+  if (o <= 0 || v <= 0) throw std::invalid_argument("o and v must be positive");
+  if ((long)v * v * o * o >= (1L << 31)) throw std::invalid_argument("v^2 o^2 must be < 2^31");
+  if (precision < 0 || precision > 2) throw std::invalid_argument("unknown precision");
+  prec_ = (GemmPrecision)precision;
+  auto take_hm = [](const int64_t* hm, int nt, int n, std::vector<int64_t>& out) {
+    if (hm == nullptr) { out = {0, n}; return; }
+    if (nt <= 0) throw std::invalid_argument("tile count must be positive");
+    out.assign(hm, hm + nt + 1);
+    if (out.front() != 0 || out.back() != n) throw std::invalid_argument("hashmarks must span [0, n]");
+    for (int t = 0; t < nt; ++t)
+      if (out[t + 1] <= out[t]) throw std::invalid_argument("hashmarks must increase");
+  };
+  take_hm(occ_hm, n_occ_tiles, o, occ_hm_);
+  take_hm(vir_hm, n_vir_tiles, v, vir_hm_);
+  MPQC_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+
+  const long O = o, V = v;
+  const long ov = O * V, oo = O * O, vv = V * V;
+  const long oovv = oo * vv, ooov = oo * ov, ovvv = ov * vv, oooo = oo * oo;
+  Fia_ = alloc(ov); Fij_ = alloc(oo); Fab_ = alloc(vv);
+  Gijab_ = alloc(oovv); Gijkl_ = alloc(oooo); Giajb_ = alloc(oovv); Giabc_ = alloc(ovvv);
+  Gijka_ = alloc(ooov);
+  // Gabcd allocated on upload (largest tensor)
+  f_ai_ = alloc(ov); g_abij_ = alloc(oovv); Gbar_ = alloc(oovv); Gbar_cKLD_ = alloc(oovv);
+  Gbar_kcld_ = alloc(oovv); P6_ = alloc(oovv); Gbar_akcd_ = alloc(ovvv); P8_ = alloc(ooov);
+  S11_ = alloc(ooov); P13_ = alloc(ooov); P14_ = alloc(ovvv); J0_ = alloc(oovv); K0_ = alloc(oovv);
+  X18_ = alloc(ooov); X19_ = alloc(ooov); GA_ = alloc(oovv); GB_ = alloc(oovv); GC_ = alloc(oovv);
+  eo_ = alloc(O); ev_ = alloc(V);
+  t1_ = alloc(ov); t2_ = alloc(oovv); tau_ = alloc(oovv); r1_ = alloc(ov); r2_ = alloc(oovv);
+  FIJ_ = alloc(oo); FAB_ = alloc(vv);
+  Xoo_ = alloc(oo); Zoo_ = alloc(oo); hkc_ = alloc(ov); hT_ = alloc(ov); t1T_ = alloc(ov);
+  tau_akld_ = alloc(oovv); tau_ilab_ = alloc(oovv); Y5_ = alloc(oovv); tau_kcdi_ = alloc(oovv);
+  tau_ackl_ = alloc(oovv); Rd_ = alloc(oovv); Q_ = alloc(ooov); M12_ = alloc(ooov);
+  S_ = alloc(ooov); gki_ = alloc(oo); gac_ = alloc(vv); t2_abjk_ = alloc(oovv); X16_ = alloc(oovv);
+  TP_ = alloc(oovv); t2P_ = alloc(oovv); J_ = alloc(oovv); K_ = alloc(oovv); Zj_ = alloc(oovv);
+  Zk_ = alloc(oovv); Xjk_ = alloc(oovv); Y1_ = alloc(oovv); Y2_ = alloc(oovv); RR_ = alloc(oovv);
+  tmpR_ = alloc(oovv); Y23_ = alloc(oooo); W_ = alloc(oooo); tmp24_ = alloc(ooov);
+  Y24_ = alloc(oovv);
+  // split-K workspace: enough for 64 slices of the largest small-output GEMM
+  const long ws_elems = std::max<long>(64L * std::max({vv, ov, oo * oo, 4096L}), 1L << 20);
+  ws_.ptr = alloc(ws_elems);
+  ws_.bytes = ws_elems * sizeof(double);
+  partials_ = alloc(kReducePartials);
+  scalars_ = alloc(64);
+  solver_init(1, 8, 0.0, 1, 1);
+  MPQC_HIP_CHECK(hipMemsetAsync(t1_, 0, ov * sizeof(double), stream_));
+  MPQC_HIP_CHECK(hipMemsetAsync(t2_, 0, oovv * sizeof(double), stream_));
+  MPQC_HIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+CCSDContext::~CCSDContext() { release(); }
+
+double* CCSDContext::alloc(size_t n) {
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(double));
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    throw std::bad_alloc();
+  }
+  allocations_.push_back((double*)p);
+  alloc_sizes_.push_back(std::max<size_t>(n, 1));
+  return (double*)p;
+}
+
+void CCSDContext::check_access(const double* p, long max_off, const char* what) const {
+  for (size_t k = 0; k < allocations_.size(); ++k) {
+    const double* base = allocations_[k];
+    if (p >= base && p < base + alloc_sizes_[k]) {
+      if (max_off < 0 || (size_t)(p - base) + (size_t)max_off >= alloc_sizes_[k])
+        throw std::logic_error(std::string("out-of-bounds operand in ") + what);
+      return;
+    }
+  }
+  throw std::logic_error(std::string("operand not owned by the context in ") + what);
+}
+
+void CCSDContext::set_option(const std::string& key, double value) {
+  if (key == "fov_sign") {
+    if (value != 1.0 && value != -1.0) throw std::invalid_argument("fov_sign must be +1 or -1");
+    fov_sign_ = value;
+  } else if (key == "precision") {
+    const int p = (int)value;
+    if (p < 0 || p > 2 || p != value) throw std::invalid_argument("unknown precision");
+    prec_ = (GemmPrecision)p;
+  } else {
+    throw std::invalid_argument("unknown option: " + key);
+  }
+}
+
+// ---------------------------------------------------------------- transfers
+void CCSDContext::upload(double* dst, const double* src, const std::array<int, 4>& modes, int layout,
+                         int memspace) {
+  long n[4];
+  long total = 1;
+  for (int d = 0; d < 4; ++d) {
+    n[d] = modes[d] == kOcc ? o_ : modes[d] == kVir ? v_ : 1;
+    total *= n[d];
+  }
+  if (memspace != 0 && memspace != 1) throw std::invalid_argument("memspace must be host or device");
+  if (layout == 0) {
+    MPQC_HIP_CHECK(hipMemcpyAsync(dst, src, total * sizeof(double),
+                                  memspace ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream_));
+    MPQC_HIP_CHECK(hipStreamSynchronize(stream_));
+    return;
+  }
+  if (layout != 1) throw std::invalid_argument("layout must be dense or TA tiles");
+  // tiled: stage raw tiles on the device, then gather into the dense layout
+  const double* dsrc = src;
+  if (!memspace) {
+    if (staging_n_ < (size_t)total) {
+      if (staging_) MPQC_HIP_CHECK(hipFree(staging_));
+      staging_ = nullptr;
+      MPQC_HIP_CHECK(hipMalloc(&staging_, total * sizeof(double)));
+      staging_n_ = total;
+    }
+    MPQC_HIP_CHECK(hipMemcpyAsync(staging_, src, total * sizeof(double), hipMemcpyHostToDevice, stream_));
+    dsrc = staging_;
+  }
+  // mode info
+  const long* hms[4];
+  int nts[4];
+  for (int d = 0; d < 4; ++d) {
+    if (modes[d] == kOcc) { hms[d] = occ_hm_.data(); nts[d] = (int)occ_hm_.size() - 1; }
+    else if (modes[d] == kVir) { hms[d] = vir_hm_.data(); nts[d] = (int)vir_hm_.size() - 1; }
+    else { hms[d] = nullptr; nts[d] = 1; }
+  }
+  std::vector<long> mi(2 * (n[0] + n[1] + n[2] + n[3]));
+  static_assert(sizeof(long) == sizeof(int64_t), "long must be 64-bit");
+  make_tile_modeinfo(mi.data(), n, (const long* const*)hms, nts);
+  if (modeinfo_n_ < mi.size()) {
+    if (modeinfo_) MPQC_HIP_CHECK(hipFree(modeinfo_));
+    modeinfo_ = nullptr;
+    MPQC_HIP_CHECK(hipMalloc(&modeinfo_, mi.size() * sizeof(long)));
+    modeinfo_n_ = mi.size();
+  }
+  MPQC_HIP_CHECK(hipMemcpyAsync(modeinfo_, mi.data(), mi.size() * sizeof(long), hipMemcpyHostToDevice, stream_));
+  tiles_to_dense(dst, dsrc, n, modeinfo_, stream_);
+  MPQC_HIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+void CCSDContext::download(double* dst, const double* src, const std::array<int, 4>& modes, int layout,
+                           int memspace) {
+  long n[4];
+  long total = 1;
+  for (int d = 0; d < 4; ++d) {
+    n[d] = modes[d] == kOcc ? o_ : modes[d] == kVir ? v_ : 1;
+    total *= n[d];
+  }
+  if (memspace != 0 && memspace != 1) throw std::invalid_argument("memspace must be host or device");
+  if (layout == 0) {
+    MPQC_HIP_CHECK(hipMemcpyAsync(dst, src, total * sizeof(double),
+                                  memspace ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, stream_));
+    MPQC_HIP_CHECK(hipStreamSynchronize(stream_));
+    return;
+  }
+  if (layout != 1) throw std::invalid_argument("layout must be dense or TA tiles");
+  const long* hms[4];
+  int nts[4];
+  for (int d = 0; d < 4; ++d) {
+    if (modes[d] == kOcc) { hms[d] = occ_hm_.data(); nts[d] = (int)occ_hm_.size() - 1; }
+    else if (modes[d] == kVir) { hms[d] = vir_hm_.data(); nts[d] = (int)vir_hm_.size() - 1; }
+    else { hms[d] = nullptr; nts[d] = 1; }
+  }
+  std::vector<long> mi(2 * (n[0] + n[1] + n[2] + n[3]));
+  make_tile_modeinfo(mi.data(), n, (const long* const*)hms, nts);
+  if (modeinfo_n_ < mi.size()) {
+    if (modeinfo_) MPQC_HIP_CHECK(hipFree(modeinfo_));
+    modeinfo_ = nullptr;
+    MPQC_HIP_CHECK(hipMalloc(&modeinfo_, mi.size() * sizeof(long)));
+    modeinfo_n_ = mi.size();
+  }
+  MPQC_HIP_CHECK(hipMemcpyAsync(modeinfo_, mi.data(), mi.size() * sizeof(long), hipMemcpyHostToDevice, stream_));
+  double* ddst = dst;
+  if (!memspace) {
+    if (staging_n_ < (size_t)total) {
+      if (staging_) MPQC_HIP_CHECK(hipFree(staging_));
+      staging_ = nullptr;
+      MPQC_HIP_CHECK(hipMalloc(&staging_, total * sizeof(double)));
+      staging_n_ = total;
+    }
+    ddst = staging_;
+  }
+  dense_to_tiles(ddst, src, n, modeinfo_, stream_);
+  if (!memspace)
+    MPQC_HIP_CHECK(hipMemcpyAsync(dst, ddst, total * sizeof(double), hipMemcpyDeviceToHost, stream_));
+  MPQC_HIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+void CCSDContext::set_integral(int which, const double* data, int layout, int memspace) {
+  if (data == nullptr) throw std::invalid_argument("null integral buffer");
+  const int O = kOcc, V = kVir, X = kNone;
+  switch (which) {
+    case 0: upload(Fia_, data, {X, X, O, V}, layout, memspace); break;
+    case 1: upload(Fij_, data, {X, X, O, O}, layout, memspace); break;
+    case 2: upload(Fab_, data, {X, X, V, V}, layout, memspace); break;
+    case 3: upload(Gijab_, data, {O, O, V, V}, layout, memspace); break;
+    case 4: upload(Gijkl_, data, {O, O, O, O}, layout, memspace); break;
+    case 5: upload(Giajb_, data, {O, V, O, V}, layout, memspace); break;
+    case 6: upload(Giabc_, data, {O, V, V, V}, layout, memspace); break;
+    case 7: upload(Gijka_, data, {O, O, O, V}, layout, memspace); break;
+    case 8:
+      if (!Gabcd_) Gabcd_ = alloc((size_t)v_ * v_ * v_ * v_);
+      upload(Gabcd_, data, {V, V, V, V}, layout, memspace);
+      break;
+    default: throw std::invalid_argument("unknown integral id");
+  }
+  have_ints_ |= 1u << which;
+  prepared_ = false;
+}
+
+void CCSDContext::set_orbital_energies(const double* eo, const double* ev) {
+  if (!eo || !ev) throw std::invalid_argument("null orbital energies");
+  MPQC_HIP_CHECK(hipMemcpyAsync(eo_, eo, o_ * sizeof(double), hipMemcpyHostToDevice, stream_));
+  MPQC_HIP_CHECK(hipMemcpyAsync(ev_, ev, v_ * sizeof(double), hipMemcpyHostToDevice, stream_));
+  MPQC_HIP_CHECK(hipStreamSynchronize(stream_));
+  // zero denominators would raise SIGFPE in the reference (mpqc_init.cpp FPE traps)
+  for (int i = 0; i < o_; ++i)
+    for (int a = 0; a < v_; ++a)
+      if (eo[i] - ev[a] == 0.0) throw std::range_error("zero Jacobi denominator e_i - e_a");
+  have_eps_ = true;
+}
+
+void CCSDContext::solver_init(int diis_start, int n_diis, double damp, int ngroup, int ngroup_nstart) {
+  if (n_diis < 1 || n_diis > 16) throw std::invalid_argument("n_diis must be in [1,16]");
+  if (ngroup < 1 || diis_start < 0 || damp < 0.0) throw std::invalid_argument("invalid DIIS parameters");
+  diis_start_ = diis_start;
+  n_diis_ = n_diis;
+  diis_damp_ = damp;
+  diis_ngroup_ = ngroup;
+  diis_group_nstart_ = ngroup_nstart;
+  diis_iter_ = 0;
+  err_slots_.clear();
+  x_slots_.clear();
+  B_.assign((size_t)n_diis * n_diis, 0.0);
+  const long n1 = (long)o_ * v_, n2 = (long)o_ * o_ * v_ * v_;
+  while ((int)hist_e1_.size() < n_diis) {
+    hist_e1_.push_back(alloc(n1));
+    hist_e2_.push_back(alloc(n2));
+    hist_x1_.push_back(alloc(n1));
+    hist_x2_.push_back(alloc(n2));
+  }
+  solver_ready_ = true;
+}
+
+void CCSDContext::set_amplitudes(const double* t1, const double* t2, int layout, int memspace) {
+  upload(t1_, t1, {kNone, kNone, kVir, kOcc}, layout, memspace);
+  upload(t2_, t2, {kVir, kVir, kOcc, kOcc}, layout, memspace);
+  build_tau(tau_, t2_, t1_, o_, v_, stream_);
+  MPQC_HIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+void CCSDContext::get_amplitudes(double* t1, double* t2, int layout, int memspace) {
+  if (t1) download(t1, t1_, {kNone, kNone, kVir, kOcc}, layout, memspace);
+  if (t2) download(t2, t2_, {kVir, kVir, kOcc, kOcc}, layout, memspace);
+}
+
+void CCSDContext::get_residuals(double* r1, double* r2, int layout, int memspace) {
+  if (r1) download(r1, r1_, {kNone, kNone, kVir, kOcc}, layout, memspace);
+  if (r2) download(r2, r2_, {kVir, kVir, kOcc, kOcc}, layout, memspace);
+}
+
+void CCSDContext::get_dressed_fock(double* FIJ, double* FAB) {
+  if (FIJ) MPQC_HIP_CHECK(hipMemcpyAsync(FIJ, FIJ_, (size_t)o_ * o_ * sizeof(double), hipMemcpyDeviceToHost, stream_));
+  if (FAB) MPQC_HIP_CHECK(hipMemcpyAsync(FAB, FAB_, (size_t)v_ * v_ * sizeof(double), hipMemcpyDeviceToHost, stream_));
+  MPQC_HIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+// ---------------------------------------------------------------- helpers
+void CCSDContext::gemm(bool tA, bool tB, int M, int N, int K, double alpha, const double* A, long lda,
+                       const double* B, long ldb, double beta, double* C, long ldc, long sA, long sB,
+                       long sC, int batch) {
+  GemmDesc d = make_gemm(tA, tB, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
+  d.sA = sA; d.sB = sB; d.sC = sC; d.batch = batch;
+  if (M <= 0 || N <= 0 || K <= 0 || batch <= 0) throw std::logic_error("empty GEMM");
+  if (lda < (tA ? M : K) || ldb < (tB ? K : N) || ldc < N) throw std::logic_error("GEMM leading dimension too small");
+  const long bm = (long)(batch - 1);
+  check_access(A, (tA ? (long)(K - 1) * lda + (M - 1) : (long)(M - 1) * lda + (K - 1)) + bm * sA, "gemm A");
+  check_access(B, (tB ? (long)(N - 1) * ldb + (K - 1) : (long)(K - 1) * ldb + (N - 1)) + bm * sB, "gemm B");
+  check_access(C, (long)(M - 1) * ldc + (N - 1) + bm * sC, "gemm C");
+  mpqc_hip::gemm(d, prec_, ws_, stream_);
+}
+
+void CCSDContext::perm_add(double* out, std::array<long, 4> n, double beta,
+                           std::initializer_list<Src> srcs) {
+  TTerm terms[kMaxTerms];
+  int nt = 0;
+  if (srcs.size() > (size_t)kMaxTerms) throw std::logic_error("too many permutation terms");
+  check_access(out, n[0] * n[1] * n[2] * n[3] - 1, "perm_add out");
+  for (const Src& s : srcs) {
+    long ext[4] = {s.ext[0], s.ext[1], s.ext[2], s.ext[3]};
+    int perm[4] = {s.perm[0], s.perm[1], s.perm[2], s.perm[3]};
+    terms[nt].p = s.p;
+    permuted_strides(ext, perm, terms[nt].s);
+    {
+      long maxoff = 0, vol = 1;
+      bool seen[4] = {false, false, false, false};
+      for (int d = 0; d < 4; ++d) {
+        if (perm[d] < 0 || perm[d] > 3 || seen[perm[d]]) throw std::logic_error("bad permutation");
+        seen[perm[d]] = true;
+        if (ext[perm[d]] != n[d]) throw std::logic_error("permutation extent mismatch");
+        maxoff += (n[d] - 1) * terms[nt].s[d];
+        vol *= ext[d];
+      }
+      if (maxoff >= vol) throw std::logic_error("permutation reads past its input");
+      check_access(s.p, maxoff, "perm_add input");
+    }
+    terms[nt].alpha = s.alpha;
+    ++nt;
+  }
+  long nn[4] = {n[0], n[1], n[2], n[3]};
+  tadd(out, nn, beta, terms, nt, stream_);
+}
+
+double CCSDContext::scalar_from_device(const double* d) {
+  double h = 0.0;
+  MPQC_HIP_CHECK(hipMemcpyAsync(&h, d, sizeof(double), hipMemcpyDeviceToHost, stream_));
+  MPQC_HIP_CHECK(hipStreamSynchronize(stream_));
+  return h;
+}
+
+// ---------------------------------------------------------------- set-up
+// Integral layouts consumed by the GEMMs (constant for a solve).
+void CCSDContext::prepare() {
+  if ((have_ints_ & 0x1ff) != 0x1ff)
+    throw std::logic_error("all of Fia, Fij, Fab, Gijab, Gijkl, Giajb, Giabc, Gijka, Gabcd must be set");
+  const long O = o_, V = v_;
+  const std::array<long, 4> ovv_ext_ijab{O, O, V, V}, e_iajb{O, V, O, V}, e_iabc{O, V, V, V},
+      e_ijka{O, O, O, V};
+  const std::array<int, 4> I{0, 1, 2, 3};
+  // f_ai("a,i") = Fia("i,a")                                     ccsd.h:331-332
+  perm_add(f_ai_, {1, 1, V, O}, 0.0, {{Fia_, {1, 1, O, V}, {0, 1, 3, 2}, 1.0}});
+  // g_abij("a,b,i,j") = Gijab("i,j,a,b")                          ccsd.h:336-337
+  perm_add(g_abij_, {V, V, O, O}, 0.0, {{Gijab_, ovv_ext_ijab, {2, 3, 0, 1}, 1.0}});
+  // g_ijab_bar(k,l,c,d) = 2 G(k,l,c,d) - G(l,k,c,d)               ccsd_r1_r2.h:87
+  perm_add(Gbar_, {O, O, V, V}, 0.0,
+           {{Gijab_, ovv_ext_ijab, I, 2.0}, {Gijab_, ovv_ext_ijab, {1, 0, 2, 3}, -1.0}});
+  perm_add(Gbar_cKLD_, {V, O, O, V}, 0.0, {{Gbar_, ovv_ext_ijab, {2, 0, 1, 3}, 1.0}});
+  perm_add(Gbar_kcld_, {O, V, O, V}, 0.0, {{Gbar_, ovv_ext_ijab, {0, 2, 1, 3}, 1.0}});
+  // P6(a,i,c,k) = 2 Gijab(k,i,c,a) - Giajb(k,a,i,c)                ccsd_r1_r2.h:107-108
+  perm_add(P6_, {V, O, V, O}, 0.0,
+           {{Gijab_, ovv_ext_ijab, {3, 1, 2, 0}, 2.0}, {Giajb_, e_iajb, {1, 2, 3, 0}, -1.0}});
+  // g_iabc_bar(k,a,c,d) = 2 Giabc(k,a,c,d) - Giabc(k,a,d,c) as (a,k,c,d)   ccsd_r1_r2.h:112-113
+  perm_add(Gbar_akcd_, {V, O, V, V}, 0.0,
+           {{Giabc_, e_iabc, {1, 0, 2, 3}, 2.0}, {Giabc_, e_iabc, {1, 0, 3, 2}, -1.0}});
+  // P8(c,k,l,i) = 2 Gijka(l,k,i,c) - Gijka(k,l,i,c)                 ccsd_r1_r2.h:120-121
+  perm_add(P8_, {V, O, O, O}, 0.0,
+           {{Gijka_, e_ijka, {3, 1, 0, 2}, 2.0}, {Gijka_, e_ijka, {3, 0, 1, 2}, -1.0}});
+  // S11(a,i,j,k) = Gijka(j,i,k,a)                                   ccsd_r1_r2.h:210-211
+  perm_add(S11_, {V, O, O, O}, 0.0, {{Gijka_, e_ijka, {3, 1, 0, 2}, 1.0}});
+  // P13(k,i,l,c) = g_ijka_bar(k,l,i,c) = 2 Gijka(k,l,i,c) - Gijka(l,k,i,c)   ccsd_r1_r2.h:219
+  perm_add(P13_, {O, O, O, V}, 0.0,
+           {{Gijka_, e_ijka, {0, 2, 1, 3}, 2.0}, {Gijka_, e_ijka, {1, 2, 0, 3}, -1.0}});
+  // P14(a,c,k,d) = g_iabc_bar(k,a,d,c) = 2 Giabc(k,a,d,c) - Giabc(k,a,c,d)   ccsd_r1_r2.h:223
+  perm_add(P14_, {V, V, O, V}, 0.0,
+           {{Giabc_, e_iabc, {1, 3, 0, 2}, 2.0}, {Giabc_, e_iabc, {1, 2, 0, 3}, -1.0}});
+  // J0(a,i,k,c) = Gijab(i,k,a,c);  K0(a,i,k,c) = Giajb(k,a,i,c)     ccsd_r1_r2.h:239,245
+  perm_add(J0_, {V, O, O, V}, 0.0, {{Gijab_, ovv_ext_ijab, {2, 0, 1, 3}, 1.0}});
+  perm_add(K0_, {V, O, O, V}, 0.0, {{Giajb_, e_iajb, {1, 2, 0, 3}, 1.0}});
+  // X18(l,i,k,c) = Gijka(l,k,i,c);  X19(l,i,k,c) = Gijka(k,l,i,c)   ccsd_r1_r2.h:239,246
+  perm_add(X18_, {O, O, O, V}, 0.0, {{Gijka_, e_ijka, {0, 2, 1, 3}, 1.0}});
+  perm_add(X19_, {O, O, O, V}, 0.0, {{Gijka_, e_ijka, {1, 2, 0, 3}, 1.0}});
+  // GA(l,d,k,c) = Gijab(k,l,c,d); GB = 2 Gijab(k,l,c,d) - Gijab(k,l,d,c); GC = Gijab(k,l,d,c)
+  //                                                                 ccsd_r1_r2.h:241-243,248
+  perm_add(GA_, {O, V, O, V}, 0.0, {{Gijab_, ovv_ext_ijab, {1, 3, 0, 2}, 1.0}});
+  perm_add(GB_, {O, V, O, V}, 0.0,
+           {{Gijab_, ovv_ext_ijab, {1, 3, 0, 2}, 2.0}, {Gijab_, ovv_ext_ijab, {1, 2, 0, 3}, -1.0}});
+  perm_add(GC_, {O, V, O, V}, 0.0, {{Gijab_, ovv_ext_ijab, {1, 2, 0, 3}, 1.0}});
+  MPQC_HIP_CHECK(hipStreamSynchronize(stream_));
+  prepared_ = true;
+}
+
+void CCSDContext::init_guess() {
+  if (!prepared_) prepare();
+  const long ov = (long)o_ * v_, oovv = ov * ov;
+  // t1 = 0, t2 = 0, tau = 0; r1 = f_ai, r2 = g_abij                 ccsd.h:331-352
+  MPQC_HIP_CHECK(hipMemsetAsync(t1_, 0, ov * sizeof(double), stream_));
+  MPQC_HIP_CHECK(hipMemsetAsync(t2_, 0, oovv * sizeof(double), stream_));
+  MPQC_HIP_CHECK(hipMemsetAsync(tau_, 0, oovv * sizeof(double), stream_));
+  MPQC_HIP_CHECK(hipMemcpyAsync(r1_, f_ai_, ov * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+  MPQC_HIP_CHECK(hipMemcpyAsync(r2_, g_abij_, oovv * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+  MPQC_HIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+// ---------------------------------------------------------------- residual
+void CCSDContext::residual() {
+  if (!prepared_) prepare();
+  const int o = o_, v = v_;
+  const long O = o, V = v;
+  const long ov = O * V, oo = O * O, vv = V * V;
+  const std::array<long, 4> e_abij{V, V, O, O};
+  const std::array<int, 4> I{0, 1, 2, 3};
+
+  // tau("a,b,i,j") = t2("a,b,i,j") + t1("a,i") t1("b,j")            ccsd.h:389
+  build_tau(tau_, t2_, t1_, o, v, stream_);
+  // amplitude layouts used by several GEMMs
+  perm_add(t1T_, {1, 1, O, V}, 0.0, {{t1_, {1, 1, V, O}, {0, 1, 3, 2}, 1.0}});  // t1T(k,d) = t1(d,k)
+
+  // ===================== r1: ccsd_r1_r2.h:70-124 =====================
+  // :83-84  r1(a,i) = Fia(i,a) - 2 (Fia(k,c) t1(c,i)) t1(a,k)
+  gemm(false, false, o, o, v, 1.0, Fia_, v, t1_, o, 0.0, Xoo_, o);
+  MPQC_HIP_CHECK(hipMemcpyAsync(r1_, f_ai_, ov * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+  gemm(false, false, v, o, o, -2.0, t1_, o, Xoo_, o, 1.0, r1_, o);
+  // :89-91  h_ac = F_vv = Fab(a,c) - gbar(k,l,c,d) tau(a,d,k,l); FAB = h_ac; r1 += h_ac t1
+  //         (ccsd_intermediates.h:71-77)
+  perm_add(tau_akld_, {V, O, O, V}, 0.0, {{tau_, e_abij, {0, 2, 3, 1}, 1.0}});
+  MPQC_HIP_CHECK(hipMemcpyAsync(FAB_, Fab_, vv * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+  gemm(false, true, v, v, o * o * v, -1.0, tau_akld_, oo * V, Gbar_cKLD_, oo * V, 1.0, FAB_, v);
+  gemm(false, false, v, o, v, 1.0, FAB_, v, t1_, o, 1.0, r1_, o);
+  // :95-97  h_ki = F_oo = Fij(k,i) + gbar(k,l,a,b) tau(a,b,i,l); FIJ = h_ki; r1 -= t1 h_ki
+  //         (ccsd_intermediates.h:57-63)
+  perm_add(tau_ilab_, {O, O, V, V}, 0.0, {{tau_, e_abij, {2, 3, 0, 1}, 1.0}});
+  MPQC_HIP_CHECK(hipMemcpyAsync(FIJ_, Fij_, oo * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+  gemm(false, true, o, o, o * v * v, 1.0, Gbar_, O * vv, tau_ilab_, O * vv, 1.0, FIJ_, o);
+  gemm(false, false, v, o, o, -1.0, t1_, o, FIJ_, o, 1.0, r1_, o);
+  // :101-104 h_kc = F_ov = Fia(k,c) + gbar(k,l,c,d) t1(d,l)   (ccsd_intermediates.h:89-95)
+  //          r1 += h_kc (2 t2(c,a,k,i) - t2(c,a,i,k) + t1(c,i) t1(a,k))
+  MPQC_HIP_CHECK(hipMemcpyAsync(hkc_, Fia_, ov * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+  gemm(false, false, o * v, 1, o * v, 1.0, Gbar_kcld_, ov, t1T_, 1, 1.0, hkc_, 1);
+  perm_add(hT_, {1, 1, V, O}, 0.0, {{hkc_, {1, 1, O, V}, {0, 1, 3, 2}, 1.0}});  // hT(c,k) = h(k,c)
+  perm_add(Y5_, {V, O, V, O}, 0.0,
+           {{t2_, e_abij, {1, 3, 0, 2}, 2.0}, {t2_, e_abij, {1, 2, 0, 3}, -1.0}});
+  gemm(false, false, v * o, 1, v * o, 1.0, Y5_, ov, hT_, 1, 1.0, r1_, 1);
+  gemm(false, false, o, o, v, 1.0, hkc_, v, t1_, o, 0.0, Zoo_, o);
+  gemm(false, false, v, o, o, 1.0, t1_, o, Zoo_, o, 1.0, r1_, o);
+  // :107-108 r1 += (2 Gijab(k,i,c,a) - Giajb(k,a,i,c)) t1(c,k)
+  gemm(false, false, v * o, 1, v * o, 1.0, P6_, ov, t1_, 1, 1.0, r1_, 1);
+  // :110-114 r1 += g_iabc_bar(k,a,c,d) tau(c,d,k,i)
+  perm_add(tau_kcdi_, {O, V, V, O}, 0.0, {{tau_, e_abij, {2, 0, 1, 3}, 1.0}});
+  gemm(false, false, v, o, o * v * v, 1.0, Gbar_akcd_, O * vv, tau_kcdi_, o, 1.0, r1_, o);
+  // :120-121 r1 -= (2 Gijka(l,k,i,c) - Gijka(k,l,i,c)) tau(c,a,k,l)
+  perm_add(tau_ackl_, {V, V, O, O}, 0.0, {{tau_, e_abij, {1, 0, 2, 3}, 1.0}});
+  gemm(false, false, v, o, o * o * v, -1.0, tau_ackl_, oo * V, P8_, o, 1.0, r1_, o);
+
+  // ===================== r2: ccsd_r1_r2.h:198-293 =====================
+  // :207-208 Rd = (Giabc(i,c,a,b) - Giajb(k,b,i,c) t1(a,k)) t1(c,j)
+  //   first term, batched over i: Rd[(a,b), i, j] = sum_c Giabc[i][c,(a,b)] t1[c,j]
+  gemm(true, false, v * v, o, v, 1.0, Giabc_, vv, t1_, o, 0.0, Rd_, oo, vv * V, 0, O, o);
+  //   second term as t1(a,k) * (Giajb(k,b,i,c) t1(c,j))
+  gemm(false, false, o * o * v, o, v, 1.0, Giajb_, v, t1_, o, 0.0, Q_, o);
+  gemm(false, false, v, v * o * o, o, -1.0, t1_, o, Q_, V * oo, 1.0, Rd_, V * oo);
+  // :210-211 Rd -= (Gijka(j,i,k,a) + Gijab(i,k,a,c) t1(c,j)) t1(b,k)
+  gemm(false, false, o * o * v, o, v, 1.0, Gijab_, v, t1_, o, 0.0, M12_, o);
+  perm_add(S_, {V, O, O, O}, 0.0,
+           {{S11_, {V, O, O, O}, I, 1.0}, {M12_, {O, O, V, O}, {2, 0, 3, 1}, 1.0}});
+  gemm(false, true, v, o * o, o, -1.0, t1_, o, S_, o, 1.0, Rd_, oo, 0, O * oo, V * oo, v);
+  // :218-220 g_ki = F_OO(FIJ, Fia, g_ijka_bar, t1) = FIJ + Fia t1 + gbar(k,l,i,c) t1(c,l)
+  //          (ccsd_intermediates.h:105-112)
+  MPQC_HIP_CHECK(hipMemcpyAsync(gki_, FIJ_, oo * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+  gemm(false, false, o, o, v, 1.0, Fia_, v, t1_, o, 1.0, gki_, o);
+  gemm(false, false, o * o, 1, o * v, 1.0, P13_, ov, t1T_, 1, 1.0, gki_, 1);
+  // :222-224 g_ac = F_VV(FAB, Fia, g_iabc_bar, t1) = FAB + f_ov(k,c) t1(a,k) + gbar(k,a,d,c) t1(d,k)
+  //          (ccsd_intermediates.h:141-148; fov_sign = +1 is the reference as written)
+  MPQC_HIP_CHECK(hipMemcpyAsync(gac_, FAB_, vv * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+  gemm(false, false, v, v, o, fov_sign_, t1_, o, Fia_, v, 1.0, gac_, v);
+  gemm(false, false, v * v, 1, o * v, 1.0, P14_, ov, t1T_, 1, 1.0, gac_, 1);
+  // :226 Rd += g_ac(a,c) t2(c,b,i,j) - g_ki(k,i) t2(a,b,k,j)
+  gemm(false, false, v, v * o * o, v, 1.0, gac_, v, t2_, V * oo, 1.0, Rd_, V * oo);
+  perm_add(t2_abjk_, {V, V, O, O}, 0.0, {{t2_, e_abij, {0, 1, 3, 2}, 1.0}});
+  gemm(false, false, v * v * o, o, o, 1.0, t2_abjk_, o, gki_, o, 0.0, X16_, o);
+  perm_add(Rd_, {V, V, O, O}, 1.0, {{X16_, e_abij, {0, 1, 3, 2}, -1.0}});
+  // :236 T(d,b,i,l) = 0.5 t2(d,b,i,l) + t1(d,i) t1(b,l), as TP(a,i,l,d) = T(d,a,i,l)
+  build_TP(TP_, t2_, t1_, o, v, stream_);
+  perm_add(t2P_, {V, O, O, V}, 0.0, {{t2_, e_abij, {0, 2, 3, 1}, 1.0}});  // t2P(a,i,l,d) = t2(a,d,i,l)
+  // :238-243 j(a,k,i,c) stored as J(a,i,k,c):
+  //   Gijab(i,k,a,c) - Gijka(l,k,i,c) t1(a,l) + Giabc(k,a,c,d) t1(d,i)
+  //   - Gijab(k,l,c,d) T(d,a,i,l) + 0.5 (2 Gijab(k,l,c,d) - Gijab(k,l,d,c)) t2(a,d,i,l)
+  // :245-248 k(k,a,i,c) stored as K(a,i,k,c):
+  //   Giajb(k,a,i,c) - Gijka(k,l,i,c) t1(a,l) + Giabc(k,a,d,c) t1(d,i) - Gijab(k,l,d,c) T(d,a,i,l)
+  MPQC_HIP_CHECK(hipMemcpyAsync(J_, J0_, ov * ov * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+  MPQC_HIP_CHECK(hipMemcpyAsync(K_, K0_, ov * ov * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+  gemm(false, false, v, o * o * v, o, -1.0, t1_, o, X18_, oo * V, 1.0, J_, oo * V);
+  gemm(false, false, v, o * o * v, o, -1.0, t1_, o, X19_, oo * V, 1.0, K_, oo * V);
+  gemm(false, false, o * v * v, o, v, 1.0, Giabc_, v, t1_, o, 0.0, Zj_, o);  // Zj(k,a,c,i)
+  gemm(true, false, v, o, v, 1.0, Giabc_, v, t1_, o, 0.0, Zk_, o, vv, 0, V * O, o * v);  // Zk(k,a,c,i)
+  perm_add(J_, {V, O, O, V}, 1.0, {{Zj_, {O, V, V, O}, {1, 3, 0, 2}, 1.0}});
+  perm_add(K_, {V, O, O, V}, 1.0, {{Zk_, {O, V, V, O}, {1, 3, 0, 2}, 1.0}});
+  gemm(false, false, o * v, o * v, o * v, -1.0, TP_, ov, GA_, ov, 1.0, J_, ov);
+  gemm(false, false, o * v, o * v, o * v, 0.5, t2P_, ov, GB_, ov, 1.0, J_, ov);
+  gemm(false, false, o * v, o * v, o * v, -1.0, TP_, ov, GC_, ov, 1.0, K_, ov);
+  // :251-252 r2 += 0.5 (2 j(a,k,i,c) - k(k,a,i,c)) (2 t2(c,b,k,j) - t2(b,c,k,j))  -> RR(a,i,b,j)
+  perm_add(Xjk_, {V, O, O, V}, 0.0, {{J_, {V, O, O, V}, I, 2.0}, {K_, {V, O, O, V}, I, -1.0}});
+  perm_add(Y1_, {O, V, V, O}, 0.0,
+           {{t2_, e_abij, {2, 0, 1, 3}, 2.0}, {t2_, e_abij, {2, 1, 0, 3}, -1.0}});
+  gemm(false, false, o * v, o * v, o * v, 0.5, Xjk_, ov, Y1_, ov, 0.0, RR_, ov);
+  // :254-256 tmp(a,b,i,j) = k(k,a,i,c) t2(c,b,j,k) -> tmpR(a,i,b,j); r2 -= 0.5 tmp + tmp(b,a,i,j)
+  perm_add(Y2_, {O, V, V, O}, 0.0, {{t2_, e_abij, {3, 0, 1, 2}, 1.0}});
+  gemm(false, false, o * v, o * v, o * v, 1.0, K_, ov, Y2_, ov, 0.0, tmpR_, ov);
+  // :260 r2(a,b,i,j) = r2(a,b,i,j) + r2(b,a,j,i); :262 r2 += Gijab(i,j,a,b)
+  {
+    const std::array<long, 4> e_aibj{V, O, V, O};
+    perm_add(r2_, {V, V, O, O}, 0.0,
+             {{Rd_, e_abij, I, 1.0},
+              {Rd_, e_abij, {1, 0, 3, 2}, 1.0},
+              {RR_, e_aibj, {0, 2, 1, 3}, 1.0},
+              {RR_, e_aibj, {2, 0, 3, 1}, 1.0},
+              {tmpR_, e_aibj, {0, 2, 1, 3}, -0.5},
+              {tmpR_, e_aibj, {2, 0, 1, 3}, -1.0},
+              {tmpR_, e_aibj, {2, 0, 3, 1}, -0.5},
+              {tmpR_, e_aibj, {0, 2, 3, 1}, -1.0},
+              {Gijab_, {O, O, V, V}, {2, 3, 0, 1}, 1.0}});
+  }
+  // :266-269 a_klij = W_oooo (ccsd_intermediates.h:182-193); r2 += a_klij(k,l,i,j) tau(a,b,k,l)
+  gemm(false, false, o * o * o, o, v, 1.0, Gijka_, v, t1_, o, 0.0, Y23_, o);
+  perm_add(W_, {O, O, O, O}, 0.0,
+           {{Gijkl_, {O, O, O, O}, I, 1.0}, {Y23_, {O, O, O, O}, I, 1.0},
+            {Y23_, {O, O, O, O}, {1, 0, 3, 2}, 1.0}});
+  gemm(false, false, o * o, o * o, v * v, 1.0, Gijab_, vv, tau_, oo, 1.0, W_, oo);
+  gemm(false, false, v * v, o * o, o * o, 1.0, tau_, oo, W_, oo, 1.0, r2_, oo);
+  // :275 b(a,b,i,j) = tau(c,d,i,j) Gabcd(a,b,c,d)  -- the particle-particle ladder
+  gemm(false, false, v * v, o * o, v * v, 1.0, Gabcd_, vv, tau_, oo, 1.0, r2_, oo);
+  // :277-280 tmp(k,a,i,j) = Giabc(k,a,c,d) tau(c,d,i,j); b -= tmp(k,a,j,i) t1(b,k) + tmp(k,b,i,j) t1(a,k)
+  gemm(false, false, o * v, o * o, v * v, 1.0, Giabc_, vv, tau_, oo, 0.0, tmp24_, oo);
+  gemm(true, true, v * o * o, v, o, 1.0, tmp24_, V * oo, t1_, o, 0.0, Y24_, v);  // Y24(a,j,i,b)
+  perm_add(r2_, {V, V, O, O}, 1.0, {{Y24_, {V, O, O, V}, {0, 3, 2, 1}, -1.0}});
+  gemm(false, false, v, v * o * o, o, -1.0, t1_, o, tmp24_, V * oo, 1.0, r2_, V * oo);
+}
+
+// ---------------------------------------------------------------- energy / error / update
+void CCSDContext::energy_error(double* E, double* err) {
+  if (!prepared_) prepare();
+  const long n1 = (long)o_ * v_, n2 = n1 * n1;
+  // error = sqrt(|r1|^2 + |r2|^2) / (size(r1) + size(r2))   qc/cc/solvers.h:60-63
+  dot(r1_, r1_, n1, scalars_ + 0, partials_, stream_);
+  dot(r2_, r2_, n2, scalars_ + 1, partials_, stream_);
+  // E = 2 <f_ai + r1, t1> + <g_abij + r2, 2 tau(a,b,i,j) - tau(b,a,i,j)>   ccsd.h:377-379
+  ccsd_energy(f_ai_, r1_, t1_, g_abij_, r2_, tau_, o_, v_, scalars_ + 2, partials_, stream_);
+  double h[3];
+  MPQC_HIP_CHECK(hipMemcpyAsync(h, scalars_, 3 * sizeof(double), hipMemcpyDeviceToHost, stream_));
+  MPQC_HIP_CHECK(hipStreamSynchronize(stream_));
+  const double e = std::sqrt(h[0] + h[1]) / (double)(n1 + n2);
+  if (!std::isfinite(e) || !std::isfinite(h[2]))
+    throw std::range_error("non-finite residual or energy");
+  if (err) *err = e;
+  if (E) *E = h[2];
+}
+
+void CCSDContext::check_finite(const double* x, long n, const char* what) {
+  count_nonfinite(x, n, scalars_ + 8, partials_, stream_);
+  if (scalar_from_device(scalars_ + 8) != 0.0) throw std::range_error(std::string("non-finite ") + what);
+}
+
+// DIISSolver::update (qc/cc/solvers.h:112-121): Jacobi update_only
+// (lcao/cc/solvers.h:1153-1158), then TA::DIIS::extrapolate(t, r) with the
+// pre-update residuals as error vectors; then tau (ccsd.h:389).
+void CCSDContext::update() {
+  if (!have_eps_) throw std::logic_error("orbital energies not set");
+  if (!solver_ready_) throw std::logic_error("solver not initialised");
+  const long n1 = (long)o_ * v_, n2 = n1 * n1;
+  // --- DIIS: push the error (copy of r) into the history
+  ++diis_iter_;
+  const double scale = 1.0 + diis_damp_;
+  if ((int)err_slots_.size() == n_diis_) {
+    err_slots_.pop_front();
+    // drop first row/column of B
+    const int m = n_diis_;
+    std::vector<double> Bn((size_t)m * m, 0.0);
+    for (int i = 1; i < m; ++i)
+      for (int j = 1; j < m; ++j) Bn[(size_t)(i - 1) * m + (j - 1)] = B_[(size_t)i * m + j];
+    B_ = Bn;
+  }
+  int eslot = 0;
+  {
+    std::vector<bool> used(n_diis_, false);
+    for (int s : err_slots_) used[s] = true;
+    while (used[eslot]) ++eslot;
+  }
+  MPQC_HIP_CHECK(hipMemcpyAsync(hist_e1_[eslot], r1_, n1 * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+  MPQC_HIP_CHECK(hipMemcpyAsync(hist_e2_[eslot], r2_, n2 * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+  err_slots_.push_back(eslot);
+  const int nvec = (int)err_slots_.size();
+  {
+    std::vector<const double*> x1, x2;
+    for (int s : err_slots_) { x1.push_back(hist_e1_[s]); x2.push_back(hist_e2_[s]); }
+    multi_dot2(x1.data(), x2.data(), nvec, r1_, r2_, n1, n2, scalars_ + 16, partials_, stream_);
+    double h[16];
+    MPQC_HIP_CHECK(hipMemcpyAsync(h, scalars_ + 16, nvec * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    MPQC_HIP_CHECK(hipStreamSynchronize(stream_));
+    const int m = n_diis_;
+    for (int i = 0; i < nvec; ++i) B_[(size_t)i * m + (nvec - 1)] = B_[(size_t)(nvec - 1) * m + i] = h[i];
+  }
+  // --- Jacobi: t += r / D
+  jacobi_update(t1_, t2_, r1_, r2_, eo_, ev_, o_, v_, stream_);
+  // --- push x = t into its history
+  if ((int)x_slots_.size() == n_diis_) x_slots_.pop_front();
+  int xslot = 0;
+  {
+    std::vector<bool> used(n_diis_, false);
+    for (int s : x_slots_) used[s] = true;
+    while (used[xslot]) ++xslot;
+  }
+  MPQC_HIP_CHECK(hipMemcpyAsync(hist_x1_[xslot], t1_, n1 * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+  MPQC_HIP_CHECK(hipMemcpyAsync(hist_x2_[xslot], t2_, n2 * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+  x_slots_.push_back(xslot);
+  const bool do_extrap =
+      diis_iter_ > diis_start_ && ((diis_iter_ - diis_start_) % diis_ngroup_) < diis_group_nstart_;
+  if (do_extrap) {
+    const double zero_determinant = 1.0e-15, zero_norm = 1.0e-10;
+    const int m = n_diis_;
+    int nskip = 0;
+    double absdet = 0.0;
+    std::vector<double> C;
+    while (true) {
+      const int rank = nvec - nskip + 1;
+      std::vector<double> A((size_t)rank * rank, 0.0), rhs(rank, 0.0);
+      for (int i = 0; i < rank; ++i) { A[i] = -1.0; A[(size_t)i * rank] = -1.0; }
+      A[0] = 0.0;
+      rhs[0] = -1.0;
+      double norm = 1.0;
+      const double b00 = B_[(size_t)nskip * m + nskip];
+      if (std::fabs(b00) > zero_norm) norm = 1.0 / std::fabs(b00);
+      for (int i = 1; i < rank; ++i)
+        for (int j = 1; j < rank; ++j)
+          A[(size_t)i * rank + j] = B_[(size_t)(i - 1 + nskip) * m + (j - 1 + nskip)] * norm;
+      for (int i = 0; i < rank; ++i) A[(size_t)i * rank + i] *= scale;
+      absdet = colpiv_qr_solve(rank, A, rhs, C);
+      ++nskip;
+      if (!(absdet < zero_determinant && nskip < nvec)) break;
+    }
+    if (absdet < zero_determinant) throw std::domain_error("DIIS: poorly-conditioned system");
+    --nskip;
+    std::vector<const double*> x1, x2;
+    std::vector<double> c;
+    int k = 0;
+    for (int s : x_slots_) {
+      if (k >= nskip) {
+        x1.push_back(hist_x1_[s]);
+        x2.push_back(hist_x2_[s]);
+        c.push_back(C[k - nskip + 1]);
+      }
+      ++k;
+    }
+    lincomb(t1_, x1.data(), c.data(), (int)c.size(), n1, stream_);
+    lincomb(t2_, x2.data(), c.data(), (int)c.size(), n2, stream_);
+  }
+  // tau("a,b,i,j") = t2 + t1 t1                                     ccsd.h:389
+  build_tau(tau_, t2_, t1_, o_, v_, stream_);
+  MPQC_HIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+// CCSD::compute_ccsd solver loop (ccsd.h:344-477) with method = "standard".
+double CCSDContext::solve(int max_iter, double target_precision, int verbose, int* n_iter) {
+  if (!have_eps_) throw std::logic_error("orbital energies not set");
+  init_guess();
+  solver_init(diis_start_, n_diis_, diis_damp_, diis_ngroup_, diis_group_nstart_);
+  trace_.clear();
+  double E0 = 0.0, E1 = 0.0, dE = 0.0, err = 1.0;
+  int iter = 0;
+  auto t0 = std::chrono::steady_clock::now();
+  while (iter < max_iter) {
+    energy_error(&E1, &err);  // error (:373) and energy (:377-379); E1 uses t of this pass
+    dE = std::fabs(E0 - E1);
+    E0 = E1;
+    const bool converged = dE <= target_precision * 0.1 && err <= target_precision;  // :141-152
+    if (iter == 0 || !converged) {
+      update();
+      auto t1 = std::chrono::steady_clock::now();
+      if (iter > 0) {
+        const double sec = std::chrono::duration<double>(t1 - t0).count();
+        trace_.push_back({iter, dE, err, E1, sec});
+        if (verbose) printf("%3i \t %10.5e \t %10.5e \t %15.12f \t %10.4f \n", iter, dE, err, E1, sec);
+      }
+    } else {
+      auto t1 = std::chrono::steady_clock::now();
+      const double sec = std::chrono::duration<double>(t1 - t0).count();
+      trace_.push_back({iter, dE, err, E1, sec});
+      if (verbose) printf("%3i \t %10.5e \t %10.5e \t %15.12f \t %10.4f \n", iter, dE, err, E1, sec);
+      break;
+    }
+    t0 = std::chrono::steady_clock::now();
+    residual();
+    ++iter;
+  }
+  if (iter >= max_iter && verbose) printf("\n Warning!! Exceed Max Iteration! \n");
+  if (n_iter) *n_iter = iter;
+  return E1;
+}
+
+}  // namespace mpqc_hip

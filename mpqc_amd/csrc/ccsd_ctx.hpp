@@ -1,0 +1,126 @@
+// Device-resident CCSD solver context: integrals, amplitudes, residuals,
+// intermediates and DIIS history all live in HBM for the whole solve.
+#pragma once
+
+#include <array>
+#include <deque>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "gemm.hpp"
+#include "tensor_ops.hpp"
+
+namespace mpqc_hip {
+
+struct DeviceBuffer {
+  double* p = nullptr;
+  size_t n = 0;  // elements
+};
+
+struct TraceRow {
+  int iter;
+  double dE, error, energy, seconds;
+};
+
+class CCSDContext {
+ public:
+  CCSDContext(int o, int v, const int64_t* occ_hm, int n_occ_tiles, const int64_t* vir_hm,
+              int n_vir_tiles, int precision);
+  ~CCSDContext();
+
+  int o() const { return o_; }
+  int v() const { return v_; }
+
+  void set_option(const std::string& key, double value);
+  void set_integral(int which, const double* data, int layout, int memspace);
+  void set_orbital_energies(const double* eo, const double* ev);
+  void solver_init(int diis_start, int n_diis, double damp, int ngroup, int ngroup_nstart);
+  void set_amplitudes(const double* t1, const double* t2, int layout, int memspace);
+  void get_amplitudes(double* t1, double* t2, int layout, int memspace);
+  void get_residuals(double* r1, double* r2, int layout, int memspace);
+  void get_dressed_fock(double* FIJ, double* FAB);
+
+  void init_guess();
+  void residual();
+  void energy_error(double* E, double* err);
+  void update();
+  double solve(int max_iter, double target_precision, int verbose, int* n_iter);
+  const std::vector<TraceRow>& trace() const { return trace_; }
+
+  std::string last_error;
+
+ private:
+  // tensor upload / download in dense or TA tile layout; `modes` gives for each
+  // of the 4 modes whether it is occupied (0), virtual (1) or absent (-1).
+  void upload(double* dst, const double* src, const std::array<int, 4>& modes, int layout,
+              int memspace);
+  void download(double* dst, const double* src, const std::array<int, 4>& modes, int layout,
+                int memspace);
+  double* alloc(size_t n);
+  // Host-side guard: throws unless [p, p + max_off] lies inside one of this
+  // context's device allocations.  Every GEMM and permutation the residual
+  // issues is checked before launch.
+  void check_access(const double* p, long max_off, const char* what) const;
+  void construct(const int64_t* occ_hm, int n_occ_tiles, const int64_t* vir_hm, int n_vir_tiles,
+                 int precision);
+  void release();
+  void prepare();
+  void gemm(bool tA, bool tB, int M, int N, int K, double alpha, const double* A, long lda,
+            const double* B, long ldb, double beta, double* C, long ldc, long sA = 0, long sB = 0,
+            long sC = 0, int batch = 1);
+  struct Src {
+    const double* p;
+    std::array<long, 4> ext;
+    std::array<int, 4> perm;
+    double alpha;
+  };
+  void perm_add(double* out, std::array<long, 4> n, double beta, std::initializer_list<Src> srcs);
+  void check_finite(const double* x, long n, const char* what);
+  double scalar_from_device(const double* d);
+
+  int o_, v_;
+  GemmPrecision prec_;
+  double fov_sign_ = 1.0;
+  hipStream_t stream_ = nullptr;
+  std::vector<int64_t> occ_hm_, vir_hm_;
+  std::vector<double*> allocations_;
+  std::vector<size_t> alloc_sizes_;
+
+  // integrals
+  double *Fia_ = nullptr, *Fij_ = nullptr, *Fab_ = nullptr, *Gijab_ = nullptr, *Gijkl_ = nullptr,
+         *Giajb_ = nullptr, *Giabc_ = nullptr, *Gijka_ = nullptr, *Gabcd_ = nullptr;
+  unsigned have_ints_ = 0;
+  bool prepared_ = false;
+  // derived integral layouts (built once in prepare())
+  double *f_ai_, *g_abij_, *Gbar_, *Gbar_cKLD_, *Gbar_kcld_, *P6_, *Gbar_akcd_, *P8_, *S11_,
+      *P13_, *P14_, *J0_, *K0_, *X18_, *X19_, *GA_, *GB_, *GC_;
+  // orbital energies
+  double *eo_ = nullptr, *ev_ = nullptr;
+  bool have_eps_ = false;
+  // amplitudes / residuals
+  double *t1_, *t2_, *tau_, *r1_, *r2_, *FIJ_, *FAB_;
+  // per-iteration scratch
+  double *Xoo_, *Zoo_, *hkc_, *hT_, *t1T_, *tau_akld_, *tau_ilab_, *Y5_, *tau_kcdi_, *tau_ackl_,
+      *Rd_, *Q_, *M12_, *S_, *gki_, *gac_, *t2_abjk_, *X16_, *TP_, *t2P_, *J_, *K_, *Zj_, *Zk_,
+      *Xjk_, *Y1_, *Y2_, *RR_, *tmpR_, *Y23_, *W_, *tmp24_, *Y24_;
+  GemmWorkspace ws_;
+  double* partials_ = nullptr;
+  double* scalars_ = nullptr;  // device scalars
+  long* modeinfo_ = nullptr;   // device tile maps (scratch, sized per call)
+  size_t modeinfo_n_ = 0;
+  double* staging_ = nullptr;  // device staging for tiled / host uploads
+  size_t staging_n_ = 0;
+
+  // DIIS (TA::DIIS semantics, see DIISState in ccsd_ctx.hip)
+  int diis_start_ = 1, n_diis_ = 8, diis_ngroup_ = 1, diis_group_nstart_ = 1;
+  double diis_damp_ = 0.0;
+  int diis_iter_ = 0;
+  std::deque<int> err_slots_, x_slots_;
+  std::vector<double> B_;  // n_diis x n_diis, row-major (only top-left nvec used)
+  std::vector<double*> hist_e1_, hist_e2_, hist_x1_, hist_x2_;
+  bool solver_ready_ = false;
+  std::vector<TraceRow> trace_;
+};
+
+}  // namespace mpqc_hip

@@ -1,0 +1,259 @@
+// Batched strided split-K GEMM on gfx950 matrix cores.
+//
+// Tile: BM x BN per workgroup of 256 threads (4 wave64s arranged 2x2), each
+// wave owns a (BM/2) x (BN/2) sub-tile made of 16x16 MFMA blocks.  K is staged
+// through LDS in BK=16 slices, double buffered: the next slice is fetched into
+// registers while the matrix cores work on the current one.
+//
+// FP64 path: v_mfma_f64_16x16x4_f64 (one double of A and of B per lane per
+// instruction, 4 doubles of C per lane).
+//
+// Operand fragments for a 16x16x4 block (lane l, r = 0..3):
+//   A[m = l%16][k = l/16], B[k = l/16][n = l%16], C[m = 4*r + l/16][n = l%16]
+// (the accumulator layout was established on the GPU; tests/test_gpu_gemm.py
+// checks every tile position through full GEMMs).
+#include "gemm.hpp"
+
+#include <algorithm>
+
+namespace mpqc_hip {
+
+namespace {
+
+constexpr int kBK = 16;
+constexpr int kThreads = 256;
+
+struct KParams {
+  const double* A;
+  const double* B;
+  double* C;
+  double* W;  // split-K workspace (nsplit > 1)
+  long lda, ldb, ldc, sA, sB, sC;
+  int M, N, K;
+  int kchunk;
+  int nsplit;
+  int batch0;  // first batch index of this launch
+  int tiles_n;
+  double alpha, beta;
+};
+
+template <int BM, int BN, bool TA, bool TB>
+__global__ __launch_bounds__(kThreads, 2) void dgemm_f64_kernel(KParams p) {
+  constexpr int PAD = 1;
+  constexpr int LDA_S = BM + PAD;
+  constexpr int LDB_S = BN + PAD;
+  constexpr int A_PER = BM * kBK / kThreads;
+  constexpr int B_PER = BN * kBK / kThreads;
+  constexpr int WM = BM / 2, WN = BN / 2;  // wave sub-tile
+  constexpr int RB = WM / 16, CB = WN / 16;
+
+  __shared__ double As[2][kBK][LDA_S];
+  __shared__ double Bs[2][kBK][LDB_S];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int lr = lane & 15, lk = lane >> 4;
+
+  const int tile = blockIdx.x;
+  const int tm = tile / p.tiles_n, tn = tile - tm * p.tiles_n;
+  const int z = blockIdx.y;
+  const int b = p.batch0 + z / p.nsplit;
+  const int s = z - (z / p.nsplit) * p.nsplit;
+  const int m0 = tm * BM, n0 = tn * BN;
+  const int k_begin = s * p.kchunk;
+  const int k_end = min(p.K, k_begin + p.kchunk);
+
+  const double* __restrict__ A = p.A + (long)b * p.sA;
+  const double* __restrict__ B = p.B + (long)b * p.sB;
+
+  double ra[A_PER], rb[B_PER];
+
+  auto load_regs = [&](int k0) {
+#pragma unroll
+    for (int r = 0; r < A_PER; ++r) {
+      const int e = tid + kThreads * r;
+      int mm, kk;
+      if (!TA) { mm = e / kBK; kk = e % kBK; } else { kk = e / BM; mm = e % BM; }
+      const int gm = m0 + mm, gk = k0 + kk;
+      double x = 0.0;
+      if (gm < p.M && gk < k_end) x = TA ? A[(long)gk * p.lda + gm] : A[(long)gm * p.lda + gk];
+      ra[r] = x;
+    }
+#pragma unroll
+    for (int r = 0; r < B_PER; ++r) {
+      const int e = tid + kThreads * r;
+      int nn, kk;
+      if (!TB) { kk = e / BN; nn = e % BN; } else { nn = e / kBK; kk = e % kBK; }
+      const int gn = n0 + nn, gk = k0 + kk;
+      double x = 0.0;
+      if (gn < p.N && gk < k_end) x = TB ? B[(long)gn * p.ldb + gk] : B[(long)gk * p.ldb + gn];
+      rb[r] = x;
+    }
+  };
+  auto store_lds = [&](int buf) {
+#pragma unroll
+    for (int r = 0; r < A_PER; ++r) {
+      const int e = tid + kThreads * r;
+      int mm, kk;
+      if (!TA) { mm = e / kBK; kk = e % kBK; } else { kk = e / BM; mm = e % BM; }
+      As[buf][kk][mm] = ra[r];
+    }
+#pragma unroll
+    for (int r = 0; r < B_PER; ++r) {
+      const int e = tid + kThreads * r;
+      int nn, kk;
+      if (!TB) { kk = e / BN; nn = e % BN; } else { nn = e / kBK; kk = e % kBK; }
+      Bs[buf][kk][nn] = rb[r];
+    }
+  };
+
+  double4_t acc[RB][CB];
+#pragma unroll
+  for (int i = 0; i < RB; ++i)
+#pragma unroll
+    for (int j = 0; j < CB; ++j) acc[i][j] = double4_t{0.0, 0.0, 0.0, 0.0};
+
+  int buf = 0;
+  if (k_begin < k_end) {
+    load_regs(k_begin);
+    store_lds(0);
+  }
+  __syncthreads();
+  for (int kt = k_begin; kt < k_end; kt += kBK) {
+    const bool has_next = kt + kBK < k_end;
+    if (has_next) load_regs(kt + kBK);
+#pragma unroll
+    for (int ks = 0; ks < kBK / 4; ++ks) {
+      double fa[RB], fb[CB];
+#pragma unroll
+      for (int i = 0; i < RB; ++i) fa[i] = As[buf][ks * 4 + lk][wm * WM + i * 16 + lr];
+#pragma unroll
+      for (int j = 0; j < CB; ++j) fb[j] = Bs[buf][ks * 4 + lk][wn * WN + j * 16 + lr];
+#pragma unroll
+      for (int i = 0; i < RB; ++i)
+#pragma unroll
+        for (int j = 0; j < CB; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[i], fb[j], acc[i][j], 0, 0, 0);
+    }
+    if (has_next) store_lds(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
+  }
+
+  // epilogue
+  if (p.nsplit == 1) {
+    double* __restrict__ C = p.C + (long)b * p.sC;
+#pragma unroll
+    for (int i = 0; i < RB; ++i)
+#pragma unroll
+      for (int j = 0; j < CB; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int gm = m0 + wm * WM + i * 16 + r * 4 + lk;
+          const int gn = n0 + wn * WN + j * 16 + lr;
+          if (gm < p.M && gn < p.N) {
+            double* c = C + (long)gm * p.ldc + gn;
+            const double v = p.alpha * acc[i][j][r];
+            *c = (p.beta == 0.0) ? v : v + p.beta * (*c);
+          }
+        }
+  } else {
+    // partial sums to workspace slice z (dense M x N)
+    double* __restrict__ Wz = p.W + (long)z * p.M * p.N;
+#pragma unroll
+    for (int i = 0; i < RB; ++i)
+#pragma unroll
+      for (int j = 0; j < CB; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int gm = m0 + wm * WM + i * 16 + r * 4 + lk;
+          const int gn = n0 + wn * WN + j * 16 + lr;
+          if (gm < p.M && gn < p.N) Wz[(long)gm * p.N + gn] = p.alpha * acc[i][j][r];
+        }
+  }
+}
+
+// C[b] = beta*C[b] + sum_s W[b*nsplit + s]   (fixed summation order)
+__global__ void splitk_reduce_kernel(const double* __restrict__ W, double* __restrict__ C, long ldc,
+                                     long sC, int M, int N, int nsplit, int batch0, double beta) {
+  const long MN = (long)M * N;
+  const int bz = blockIdx.y;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < MN; e += (long)gridDim.x * blockDim.x) {
+    const int m = (int)(e / N), n = (int)(e - (long)(e / N) * N);
+    double acc = 0.0;
+    const double* w = W + (long)bz * nsplit * MN + e;
+    for (int s = 0; s < nsplit; ++s) acc += w[(long)s * MN];
+    double* c = C + (long)(batch0 + bz) * sC + (long)m * ldc + n;
+    *c = (beta == 0.0) ? acc : acc + beta * (*c);
+  }
+}
+
+template <int BM, int BN>
+void launch_f64(const GemmDesc& d, const GemmWorkspace& ws, hipStream_t stream) {
+  const int tiles_m = (int)ceil_div(d.M, BM), tiles_n = (int)ceil_div(d.N, BN);
+  const long tiles = (long)tiles_m * tiles_n;
+  // split-K: aim for >= ~2 workgroups per CU (256 CUs) when the output is small
+  int nsplit = 1;
+  const long want = 512;
+  if (tiles * d.batch < want && d.K >= 8 * kBK) {
+    nsplit = (int)std::min<long>(ceil_div(want, tiles * d.batch), d.K / (4 * kBK));
+    nsplit = std::max(1, std::min(nsplit, 64));
+  }
+  int kchunk = (int)(ceil_div(ceil_div(d.K, nsplit), kBK) * kBK);
+  if (nsplit > 1) {
+    nsplit = (int)ceil_div(d.K, kchunk);
+    // workspace capacity limits how many batches go in one launch
+  }
+  const size_t slice = (size_t)d.M * d.N * sizeof(double);
+  int max_batch_per_launch = 65535 / nsplit;
+  if (nsplit > 1) {
+    const size_t cap = ws.bytes / (slice * nsplit);
+    if (cap == 0) {  // not enough workspace: fall back to no split
+      nsplit = 1;
+      kchunk = (int)(ceil_div(d.K, kBK) * kBK);
+      max_batch_per_launch = 65535;
+    } else {
+      max_batch_per_launch = (int)std::min<size_t>(max_batch_per_launch, cap);
+    }
+  }
+  if (d.K == 0) kchunk = kBK;
+  for (int b0 = 0; b0 < d.batch; b0 += max_batch_per_launch) {
+    const int nb = std::min(max_batch_per_launch, d.batch - b0);
+    KParams p;
+    p.A = d.A; p.B = d.B; p.C = d.C; p.W = ws.ptr;
+    p.lda = d.lda; p.ldb = d.ldb; p.ldc = d.ldc; p.sA = d.sA; p.sB = d.sB; p.sC = d.sC;
+    p.M = d.M; p.N = d.N; p.K = d.K; p.kchunk = kchunk; p.nsplit = nsplit; p.batch0 = b0;
+    p.tiles_n = tiles_n; p.alpha = d.alpha; p.beta = d.beta;
+    dim3 grid((unsigned)tiles, (unsigned)(nb * nsplit));
+    if (!d.transA && !d.transB) hipLaunchKernelGGL((dgemm_f64_kernel<BM, BN, false, false>), grid, dim3(kThreads), 0, stream, p);
+    else if (!d.transA && d.transB) hipLaunchKernelGGL((dgemm_f64_kernel<BM, BN, false, true>), grid, dim3(kThreads), 0, stream, p);
+    else if (d.transA && !d.transB) hipLaunchKernelGGL((dgemm_f64_kernel<BM, BN, true, false>), grid, dim3(kThreads), 0, stream, p);
+    else hipLaunchKernelGGL((dgemm_f64_kernel<BM, BN, true, true>), grid, dim3(kThreads), 0, stream, p);
+    check_launch("dgemm_f64");
+    if (nsplit > 1) {
+      const long MN = (long)d.M * d.N;
+      dim3 g2((unsigned)std::min<long>(ceil_div(MN, 256), 4096), (unsigned)nb);
+      hipLaunchKernelGGL(splitk_reduce_kernel, g2, dim3(256), 0, stream, ws.ptr, d.C, d.ldc, d.sC,
+                         d.M, d.N, nsplit, b0, d.beta);
+      check_launch("splitk_reduce");
+    }
+  }
+}
+
+}  // namespace
+
+void gemm(const GemmDesc& d, GemmPrecision prec, const GemmWorkspace& ws, hipStream_t stream) {
+  if (d.M <= 0 || d.N <= 0 || d.batch <= 0) return;
+  if (d.K <= 0) {
+    throw std::invalid_argument("gemm: K must be positive");
+  }
+  (void)prec;  // lower-precision paths dispatch here as they are added
+  if (d.M <= 64 || d.N <= 64)
+    launch_f64<64, 64>(d, ws, stream);
+  else
+    launch_f64<128, 128>(d, ws, stream);
+}
+
+}  // namespace mpqc_hip

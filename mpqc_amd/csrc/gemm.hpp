@@ -1,0 +1,57 @@
+// Batched, strided, split-K GEMM on CDNA4 matrix cores (FP64 MFMA).
+//
+// Row-major semantics: C[m,n] = alpha * sum_k op(A)[m,k] op(B)[k,n] + beta * C[m,n]
+//   op(A) = A      : A[m*lda + k]       op(A) = A^T : A[k*lda + m]
+//   op(B) = B      : B[k*ldb + n]       op(B) = B^T : B[n*ldb + k]
+// Batch b uses A + b*sA, B + b*sB, C + b*sC.
+//
+// This is the contraction engine that replaces TiledArray's SUMMA +
+// dgemm for the CCSD residual (reference: src/mpqc/math/external/tiledarray/
+// tiledarray_impl.cpp:36-47, which instantiates TA's ContractReduce over
+// TA::Tensor<double>).  Every "*" in ccsd_r1_r2.h:70-293 lands here, after the
+// operands have been brought into GEMM layout by the permutation kernels.
+#pragma once
+
+#include "common.hpp"
+
+namespace mpqc_hip {
+
+enum class GemmPrecision : int {
+  F64 = 0,      // FP64 MFMA (v_mfma_f64_16x16x4_f64), FP64 accumulate
+  F32 = 1,      // FP32 MFMA on FP32-rounded operands, FP32 accumulate per K-chunk, FP64 across chunks
+  BF16X3 = 2,   // bf16 hi/lo split, 3 products on bf16 MFMA, FP32 accumulate per K-chunk, FP64 across chunks
+};
+
+struct GemmDesc {
+  bool transA = false, transB = false;
+  int M = 0, N = 0, K = 0;
+  double alpha = 1.0, beta = 0.0;
+  const double* A = nullptr;
+  long lda = 0, sA = 0;
+  const double* B = nullptr;
+  long ldb = 0, sB = 0;
+  double* C = nullptr;
+  long ldc = 0, sC = 0;
+  int batch = 1;
+};
+
+// Workspace used for split-K partial sums (owned by the caller).
+struct GemmWorkspace {
+  double* ptr = nullptr;
+  size_t bytes = 0;
+};
+
+// Launches on `stream`.  Chooses tile shape and split-K factor.  Deterministic:
+// split-K partials are summed in a fixed order by a separate pass.
+void gemm(const GemmDesc& d, GemmPrecision prec, const GemmWorkspace& ws, hipStream_t stream);
+
+// Convenience for plain (non-batched) calls.
+inline GemmDesc make_gemm(bool tA, bool tB, int M, int N, int K, double alpha, const double* A,
+                          long lda, const double* B, long ldb, double beta, double* C, long ldc) {
+  GemmDesc d;
+  d.transA = tA; d.transB = tB; d.M = M; d.N = N; d.K = K; d.alpha = alpha; d.beta = beta;
+  d.A = A; d.lda = lda; d.B = B; d.ldb = ldb; d.C = C; d.ldc = ldc;
+  return d;
+}
+
+}  // namespace mpqc_hip

@@ -1,0 +1,389 @@
+// HBM-streaming kernels of the CCSD iteration.  All are bandwidth bound; they
+// read/write each element once with unit-stride writes and wave-contiguous
+// reads along the innermost output index.
+#include "tensor_ops.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+namespace mpqc_hip {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+struct TAddArgs {
+  double* out;
+  long n[4];
+  double beta;
+  int nterms;
+  TTerm t[kMaxTerms];
+};
+
+// grid: x over the (i2,i3) plane, y = i1, z = i0
+__global__ __launch_bounds__(kBlock) void tadd_kernel(TAddArgs a) {
+  const long plane = a.n[2] * a.n[3];
+  const long e = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (e >= plane) return;
+  const long i0 = blockIdx.z, i1 = blockIdx.y;
+  const long i2 = e / a.n[3], i3 = e - i2 * a.n[3];
+  const long o = ((i0 * a.n[1] + i1) * a.n[2] + i2) * a.n[3] + i3;
+  double acc = (a.beta == 0.0) ? 0.0 : a.beta * a.out[o];
+  for (int t = 0; t < a.nterms; ++t) {
+    const TTerm& tt = a.t[t];
+    acc += tt.alpha * tt.p[i0 * tt.s[0] + i1 * tt.s[1] + i2 * tt.s[2] + i3 * tt.s[3]];
+  }
+  a.out[o] = acc;
+}
+
+// Element-wise kernels use 1-D grid-stride loops with 32-bit indices (the
+// host checks v^2 o^2 < 2^31).
+__global__ __launch_bounds__(kBlock) void build_tau_kernel(double* __restrict__ tau,
+                                                           const double* __restrict__ t2,
+                                                           const double* __restrict__ t1, int o,
+                                                           int v) {
+  // tau(a,b,i,j) = t2 + t1(a,i) t1(b,j)
+  const int oo = o * o, n = v * v * oo;
+  for (int e = blockIdx.x * kBlock + threadIdx.x; e < n; e += gridDim.x * kBlock) {
+    const int ab = e / oo, ij = e - ab * oo;
+    const int a = ab / v, b = ab - a * v;
+    const int i = ij / o, j = ij - i * o;
+    tau[e] = t2[e] + t1[a * o + i] * t1[b * o + j];
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void build_TP_kernel(double* __restrict__ TP,
+                                                          const double* __restrict__ t2,
+                                                          const double* __restrict__ t1, int o,
+                                                          int v) {
+  // TP(a,i,l,d) = 0.5 t2(d,a,i,l) + t1(d,i) t1(a,l)
+  const int ov = o * v, n = ov * ov;
+  for (int e = blockIdx.x * kBlock + threadIdx.x; e < n; e += gridDim.x * kBlock) {
+    const int ai = e / ov, ld = e - ai * ov;
+    const int a = ai / o, i = ai - a * o;
+    const int l = ld / v, d = ld - l * v;
+    const int src = ((d * v + a) * o + i) * o + l;
+    TP[e] = 0.5 * t2[src] + t1[d * o + i] * t1[a * o + l];
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void jacobi_kernel(double* __restrict__ t1, double* __restrict__ t2,
+                                                        const double* __restrict__ r1,
+                                                        const double* __restrict__ r2,
+                                                        const double* __restrict__ eo,
+                                                        const double* __restrict__ ev, int o, int v) {
+  const int oo = o * o, n = v * v * oo;
+  for (int e = blockIdx.x * kBlock + threadIdx.x; e < n; e += gridDim.x * kBlock) {
+    const int ab = e / oo, ij = e - ab * oo;
+    const int a = ab / v, b = ab - a * v;
+    const int i = ij / o, j = ij - i * o;
+    t2[e] += r2[e] / (eo[i] + eo[j] - ev[a] - ev[b]);
+  }
+  for (int e = blockIdx.x * kBlock + threadIdx.x; e < v * o; e += gridDim.x * kBlock) {
+    const int a = e / o, i = e - a * o;
+    t1[e] += r1[e] / (eo[i] - ev[a]);
+  }
+}
+
+// ---------------------------------------------------------------- reductions
+constexpr int kRedGrid = 1024;
+
+__device__ inline double block_sum(double x) {
+  __shared__ double sh[kBlock / 64];
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) sh[w] = x;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < kBlock / 64; ++k) r += sh[k];
+  }
+  __syncthreads();
+  return r;  // valid in thread 0
+}
+
+__global__ __launch_bounds__(kBlock) void dot_stage1(const double* __restrict__ x,
+                                                     const double* __restrict__ y, long n,
+                                                     double* __restrict__ partials) {
+  double acc = 0.0;
+  for (long e = (long)blockIdx.x * kBlock + threadIdx.x; e < n; e += (long)gridDim.x * kBlock)
+    acc += x[e] * y[e];
+  const double r = block_sum(acc);
+  if (threadIdx.x == 0) partials[blockIdx.x] = r;
+}
+
+// out[k] = sum_b partials[k*nb + b]  (one block per k)
+__global__ __launch_bounds__(kBlock) void reduce_stage2(const double* __restrict__ partials, int nb,
+                                                        double* __restrict__ out, double scale) {
+  double acc = 0.0;
+  const double* p = partials + (long)blockIdx.x * nb;
+  for (int b = threadIdx.x; b < nb; b += kBlock) acc += p[b];
+  const double r = block_sum(acc);
+  if (threadIdx.x == 0) out[blockIdx.x] = scale * r;
+}
+
+__global__ __launch_bounds__(kBlock) void energy_stage1(
+    const double* __restrict__ f_ai, const double* __restrict__ r1, const double* __restrict__ t1,
+    const double* __restrict__ g, const double* __restrict__ r2, const double* __restrict__ tau,
+    int o, int v, double* __restrict__ partials) {
+  // sizes checked on the host: v*v*o*o < 2^31
+  const int n2 = v * v * o * o;
+  const int oo = o * o;
+  double acc = 0.0;
+  for (int e = blockIdx.x * kBlock + threadIdx.x; e < n2; e += gridDim.x * kBlock) {
+    const int ab = e / oo, ij = e - ab * oo;
+    const int a = ab / v, b = ab - a * v;
+    const double tba = tau[(b * v + a) * oo + ij];
+    acc += (g[e] + r2[e]) * (2.0 * tau[e] - tba);
+  }
+  double acc1 = 0.0;
+  for (long e = (long)blockIdx.x * kBlock + threadIdx.x; e < (long)v * o; e += (long)gridDim.x * kBlock)
+    acc1 += (f_ai[e] + r1[e]) * t1[e];
+  const double r = block_sum(acc + 2.0 * acc1);
+  if (threadIdx.x == 0) partials[blockIdx.x] = r;
+}
+
+struct MultiDotArgs {
+  const double* x1[16];
+  const double* x2[16];
+  const double* y1;
+  const double* y2;
+  long n1, n2;
+  int nvec;
+};
+
+// partials[k*gridDim.x + block] = partial <x_k, y>
+__global__ __launch_bounds__(kBlock) void multidot_stage1(MultiDotArgs a, double* __restrict__ partials) {
+  double acc[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) acc[k] = 0.0;
+  for (long e = (long)blockIdx.x * kBlock + threadIdx.x; e < a.n2; e += (long)gridDim.x * kBlock) {
+    const double y = a.y2[e];
+#pragma unroll
+    for (int k = 0; k < 16; ++k)
+      if (k < a.nvec) acc[k] += a.x2[k][e] * y;
+  }
+  for (long e = (long)blockIdx.x * kBlock + threadIdx.x; e < a.n1; e += (long)gridDim.x * kBlock) {
+    const double y = a.y1[e];
+#pragma unroll
+    for (int k = 0; k < 16; ++k)
+      if (k < a.nvec) acc[k] += a.x1[k][e] * y;
+  }
+  for (int k = 0; k < a.nvec; ++k) {
+    const double r = block_sum(acc[k]);
+    if (threadIdx.x == 0) partials[(long)k * gridDim.x + blockIdx.x] = r;
+  }
+}
+
+struct LincombArgs {
+  const double* x[16];
+  double c[16];
+  int nvec;
+};
+
+__global__ __launch_bounds__(kBlock) void lincomb_kernel(double* __restrict__ y, LincombArgs a, long n) {
+  for (long e = (long)blockIdx.x * kBlock + threadIdx.x; e < n; e += (long)gridDim.x * kBlock) {
+    double acc = 0.0;
+    for (int k = 0; k < a.nvec; ++k) acc += a.c[k] * a.x[k][e];
+    y[e] = acc;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void nonfinite_stage1(const double* __restrict__ x, long n,
+                                                           double* __restrict__ partials) {
+  double acc = 0.0;
+  for (long e = (long)blockIdx.x * kBlock + threadIdx.x; e < n; e += (long)gridDim.x * kBlock)
+    acc += isfinite(x[e]) ? 0.0 : 1.0;
+  const double r = block_sum(acc);
+  if (threadIdx.x == 0) partials[blockIdx.x] = r;
+}
+
+__device__ inline long tiled_offset(const long* __restrict__ mi, const long n[4], long i0, long i1,
+                                    long i2, long i3) {
+  const long* m0 = mi;
+  const long* m1 = m0 + 2 * n[0];
+  const long* m2 = m1 + 2 * n[1];
+  const long* m3 = m2 + 2 * n[2];
+  const long lo0 = m0[i0], e0 = m0[n[0] + i0];
+  const long lo1 = m1[i1], e1 = m1[n[1] + i1];
+  const long lo2 = m2[i2], e2 = m2[n[2] + i2];
+  const long lo3 = m3[i3], e3 = m3[n[3] + i3];
+  const long base = lo0 * n[1] * n[2] * n[3] + e0 * lo1 * n[2] * n[3] + e0 * e1 * lo2 * n[3] +
+                    e0 * e1 * e2 * lo3;
+  return base + (((i0 - lo0) * e1 + (i1 - lo1)) * e2 + (i2 - lo2)) * e3 + (i3 - lo3);
+}
+
+struct TileArgs {
+  long n[4];
+};
+
+template <bool TO_DENSE>
+__global__ __launch_bounds__(kBlock) void tiles_kernel(double* __restrict__ dst,
+                                                       const double* __restrict__ src,
+                                                       const long* __restrict__ mi, TileArgs a) {
+  const long plane = a.n[2] * a.n[3];
+  const long e = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (e >= plane) return;
+  const long i0 = blockIdx.z, i1 = blockIdx.y;
+  const long i2 = e / a.n[3], i3 = e - i2 * a.n[3];
+  const long d = ((i0 * a.n[1] + i1) * a.n[2] + i2) * a.n[3] + i3;
+  const long t = tiled_offset(mi, a.n, i0, i1, i2, i3);
+  if (TO_DENSE) dst[d] = src[t];
+  else dst[t] = src[d];
+}
+
+int red_grid(long n) { return (int)std::max<long>(1, std::min<long>(kRedGrid, ceil_div(n, kBlock))); }
+
+void check_grid_yz(long y, long z) {
+  if (y > 65535 || z > 65535) throw std::invalid_argument("tensor kernel: extent exceeds grid limit");
+}
+
+}  // namespace
+
+void permuted_strides(const long in_extents[4], const int perm[4], long out_s[4]) {
+  long st[4];
+  st[3] = 1;
+  for (int d = 2; d >= 0; --d) st[d] = st[d + 1] * in_extents[d + 1];
+  for (int d = 0; d < 4; ++d) out_s[d] = st[perm[d]];
+}
+
+void tadd(double* out, const long n[4], double beta, const TTerm* terms, int nterms,
+          hipStream_t stream) {
+  if (nterms > kMaxTerms) throw std::invalid_argument("tadd: too many terms");
+  TAddArgs a;
+  a.out = out;
+  for (int d = 0; d < 4; ++d) a.n[d] = n[d];
+  a.beta = beta;
+  a.nterms = nterms;
+  for (int t = 0; t < nterms; ++t) a.t[t] = terms[t];
+  const long plane = n[2] * n[3];
+  if (plane == 0 || n[0] == 0 || n[1] == 0) return;
+  check_grid_yz(n[1], n[0]);
+  dim3 grid((unsigned)ceil_div(plane, kBlock), (unsigned)n[1], (unsigned)n[0]);
+  hipLaunchKernelGGL(tadd_kernel, grid, dim3(kBlock), 0, stream, a);
+  check_launch("tadd");
+}
+
+int elem_grid(long n) { return (int)std::max<long>(1, std::min<long>(ceil_div(n, kBlock), 16384)); }
+
+void check_small(int o, int v) {
+  if ((long)v * v * o * o >= (1L << 31)) throw std::invalid_argument("v^2 o^2 must be < 2^31");
+}
+
+void build_tau(double* tau, const double* t2, const double* t1, int o, int v, hipStream_t s) {
+  check_small(o, v);
+  hipLaunchKernelGGL(build_tau_kernel, dim3(elem_grid((long)v * v * o * o)), dim3(kBlock), 0, s, tau,
+                     t2, t1, o, v);
+  check_launch("build_tau");
+}
+
+void build_TP(double* TP, const double* t2, const double* t1, int o, int v, hipStream_t s) {
+  check_small(o, v);
+  hipLaunchKernelGGL(build_TP_kernel, dim3(elem_grid((long)v * v * o * o)), dim3(kBlock), 0, s, TP,
+                     t2, t1, o, v);
+  check_launch("build_TP");
+}
+
+void jacobi_update(double* t1, double* t2, const double* r1, const double* r2, const double* eps_o,
+                   const double* eps_v, int o, int v, hipStream_t s) {
+  check_small(o, v);
+  hipLaunchKernelGGL(jacobi_kernel, dim3(elem_grid((long)v * v * o * o)), dim3(kBlock), 0, s, t1, t2,
+                     r1, r2, eps_o, eps_v, o, v);
+  check_launch("jacobi");
+}
+
+void dot(const double* x, const double* y, long n, double* out, double* partials, hipStream_t s) {
+  const int g = red_grid(n);
+  hipLaunchKernelGGL(dot_stage1, dim3(g), dim3(kBlock), 0, s, x, y, n, partials);
+  hipLaunchKernelGGL(reduce_stage2, dim3(1), dim3(kBlock), 0, s, partials, g, out, 1.0);
+  check_launch("dot");
+}
+
+void ccsd_energy(const double* f_ai, const double* r1, const double* t1, const double* g_abij,
+                 const double* r2, const double* tau, int o, int v, double* out, double* partials,
+                 hipStream_t s) {
+  if ((long)v * v * o * o >= (1L << 31)) throw std::invalid_argument("ccsd_energy: v^2 o^2 too large");
+  const int g = red_grid((long)v * v * o * o);
+  hipLaunchKernelGGL(energy_stage1, dim3(g), dim3(kBlock), 0, s, f_ai, r1, t1, g_abij, r2, tau, o, v,
+                     partials);
+  hipLaunchKernelGGL(reduce_stage2, dim3(1), dim3(kBlock), 0, s, partials, g, out, 1.0);
+  check_launch("ccsd_energy");
+}
+
+void multi_dot2(const double* const* xs1, const double* const* xs2, int nvec, const double* y1,
+                const double* y2, long n1, long n2, double* out, double* partials, hipStream_t s) {
+  if (nvec > 16) throw std::invalid_argument("multi_dot2: at most 16 vectors");
+  MultiDotArgs a;
+  for (int k = 0; k < 16; ++k) {
+    a.x1[k] = k < nvec ? xs1[k] : nullptr;
+    a.x2[k] = k < nvec ? xs2[k] : nullptr;
+  }
+  a.y1 = y1; a.y2 = y2; a.n1 = n1; a.n2 = n2; a.nvec = nvec;
+  const int g = red_grid(std::max(n1, n2));
+  hipLaunchKernelGGL(multidot_stage1, dim3(g), dim3(kBlock), 0, s, a, partials);
+  hipLaunchKernelGGL(reduce_stage2, dim3(nvec), dim3(kBlock), 0, s, partials, g, out, 1.0);
+  check_launch("multi_dot2");
+}
+
+void lincomb(double* y, const double* const* xs, const double* c, int nvec, long n, hipStream_t s) {
+  if (nvec > 16) throw std::invalid_argument("lincomb: at most 16 vectors");
+  LincombArgs a;
+  for (int k = 0; k < 16; ++k) {
+    a.x[k] = k < nvec ? xs[k] : nullptr;
+    a.c[k] = k < nvec ? c[k] : 0.0;
+  }
+  a.nvec = nvec;
+  const int g = (int)std::min<long>(ceil_div(n, kBlock), 8192);
+  hipLaunchKernelGGL(lincomb_kernel, dim3(g), dim3(kBlock), 0, s, y, a, n);
+  check_launch("lincomb");
+}
+
+void count_nonfinite(const double* x, long n, double* out, double* partials, hipStream_t s) {
+  const int g = red_grid(n);
+  hipLaunchKernelGGL(nonfinite_stage1, dim3(g), dim3(kBlock), 0, s, x, n, partials);
+  hipLaunchKernelGGL(reduce_stage2, dim3(1), dim3(kBlock), 0, s, partials, g, out, 1.0);
+  check_launch("count_nonfinite");
+}
+
+void make_tile_modeinfo(long* host_out, const long n[4], const long* const hashmarks[4],
+                        const int ntiles[4]) {
+  long* p = host_out;
+  for (int d = 0; d < 4; ++d) {
+    const long nd = n[d];
+    if (hashmarks[d] == nullptr) {  // one tile covering the mode
+      for (long i = 0; i < nd; ++i) { p[i] = 0; p[nd + i] = nd; }
+    } else {
+      const long* hm = hashmarks[d];
+      if (hm[0] != 0 || hm[ntiles[d]] != nd)
+        throw std::invalid_argument("tile hashmarks must start at 0 and end at the extent");
+      for (int t = 0; t < ntiles[d]; ++t) {
+        if (hm[t + 1] <= hm[t]) throw std::invalid_argument("tile hashmarks must increase");
+        for (long i = hm[t]; i < hm[t + 1]; ++i) { p[i] = hm[t]; p[nd + i] = hm[t + 1] - hm[t]; }
+      }
+    }
+    p += 2 * nd;
+  }
+}
+
+void tiles_to_dense(double* dense, const double* tiled, const long n[4], const long* modeinfo,
+                    hipStream_t s) {
+  TileArgs a;
+  for (int d = 0; d < 4; ++d) a.n[d] = n[d];
+  check_grid_yz(n[1], n[0]);
+  dim3 grid((unsigned)ceil_div(n[2] * n[3], kBlock), (unsigned)n[1], (unsigned)n[0]);
+  hipLaunchKernelGGL(tiles_kernel<true>, grid, dim3(kBlock), 0, s, dense, tiled, modeinfo, a);
+  check_launch("tiles_to_dense");
+}
+
+void dense_to_tiles(double* tiled, const double* dense, const long n[4], const long* modeinfo,
+                    hipStream_t s) {
+  TileArgs a;
+  for (int d = 0; d < 4; ++d) a.n[d] = n[d];
+  check_grid_yz(n[1], n[0]);
+  dim3 grid((unsigned)ceil_div(n[2] * n[3], kBlock), (unsigned)n[1], (unsigned)n[0]);
+  hipLaunchKernelGGL(tiles_kernel<false>, grid, dim3(kBlock), 0, s, tiled, dense, modeinfo, a);
+  check_launch("dense_to_tiles");
+}
+
+}  // namespace mpqc_hip
