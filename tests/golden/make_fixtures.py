@@ -1,0 +1,53 @@
+"""Writes tests/golden/h2o_ccsd.json from the reference's committed validation
+outputs (run in the development container, where the reference tree is
+mounted; the tests only read the JSON).
+
+Data taken: the h2o.xyz geometry, RHF energies, CCSD iteration tables
+(iter, deltaE, residual, energy) and final CCSD energies of
+  tests/validation/reference/outputs/h2o-eom-ccsd-direct-631g.out  (6-31G, frozen core)
+  tests/validation/reference/outputs/h2o-eom-ip-ccsd-pvdz.out      (cc-pVDZ, all electron)
+  tests/validation/reference/outputs/h2o-eom-ea-ccsd-pvdz.out      (cc-pVDZ, frozen core)
+"""
+import json
+import os
+import re
+
+REF = "/root/reference/tests/validation/reference"
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def parse(name):
+    txt = open(os.path.join(REF, "outputs", name)).read()
+    rev = re.search(r"Revision (\w+)", txt).group(1)
+    scf = [float(x) for x in re.findall(r"Energy: (-?\d+\.\d+)", txt)]
+    rows = []
+    m = re.search(r"iter \t +deltaE.*?\n(.*?)CCSD Energy", txt, re.S)
+    for line in m.group(1).strip().splitlines():
+        p = line.split()
+        rows.append({"iter": int(p[0]), "dE": float(p[1]), "residual": float(p[2]),
+                     "energy": float(p[3])})
+    e = float(re.search(r"CCSD Energy\s+(-?\d+\.\d+)", txt).group(1))
+    return rev, scf[-1], rows, e
+
+
+cases = [
+    ("h2o-eom-ccsd-direct-631g.out", "6-31G", True, 4, 8, 1e-9, 1.0,
+     "current convergence rule: dE <= 0.1*precision and error <= precision"),
+    ("h2o-eom-ip-ccsd-pvdz.out", "cc-pVDZ", False, 5, 19, 1e-9, 10.0,
+     "revision 7efc976 stops at dE <= precision (no 0.1 margin): equivalent to the current "
+     "rule with precision x10 for this trace"),
+    ("h2o-eom-ea-ccsd-pvdz.out", "cc-pVDZ", True, 4, 19, 1e-9, 10.0,
+     "revision 7efc976 stops at dE <= precision (no 0.1 margin): equivalent to the current "
+     "rule with precision x10 for this trace"),
+]
+out = {"geometry_angstrom": open(os.path.join(REF, "inputs", "h2o.xyz")).read(),
+       "units": "2010CODATA", "cases": []}
+for fname, basis, fc, o, v, prec, scale, note in cases:
+    rev, e_scf, rows, e = parse(fname)
+    out["cases"].append({"source": f"tests/validation/reference/outputs/{fname}", "revision": rev,
+                         "basis": basis, "frozen_core": fc, "o": o, "v": v, "e_rhf": e_scf,
+                         "e_ccsd": e, "precision": prec, "precision_scale_for_current_rule": scale,
+                         "note": note, "trace": rows})
+with open(os.path.join(HERE, "h2o_ccsd.json"), "w") as f:
+    json.dump(out, f, indent=1)
+print("wrote", len(out["cases"]), "cases")

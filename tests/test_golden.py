@@ -1,0 +1,61 @@
+"""Oracle and host front-end against the reference's golden validation data
+(tests/golden/h2o_ccsd.json, extracted by tests/golden/make_fixtures.py).
+
+* RHF energies pin the basis sets and the integral engine.
+* CCSD iteration tables pin the residual, the Jacobi update, the energy
+  expression, the convergence test and TA::DIIS's skip threshold row by row.
+Tolerances: dE / residual are printed with 6 significant digits; energies with
+12 decimals.  The reference computed its integrals with Schwarz screening at
+1e-12 (its outputs print "ScreenThreshold = 1e-12"), which shifts cc-pVDZ
+energies by ~1e-11 Eh; that bounds the energy tolerance.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import ccsd as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIX = json.load(open(os.path.join(HERE, "golden", "h2o_ccsd.json")))
+
+
+def _inputs(case):
+    from mpqc_amd.chem.molecule import Molecule
+    from mpqc_amd.chem.scf import rhf
+    from mpqc_amd.chem.mo import ccsd_integrals
+    mol = Molecule.from_xyz_text(FIX["geometry_angstrom"])
+    scf = rhf(mol, case["basis"])
+    return scf, ccsd_integrals(scf, frozen_core=case["frozen_core"])
+
+
+def check_trace(rows, case, e_tol):
+    gold = case["trace"]
+    assert len(rows) >= len(gold)
+    for r, g in zip(rows, gold):
+        assert r.iter == g["iter"]
+        assert abs(r.dE - g["dE"]) <= 2e-5 * g["dE"] + 1e-15, (r, g)
+        assert abs(r.error - g["residual"]) <= 2e-5 * g["residual"] + 1e-16, (r, g)
+        assert abs(r.energy - g["energy"]) <= e_tol + 5e-13, (r, g)
+
+
+@pytest.mark.parametrize("idx", range(len(FIX["cases"])))
+def test_rhf_energy_matches_reference(idx):
+    case = FIX["cases"][idx]
+    scf, inp = _inputs(case)
+    assert abs(scf.energy - case["e_rhf"]) < 3e-12
+    assert (inp.o, inp.v) == (case["o"], case["v"])
+
+
+@pytest.mark.parametrize("idx", range(len(FIX["cases"])))
+def test_oracle_ccsd_trace_matches_reference(idx):
+    case = FIX["cases"][idx]
+    _, inp = _inputs(case)
+    tp = case["precision"] * case["precision_scale_for_current_rule"]
+    E, t1, t2, rows = O.compute_ccsd(O.Integrals(**inp.blocks), inp.eps_occ, inp.eps_vir,
+                                     target_precision=tp)
+    e_tol = 2e-12 if case["basis"] == "6-31G" else 2e-11
+    check_trace(rows, case, e_tol)
+    assert len(rows) == len(case["trace"])  # stops on the same row
+    assert abs(E - case["e_ccsd"]) < e_tol
