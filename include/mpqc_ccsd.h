@@ -99,7 +99,8 @@ int mpqc_ccsd_ctx_destroy(mpqc_ccsd_ctx* ctx);
 const char* mpqc_ccsd_last_error(const mpqc_ccsd_ctx* ctx);
 
 /* Numeric options: "fov_sign" (+1 reference F_VV as written, -1 exact CCSD for
- * non-Brillouin references), "precision" (MPQC_CCSD_PREC_*). */
+ * non-Brillouin references), "precision" (MPQC_CCSD_PREC_*), "kernel_timing"
+ * (1 = time every contraction with HIP events on the context stream). */
 int mpqc_ccsd_set_option(mpqc_ccsd_ctx* ctx, const char* key, double value);
 
 int mpqc_ccsd_set_integral(mpqc_ccsd_ctx* ctx, int which, const double* data, int layout,
@@ -133,6 +134,14 @@ int mpqc_ccsd_solve(mpqc_ccsd_ctx* ctx, int max_iter, double target_precision, i
  * energy, wall seconds of that iteration.  Returns the row count in *n_rows. */
 int mpqc_ccsd_get_trace(mpqc_ccsd_ctx* ctx, int max_rows, int* n_rows, int* iters, double* dE,
                         double* error, double* energy, double* seconds);
+
+/* Accumulated contraction timings (when "kernel_timing" is on): for each tag
+ * (the reference expression, e.g. "r2:pp_ladder"), the number of launches,
+ * total milliseconds and total FLOPs (2*M*N*K).  names receives max_rows
+ * NUL-terminated strings of 64 bytes each.  This is the measurement the
+ * reference gets from its per-phase timers (ccsd.h:392-394,442-444,463-465). */
+int mpqc_ccsd_get_kernel_times(mpqc_ccsd_ctx* ctx, int max_rows, int* n_rows, char* names,
+                               int64_t* calls, double* ms, double* flops);
 
 /* Device-pointer kernel entry points (stream may be NULL for the null stream). */
 int mpqc_ccsd_pp_ladder(int o, int v, const double* gabcd_dev, const double* tau_dev,

@@ -29,6 +29,7 @@ EXPORTS = [
     "mpqc_ccsd_get_residuals", "mpqc_ccsd_get_dressed_fock", "mpqc_ccsd_init_guess",
     "mpqc_ccsd_residual", "mpqc_ccsd_energy_error", "mpqc_ccsd_update", "mpqc_ccsd_solve",
     "mpqc_ccsd_get_trace", "mpqc_ccsd_pp_ladder", "mpqc_ccsd_gemm",
+    "mpqc_ccsd_get_kernel_times",
 ]
 
 _lib = None
@@ -76,6 +77,7 @@ def load():
         "mpqc_ccsd_solve": (I, [P, I, D, I, pD, pI]),
         "mpqc_ccsd_get_trace": (I, [P, I, pI, P, P, P, P, P]),
         "mpqc_ccsd_pp_ladder": (I, [I, I, P, P, P, D, D, I, P]),
+        "mpqc_ccsd_get_kernel_times": (I, [P, I, pI, P, P, P, P]),
         "mpqc_ccsd_gemm": (I, [I, I, I, I, I, D, P, I64, I64, P, I64, I64, D, P, I64, I64, I, I, P]),
     }
     for name, (res, args) in sig.items():

@@ -268,6 +268,28 @@ class CCSD:
         return [TraceRow(it[k], arrs[0][k], arrs[1][k], arrs[2][k], arrs[3][k]) for k in range(m)]
 
 
+def _kernel_times(self):
+    n = ctypes.c_int()
+    self._check(self.lib.mpqc_ccsd_get_kernel_times(self.ctx, 0, ctypes.byref(n), None, None,
+                                                    None, None))
+    m = max(n.value, 1)
+    names = ctypes.create_string_buffer(64 * m)
+    calls = np.zeros(m, dtype=np.int64)
+    ms = np.zeros(m)
+    fl = np.zeros(m)
+    self._check(self.lib.mpqc_ccsd_get_kernel_times(
+        self.ctx, m, ctypes.byref(n), names, calls.ctypes.data_as(ctypes.c_void_p),
+        ms.ctypes.data_as(ctypes.c_void_p), fl.ctypes.data_as(ctypes.c_void_p)))
+    out = {}
+    for k in range(n.value):
+        tag = names.raw[64 * k:64 * (k + 1)].split(b"\0", 1)[0].decode()
+        out[tag] = {"calls": int(calls[k]), "ms": float(ms[k]), "flops": float(fl[k])}
+    return out
+
+
+CCSD.kernel_times = _kernel_times
+
+
 def pp_ladder(gabcd_ptr: int, tau_ptr: int, r2_ptr: int, o: int, v: int, alpha=1.0, beta=0.0,
               precision: str = "f64", stream: Optional[int] = None):
     """b("a,b,i,j") = alpha * tau("c,d,i,j") * Gabcd("a,b,c,d") + beta * b, on

@@ -28,6 +28,8 @@ def _load():
         lib.chem_one_electron.restype = ctypes.c_int
         lib.chem_one_electron.argtypes = [ctypes.c_int, P, P, P, P, P, P, P, ctypes.c_int,
                                           ctypes.c_int, P, P, P, P, P, P, P, P, P]
+        lib.chem_jk.restype = ctypes.c_int
+        lib.chem_jk.argtypes = [ctypes.c_int, P, P, P, P]
         lib.chem_eri.restype = ctypes.c_int
         lib.chem_eri.argtypes = [ctypes.c_int, P, P, P, P, P, P, P, ctypes.c_int,
                                  ctypes.c_double, P]
@@ -111,3 +113,16 @@ class BasisSet:
         if st != 0:
             raise RuntimeError("ERI evaluation failed")
         return out
+
+
+def jk(eri: np.ndarray, D: np.ndarray):
+    """Coulomb J and exchange K from dense (ij|kl) (host library, one pass)."""
+    lib = _load()
+    n = D.shape[0]
+    D = np.ascontiguousarray(D, dtype=np.float64)
+    J = np.empty((n, n))
+    K = np.empty((n, n))
+    p = BasisSet._p
+    if lib.chem_jk(n, p(eri), p(D), p(J), p(K)) != 0:
+        raise RuntimeError("J/K build failed")
+    return J, K

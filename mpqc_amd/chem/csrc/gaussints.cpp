@@ -640,3 +640,33 @@ int chem_eri(int nshell, const int* l, const int* pure, const int* nprim, const 
 }
 
 }  // extern "C"
+
+extern "C" {
+// Coulomb and exchange matrices from dense (ij|kl) and a symmetric density:
+// J_ij = sum_kl (ij|kl) D_kl,  K_ij = sum_kl (ik|jl) D_kl.  One pass over the
+// integrals, OpenMP over i.
+int chem_jk(int n, const double* eri, const double* D, double* J, double* K) {
+  const size_t N = (size_t)n;
+#pragma omp parallel for schedule(dynamic)
+  for (int i = 0; i < n; ++i) {
+    for (int j = 0; j < n; ++j) {
+      const double* e = eri + (((size_t)i * N + j) * N) * N;
+      double s = 0.0;
+      for (size_t kl = 0; kl < N * N; ++kl) s += e[kl] * D[kl];
+      J[(size_t)i * N + j] = s;
+    }
+    std::vector<double> row(N, 0.0);
+    for (int k = 0; k < n; ++k) {
+      const double* Dk = D + (size_t)k * N;
+      for (int j = 0; j < n; ++j) {
+        const double* e = eri + (((size_t)i * N + k) * N + j) * N;
+        double s = 0.0;
+        for (int l = 0; l < n; ++l) s += e[l] * Dk[l];
+        row[j] += s;
+      }
+    }
+    for (int j = 0; j < n; ++j) K[(size_t)i * N + j] = row[j];
+  }
+  return 0;
+}
+}  // extern "C"

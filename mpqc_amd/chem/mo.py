@@ -34,13 +34,17 @@ class CCSDInputs:
 
 
 def _transform(eri, C1, C2, C3, C4):
-    """(pq|rs) = sum C1[mu,p] C2[nu,q] C3[la,r] C4[si,s] (mu nu|la si)."""
-    n = eri.shape[0]
-    x = (C1.T @ eri.reshape(n, -1)).reshape(C1.shape[1], n, n, n)
-    x = np.tensordot(x, C2, axes=([1], [0]))          # p la si q
-    x = np.tensordot(x, C3, axes=([1], [0]))          # p si q r
-    x = np.tensordot(x, C4, axes=([1], [0]))          # p q r s
-    return np.ascontiguousarray(x)
+    """(pq|rs) = sum C1[mu,p] C2[nu,q] C3[la,r] C4[si,s] (mu nu|la si).
+
+    Four quarter transforms, each one GEMM that contracts the leading index
+    and appends the new one: (mu nu la si) -> (nu la si p) -> ... -> (p q r s).
+    """
+    x = eri
+    for Cq in (C1, C2, C3, C4):
+        n0 = x.shape[0]
+        rest = x.shape[1:]
+        x = (x.reshape(n0, -1).T @ Cq).reshape(*rest, Cq.shape[1])
+    return x
 
 
 def ccsd_integrals(scf: RHFResult, frozen_core: bool = False, fock_ao=None) -> CCSDInputs:
@@ -55,12 +59,17 @@ def ccsd_integrals(scf: RHFResult, frozen_core: bool = False, fock_ao=None) -> C
     Fmo_ii = Ci.T @ F @ Ci
     Fmo_ia = Ci.T @ F @ Ca
     Fmo_aa = Ca.T @ F @ Ca
-    eri = scf.eri
+    # full MO-basis (pq|rs) once, then the blocks
+    nmo = C.shape[1]
+    mo = _transform(scf.eri, C, C, C, C)
+    occ = slice(nfz, nocc)
+    vir = slice(nocc, nmo)
 
     def phys(A, B, Cc, D):
-        # <pq|rs> = (pr|qs): transform (A C | B D) then reorder
-        x = _transform(eri, A, Cc, B, D)  # (p r | q s) -> indices p r q s
-        return np.ascontiguousarray(x.transpose(0, 2, 1, 3))
+        # <pq|rs> = (pr|qs)
+        return np.ascontiguousarray(mo[A, Cc, B, D].transpose(0, 2, 1, 3))
+
+    Ci, Ca = occ, vir
 
     blocks = dict(
         Fia=np.ascontiguousarray(Fmo_ia), Fij=np.ascontiguousarray(Fmo_ii),

@@ -31,10 +31,8 @@ class RHFResult:
 
 
 def _jk(eri, D):
-    n = D.shape[0]
-    J = (eri.reshape(n * n, n * n) @ D.reshape(-1)).reshape(n, n)
-    K = np.einsum("ikjl,kl->ij", eri, D, optimize=True)
-    return J, K
+    from .basis import jk
+    return jk(eri, D)
 
 
 def rhf(mol: Molecule, basis_name: str, conv_e=1e-13, conv_g=1e-11, max_iter=100,
@@ -87,6 +85,9 @@ def rhf(mol: Molecule, basis_name: str, conv_e=1e-13, conv_g=1e-11, max_iter=100
             for a in range(m):
                 for b in range(m):
                     B[a, b] = float(np.sum(es[a] * es[b]))
+            # scale the error overlaps to O(1) so the bordered system stays
+            # well conditioned as the errors shrink
+            B[:m, :m] /= np.max(np.diag(B[:m, :m]))
             rhs = np.zeros(m + 1)
             rhs[m] = -1.0
             c = np.linalg.lstsq(B, rhs, rcond=None)[0][:m]

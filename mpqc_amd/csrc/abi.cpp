@@ -217,6 +217,24 @@ int mpqc_ccsd_get_trace(mpqc_ccsd_ctx* ctx, int max_rows, int* n_rows, int* iter
   });
 }
 
+int mpqc_ccsd_get_kernel_times(mpqc_ccsd_ctx* ctx, int max_rows, int* n_rows, char* names,
+                               int64_t* calls, double* ms, double* flops) {
+  return guarded(ctx, [&](mpqc_hip::CCSDContext& c) {
+    const auto kt = c.kernel_times();
+    const int n = (int)std::min<size_t>(kt.size(), (size_t)std::max(0, max_rows));
+    for (int k = 0; k < n; ++k) {
+      if (names) {
+        std::memset(names + 64 * k, 0, 64);
+        std::strncpy(names + 64 * k, kt[k].tag.c_str(), 63);
+      }
+      if (calls) calls[k] = kt[k].calls;
+      if (ms) ms[k] = kt[k].ms;
+      if (flops) flops[k] = kt[k].flops;
+    }
+    if (n_rows) *n_rows = (int)kt.size();
+  });
+}
+
 int mpqc_ccsd_gemm(int transa, int transb, int m, int n, int k, double alpha, const double* a,
                    int64_t lda, int64_t stride_a, const double* b, int64_t ldb, int64_t stride_b,
                    double beta, double* c, int64_t ldc, int64_t stride_c, int batch, int precision,

@@ -103,6 +103,13 @@ CCSDContext::CCSDContext(int o, int v, const int64_t* occ_hm, int n_occ_tiles, c
 
 void CCSDContext::release() {
   if (stream_) hipStreamSynchronize(stream_);
+  for (auto& pe : pending_) {
+    hipEventDestroy(pe.a);
+    hipEventDestroy(pe.b);
+  }
+  pending_.clear();
+  for (auto e : event_pool_) hipEventDestroy(e);
+  event_pool_.clear();
   for (double* p : allocations_) hipFree(p);
   allocations_.clear();
   alloc_sizes_.clear();
@@ -197,6 +204,8 @@ void CCSDContext::set_option(const std::string& key, double value) {
   if (key == "fov_sign") {
     if (value != 1.0 && value != -1.0) throw std::invalid_argument("fov_sign must be +1 or -1");
     fov_sign_ = value;
+  } else if (key == "kernel_timing") {
+    kernel_timing_ = value != 0.0;
   } else if (key == "precision") {
     const int p = (int)value;
     if (p < 0 || p > 2 || p != value) throw std::invalid_argument("unknown precision");
@@ -387,7 +396,7 @@ void CCSDContext::get_dressed_fock(double* FIJ, double* FAB) {
 // ---------------------------------------------------------------- helpers
 void CCSDContext::gemm(bool tA, bool tB, int M, int N, int K, double alpha, const double* A, long lda,
                        const double* B, long ldb, double beta, double* C, long ldc, long sA, long sB,
-                       long sC, int batch) {
+                       long sC, int batch, const char* tag) {
   GemmDesc d = make_gemm(tA, tB, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
   d.sA = sA; d.sB = sB; d.sC = sC; d.batch = batch;
   if (M <= 0 || N <= 0 || K <= 0 || batch <= 0) throw std::logic_error("empty GEMM");
@@ -396,7 +405,58 @@ void CCSDContext::gemm(bool tA, bool tB, int M, int N, int K, double alpha, cons
   check_access(A, (tA ? (long)(K - 1) * lda + (M - 1) : (long)(M - 1) * lda + (K - 1)) + bm * sA, "gemm A");
   check_access(B, (tB ? (long)(N - 1) * ldb + (K - 1) : (long)(K - 1) * ldb + (N - 1)) + bm * sB, "gemm B");
   check_access(C, (long)(M - 1) * ldc + (N - 1) + bm * sC, "gemm C");
+  if (!kernel_timing_) {
+    mpqc_hip::gemm(d, prec_, ws_, stream_);
+    return;
+  }
+  const std::string t = tag ? tag : "gemm";
+  int slot = -1;
+  for (size_t k = 0; k < ktimes_.size(); ++k)
+    if (ktimes_[k].tag == t) slot = (int)k;
+  if (slot < 0) {
+    ktimes_.push_back({t, 0, 0.0, 0.0});
+    slot = (int)ktimes_.size() - 1;
+  }
+  auto take = [&]() {
+    hipEvent_t e;
+    if (!event_pool_.empty()) {
+      e = event_pool_.back();
+      event_pool_.pop_back();
+    } else {
+      MPQC_HIP_CHECK(hipEventCreate(&e));
+    }
+    return e;
+  };
+  PendingEvent pe{slot, take(), take(), 2.0 * M * (double)N * K * batch};
+  MPQC_HIP_CHECK(hipEventRecord(pe.a, stream_));
   mpqc_hip::gemm(d, prec_, ws_, stream_);
+  MPQC_HIP_CHECK(hipEventRecord(pe.b, stream_));
+  pending_.push_back(pe);
+}
+
+void CCSDContext::resolve_events() {
+  if (pending_.empty()) return;
+  MPQC_HIP_CHECK(hipStreamSynchronize(stream_));
+  for (auto& pe : pending_) {
+    float ms = 0.f;
+    MPQC_HIP_CHECK(hipEventElapsedTime(&ms, pe.a, pe.b));
+    ktimes_[pe.slot].calls += 1;
+    ktimes_[pe.slot].ms += ms;
+    ktimes_[pe.slot].flops += pe.flops;
+    event_pool_.push_back(pe.a);
+    event_pool_.push_back(pe.b);
+  }
+  pending_.clear();
+}
+
+std::vector<CCSDContext::KernelTime> CCSDContext::kernel_times() {
+  resolve_events();
+  return ktimes_;
+}
+
+void CCSDContext::reset_kernel_times() {
+  resolve_events();
+  ktimes_.clear();
 }
 
 void CCSDContext::perm_add(double* out, std::array<long, 4> n, double beta,
@@ -523,13 +583,13 @@ void CCSDContext::residual() {
   //         (ccsd_intermediates.h:71-77)
   perm_add(tau_akld_, {V, O, O, V}, 0.0, {{tau_, e_abij, {0, 2, 3, 1}, 1.0}});
   MPQC_HIP_CHECK(hipMemcpyAsync(FAB_, Fab_, vv * sizeof(double), hipMemcpyDeviceToDevice, stream_));
-  gemm(false, true, v, v, o * o * v, -1.0, tau_akld_, oo * V, Gbar_cKLD_, oo * V, 1.0, FAB_, v);
+  gemm(false, true, v, v, o * o * v, -1.0, tau_akld_, oo * V, Gbar_cKLD_, oo * V, 1.0, FAB_, v, 0, 0, 0, 1, "r1:F_vv");
   gemm(false, false, v, o, v, 1.0, FAB_, v, t1_, o, 1.0, r1_, o);
   // :95-97  h_ki = F_oo = Fij(k,i) + gbar(k,l,a,b) tau(a,b,i,l); FIJ = h_ki; r1 -= t1 h_ki
   //         (ccsd_intermediates.h:57-63)
   perm_add(tau_ilab_, {O, O, V, V}, 0.0, {{tau_, e_abij, {2, 3, 0, 1}, 1.0}});
   MPQC_HIP_CHECK(hipMemcpyAsync(FIJ_, Fij_, oo * sizeof(double), hipMemcpyDeviceToDevice, stream_));
-  gemm(false, true, o, o, o * v * v, 1.0, Gbar_, O * vv, tau_ilab_, O * vv, 1.0, FIJ_, o);
+  gemm(false, true, o, o, o * v * v, 1.0, Gbar_, O * vv, tau_ilab_, O * vv, 1.0, FIJ_, o, 0, 0, 0, 1, "r1:F_oo");
   gemm(false, false, v, o, o, -1.0, t1_, o, FIJ_, o, 1.0, r1_, o);
   // :101-104 h_kc = F_ov = Fia(k,c) + gbar(k,l,c,d) t1(d,l)   (ccsd_intermediates.h:89-95)
   //          r1 += h_kc (2 t2(c,a,k,i) - t2(c,a,i,k) + t1(c,i) t1(a,k))
@@ -545,15 +605,15 @@ void CCSDContext::residual() {
   gemm(false, false, v * o, 1, v * o, 1.0, P6_, ov, t1_, 1, 1.0, r1_, 1);
   // :110-114 r1 += g_iabc_bar(k,a,c,d) tau(c,d,k,i)
   perm_add(tau_kcdi_, {O, V, V, O}, 0.0, {{tau_, e_abij, {2, 0, 1, 3}, 1.0}});
-  gemm(false, false, v, o, o * v * v, 1.0, Gbar_akcd_, O * vv, tau_kcdi_, o, 1.0, r1_, o);
+  gemm(false, false, v, o, o * v * v, 1.0, Gbar_akcd_, O * vv, tau_kcdi_, o, 1.0, r1_, o, 0, 0, 0, 1, "r1:giabc_tau");
   // :120-121 r1 -= (2 Gijka(l,k,i,c) - Gijka(k,l,i,c)) tau(c,a,k,l)
   perm_add(tau_ackl_, {V, V, O, O}, 0.0, {{tau_, e_abij, {1, 0, 2, 3}, 1.0}});
-  gemm(false, false, v, o, o * o * v, -1.0, tau_ackl_, oo * V, P8_, o, 1.0, r1_, o);
+  gemm(false, false, v, o, o * o * v, -1.0, tau_ackl_, oo * V, P8_, o, 1.0, r1_, o, 0, 0, 0, 1, "r1:gijka_tau");
 
   // ===================== r2: ccsd_r1_r2.h:198-293 =====================
   // :207-208 Rd = (Giabc(i,c,a,b) - Giajb(k,b,i,c) t1(a,k)) t1(c,j)
   //   first term, batched over i: Rd[(a,b), i, j] = sum_c Giabc[i][c,(a,b)] t1[c,j]
-  gemm(true, false, v * v, o, v, 1.0, Giabc_, vv, t1_, o, 0.0, Rd_, oo, vv * V, 0, O, o);
+  gemm(true, false, v * v, o, v, 1.0, Giabc_, vv, t1_, o, 0.0, Rd_, oo, vv * V, 0, O, o, "r2:giabc_t1");
   //   second term as t1(a,k) * (Giajb(k,b,i,c) t1(c,j))
   gemm(false, false, o * o * v, o, v, 1.0, Giajb_, v, t1_, o, 0.0, Q_, o);
   gemm(false, false, v, v * o * o, o, -1.0, t1_, o, Q_, V * oo, 1.0, Rd_, V * oo);
@@ -573,7 +633,7 @@ void CCSDContext::residual() {
   gemm(false, false, v, v, o, fov_sign_, t1_, o, Fia_, v, 1.0, gac_, v);
   gemm(false, false, v * v, 1, o * v, 1.0, P14_, ov, t1T_, 1, 1.0, gac_, 1);
   // :226 Rd += g_ac(a,c) t2(c,b,i,j) - g_ki(k,i) t2(a,b,k,j)
-  gemm(false, false, v, v * o * o, v, 1.0, gac_, v, t2_, V * oo, 1.0, Rd_, V * oo);
+  gemm(false, false, v, v * o * o, v, 1.0, gac_, v, t2_, V * oo, 1.0, Rd_, V * oo, 0, 0, 0, 1, "r2:g_ac_t2");
   perm_add(t2_abjk_, {V, V, O, O}, 0.0, {{t2_, e_abij, {0, 1, 3, 2}, 1.0}});
   gemm(false, false, v * v * o, o, o, 1.0, t2_abjk_, o, gki_, o, 0.0, X16_, o);
   perm_add(Rd_, {V, V, O, O}, 1.0, {{X16_, e_abij, {0, 1, 3, 2}, -1.0}});
@@ -589,21 +649,21 @@ void CCSDContext::residual() {
   MPQC_HIP_CHECK(hipMemcpyAsync(K_, K0_, ov * ov * sizeof(double), hipMemcpyDeviceToDevice, stream_));
   gemm(false, false, v, o * o * v, o, -1.0, t1_, o, X18_, oo * V, 1.0, J_, oo * V);
   gemm(false, false, v, o * o * v, o, -1.0, t1_, o, X19_, oo * V, 1.0, K_, oo * V);
-  gemm(false, false, o * v * v, o, v, 1.0, Giabc_, v, t1_, o, 0.0, Zj_, o);  // Zj(k,a,c,i)
-  gemm(true, false, v, o, v, 1.0, Giabc_, v, t1_, o, 0.0, Zk_, o, vv, 0, V * O, o * v);  // Zk(k,a,c,i)
+  gemm(false, false, o * v * v, o, v, 1.0, Giabc_, v, t1_, o, 0.0, Zj_, o, 0, 0, 0, 1, "r2:j_giabc_t1");  // Zj(k,a,c,i)
+  gemm(true, false, v, o, v, 1.0, Giabc_, v, t1_, o, 0.0, Zk_, o, vv, 0, V * O, o * v, "r2:k_giabc_t1");  // Zk(k,a,c,i)
   perm_add(J_, {V, O, O, V}, 1.0, {{Zj_, {O, V, V, O}, {1, 3, 0, 2}, 1.0}});
   perm_add(K_, {V, O, O, V}, 1.0, {{Zk_, {O, V, V, O}, {1, 3, 0, 2}, 1.0}});
-  gemm(false, false, o * v, o * v, o * v, -1.0, TP_, ov, GA_, ov, 1.0, J_, ov);
-  gemm(false, false, o * v, o * v, o * v, 0.5, t2P_, ov, GB_, ov, 1.0, J_, ov);
-  gemm(false, false, o * v, o * v, o * v, -1.0, TP_, ov, GC_, ov, 1.0, K_, ov);
+  gemm(false, false, o * v, o * v, o * v, -1.0, TP_, ov, GA_, ov, 1.0, J_, ov, 0, 0, 0, 1, "r2:j_gT");
+  gemm(false, false, o * v, o * v, o * v, 0.5, t2P_, ov, GB_, ov, 1.0, J_, ov, 0, 0, 0, 1, "r2:j_gbar_t2");
+  gemm(false, false, o * v, o * v, o * v, -1.0, TP_, ov, GC_, ov, 1.0, K_, ov, 0, 0, 0, 1, "r2:k_gT");
   // :251-252 r2 += 0.5 (2 j(a,k,i,c) - k(k,a,i,c)) (2 t2(c,b,k,j) - t2(b,c,k,j))  -> RR(a,i,b,j)
   perm_add(Xjk_, {V, O, O, V}, 0.0, {{J_, {V, O, O, V}, I, 2.0}, {K_, {V, O, O, V}, I, -1.0}});
   perm_add(Y1_, {O, V, V, O}, 0.0,
            {{t2_, e_abij, {2, 0, 1, 3}, 2.0}, {t2_, e_abij, {2, 1, 0, 3}, -1.0}});
-  gemm(false, false, o * v, o * v, o * v, 0.5, Xjk_, ov, Y1_, ov, 0.0, RR_, ov);
+  gemm(false, false, o * v, o * v, o * v, 0.5, Xjk_, ov, Y1_, ov, 0.0, RR_, ov, 0, 0, 0, 1, "r2:ring");
   // :254-256 tmp(a,b,i,j) = k(k,a,i,c) t2(c,b,j,k) -> tmpR(a,i,b,j); r2 -= 0.5 tmp + tmp(b,a,i,j)
   perm_add(Y2_, {O, V, V, O}, 0.0, {{t2_, e_abij, {3, 0, 1, 2}, 1.0}});
-  gemm(false, false, o * v, o * v, o * v, 1.0, K_, ov, Y2_, ov, 0.0, tmpR_, ov);
+  gemm(false, false, o * v, o * v, o * v, 1.0, K_, ov, Y2_, ov, 0.0, tmpR_, ov, 0, 0, 0, 1, "r2:crossed_ring");
   // :260 r2(a,b,i,j) = r2(a,b,i,j) + r2(b,a,j,i); :262 r2 += Gijab(i,j,a,b)
   {
     const std::array<long, 4> e_aibj{V, O, V, O};
@@ -623,12 +683,12 @@ void CCSDContext::residual() {
   perm_add(W_, {O, O, O, O}, 0.0,
            {{Gijkl_, {O, O, O, O}, I, 1.0}, {Y23_, {O, O, O, O}, I, 1.0},
             {Y23_, {O, O, O, O}, {1, 0, 3, 2}, 1.0}});
-  gemm(false, false, o * o, o * o, v * v, 1.0, Gijab_, vv, tau_, oo, 1.0, W_, oo);
-  gemm(false, false, v * v, o * o, o * o, 1.0, tau_, oo, W_, oo, 1.0, r2_, oo);
+  gemm(false, false, o * o, o * o, v * v, 1.0, Gijab_, vv, tau_, oo, 1.0, W_, oo, 0, 0, 0, 1, "r2:W_oooo");
+  gemm(false, false, v * v, o * o, o * o, 1.0, tau_, oo, W_, oo, 1.0, r2_, oo, 0, 0, 0, 1, "r2:hh_ladder");
   // :275 b(a,b,i,j) = tau(c,d,i,j) Gabcd(a,b,c,d)  -- the particle-particle ladder
-  gemm(false, false, v * v, o * o, v * v, 1.0, Gabcd_, vv, tau_, oo, 1.0, r2_, oo);
+  gemm(false, false, v * v, o * o, v * v, 1.0, Gabcd_, vv, tau_, oo, 1.0, r2_, oo, 0, 0, 0, 1, "r2:pp_ladder");
   // :277-280 tmp(k,a,i,j) = Giabc(k,a,c,d) tau(c,d,i,j); b -= tmp(k,a,j,i) t1(b,k) + tmp(k,b,i,j) t1(a,k)
-  gemm(false, false, o * v, o * o, v * v, 1.0, Giabc_, vv, tau_, oo, 0.0, tmp24_, oo);
+  gemm(false, false, o * v, o * o, v * v, 1.0, Giabc_, vv, tau_, oo, 0.0, tmp24_, oo, 0, 0, 0, 1, "r2:pp_t1_giabc_tau");
   gemm(true, true, v * o * o, v, o, 1.0, tmp24_, V * oo, t1_, o, 0.0, Y24_, v);  // Y24(a,j,i,b)
   perm_add(r2_, {V, V, O, O}, 1.0, {{Y24_, {V, O, O, V}, {0, 3, 2, 1}, -1.0}});
   gemm(false, false, v, v * o * o, o, -1.0, t1_, o, tmp24_, V * oo, 1.0, r2_, V * oo);

