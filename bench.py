@@ -245,26 +245,28 @@ def bench_ccsd(args):
         E = c.compute_ccsd(target_precision=1e-8)
         rows = c.trace()
         log(f"{prec}: E_corr={E:.12f} after {len(rows)} iterations")
-        # timed iterations: W untimed, then K timed (energy + update + residual)
+        # timed iterations: W untimed, then K timed; one iteration = Jacobi/DIIS
+        # update + tau + residual + energy/error (the last call synchronises)
         c.init_guess()
+        c.energy_error()
         for _ in range(args.warmup):
-            c.energy_error()
             c.update()
             c.residual()
+            c.energy_error()
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            c.energy_error()
             c.update()
             c.residual()
-        c.energy_error()  # synchronises the stream
+            c.energy_error()
         ms = (time.perf_counter() - t0) * 1e3 / args.steps
         # the same iterations again with per-contraction HIP-event timing
         c.set_option("kernel_timing", 1)
         c.init_guess()
+        c.energy_error()
         for _ in range(args.warmup + args.steps):
-            c.energy_error()
             c.update()
             c.residual()
+            c.energy_error()
         kt = c.kernel_times()
         c.close()
         forms[prec] = {"E_corr": E, "iterations": len(rows), "ms_per_iteration": ms,

@@ -69,7 +69,19 @@ enum {
   MPQC_CCSD_ERR_NOMEM = 6      /* device allocation failed */
 };
 
-enum { MPQC_CCSD_PREC_F64 = 0, MPQC_CCSD_PREC_F32 = 1, MPQC_CCSD_PREC_BF16X3 = 2 };
+/* Contraction precisions (the Jacobi update, DIIS and energy are always FP64):
+ * F64    FP64 MFMA
+ * F32    FP32 MFMA on FP32-rounded operands, FP64 accumulation across K chunks
+ * BF16X3 bf16 hi+lo split, 3 products on bf16 MFMA (~16 significant bits)
+ * F16X3  fp16 hi+lo split of power-of-two scaled operands, 3 products (~22 bits)
+ * BF16X6 bf16 hi+mid+lo split, 6 products (~24 bits) */
+enum {
+  MPQC_CCSD_PREC_F64 = 0,
+  MPQC_CCSD_PREC_F32 = 1,
+  MPQC_CCSD_PREC_BF16X3 = 2,
+  MPQC_CCSD_PREC_F16X3 = 3,
+  MPQC_CCSD_PREC_BF16X6 = 4
+};
 enum { MPQC_LAYOUT_DENSE = 0, MPQC_LAYOUT_TA_TILES = 1 };
 enum { MPQC_MEM_HOST = 0, MPQC_MEM_DEVICE = 1 };
 enum {

@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(_HERE, "lib", "libmpqc_ccsd.so")
 
 OK = 0
 ERR_ARG, ERR_HIP, ERR_STATE, ERR_NONFINITE, ERR_DIIS, ERR_NOMEM = 1, 2, 3, 4, 5, 6
-PREC_F64, PREC_F32, PREC_BF16X3 = 0, 1, 2
+PREC_F64, PREC_F32, PREC_BF16X3, PREC_F16X3, PREC_BF16X6 = 0, 1, 2, 3, 4
 LAYOUT_DENSE, LAYOUT_TA_TILES = 0, 1
 MEM_HOST, MEM_DEVICE = 0, 1
 INT_FIA, INT_FIJ, INT_FAB, INT_GIJAB, INT_GIJKL, INT_GIAJB, INT_GIABC, INT_GIJKA, INT_GABCD = range(9)

@@ -16,7 +16,8 @@ import numpy as np
 
 from . import _lib as L
 
-PRECISIONS = {"f64": L.PREC_F64, "f32": L.PREC_F32, "bf16x3": L.PREC_BF16X3}
+PRECISIONS = {"f64": L.PREC_F64, "f32": L.PREC_F32, "bf16x3": L.PREC_BF16X3,
+              "f16x3": L.PREC_F16X3, "bf16x6": L.PREC_BF16X6}
 
 
 def compute_trange1(n: int, block_size: int) -> List[int]:

@@ -242,7 +242,7 @@ int mpqc_ccsd_gemm(int transa, int transb, int m, int n, int k, double alpha, co
   return guarded_free([&] {
     if (m < 0 || n < 0 || k <= 0 || batch < 1) throw std::invalid_argument("invalid GEMM sizes");
     if (!a || !b || !c) throw std::invalid_argument("null GEMM operand");
-    if (precision < 0 || precision > 2) throw std::invalid_argument("unknown precision");
+    if (precision < 0 || precision > 4) throw std::invalid_argument("unknown precision");
     mpqc_hip::GemmDesc d = mpqc_hip::make_gemm(transa != 0, transb != 0, m, n, k, alpha, a, lda, b,
                                                ldb, beta, c, ldc);
     d.sA = stride_a; d.sB = stride_b; d.sC = stride_c; d.batch = batch;
@@ -257,7 +257,7 @@ int mpqc_ccsd_pp_ladder(int o, int v, const double* gabcd_dev, const double* tau
   return guarded_free([&] {
     if (o <= 0 || v <= 0) throw std::invalid_argument("o and v must be positive");
     if (!gabcd_dev || !tau_dev || !r2_dev) throw std::invalid_argument("null operand");
-    if (precision < 0 || precision > 2) throw std::invalid_argument("unknown precision");
+    if (precision < 0 || precision > 4) throw std::invalid_argument("unknown precision");
     const long vv = (long)v * v, oo = (long)o * o;
     if (vv > 0x7fffffffL) throw std::invalid_argument("v^2 too large");
     // b("a,b,i,j") = tau("c,d,i,j") * Gabcd("a,b,c,d"): M = v^2, N = o^2, K = v^2

@@ -113,6 +113,8 @@ void CCSDContext::release() {
   for (double* p : allocations_) hipFree(p);
   allocations_.clear();
   alloc_sizes_.clear();
+  for (auto& pa : plane_allocs_) hipFree(pa.first);
+  plane_allocs_.clear();
   if (modeinfo_) hipFree(modeinfo_);
   modeinfo_ = nullptr;
   if (staging_) hipFree(staging_);
@@ -126,7 +128,7 @@ void CCSDContext::construct(const int64_t* occ_hm, int n_occ_tiles, const int64_
   const int o = o_, v = v_;
   if (o <= 0 || v <= 0) throw std::invalid_argument("o and v must be positive");
   if ((long)v * v * o * o >= (1L << 31)) throw std::invalid_argument("v^2 o^2 must be < 2^31");
-  if (precision < 0 || precision > 2) throw std::invalid_argument("unknown precision");
+  if (precision < 0 || precision > 4) throw std::invalid_argument("unknown precision");
   prec_ = (GemmPrecision)precision;
   auto take_hm = [](const int64_t* hm, int nt, int n, std::vector<int64_t>& out) {
     if (hm == nullptr) { out = {0, n}; return; }
@@ -208,8 +210,16 @@ void CCSDContext::set_option(const std::string& key, double value) {
     kernel_timing_ = value != 0.0;
   } else if (key == "precision") {
     const int p = (int)value;
-    if (p < 0 || p > 2 || p != value) throw std::invalid_argument("unknown precision");
-    prec_ = (GemmPrecision)p;
+    if (p < 0 || p > 4 || p != value) throw std::invalid_argument("unknown precision");
+    if ((GemmPrecision)p != prec_) {
+      prec_ = (GemmPrecision)p;
+      planes_ready_ = false;
+      if (Gabcd_P_.p[0]) {  // plane kind may change: re-allocate
+        for (auto& pa : plane_allocs_) hipFree(pa.first);
+        plane_allocs_.clear();
+        Gabcd_P_ = Planes();
+      }
+    }
   } else {
     throw std::invalid_argument("unknown option: " + key);
   }
@@ -399,6 +409,9 @@ void CCSDContext::gemm(bool tA, bool tB, int M, int N, int K, double alpha, cons
                        long sC, int batch, const char* tag) {
   GemmDesc d = make_gemm(tA, tB, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
   d.sA = sA; d.sB = sB; d.sC = sC; d.batch = batch;
+  // in the split-precision forms only the large contractions use 16-bit planes;
+  // everything issued through this path runs in FP64
+  const GemmPrecision prec = split_mode() ? GemmPrecision::F64 : prec_;
   if (M <= 0 || N <= 0 || K <= 0 || batch <= 0) throw std::logic_error("empty GEMM");
   if (lda < (tA ? M : K) || ldb < (tB ? K : N) || ldc < N) throw std::logic_error("GEMM leading dimension too small");
   const long bm = (long)(batch - 1);
@@ -406,7 +419,7 @@ void CCSDContext::gemm(bool tA, bool tB, int M, int N, int K, double alpha, cons
   check_access(B, (tB ? (long)(N - 1) * ldb + (K - 1) : (long)(K - 1) * ldb + (N - 1)) + bm * sB, "gemm B");
   check_access(C, (long)(M - 1) * ldc + (N - 1) + bm * sC, "gemm C");
   if (!kernel_timing_) {
-    mpqc_hip::gemm(d, prec_, ws_, stream_);
+    mpqc_hip::gemm(d, prec, ws_, stream_);
     return;
   }
   const std::string t = tag ? tag : "gemm";
@@ -429,7 +442,7 @@ void CCSDContext::gemm(bool tA, bool tB, int M, int N, int K, double alpha, cons
   };
   PendingEvent pe{slot, take(), take(), 2.0 * M * (double)N * K * batch};
   MPQC_HIP_CHECK(hipEventRecord(pe.a, stream_));
-  mpqc_hip::gemm(d, prec_, ws_, stream_);
+  mpqc_hip::gemm(d, prec, ws_, stream_);
   MPQC_HIP_CHECK(hipEventRecord(pe.b, stream_));
   pending_.push_back(pe);
 }
@@ -459,12 +472,9 @@ void CCSDContext::reset_kernel_times() {
   ktimes_.clear();
 }
 
-void CCSDContext::perm_add(double* out, std::array<long, 4> n, double beta,
-                           std::initializer_list<Src> srcs) {
-  TTerm terms[kMaxTerms];
+int CCSDContext::make_terms(std::array<long, 4> n, std::initializer_list<Src> srcs, TTerm* terms) {
   int nt = 0;
   if (srcs.size() > (size_t)kMaxTerms) throw std::logic_error("too many permutation terms");
-  check_access(out, n[0] * n[1] * n[2] * n[3] - 1, "perm_add out");
   for (const Src& s : srcs) {
     long ext[4] = {s.ext[0], s.ext[1], s.ext[2], s.ext[3]};
     int perm[4] = {s.perm[0], s.perm[1], s.perm[2], s.perm[3]};
@@ -481,13 +491,119 @@ void CCSDContext::perm_add(double* out, std::array<long, 4> n, double beta,
         vol *= ext[d];
       }
       if (maxoff >= vol) throw std::logic_error("permutation reads past its input");
-      check_access(s.p, maxoff, "perm_add input");
+      check_access(s.p, maxoff, "permutation input");
     }
     terms[nt].alpha = s.alpha;
     ++nt;
   }
+  return nt;
+}
+
+void CCSDContext::perm_add(double* out, std::array<long, 4> n, double beta,
+                           std::initializer_list<Src> srcs) {
+  TTerm terms[kMaxTerms];
+  check_access(out, n[0] * n[1] * n[2] * n[3] - 1, "perm_add out");
+  const int nt = make_terms(n, srcs, terms);
   long nn[4] = {n[0], n[1], n[2], n[3]};
   tadd(out, nn, beta, terms, nt, stream_);
+}
+
+// ---------------------------------------------------------------- planes
+bool CCSDContext::split_mode() const {
+  return prec_ == GemmPrecision::BF16X3 || prec_ == GemmPrecision::F16X3 ||
+         prec_ == GemmPrecision::BF16X6;
+}
+
+PlaneKind CCSDContext::plane_kind() const {
+  switch (prec_) {
+    case GemmPrecision::F16X3: return PlaneKind::F16x2;
+    case GemmPrecision::BF16X6: return PlaneKind::BF16x3;
+    default: return PlaneKind::BF16x2;
+  }
+}
+
+Planes CCSDContext::alloc_planes(long n) {
+  Planes P;
+  P.n = n;
+  for (int q = 0; q < plane_count(plane_kind()); ++q) {
+    void* ptr = nullptr;
+    MPQC_HIP_CHECK(hipMalloc(&ptr, std::max<long>(n, 8) * sizeof(uint16_t)));
+    P.p[q] = (uint16_t*)ptr;
+    plane_allocs_.push_back({P.p[q], (size_t)std::max<long>(n, 8)});
+  }
+  P.scale = alloc(1);
+  return P;
+}
+
+void CCSDContext::split(const Planes& out, std::array<long, 4> n, std::initializer_list<Src> srcs) {
+  TTerm terms[kMaxTerms];
+  const int nt = make_terms(n, srcs, terms);
+  long nn[4] = {n[0], n[1], n[2], n[3]};
+  split_to_planes(plane_kind(), out, nn, terms, nt, partials_, stream_);
+}
+
+void CCSDContext::gemmP(int M, int N, int K, double alpha, const Planes& A, long lda, const Planes& B,
+                        long ldb, double beta, double* C, long ldc, const char* tag) {
+  if (M <= 0 || N <= 0 || K <= 0) throw std::logic_error("empty GEMM");
+  if (lda < K || ldb < K || ldc < N) throw std::logic_error("GEMM leading dimension too small");
+  if ((long)(M - 1) * lda + K - 1 >= A.n || (long)(N - 1) * ldb + K - 1 >= B.n)
+    throw std::logic_error(std::string("out-of-bounds plane operand in ") + tag);
+  check_access(C, (long)(M - 1) * ldc + (N - 1), "gemmP C");
+  PlaneGemmDesc d;
+  d.kind = plane_kind();
+  d.M = M; d.N = N; d.K = K; d.alpha = alpha; d.beta = beta;
+  d.A = A; d.lda = lda; d.sA = 0; d.B = B; d.ldb = ldb; d.sB = 0;
+  d.C = C; d.ldc = ldc; d.sC = 0; d.batch = 1;
+  hipEvent_t ea = nullptr, eb = nullptr;
+  int slot = -1;
+  if (kernel_timing_) {
+    const std::string t = tag;
+    for (size_t k = 0; k < ktimes_.size(); ++k)
+      if (ktimes_[k].tag == t) slot = (int)k;
+    if (slot < 0) {
+      ktimes_.push_back({t, 0, 0.0, 0.0});
+      slot = (int)ktimes_.size() - 1;
+    }
+    for (hipEvent_t* e : {&ea, &eb}) {
+      if (!event_pool_.empty()) {
+        *e = event_pool_.back();
+        event_pool_.pop_back();
+      } else {
+        MPQC_HIP_CHECK(hipEventCreate(e));
+      }
+    }
+    MPQC_HIP_CHECK(hipEventRecord(ea, stream_));
+  }
+  gemm_planes(d, ws_.ptr, ws_.bytes, stream_);
+  if (kernel_timing_) {
+    MPQC_HIP_CHECK(hipEventRecord(eb, stream_));
+    pending_.push_back({slot, ea, eb, 2.0 * M * (double)N * K});
+  }
+}
+
+// constant operands of the large contractions, split once per solve
+void CCSDContext::prepare_planes() {
+  const long O = o_, V = v_;
+  const long oovv = O * O * V * V, ovvv = O * V * V * V, vvvv = V * V * V * V, oooo = O * O * O * O;
+  if (!Gabcd_P_.p[0]) {
+    Gabcd_P_ = alloc_planes(vvvv); Giabc_P_ = alloc_planes(ovvv); Gijab_P_ = alloc_planes(oovv);
+    GAt_P_ = alloc_planes(oovv); GBt_P_ = alloc_planes(oovv); GCt_P_ = alloc_planes(oovv);
+    tau_P_ = alloc_planes(oovv); tauT_P_ = alloc_planes(oovv); TP_P_ = alloc_planes(oovv);
+    t2P_P_ = alloc_planes(oovv); Xjk_P_ = alloc_planes(oovv); K_P_ = alloc_planes(oovv);
+    Y1t_P_ = alloc_planes(oovv); Y2t_P_ = alloc_planes(oovv); WT_P_ = alloc_planes(oooo);
+  }
+  const std::array<long, 4> e_ijab{O, O, V, V};
+  const std::array<int, 4> I{0, 1, 2, 3};
+  split(Gabcd_P_, {V, V, V, V}, {{Gabcd_, {V, V, V, V}, I, 1.0}});
+  split(Giabc_P_, {O, V, V, V}, {{Giabc_, {O, V, V, V}, I, 1.0}});
+  split(Gijab_P_, {O, O, V, V}, {{Gijab_, e_ijab, I, 1.0}});
+  // B^T operands of the J/K GEMMs: GA^T(k,c,l,d) = Gijab(k,l,c,d); GB^T = 2 Gijab(k,l,c,d) -
+  // Gijab(k,l,d,c); GC^T(k,c,l,d) = Gijab(k,l,d,c)
+  split(GAt_P_, {O, V, O, V}, {{Gijab_, e_ijab, {0, 2, 1, 3}, 1.0}});
+  split(GBt_P_, {O, V, O, V}, {{Gijab_, e_ijab, {0, 2, 1, 3}, 2.0}, {Gijab_, e_ijab, {0, 3, 1, 2}, -1.0}});
+  split(GCt_P_, {O, V, O, V}, {{Gijab_, e_ijab, {0, 3, 1, 2}, 1.0}});
+  MPQC_HIP_CHECK(hipStreamSynchronize(stream_));
+  planes_ready_ = true;
 }
 
 double CCSDContext::scalar_from_device(const double* d) {
@@ -546,6 +662,7 @@ void CCSDContext::prepare() {
   perm_add(GC_, {O, V, O, V}, 0.0, {{Gijab_, ovv_ext_ijab, {1, 2, 0, 3}, 1.0}});
   MPQC_HIP_CHECK(hipStreamSynchronize(stream_));
   prepared_ = true;
+  planes_ready_ = false;
 }
 
 void CCSDContext::init_guess() {
@@ -653,17 +770,40 @@ void CCSDContext::residual() {
   gemm(true, false, v, o, v, 1.0, Giabc_, v, t1_, o, 0.0, Zk_, o, vv, 0, V * O, o * v, "r2:k_giabc_t1");  // Zk(k,a,c,i)
   perm_add(J_, {V, O, O, V}, 1.0, {{Zj_, {O, V, V, O}, {1, 3, 0, 2}, 1.0}});
   perm_add(K_, {V, O, O, V}, 1.0, {{Zk_, {O, V, V, O}, {1, 3, 0, 2}, 1.0}});
-  gemm(false, false, o * v, o * v, o * v, -1.0, TP_, ov, GA_, ov, 1.0, J_, ov, 0, 0, 0, 1, "r2:j_gT");
-  gemm(false, false, o * v, o * v, o * v, 0.5, t2P_, ov, GB_, ov, 1.0, J_, ov, 0, 0, 0, 1, "r2:j_gbar_t2");
-  gemm(false, false, o * v, o * v, o * v, -1.0, TP_, ov, GC_, ov, 1.0, K_, ov, 0, 0, 0, 1, "r2:k_gT");
+  const bool sm = split_mode();
+  if (sm) {
+    if (!planes_ready_) prepare_planes();
+    const std::array<long, 4> e_aild{V, O, O, V};
+    split(TP_P_, e_aild, {{TP_, e_aild, I, 1.0}});
+    split(t2P_P_, e_aild, {{t2P_, e_aild, I, 1.0}});
+    gemmP(o * v, o * v, o * v, -1.0, TP_P_, ov, GAt_P_, ov, 1.0, J_, ov, "r2:j_gT");
+    gemmP(o * v, o * v, o * v, 0.5, t2P_P_, ov, GBt_P_, ov, 1.0, J_, ov, "r2:j_gbar_t2");
+    gemmP(o * v, o * v, o * v, -1.0, TP_P_, ov, GCt_P_, ov, 1.0, K_, ov, "r2:k_gT");
+  } else {
+    gemm(false, false, o * v, o * v, o * v, -1.0, TP_, ov, GA_, ov, 1.0, J_, ov, 0, 0, 0, 1, "r2:j_gT");
+    gemm(false, false, o * v, o * v, o * v, 0.5, t2P_, ov, GB_, ov, 1.0, J_, ov, 0, 0, 0, 1, "r2:j_gbar_t2");
+    gemm(false, false, o * v, o * v, o * v, -1.0, TP_, ov, GC_, ov, 1.0, K_, ov, 0, 0, 0, 1, "r2:k_gT");
+  }
   // :251-252 r2 += 0.5 (2 j(a,k,i,c) - k(k,a,i,c)) (2 t2(c,b,k,j) - t2(b,c,k,j))  -> RR(a,i,b,j)
-  perm_add(Xjk_, {V, O, O, V}, 0.0, {{J_, {V, O, O, V}, I, 2.0}, {K_, {V, O, O, V}, I, -1.0}});
-  perm_add(Y1_, {O, V, V, O}, 0.0,
-           {{t2_, e_abij, {2, 0, 1, 3}, 2.0}, {t2_, e_abij, {2, 1, 0, 3}, -1.0}});
-  gemm(false, false, o * v, o * v, o * v, 0.5, Xjk_, ov, Y1_, ov, 0.0, RR_, ov, 0, 0, 0, 1, "r2:ring");
-  // :254-256 tmp(a,b,i,j) = k(k,a,i,c) t2(c,b,j,k) -> tmpR(a,i,b,j); r2 -= 0.5 tmp + tmp(b,a,i,j)
-  perm_add(Y2_, {O, V, V, O}, 0.0, {{t2_, e_abij, {3, 0, 1, 2}, 1.0}});
-  gemm(false, false, o * v, o * v, o * v, 1.0, K_, ov, Y2_, ov, 0.0, tmpR_, ov, 0, 0, 0, 1, "r2:crossed_ring");
+  if (sm) {
+    const std::array<long, 4> e_aikc{V, O, O, V};
+    // NT operands: Xjk(a,i,k,c) = 2J - K; Y1^T(b,j,k,c) = 2 t2(c,b,k,j) - t2(b,c,k,j);
+    // Y2^T(b,j,k,c) = t2(c,b,j,k)
+    split(Xjk_P_, e_aikc, {{J_, e_aikc, I, 2.0}, {K_, e_aikc, I, -1.0}});
+    split(Y1t_P_, {V, O, O, V}, {{t2_, e_abij, {1, 3, 2, 0}, 2.0}, {t2_, e_abij, {0, 3, 2, 1}, -1.0}});
+    gemmP(o * v, o * v, o * v, 0.5, Xjk_P_, ov, Y1t_P_, ov, 0.0, RR_, ov, "r2:ring");
+    split(K_P_, e_aikc, {{K_, e_aikc, I, 1.0}});
+    split(Y2t_P_, {V, O, O, V}, {{t2_, e_abij, {1, 2, 3, 0}, 1.0}});
+    gemmP(o * v, o * v, o * v, 1.0, K_P_, ov, Y2t_P_, ov, 0.0, tmpR_, ov, "r2:crossed_ring");
+  } else {
+    perm_add(Xjk_, {V, O, O, V}, 0.0, {{J_, {V, O, O, V}, I, 2.0}, {K_, {V, O, O, V}, I, -1.0}});
+    perm_add(Y1_, {O, V, V, O}, 0.0,
+             {{t2_, e_abij, {2, 0, 1, 3}, 2.0}, {t2_, e_abij, {2, 1, 0, 3}, -1.0}});
+    gemm(false, false, o * v, o * v, o * v, 0.5, Xjk_, ov, Y1_, ov, 0.0, RR_, ov, 0, 0, 0, 1, "r2:ring");
+    // :254-256 tmp(a,b,i,j) = k(k,a,i,c) t2(c,b,j,k) -> tmpR(a,i,b,j); r2 -= 0.5 tmp + tmp(b,a,i,j)
+    perm_add(Y2_, {O, V, V, O}, 0.0, {{t2_, e_abij, {3, 0, 1, 2}, 1.0}});
+    gemm(false, false, o * v, o * v, o * v, 1.0, K_, ov, Y2_, ov, 0.0, tmpR_, ov, 0, 0, 0, 1, "r2:crossed_ring");
+  }
   // :260 r2(a,b,i,j) = r2(a,b,i,j) + r2(b,a,j,i); :262 r2 += Gijab(i,j,a,b)
   {
     const std::array<long, 4> e_aibj{V, O, V, O};
@@ -683,12 +823,27 @@ void CCSDContext::residual() {
   perm_add(W_, {O, O, O, O}, 0.0,
            {{Gijkl_, {O, O, O, O}, I, 1.0}, {Y23_, {O, O, O, O}, I, 1.0},
             {Y23_, {O, O, O, O}, {1, 0, 3, 2}, 1.0}});
-  gemm(false, false, o * o, o * o, v * v, 1.0, Gijab_, vv, tau_, oo, 1.0, W_, oo, 0, 0, 0, 1, "r2:W_oooo");
-  gemm(false, false, v * v, o * o, o * o, 1.0, tau_, oo, W_, oo, 1.0, r2_, oo, 0, 0, 0, 1, "r2:hh_ladder");
-  // :275 b(a,b,i,j) = tau(c,d,i,j) Gabcd(a,b,c,d)  -- the particle-particle ladder
-  gemm(false, false, v * v, o * o, v * v, 1.0, Gabcd_, vv, tau_, oo, 1.0, r2_, oo, 0, 0, 0, 1, "r2:pp_ladder");
-  // :277-280 tmp(k,a,i,j) = Giabc(k,a,c,d) tau(c,d,i,j); b -= tmp(k,a,j,i) t1(b,k) + tmp(k,b,i,j) t1(a,k)
-  gemm(false, false, o * v, o * o, v * v, 1.0, Giabc_, vv, tau_, oo, 0.0, tmp24_, oo, 0, 0, 0, 1, "r2:pp_t1_giabc_tau");
+  if (sm) {
+    // tau^T(i,j,c,d) = tau_ilab, the B^T operand of every tau-contraction; tau itself for hh
+    split(tauT_P_, {O, O, V, V}, {{tau_ilab_, {O, O, V, V}, I, 1.0}});
+    split(tau_P_, e_abij, {{tau_, e_abij, I, 1.0}});
+    gemmP(o * o, o * o, v * v, 1.0, Gijab_P_, vv, tauT_P_, vv, 1.0, W_, oo, "r2:W_oooo");
+    // W^T(i,j,k,l) = W(k,l,i,j)
+    split(WT_P_, {O, O, O, O}, {{W_, {O, O, O, O}, {2, 3, 0, 1}, 1.0}});
+    gemmP(v * v, o * o, o * o, 1.0, tau_P_, oo, WT_P_, oo, 1.0, r2_, oo, "r2:hh_ladder");
+    // :275 b(a,b,i,j) = tau(c,d,i,j) Gabcd(a,b,c,d)  -- the particle-particle ladder
+    gemmP(v * v, o * o, v * v, 1.0, Gabcd_P_, vv, tauT_P_, vv, 1.0, r2_, oo, "r2:pp_ladder");
+    // :277 tmp(k,a,i,j) = Giabc(k,a,c,d) tau(c,d,i,j)
+    gemmP(o * v, o * o, v * v, 1.0, Giabc_P_, vv, tauT_P_, vv, 0.0, tmp24_, oo, "r2:pp_t1_giabc_tau");
+  } else {
+    gemm(false, false, o * o, o * o, v * v, 1.0, Gijab_, vv, tau_, oo, 1.0, W_, oo, 0, 0, 0, 1, "r2:W_oooo");
+    gemm(false, false, v * v, o * o, o * o, 1.0, tau_, oo, W_, oo, 1.0, r2_, oo, 0, 0, 0, 1, "r2:hh_ladder");
+    // :275 b(a,b,i,j) = tau(c,d,i,j) Gabcd(a,b,c,d)  -- the particle-particle ladder
+    gemm(false, false, v * v, o * o, v * v, 1.0, Gabcd_, vv, tau_, oo, 1.0, r2_, oo, 0, 0, 0, 1, "r2:pp_ladder");
+    // :277 tmp(k,a,i,j) = Giabc(k,a,c,d) tau(c,d,i,j)
+    gemm(false, false, o * v, o * o, v * v, 1.0, Giabc_, vv, tau_, oo, 0.0, tmp24_, oo, 0, 0, 0, 1, "r2:pp_t1_giabc_tau");
+  }
+  // :278-280 b -= tmp(k,a,j,i) t1(b,k) + tmp(k,b,i,j) t1(a,k)
   gemm(true, true, v * o * o, v, o, 1.0, tmp24_, V * oo, t1_, o, 0.0, Y24_, v);  // Y24(a,j,i,b)
   perm_add(r2_, {V, V, O, O}, 1.0, {{Y24_, {V, O, O, V}, {0, 3, 2, 1}, -1.0}});
   gemm(false, false, v, v * o * o, o, -1.0, t1_, o, tmp24_, V * oo, 1.0, r2_, V * oo);

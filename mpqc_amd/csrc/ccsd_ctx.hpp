@@ -9,6 +9,7 @@
 
 #include "common.hpp"
 #include "gemm.hpp"
+#include "planes.hpp"
 #include "tensor_ops.hpp"
 
 namespace mpqc_hip {
@@ -87,6 +88,21 @@ class CCSDContext {
     double alpha;
   };
   void perm_add(double* out, std::array<long, 4> n, double beta, std::initializer_list<Src> srcs);
+  // validated terms of a permuted linear combination
+  int make_terms(std::array<long, 4> n, std::initializer_list<Src> srcs, TTerm* terms);
+
+  // ---- mixed precision: 16-bit planes of the operands of the large contractions
+  bool split_mode() const;
+  PlaneKind plane_kind() const;
+  Planes alloc_planes(long n);
+  void split(const Planes& out, std::array<long, 4> n, std::initializer_list<Src> srcs);
+  void gemmP(int M, int N, int K, double alpha, const Planes& A, long lda, const Planes& B, long ldb,
+             double beta, double* C, long ldc, const char* tag);
+  void prepare_planes();
+  bool planes_ready_ = false;
+  std::vector<std::pair<uint16_t*, size_t>> plane_allocs_;
+  Planes Gabcd_P_, Giabc_P_, Gijab_P_, GAt_P_, GBt_P_, GCt_P_;
+  Planes tau_P_, tauT_P_, TP_P_, t2P_P_, Xjk_P_, K_P_, Y1t_P_, Y2t_P_, WT_P_;
   void check_finite(const double* x, long n, const char* what);
   double scalar_from_device(const double* d);
 

@@ -405,7 +405,10 @@ void gemm(const GemmDesc& d, GemmPrecision prec, const GemmWorkspace& ws, hipStr
   if (d.K <= 0) {
     throw std::invalid_argument("gemm: K must be positive");
   }
-  if (prec == GemmPrecision::BF16X3) throw std::invalid_argument("bf16x3 contractions not built yet");
+  if (prec == GemmPrecision::BF16X3 || prec == GemmPrecision::F16X3 || prec == GemmPrecision::BF16X6) {
+    gemm_split(d, prec, ws, stream);
+    return;
+  }
   if (d.M <= 64 || d.N <= 64)
     launch<64, 64>(d, prec, ws, stream);
   else

@@ -19,7 +19,9 @@ namespace mpqc_hip {
 enum class GemmPrecision : int {
   F64 = 0,      // FP64 MFMA (v_mfma_f64_16x16x4_f64), FP64 accumulate
   F32 = 1,      // FP32 MFMA on FP32-rounded operands, FP32 accumulate per K-chunk, FP64 across chunks
-  BF16X3 = 2,   // bf16 hi/lo split, 3 products on bf16 MFMA, FP32 accumulate per K-chunk, FP64 across chunks
+  BF16X3 = 2,   // bf16 hi+lo split, 3 products (hh, hl, lh) on bf16 MFMA (~16 bits)
+  F16X3 = 3,    // fp16 hi+lo split of power-of-two scaled operands, 3 products (~22 bits)
+  BF16X6 = 4,   // bf16 hi+mid+lo split, 6 products (~24 bits)
 };
 
 struct GemmDesc {
@@ -44,6 +46,9 @@ struct GemmWorkspace {
 // Launches on `stream`.  Chooses tile shape and split-K factor.  Deterministic:
 // split-K partials are summed in a fixed order by a separate pass.
 void gemm(const GemmDesc& d, GemmPrecision prec, const GemmWorkspace& ws, hipStream_t stream);
+
+// Split-precision paths (BF16X3, F16X3, BF16X6); called by gemm().
+void gemm_split(const GemmDesc& d, GemmPrecision prec, const GemmWorkspace& ws, hipStream_t stream);
 
 // Convenience for plain (non-batched) calls.
 inline GemmDesc make_gemm(bool tA, bool tB, int M, int N, int K, double alpha, const double* A,

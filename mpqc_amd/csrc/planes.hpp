@@ -1,0 +1,53 @@
+// Mixed-precision operands kept in HBM as 16-bit "planes".
+//
+// A tensor x (FP64) is stored as NP half-precision planes p_0..p_{NP-1} with the
+// same element layout, such that  x ~= (p_0 + p_1 [+ p_2]) / s  where s is a
+// power of two kept in device memory (s = 1 for bf16).  Contractions then read
+// 2*NP bytes per element instead of 8 and need no conversion in the GEMM.
+#pragma once
+
+#include <cstdint>
+
+#include "common.hpp"
+#include "tensor_ops.hpp"
+
+namespace mpqc_hip {
+
+enum class PlaneKind : int {
+  BF16x2 = 0,  // hi + lo bf16, products hh + hl + lh
+  F16x2 = 1,   // hi + lo fp16 of the power-of-two scaled value, products hh + hl + lh
+  BF16x3 = 2,  // hi + mid + lo bf16, six products
+};
+
+inline int plane_count(PlaneKind k) { return k == PlaneKind::BF16x3 ? 3 : 2; }
+
+struct Planes {
+  uint16_t* p[3] = {nullptr, nullptr, nullptr};
+  double* scale = nullptr;  // device scalar (power of two; 1 for bf16)
+  long n = 0;               // elements per plane
+};
+
+// Writes planes of  out[i0,i1,i2,i3] = sum_t alpha_t in_t[...]  (permuted linear
+// combination, as tadd) with extents n; computes the scale on the device first
+// (max |out| over the tensor).  `partials` needs kReducePartials doubles.
+void split_to_planes(PlaneKind kind, const Planes& out, const long n[4], const TTerm* terms,
+                     int nterms, double* partials, hipStream_t stream);
+
+// C[m,n] (FP64) = alpha * sum_k A[m,k] B[n,k] / (sA sB) + beta C, batched, with A and
+// B given as planes in k-contiguous layout (A: row m at A + m*lda, B: row n at
+// B + n*ldb).  Split-K partial sums go to `ws` when the output is small.
+struct PlaneGemmDesc {
+  PlaneKind kind;
+  int M, N, K;
+  double alpha, beta;
+  Planes A;
+  long lda, sA;
+  Planes B;
+  long ldb, sB;
+  double* C;
+  long ldc, sC;
+  int batch = 1;
+};
+void gemm_planes(const PlaneGemmDesc& d, double* ws, size_t ws_bytes, hipStream_t stream);
+
+}  // namespace mpqc_hip

@@ -38,22 +38,41 @@ def _dot(a, b):
 
 
 def colpiv_qr_solve(A, rhs):
-    """Column-pivoted Householder QR solve with rank truncation at
-    ``eps * n * max|R_ii|`` (zero pivots give zero solution components).
-    Returns (x, |det A|)."""
+    """Column-pivoted Householder QR solve (the factorisation TA::DIIS uses,
+    Eigen::ColPivHouseholderQR): at step k pick the remaining column of
+    largest norm, reflect rows k.. with v = x - alpha e_k,
+    alpha = -sign(x_k) |x|; rank = #{k : |R_kk| > eps * n * max|R_ii|};
+    components beyond the rank are zero.  Returns (x, |det A|)."""
+    A = np.array(A, dtype=np.float64)
+    b = np.array(rhs, dtype=np.float64)
     n = A.shape[0]
-    Q, R, P = scipy.linalg.qr(A, pivoting=True)
-    d = np.abs(np.diag(R))
-    absdet = float(np.prod(d))
-    maxpiv = d.max() if n else 0.0
-    thresh = np.finfo(np.float64).eps * n
-    rank = int(np.sum(d > maxpiv * thresh))
-    y = Q.T @ rhs
+    perm = np.arange(n)
+    diag = np.zeros(n)
+    for k in range(n):
+        norms = np.sum(A[k:, k:] ** 2, axis=0)
+        p = k + int(np.argmax(norms))
+        if p != k:
+            A[:, [k, p]] = A[:, [p, k]]
+            perm[[k, p]] = perm[[p, k]]
+        norm = np.sqrt(np.sum(A[k:, k] ** 2))
+        if norm == 0.0:
+            continue
+        alpha = -norm if A[k, k] > 0 else norm
+        vh = A[k:, k].copy()
+        vh[0] -= alpha
+        vn = float(vh @ vh)
+        if vn > 0.0:
+            A[k:, k:] -= np.outer(vh, (2.0 / vn) * (vh @ A[k:, k:]))
+            b[k:] -= (2.0 / vn) * (vh @ b[k:]) * vh
+        diag[k] = A[k, k]
+    absdet = float(np.prod(np.abs(diag)))
+    maxpiv = float(np.max(np.abs(diag))) if n else 0.0
+    rank = int(np.sum(np.abs(diag) > maxpiv * np.finfo(np.float64).eps * n))
     z = np.zeros(n)
     if rank:
-        z[:rank] = scipy.linalg.solve_triangular(R[:rank, :rank], y[:rank])
+        z[:rank] = scipy.linalg.solve_triangular(np.triu(A[:rank, :rank]), b[:rank])
     x = np.zeros(n)
-    x[P] = z
+    x[perm] = z
     return x, absdet
 
 

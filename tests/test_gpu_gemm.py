@@ -86,6 +86,33 @@ def test_gemm_f32_mfma_error_bound(ta, tb, m, n, k):
     torch.cuda.synchronize()
     err = np.abs(dC.cpu().numpy() - ref)
     assert np.all(err <= bound + 1e-300), float((err / bound).max())
-    # and it is not secretly FP64: a product of FP32-rounded inputs
-    ref32 = Ao.astype(np.float32).astype(np.float64) @ Bo.astype(np.float32).astype(np.float64)
-    assert np.abs(dC.cpu().numpy() - ref32).max() <= 1e-9 * np.abs(ref32).max() * np.sqrt(k / 64 + 1)
+    # and it really ran in reduced precision (FP64 would be ~1e-16 relative)
+    assert err.max() > 1e-12 * np.abs(ref).max()
+
+
+SPLIT_BOUNDS = {"bf16x3": 2.0 ** -14, "f16x3": 2.0 ** -19, "bf16x6": 2.0 ** -19}
+
+
+@pytest.mark.parametrize("prec", ["bf16x3", "f16x3", "bf16x6"])
+@pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
+@pytest.mark.parametrize("m,n,k", [(17, 33, 5), (130, 70, 257), (300, 200, 3000), (40, 40, 20000)])
+def test_gemm_split_error_bound(prec, ta, tb, m, n, k):
+    """Split-precision contractions: |C - A B| <= bound(prec) * |A| |B| elementwise
+    (plus a tiny absolute term for FP16 subnormal parts), and the result is
+    measurably not FP64."""
+    from mpqc_amd.ccsd import gemm
+    rng = np.random.default_rng(m * 7 + n + k)
+    A = rng.standard_normal((k, m) if ta else (m, k)) * np.exp(rng.uniform(-3, 3, (k, m) if ta else (m, k)))
+    B = rng.standard_normal((n, k) if tb else (k, n)) * 1e-3
+    Ao, Bo = (A.T if ta else A), (B.T if tb else B)
+    ref = Ao @ Bo
+    absprod = np.abs(Ao) @ np.abs(Bo)
+    bound = SPLIT_BOUNDS[prec] * absprod + 2.0 ** -30 * np.abs(A).max() * np.abs(B).max() * k
+    dA, dB = _dev(A), _dev(B)
+    dC = torch.full((m, n), 7.0, dtype=torch.float64, device="cuda")
+    gemm(ta, tb, m, n, k, 1.0, dA.data_ptr(), A.shape[1], dB.data_ptr(), B.shape[1], 0.0,
+         dC.data_ptr(), n, precision=prec)
+    torch.cuda.synchronize()
+    err = np.abs(dC.cpu().numpy() - ref)
+    assert np.all(err <= bound), float((err / bound).max())
+    assert err.max() > 1e-15 * np.abs(ref).max()

@@ -1,0 +1,60 @@
+"""Mixed-precision forms of the CCSD iteration: contractions in reduced
+precision on the matrix cores, Jacobi update / DIIS / energy in FP64.  Each
+form's converged correlation energy is compared with the FP64 GPU form and the
+FP64 oracle on real molecules; the error each form gives is printed."""
+import numpy as np
+import pytest
+
+from test_golden import FIX, _inputs
+
+pytestmark = pytest.mark.gpu
+
+# |E_form - E_fp64| bounds (Eh) for a ~0.2 Eh correlation energy
+BOUNDS = {"f32": 1e-8, "f16x3": 1e-7, "bf16x6": 1e-8, "bf16x3": 5e-5}
+
+
+@pytest.mark.parametrize("prec", ["f32", "f16x3", "bf16x6", "bf16x3"])
+def test_mixed_precision_energy_h2o_pvdz(prec):
+    from mpqc_amd.ccsd import CCSD, Integrals
+    case = FIX["cases"][1]  # H2O cc-pVDZ, all electrons
+    _, inp = _inputs(case)
+    E = {}
+    for p in ("f64", prec):
+        with CCSD(inp.o, inp.v, precision=p) as c:
+            c.set_integrals(Integrals(**inp.blocks))
+            c.set_orbital_energies(inp.eps_occ, inp.eps_vir)
+            E[p] = c.compute_ccsd(target_precision=1e-10)
+    d = E[prec] - E["f64"]
+    print(f"{prec}: E={E[prec]:.12f} f64={E['f64']:.12f} dE={d:.3e}")
+    assert abs(d) < BOUNDS[prec]
+
+
+RES_BOUNDS = {"f32": 1e-6, "f16x3": 2e-6, "bf16x6": 1e-6, "bf16x3": 1e-3}
+
+
+@pytest.mark.parametrize("prec", ["f32", "f16x3", "bf16x6", "bf16x3"])
+def test_mixed_precision_residual(prec):
+    """Residual of each form at the same amplitudes against the FP64 oracle,
+    relative to the largest residual contribution."""
+    from oracle import ccsd as O
+    from mpqc_amd.ccsd import CCSD, Integrals
+    from spinorb_ccsd import mpqc_blocks, random_hamiltonian
+    no, nv = 6, 33
+    f, g = random_hamiltonian(no, nv, seed=21, eri_scale=0.03)
+    b = mpqc_blocks(f, g, no)
+    rng = np.random.default_rng(4)
+    t1 = rng.normal(0, 0.01, (nv, no))
+    t2 = rng.normal(0, 0.01, (nv, nv, no, no))
+    ints = O.Integrals(**b)
+    tau = O.make_tau(t1, t2)
+    r1r = O.cs_ccsd_r1(t1, t2, tau, ints)
+    r2r = O.cs_ccsd_r2(t1, t2, tau, ints)
+    with CCSD(no, nv, precision=prec) as c:
+        c.set_integrals(Integrals(**b))
+        c.set_orbital_energies(np.diag(f)[:no], np.diag(f)[no:])
+        r1, r2 = c.compute_residuals(t1, t2)
+    e1 = np.abs(r1 - r1r).max() / np.abs(r1r).max()
+    e2 = np.abs(r2 - r2r).max() / np.abs(r2r).max()
+    print(f"{prec}: rel err r1 {e1:.2e} r2 {e2:.2e}")
+    assert e1 < RES_BOUNDS[prec] and e2 < RES_BOUNDS[prec]
+    assert max(e1, e2) > 1e-13  # reduced precision really used
