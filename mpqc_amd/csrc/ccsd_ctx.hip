@@ -574,7 +574,7 @@ void CCSDContext::gemmP(int M, int N, int K, double alpha, const Planes& A, long
     }
     MPQC_HIP_CHECK(hipEventRecord(ea, stream_));
   }
-  gemm_planes(d, ws_.ptr, ws_.bytes, stream_);
+  gemm_planes(d, pws_, pws_bytes_, stream_);
   if (kernel_timing_) {
     MPQC_HIP_CHECK(hipEventRecord(eb, stream_));
     pending_.push_back({slot, ea, eb, 2.0 * M * (double)N * K});
@@ -591,6 +591,18 @@ void CCSDContext::prepare_planes() {
     tau_P_ = alloc_planes(oovv); tauT_P_ = alloc_planes(oovv); TP_P_ = alloc_planes(oovv);
     t2P_P_ = alloc_planes(oovv); Xjk_P_ = alloc_planes(oovv); K_P_ = alloc_planes(oovv);
     Y1t_P_ = alloc_planes(oovv); Y2t_P_ = alloc_planes(oovv); WT_P_ = alloc_planes(oooo);
+  }
+  {  // split-K workspace for the largest plane contraction shapes
+    const int o = o_, v = v_;
+    size_t need = 0;
+    const int shapes[][3] = {{v * v, o * o, v * v}, {o * v, o * o, v * v}, {o * o, o * o, v * v},
+                             {v * v, o * o, o * o}, {o * v, o * v, o * v}};
+    for (auto& sh : shapes) need = std::max(need, gemm_planes_workspace_bytes(sh[0], sh[1], sh[2], 1));
+    if (need > pws_bytes_) {
+      const size_t n = (need + sizeof(double) - 1) / sizeof(double);
+      pws_ = alloc(n);
+      pws_bytes_ = n * sizeof(double);
+    }
   }
   const std::array<long, 4> e_ijab{O, O, V, V};
   const std::array<int, 4> I{0, 1, 2, 3};
@@ -693,33 +705,33 @@ void CCSDContext::residual() {
 
   // ===================== r1: ccsd_r1_r2.h:70-124 =====================
   // :83-84  r1(a,i) = Fia(i,a) - 2 (Fia(k,c) t1(c,i)) t1(a,k)
-  gemm(false, false, o, o, v, 1.0, Fia_, v, t1_, o, 0.0, Xoo_, o);
+  gemm(false, false, o, o, v, 1.0, Fia_, v, t1_, o, 0.0, Xoo_, o, 0, 0, 0, 1, "r1:fia_t1");
   MPQC_HIP_CHECK(hipMemcpyAsync(r1_, f_ai_, ov * sizeof(double), hipMemcpyDeviceToDevice, stream_));
-  gemm(false, false, v, o, o, -2.0, t1_, o, Xoo_, o, 1.0, r1_, o);
+  gemm(false, false, v, o, o, -2.0, t1_, o, Xoo_, o, 1.0, r1_, o, 0, 0, 0, 1, "r1:t1_fia_t1");
   // :89-91  h_ac = F_vv = Fab(a,c) - gbar(k,l,c,d) tau(a,d,k,l); FAB = h_ac; r1 += h_ac t1
   //         (ccsd_intermediates.h:71-77)
   perm_add(tau_akld_, {V, O, O, V}, 0.0, {{tau_, e_abij, {0, 2, 3, 1}, 1.0}});
   MPQC_HIP_CHECK(hipMemcpyAsync(FAB_, Fab_, vv * sizeof(double), hipMemcpyDeviceToDevice, stream_));
   gemm(false, true, v, v, o * o * v, -1.0, tau_akld_, oo * V, Gbar_cKLD_, oo * V, 1.0, FAB_, v, 0, 0, 0, 1, "r1:F_vv");
-  gemm(false, false, v, o, v, 1.0, FAB_, v, t1_, o, 1.0, r1_, o);
+  gemm(false, false, v, o, v, 1.0, FAB_, v, t1_, o, 1.0, r1_, o, 0, 0, 0, 1, "r1:FAB_t1");
   // :95-97  h_ki = F_oo = Fij(k,i) + gbar(k,l,a,b) tau(a,b,i,l); FIJ = h_ki; r1 -= t1 h_ki
   //         (ccsd_intermediates.h:57-63)
   perm_add(tau_ilab_, {O, O, V, V}, 0.0, {{tau_, e_abij, {2, 3, 0, 1}, 1.0}});
   MPQC_HIP_CHECK(hipMemcpyAsync(FIJ_, Fij_, oo * sizeof(double), hipMemcpyDeviceToDevice, stream_));
   gemm(false, true, o, o, o * v * v, 1.0, Gbar_, O * vv, tau_ilab_, O * vv, 1.0, FIJ_, o, 0, 0, 0, 1, "r1:F_oo");
-  gemm(false, false, v, o, o, -1.0, t1_, o, FIJ_, o, 1.0, r1_, o);
+  gemm(false, false, v, o, o, -1.0, t1_, o, FIJ_, o, 1.0, r1_, o, 0, 0, 0, 1, "r1:t1_FIJ");
   // :101-104 h_kc = F_ov = Fia(k,c) + gbar(k,l,c,d) t1(d,l)   (ccsd_intermediates.h:89-95)
   //          r1 += h_kc (2 t2(c,a,k,i) - t2(c,a,i,k) + t1(c,i) t1(a,k))
   MPQC_HIP_CHECK(hipMemcpyAsync(hkc_, Fia_, ov * sizeof(double), hipMemcpyDeviceToDevice, stream_));
-  gemm(false, false, o * v, 1, o * v, 1.0, Gbar_kcld_, ov, t1T_, 1, 1.0, hkc_, 1);
+  gemm(false, false, o * v, 1, o * v, 1.0, Gbar_kcld_, ov, t1T_, 1, 1.0, hkc_, 1, 0, 0, 0, 1, "r1:F_ov");
   perm_add(hT_, {1, 1, V, O}, 0.0, {{hkc_, {1, 1, O, V}, {0, 1, 3, 2}, 1.0}});  // hT(c,k) = h(k,c)
   perm_add(Y5_, {V, O, V, O}, 0.0,
            {{t2_, e_abij, {1, 3, 0, 2}, 2.0}, {t2_, e_abij, {1, 2, 0, 3}, -1.0}});
-  gemm(false, false, v * o, 1, v * o, 1.0, Y5_, ov, hT_, 1, 1.0, r1_, 1);
-  gemm(false, false, o, o, v, 1.0, hkc_, v, t1_, o, 0.0, Zoo_, o);
-  gemm(false, false, v, o, o, 1.0, t1_, o, Zoo_, o, 1.0, r1_, o);
+  gemm(false, false, v * o, 1, v * o, 1.0, Y5_, ov, hT_, 1, 1.0, r1_, 1, 0, 0, 0, 1, "r1:hkc_t2bar");
+  gemm(false, false, o, o, v, 1.0, hkc_, v, t1_, o, 0.0, Zoo_, o, 0, 0, 0, 1, "r1:hkc_t1");
+  gemm(false, false, v, o, o, 1.0, t1_, o, Zoo_, o, 1.0, r1_, o, 0, 0, 0, 1, "r1:t1_hkc_t1");
   // :107-108 r1 += (2 Gijab(k,i,c,a) - Giajb(k,a,i,c)) t1(c,k)
-  gemm(false, false, v * o, 1, v * o, 1.0, P6_, ov, t1_, 1, 1.0, r1_, 1);
+  gemm(false, false, v * o, 1, v * o, 1.0, P6_, ov, t1_, 1, 1.0, r1_, 1, 0, 0, 0, 1, "r1:g_t1");
   // :110-114 r1 += g_iabc_bar(k,a,c,d) tau(c,d,k,i)
   perm_add(tau_kcdi_, {O, V, V, O}, 0.0, {{tau_, e_abij, {2, 0, 1, 3}, 1.0}});
   gemm(false, false, v, o, o * v * v, 1.0, Gbar_akcd_, O * vv, tau_kcdi_, o, 1.0, r1_, o, 0, 0, 0, 1, "r1:giabc_tau");
@@ -732,27 +744,27 @@ void CCSDContext::residual() {
   //   first term, batched over i: Rd[(a,b), i, j] = sum_c Giabc[i][c,(a,b)] t1[c,j]
   gemm(true, false, v * v, o, v, 1.0, Giabc_, vv, t1_, o, 0.0, Rd_, oo, vv * V, 0, O, o, "r2:giabc_t1");
   //   second term as t1(a,k) * (Giajb(k,b,i,c) t1(c,j))
-  gemm(false, false, o * o * v, o, v, 1.0, Giajb_, v, t1_, o, 0.0, Q_, o);
-  gemm(false, false, v, v * o * o, o, -1.0, t1_, o, Q_, V * oo, 1.0, Rd_, V * oo);
+  gemm(false, false, o * o * v, o, v, 1.0, Giajb_, v, t1_, o, 0.0, Q_, o, 0, 0, 0, 1, "r2:giajb_t1");
+  gemm(false, false, v, v * o * o, o, -1.0, t1_, o, Q_, V * oo, 1.0, Rd_, V * oo, 0, 0, 0, 1, "r2:t1_giajb_t1");
   // :210-211 Rd -= (Gijka(j,i,k,a) + Gijab(i,k,a,c) t1(c,j)) t1(b,k)
-  gemm(false, false, o * o * v, o, v, 1.0, Gijab_, v, t1_, o, 0.0, M12_, o);
+  gemm(false, false, o * o * v, o, v, 1.0, Gijab_, v, t1_, o, 0.0, M12_, o, 0, 0, 0, 1, "r2:gijab_t1");
   perm_add(S_, {V, O, O, O}, 0.0,
            {{S11_, {V, O, O, O}, I, 1.0}, {M12_, {O, O, V, O}, {2, 0, 3, 1}, 1.0}});
-  gemm(false, true, v, o * o, o, -1.0, t1_, o, S_, o, 1.0, Rd_, oo, 0, O * oo, V * oo, v);
+  gemm(false, true, v, o * o, o, -1.0, t1_, o, S_, o, 1.0, Rd_, oo, 0, O * oo, V * oo, v, "r2:t1_S");
   // :218-220 g_ki = F_OO(FIJ, Fia, g_ijka_bar, t1) = FIJ + Fia t1 + gbar(k,l,i,c) t1(c,l)
   //          (ccsd_intermediates.h:105-112)
   MPQC_HIP_CHECK(hipMemcpyAsync(gki_, FIJ_, oo * sizeof(double), hipMemcpyDeviceToDevice, stream_));
-  gemm(false, false, o, o, v, 1.0, Fia_, v, t1_, o, 1.0, gki_, o);
-  gemm(false, false, o * o, 1, o * v, 1.0, P13_, ov, t1T_, 1, 1.0, gki_, 1);
+  gemm(false, false, o, o, v, 1.0, Fia_, v, t1_, o, 1.0, gki_, o, 0, 0, 0, 1, "r2:gki_fia_t1");
+  gemm(false, false, o * o, 1, o * v, 1.0, P13_, ov, t1T_, 1, 1.0, gki_, 1, 0, 0, 0, 1, "r2:gki_g_t1");
   // :222-224 g_ac = F_VV(FAB, Fia, g_iabc_bar, t1) = FAB + f_ov(k,c) t1(a,k) + gbar(k,a,d,c) t1(d,k)
   //          (ccsd_intermediates.h:141-148; fov_sign = +1 is the reference as written)
   MPQC_HIP_CHECK(hipMemcpyAsync(gac_, FAB_, vv * sizeof(double), hipMemcpyDeviceToDevice, stream_));
-  gemm(false, false, v, v, o, fov_sign_, t1_, o, Fia_, v, 1.0, gac_, v);
-  gemm(false, false, v * v, 1, o * v, 1.0, P14_, ov, t1T_, 1, 1.0, gac_, 1);
+  gemm(false, false, v, v, o, fov_sign_, t1_, o, Fia_, v, 1.0, gac_, v, 0, 0, 0, 1, "r2:gac_t1_fia");
+  gemm(false, false, v * v, 1, o * v, 1.0, P14_, ov, t1T_, 1, 1.0, gac_, 1, 0, 0, 0, 1, "r2:gac_g_t1");
   // :226 Rd += g_ac(a,c) t2(c,b,i,j) - g_ki(k,i) t2(a,b,k,j)
   gemm(false, false, v, v * o * o, v, 1.0, gac_, v, t2_, V * oo, 1.0, Rd_, V * oo, 0, 0, 0, 1, "r2:g_ac_t2");
   perm_add(t2_abjk_, {V, V, O, O}, 0.0, {{t2_, e_abij, {0, 1, 3, 2}, 1.0}});
-  gemm(false, false, v * v * o, o, o, 1.0, t2_abjk_, o, gki_, o, 0.0, X16_, o);
+  gemm(false, false, v * v * o, o, o, 1.0, t2_abjk_, o, gki_, o, 0.0, X16_, o, 0, 0, 0, 1, "r2:t2_gki");
   perm_add(Rd_, {V, V, O, O}, 1.0, {{X16_, e_abij, {0, 1, 3, 2}, -1.0}});
   // :236 T(d,b,i,l) = 0.5 t2(d,b,i,l) + t1(d,i) t1(b,l), as TP(a,i,l,d) = T(d,a,i,l)
   build_TP(TP_, t2_, t1_, o, v, stream_);
@@ -764,8 +776,8 @@ void CCSDContext::residual() {
   //   Giajb(k,a,i,c) - Gijka(k,l,i,c) t1(a,l) + Giabc(k,a,d,c) t1(d,i) - Gijab(k,l,d,c) T(d,a,i,l)
   MPQC_HIP_CHECK(hipMemcpyAsync(J_, J0_, ov * ov * sizeof(double), hipMemcpyDeviceToDevice, stream_));
   MPQC_HIP_CHECK(hipMemcpyAsync(K_, K0_, ov * ov * sizeof(double), hipMemcpyDeviceToDevice, stream_));
-  gemm(false, false, v, o * o * v, o, -1.0, t1_, o, X18_, oo * V, 1.0, J_, oo * V);
-  gemm(false, false, v, o * o * v, o, -1.0, t1_, o, X19_, oo * V, 1.0, K_, oo * V);
+  gemm(false, false, v, o * o * v, o, -1.0, t1_, o, X18_, oo * V, 1.0, J_, oo * V, 0, 0, 0, 1, "r2:j_gijka_t1");
+  gemm(false, false, v, o * o * v, o, -1.0, t1_, o, X19_, oo * V, 1.0, K_, oo * V, 0, 0, 0, 1, "r2:k_gijka_t1");
   gemm(false, false, o * v * v, o, v, 1.0, Giabc_, v, t1_, o, 0.0, Zj_, o, 0, 0, 0, 1, "r2:j_giabc_t1");  // Zj(k,a,c,i)
   gemm(true, false, v, o, v, 1.0, Giabc_, v, t1_, o, 0.0, Zk_, o, vv, 0, V * O, o * v, "r2:k_giabc_t1");  // Zk(k,a,c,i)
   perm_add(J_, {V, O, O, V}, 1.0, {{Zj_, {O, V, V, O}, {1, 3, 0, 2}, 1.0}});
@@ -819,7 +831,7 @@ void CCSDContext::residual() {
               {Gijab_, {O, O, V, V}, {2, 3, 0, 1}, 1.0}});
   }
   // :266-269 a_klij = W_oooo (ccsd_intermediates.h:182-193); r2 += a_klij(k,l,i,j) tau(a,b,k,l)
-  gemm(false, false, o * o * o, o, v, 1.0, Gijka_, v, t1_, o, 0.0, Y23_, o);
+  gemm(false, false, o * o * o, o, v, 1.0, Gijka_, v, t1_, o, 0.0, Y23_, o, 0, 0, 0, 1, "r2:W_gijka_t1");
   perm_add(W_, {O, O, O, O}, 0.0,
            {{Gijkl_, {O, O, O, O}, I, 1.0}, {Y23_, {O, O, O, O}, I, 1.0},
             {Y23_, {O, O, O, O}, {1, 0, 3, 2}, 1.0}});
@@ -844,9 +856,9 @@ void CCSDContext::residual() {
     gemm(false, false, o * v, o * o, v * v, 1.0, Giabc_, vv, tau_, oo, 0.0, tmp24_, oo, 0, 0, 0, 1, "r2:pp_t1_giabc_tau");
   }
   // :278-280 b -= tmp(k,a,j,i) t1(b,k) + tmp(k,b,i,j) t1(a,k)
-  gemm(true, true, v * o * o, v, o, 1.0, tmp24_, V * oo, t1_, o, 0.0, Y24_, v);  // Y24(a,j,i,b)
+  gemm(true, true, v * o * o, v, o, 1.0, tmp24_, V * oo, t1_, o, 0.0, Y24_, v, 0, 0, 0, 1, "r2:pp_t1_b");  // Y24(a,j,i,b)
   perm_add(r2_, {V, V, O, O}, 1.0, {{Y24_, {V, O, O, V}, {0, 3, 2, 1}, -1.0}});
-  gemm(false, false, v, v * o * o, o, -1.0, t1_, o, tmp24_, V * oo, 1.0, r2_, V * oo);
+  gemm(false, false, v, v * o * o, o, -1.0, t1_, o, tmp24_, V * oo, 1.0, r2_, V * oo, 0, 0, 0, 1, "r2:pp_t1_a");
 }
 
 // ---------------------------------------------------------------- energy / error / update

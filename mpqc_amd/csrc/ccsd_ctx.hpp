@@ -100,6 +100,8 @@ class CCSDContext {
              double beta, double* C, long ldc, const char* tag);
   void prepare_planes();
   bool planes_ready_ = false;
+  double* pws_ = nullptr;  // split-K workspace of the plane GEMMs
+  size_t pws_bytes_ = 0;
   std::vector<std::pair<uint16_t*, size_t>> plane_allocs_;
   Planes Gabcd_P_, Giabc_P_, Gijab_P_, GAt_P_, GBt_P_, GCt_P_;
   Planes tau_P_, tauT_P_, TP_P_, t2P_P_, Xjk_P_, K_P_, Y1t_P_, Y2t_P_, WT_P_;

@@ -33,9 +33,21 @@ struct KParams {
   int kchunk;
   int nsplit;
   int batch0;  // first batch index of this launch
-  int tiles_n;
+  int tiles_m, tiles_n;
+  int per_xcd;  // tiles per XCD (workgroup w runs on XCD w % 8)
   double alpha, beta;
 };
+
+constexpr int kXcdCount = 8;
+
+// XCD-aware tile order: workgroups are dispatched round-robin over the 8 XCDs;
+// giving each XCD a contiguous range of row-major tiles keeps the operand rows
+// those tiles share inside one XCD's L2.  Returns -1 for padding workgroups.
+__device__ inline int xcd_tile(const KParams& p) {
+  const int w = blockIdx.x;
+  const int t = (w % kXcdCount) * p.per_xcd + w / kXcdCount;
+  return t < p.tiles_m * p.tiles_n ? t : -1;
+}
 
 template <int BM, int BN, bool TA, bool TB>
 __global__ __launch_bounds__(kThreads, 2) void dgemm_f64_kernel(KParams p) {
@@ -56,7 +68,8 @@ __global__ __launch_bounds__(kThreads, 2) void dgemm_f64_kernel(KParams p) {
   const int wm = wave >> 1, wn = wave & 1;
   const int lr = lane & 15, lk = lane >> 4;
 
-  const int tile = blockIdx.x;
+  const int tile = xcd_tile(p);
+  if (tile < 0) return;
   const int tm = tile / p.tiles_n, tn = tile - tm * p.tiles_n;
   const int z = blockIdx.y;
   const int b = p.batch0 + z / p.nsplit;
@@ -201,7 +214,8 @@ __global__ __launch_bounds__(kThreads, 2) void sgemm_f32acc64_kernel(KParams p) 
   const int wm = wave >> 1, wn = wave & 1;
   const int lr = lane & 15, lk = lane >> 4;
 
-  const int tile = blockIdx.x;
+  const int tile = xcd_tile(p);
+  if (tile < 0) return;
   const int tm = tile / p.tiles_n, tn = tile - tm * p.tiles_n;
   const int z = blockIdx.y;
   const int b = p.batch0 + z / p.nsplit;
@@ -373,8 +387,9 @@ void launch(const GemmDesc& d, GemmPrecision prec, const GemmWorkspace& ws, hipS
     p.A = d.A; p.B = d.B; p.C = d.C; p.W = ws.ptr;
     p.lda = d.lda; p.ldb = d.ldb; p.ldc = d.ldc; p.sA = d.sA; p.sB = d.sB; p.sC = d.sC;
     p.M = d.M; p.N = d.N; p.K = d.K; p.kchunk = kchunk; p.nsplit = nsplit; p.batch0 = b0;
-    p.tiles_n = tiles_n; p.alpha = d.alpha; p.beta = d.beta;
-    dim3 grid((unsigned)tiles, (unsigned)(nb * nsplit));
+    p.tiles_m = tiles_m; p.tiles_n = tiles_n; p.alpha = d.alpha; p.beta = d.beta;
+    p.per_xcd = (int)ceil_div(tiles, kXcdCount);
+    dim3 grid((unsigned)(p.per_xcd * kXcdCount), (unsigned)(nb * nsplit));
     if (prec == GemmPrecision::F32) {
       if (!d.transA && !d.transB) hipLaunchKernelGGL((sgemm_f32acc64_kernel<BM, BN, false, false>), grid, dim3(kThreads), 0, stream, p);
       else if (!d.transA && d.transB) hipLaunchKernelGGL((sgemm_f32acc64_kernel<BM, BN, false, true>), grid, dim3(kThreads), 0, stream, p);

@@ -5,9 +5,11 @@
 // through LDS in 32-wide slices (one MFMA K), double buffered.  Both operands
 // are k-contiguous in HBM and in LDS, so a slice row is copied with 16-byte
 // loads and every MFMA fragment (8 consecutive k of one row) is one 16-byte
-// LDS read.  FP32 partial sums are folded into FP64 accumulators every 4
-// slices.  Workgroups are mapped to tiles XCD by XCD (consecutive tiles on the
-// same XCD share operand rows in that XCD's L2).
+// LDS read.  The matrix cores accumulate in FP32 over K chunks of at most
+// kMaxChunk; chunks are summed in FP64 by the split-K pass, which bounds the
+// FP32 accumulation length without FP64 registers in the kernel (two
+// workgroups fit per CU).  Workgroups are mapped to tiles XCD by XCD
+// (consecutive tiles on the same XCD share operand rows in that XCD's L2).
 #include "planes.hpp"
 
 #include <algorithm>
@@ -21,6 +23,7 @@ constexpr int kThr = 256;
 constexpr int kBK = 32;
 constexpr int kRow = kBK + 8;  // LDS row in halves (16-byte aligned rows)
 constexpr int kXcd = 8;
+constexpr int kMaxChunk = 2048;  // longest K summed in FP32
 
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef _Float16 f16x8_t __attribute__((ext_vector_type(8)));
@@ -152,12 +155,11 @@ __device__ inline void mma3(const u16x8_t* a, const u16x8_t* b, float4_t& c) {
 }
 
 template <int BM, int BN, int NP, bool F16, bool VEC>
-__global__ __launch_bounds__(kThr, 1) void gemm_planes_kernel(PParams p) {
+__global__ __launch_bounds__(kThr, 2) void gemm_planes_kernel(PParams p) {
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int RB = WM / 16, CB = WN / 16;
   constexpr int ACH = BM * (kBK / 8) / kThr;  // 16-byte chunks per thread per plane (A)
   constexpr int BCH = BN * (kBK / 8) / kThr;
-  constexpr int kFlush = 4;
 
   __shared__ __attribute__((aligned(16))) uint16_t As[2][NP][BM][kRow];
   __shared__ __attribute__((aligned(16))) uint16_t Bs[2][NP][BN][kRow];
@@ -243,17 +245,13 @@ __global__ __launch_bounds__(kThr, 1) void gemm_planes_kernel(PParams p) {
     }
   };
 
-  double4_t acc[RB][CB];
   float4_t facc[RB][CB];
 #pragma unroll
   for (int i = 0; i < RB; ++i)
 #pragma unroll
-    for (int j = 0; j < CB; ++j) {
-      acc[i][j] = double4_t{0.0, 0.0, 0.0, 0.0};
-      facc[i][j] = float4_t{0.f, 0.f, 0.f, 0.f};
-    }
+    for (int j = 0; j < CB; ++j) facc[i][j] = float4_t{0.f, 0.f, 0.f, 0.f};
 
-  int buf = 0, since = 0;
+  int buf = 0;
   if (k_begin < k_end) {
     load_regs(k_begin);
     store_lds(0);
@@ -277,17 +275,6 @@ __global__ __launch_bounds__(kThr, 1) void gemm_planes_kernel(PParams p) {
 #pragma unroll
       for (int j = 0; j < CB; ++j) mma3<NP, F16>(fa, fb[j], facc[i][j]);
     }
-    if (++since == kFlush || !has_next) {
-      since = 0;
-#pragma unroll
-      for (int i = 0; i < RB; ++i)
-#pragma unroll
-        for (int j = 0; j < CB; ++j) {
-#pragma unroll
-          for (int r = 0; r < 4; ++r) acc[i][j][r] += (double)facc[i][j][r];
-          facc[i][j] = float4_t{0.f, 0.f, 0.f, 0.f};
-        }
-    }
     if (has_next) store_lds(buf ^ 1);
     __syncthreads();
     buf ^= 1;
@@ -305,7 +292,7 @@ __global__ __launch_bounds__(kThr, 1) void gemm_planes_kernel(PParams p) {
         const int gm = m0 + wm * WM + i * 16 + lk * 4 + r;
         const int gn = n0 + wn * WN + j * 16 + lr;
         if (gm < p.M && gn < p.N) {
-          const double v = unscale * acc[i][j][r];
+          const double v = unscale * (double)facc[i][j][r];
           if (p.nsplit == 1) {
             double* c = C + (long)gm * p.ldc + gn;
             *c = (p.beta == 0.0) ? v : v + p.beta * (*c);
@@ -340,19 +327,16 @@ void launch_planes(const PlaneGemmDesc& d, double* ws, size_t ws_bytes, hipStrea
     nsplit = (int)std::min<long>(ceil_div(want, tiles * d.batch), d.K / (4 * kBK));
     nsplit = std::max(1, std::min(nsplit, 64));
   }
+  // FP32 accumulation length bounded by kMaxChunk: more chunks, summed in FP64
+  nsplit = std::max<int>(nsplit, (int)ceil_div(d.K, kMaxChunk));
   int kchunk = (int)(ceil_div(ceil_div(d.K, nsplit), kBK) * kBK);
   if (nsplit > 1) nsplit = (int)ceil_div(d.K, kchunk);
   const size_t slice = (size_t)d.M * d.N * sizeof(double);
   int max_batch = 65535 / nsplit;
   if (nsplit > 1) {
     const size_t cap = ws_bytes / (slice * nsplit);
-    if (cap == 0) {
-      nsplit = 1;
-      kchunk = (int)(ceil_div(d.K, kBK) * kBK);
-      max_batch = 65535;
-    } else {
-      max_batch = (int)std::min<size_t>(max_batch, cap);
-    }
+    if (cap == 0) throw std::runtime_error("gemm_planes: split-K workspace too small");
+    max_batch = (int)std::min<size_t>(max_batch, cap);
   }
   const bool vec = d.K % 8 == 0 && d.lda % 8 == 0 && d.ldb % 8 == 0 && d.sA % 8 == 0 &&
                    d.sB % 8 == 0;
@@ -413,6 +397,14 @@ void split_to_planes(PlaneKind kind, const Planes& out, const long n[4], const T
   const int g2 = (int)std::max<long>(1, std::min<long>(16384, ceil_div(total, 256)));
   hipLaunchKernelGGL(split_write, dim3(g2), dim3(256), 0, stream, a);
   check_launch("split_to_planes");
+}
+
+size_t gemm_planes_workspace_bytes(int M, int N, int K, int batch) {
+  const long tiles = ceil_div(M, 128) * ceil_div(N, 128);
+  long nsplit = 1;
+  if (tiles * batch < 512 && K >= 8 * kBK) nsplit = std::min<long>(64, std::max<long>(1, std::min<long>(ceil_div(512, tiles * batch), K / (4 * kBK))));
+  nsplit = std::max<long>(nsplit, ceil_div(K, kMaxChunk)) + 1;
+  return nsplit > 2 ? (size_t)nsplit * M * N * batch * sizeof(double) : 0;
 }
 
 void gemm_planes(const PlaneGemmDesc& d, double* ws, size_t ws_bytes, hipStream_t stream) {

@@ -49,5 +49,7 @@ struct PlaneGemmDesc {
   int batch = 1;
 };
 void gemm_planes(const PlaneGemmDesc& d, double* ws, size_t ws_bytes, hipStream_t stream);
+// Workspace (bytes) gemm_planes needs for its split-K partial sums at this shape.
+size_t gemm_planes_workspace_bytes(int M, int N, int K, int batch);
 
 }  // namespace mpqc_hip
