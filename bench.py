@@ -1,19 +1,22 @@
 #!/usr/bin/env python3
 """Benchmark of the CCSD amplitude-residual hot path on one MI355X.
 
-Default workload (one "step" = one launch of the hot kernel over resident data):
-  pp-ladder  R[a,b,i,j] += sum_cd tau[c,d,i,j] V[a,b,c,d], o=32, v=256, FP64
-             (ccsd_r1_r2.h:275).  V ~ N(0, 0.05) symmetrised to the 8-fold
-             (ac|bd) symmetry, tau ~ N(0, 1e-2) / (e_c + e_d - e_i - e_j),
-             e_occ ~ U[-1.5, -0.3], e_vir ~ U[0.1, 3.0]; V and tau resident in
-             HBM before timing.
+Default workload (one "step" = one CCSD iteration over resident data):
+  ccsd-h2o8  (H2O)8 on a 2x2x2 lattice (O-O 2.8 A, seeded 0.1 A jitter, random
+             orientations), cc-pVDZ, o=40, v=152: RHF and AO->MO on the host,
+             then the converged solve to 1e-8 and K timed iterations
+             (update + tau + residual + energy/error) in FP64 and in each
+             mixed-precision form; the headline is the fastest form within
+             1 uEh of FP64.
 
-Prints ONE JSON line with the metric, the roofline of the dominant kernel
-(MFMA-bound, algorithmic FLOPs 2 o^2 v^4 per launch) and the CPU baseline (the
-oracle's contraction on the host cores over a bounded row sample).
+Prints ONE JSON line with the metric, the roofline of the dominant kernel (the
+pp-ladder contraction, algorithmic FLOPs 2 o^2 v^4 per launch) and the CPU
+baseline (one oracle residual on the host cores).
 
-Other workloads: --workload ccsd-h2o8 runs the full CCSD iteration on the
-(H2O)8 cc-pVDZ system (o=40, v=152) in FP64 and mixed precision.
+Other workloads: pp-ladder (R[a,b,i,j] += sum_cd tau[c,d,i,j] V[a,b,c,d],
+o=32, v=256, FP64, ccsd_r1_r2.h:275; V ~ N(0, 0.05) symmetrised to the 8-fold
+(ac|bd) symmetry, tau ~ N(0, 1e-2) / (e_c + e_d - e_i - e_j)); ccsd-h2o6
+(o=30, v=114); ccsd-h20 (H20 chain, o=10, v=90); copy (counter calibration).
 """
 from __future__ import annotations
 
@@ -26,7 +29,13 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK = {"f64": 78.6, "f32": 157.3, "bf16x3": 2500.0}  # TFLOP/s dense MFMA, MI355X
+# TFLOP/s dense MFMA, MI355X, of the unit each form's contractions run on
+PEAK = {"f64": 78.6, "f32": 157.3, "bf16x3": 2500.0, "f16x3": 2500.0, "bf16x6": 2500.0}
+DTYPE = {"f64": "f64",
+         "f32": "f32-mfma contractions / f64 update, DIIS, energy",
+         "bf16x3": "bf16 hi+lo split contractions (3 products) / f64 update, DIIS, energy",
+         "f16x3": "fp16 hi+lo split contractions (3 products) / f64 update, DIIS, energy",
+         "bf16x6": "bf16 hi+mid+lo split contractions (6 products) / f64 update, DIIS, energy"}
 
 
 def _args():
@@ -36,7 +45,7 @@ def _args():
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--workload", default="ccsd-h2o8",
                    choices=["ccsd-h2o8", "ccsd-h2o6", "ccsd-h20", "pp-ladder", "copy"])
-    p.add_argument("--forms", default="f64,f32",
+    p.add_argument("--forms", default="f64,f16x3",
                    help="contraction precisions to run for the CCSD workloads (first = reference)")
     p.add_argument("--max-iter", type=int, default=40)
     p.add_argument("--copy-gib", type=float, default=4.0,
@@ -269,6 +278,12 @@ def bench_ccsd(args):
             c.energy_error()
         kt = c.kernel_times()
         c.close()
+        n_it = args.warmup + args.steps
+        tot = sum(x["ms"] for x in kt.values()) / n_it
+        log(f"{prec}: {ms:.2f} ms/iteration; event-timed phases sum to {tot:.2f} ms/iteration")
+        for k, x in sorted(kt.items(), key=lambda kv: -kv[1]["ms"])[:40]:
+            log(f"   {k:24s} {x['ms'] / n_it:8.3f} ms/it  {x['calls'] / n_it:5.1f} calls/it"
+                + (f"  {x['flops'] / max(1e-12, x['ms'] * 1e-3) / 1e12:7.2f} TF" if x["flops"] else ""))
         forms[prec] = {"E_corr": E, "iterations": len(rows), "ms_per_iteration": ms,
                        "tflops": flops / (ms * 1e-3) / 1e12,
                        "solve_iteration_ms": [round(r.seconds * 1e3, 3) for r in rows],
@@ -297,8 +312,7 @@ def bench_ccsd(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"f64": "f64", "f32": "f32-mfma contractions / f64 update, DIIS, energy",
-                  "bf16x3": "bf16x3-split contractions / f64 update"}[head],
+        "dtype": DTYPE[head],
         "data": "synthetic molecule (seeded geometry), RHF + AO->MO on the host",
         "config": {"workload": SYSTEMS[args.workload], "o": o, "v": v,
                    "algorithmic_flops_per_iteration": flops, "form": head,

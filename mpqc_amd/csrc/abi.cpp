@@ -246,7 +246,7 @@ int mpqc_ccsd_gemm(int transa, int transb, int m, int n, int k, double alpha, co
     mpqc_hip::GemmDesc d = mpqc_hip::make_gemm(transa != 0, transb != 0, m, n, k, alpha, a, lda, b,
                                                ldb, beta, c, ldc);
     d.sA = stride_a; d.sB = stride_b; d.sC = stride_c; d.batch = batch;
-    const size_t want = std::max<size_t>((size_t)m * n * batch * 64 * sizeof(double), 1u << 20);
+    const size_t want = std::max<size_t>((size_t)m * n * batch * 64 * sizeof(double), (size_t)64 << 20);
     auto ws = workspace_for_current_device(std::min<size_t>(want, (size_t)1 << 30));
     mpqc_hip::gemm(d, (mpqc_hip::GemmPrecision)precision, ws, (hipStream_t)stream);
   });

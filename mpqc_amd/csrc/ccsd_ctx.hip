@@ -418,33 +418,27 @@ void CCSDContext::gemm(bool tA, bool tB, int M, int N, int K, double alpha, cons
   check_access(A, (tA ? (long)(K - 1) * lda + (M - 1) : (long)(M - 1) * lda + (K - 1)) + bm * sA, "gemm A");
   check_access(B, (tB ? (long)(N - 1) * ldb + (K - 1) : (long)(K - 1) * ldb + (N - 1)) + bm * sB, "gemm B");
   check_access(C, (long)(M - 1) * ldc + (N - 1) + bm * sC, "gemm C");
-  if (!kernel_timing_) {
-    mpqc_hip::gemm(d, prec, ws_, stream_);
-    return;
-  }
-  const std::string t = tag ? tag : "gemm";
-  int slot = -1;
+  timed(tag ? tag : "gemm", 2.0 * M * (double)N * K * batch,
+        [&] { mpqc_hip::gemm(d, prec, ws_, stream_); });
+}
+
+int CCSDContext::timing_slot(const char* tag) {
+  const std::string t = tag;
   for (size_t k = 0; k < ktimes_.size(); ++k)
-    if (ktimes_[k].tag == t) slot = (int)k;
-  if (slot < 0) {
-    ktimes_.push_back({t, 0, 0.0, 0.0});
-    slot = (int)ktimes_.size() - 1;
+    if (ktimes_[k].tag == t) return (int)k;
+  ktimes_.push_back({t, 0, 0.0, 0.0});
+  return (int)ktimes_.size() - 1;
+}
+
+hipEvent_t CCSDContext::take_event() {
+  hipEvent_t e;
+  if (!event_pool_.empty()) {
+    e = event_pool_.back();
+    event_pool_.pop_back();
+  } else {
+    MPQC_HIP_CHECK(hipEventCreate(&e));
   }
-  auto take = [&]() {
-    hipEvent_t e;
-    if (!event_pool_.empty()) {
-      e = event_pool_.back();
-      event_pool_.pop_back();
-    } else {
-      MPQC_HIP_CHECK(hipEventCreate(&e));
-    }
-    return e;
-  };
-  PendingEvent pe{slot, take(), take(), 2.0 * M * (double)N * K * batch};
-  MPQC_HIP_CHECK(hipEventRecord(pe.a, stream_));
-  mpqc_hip::gemm(d, prec, ws_, stream_);
-  MPQC_HIP_CHECK(hipEventRecord(pe.b, stream_));
-  pending_.push_back(pe);
+  return e;
 }
 
 void CCSDContext::resolve_events() {
@@ -494,18 +488,21 @@ int CCSDContext::make_terms(std::array<long, 4> n, std::initializer_list<Src> sr
       check_access(s.p, maxoff, "permutation input");
     }
     terms[nt].alpha = s.alpha;
+    terms[nt].vol = s.ext[0] * s.ext[1] * s.ext[2] * s.ext[3];
     ++nt;
   }
   return nt;
 }
 
 void CCSDContext::perm_add(double* out, std::array<long, 4> n, double beta,
-                           std::initializer_list<Src> srcs) {
+                           std::initializer_list<Src> srcs, const char* tag, int line) {
   TTerm terms[kMaxTerms];
   check_access(out, n[0] * n[1] * n[2] * n[3] - 1, "perm_add out");
   const int nt = make_terms(n, srcs, terms);
   long nn[4] = {n[0], n[1], n[2], n[3]};
-  tadd(out, nn, beta, terms, nt, stream_);
+  std::string t;
+  if (kernel_timing_) t = tag ? std::string(tag) : "perm@" + std::to_string(line);
+  timed(t.c_str(), 0.0, [&] { tadd(out, nn, beta, terms, nt, stream_); });
 }
 
 // ---------------------------------------------------------------- planes
@@ -535,11 +532,14 @@ Planes CCSDContext::alloc_planes(long n) {
   return P;
 }
 
-void CCSDContext::split(const Planes& out, std::array<long, 4> n, std::initializer_list<Src> srcs) {
+void CCSDContext::split(const Planes& out, std::array<long, 4> n, std::initializer_list<Src> srcs,
+                        const char* tag, int line) {
   TTerm terms[kMaxTerms];
   const int nt = make_terms(n, srcs, terms);
   long nn[4] = {n[0], n[1], n[2], n[3]};
-  split_to_planes(plane_kind(), out, nn, terms, nt, partials_, stream_);
+  std::string t;
+  if (kernel_timing_) t = tag ? std::string(tag) : "split@" + std::to_string(line);
+  timed(t.c_str(), 0.0, [&] { split_to_planes(plane_kind(), out, nn, terms, nt, partials_, stream_); });
 }
 
 void CCSDContext::gemmP(int M, int N, int K, double alpha, const Planes& A, long lda, const Planes& B,
@@ -554,31 +554,7 @@ void CCSDContext::gemmP(int M, int N, int K, double alpha, const Planes& A, long
   d.M = M; d.N = N; d.K = K; d.alpha = alpha; d.beta = beta;
   d.A = A; d.lda = lda; d.sA = 0; d.B = B; d.ldb = ldb; d.sB = 0;
   d.C = C; d.ldc = ldc; d.sC = 0; d.batch = 1;
-  hipEvent_t ea = nullptr, eb = nullptr;
-  int slot = -1;
-  if (kernel_timing_) {
-    const std::string t = tag;
-    for (size_t k = 0; k < ktimes_.size(); ++k)
-      if (ktimes_[k].tag == t) slot = (int)k;
-    if (slot < 0) {
-      ktimes_.push_back({t, 0, 0.0, 0.0});
-      slot = (int)ktimes_.size() - 1;
-    }
-    for (hipEvent_t* e : {&ea, &eb}) {
-      if (!event_pool_.empty()) {
-        *e = event_pool_.back();
-        event_pool_.pop_back();
-      } else {
-        MPQC_HIP_CHECK(hipEventCreate(e));
-      }
-    }
-    MPQC_HIP_CHECK(hipEventRecord(ea, stream_));
-  }
-  gemm_planes(d, pws_, pws_bytes_, stream_);
-  if (kernel_timing_) {
-    MPQC_HIP_CHECK(hipEventRecord(eb, stream_));
-    pending_.push_back({slot, ea, eb, 2.0 * M * (double)N * K});
-  }
+  timed(tag, 2.0 * M * (double)N * K, [&] { gemm_planes(d, pws_, pws_bytes_, stream_); });
 }
 
 // constant operands of the large contractions, split once per solve
@@ -591,12 +567,18 @@ void CCSDContext::prepare_planes() {
     tau_P_ = alloc_planes(oovv); tauT_P_ = alloc_planes(oovv); TP_P_ = alloc_planes(oovv);
     t2P_P_ = alloc_planes(oovv); Xjk_P_ = alloc_planes(oovv); K_P_ = alloc_planes(oovv);
     Y1t_P_ = alloc_planes(oovv); Y2t_P_ = alloc_planes(oovv); WT_P_ = alloc_planes(oooo);
+    GbarT_P_ = alloc_planes(oovv); Gbar_P_ = alloc_planes(oovv); Gbar_akcd_P_ = alloc_planes(ovvv);
+    tau_ikcd_P_ = alloc_planes(oovv); tau_ackl_P_ = alloc_planes(oovv);
+    P8T_P_ = alloc_planes(O * O * O * V); GiabcT_P_ = alloc_planes(ovvv);
+    t1T_P_ = alloc_planes(O * V); gac_P_ = alloc_planes(V * V);
   }
   {  // split-K workspace for the largest plane contraction shapes
     const int o = o_, v = v_;
     size_t need = 0;
     const int shapes[][3] = {{v * v, o * o, v * v}, {o * v, o * o, v * v}, {o * o, o * o, v * v},
-                             {v * v, o * o, o * o}, {o * v, o * v, o * v}};
+                             {v * v, o * o, o * o}, {o * v, o * v, o * v}, {v, v, o * o * v},
+                             {o, o, o * v * v},     {v, o, o * v * v},     {v, o, v * o * o},
+                             {o * v * v, o, v},     {v, v * o * o, v}};
     for (auto& sh : shapes) need = std::max(need, gemm_planes_workspace_bytes(sh[0], sh[1], sh[2], 1));
     if (need > pws_bytes_) {
       const size_t n = (need + sizeof(double) - 1) / sizeof(double);
@@ -614,6 +596,14 @@ void CCSDContext::prepare_planes() {
   split(GAt_P_, {O, V, O, V}, {{Gijab_, e_ijab, {0, 2, 1, 3}, 1.0}});
   split(GBt_P_, {O, V, O, V}, {{Gijab_, e_ijab, {0, 2, 1, 3}, 2.0}, {Gijab_, e_ijab, {0, 3, 1, 2}, -1.0}});
   split(GCt_P_, {O, V, O, V}, {{Gijab_, e_ijab, {0, 3, 1, 2}, 1.0}});
+  // r1 operands: GbarT(c,d,k,l) = gbar(k,l,c,d); gbar(k,l,a,b) and g_iabc_bar(a,k,c,d)
+  // themselves; P8T(i,c,k,l) = P8(c,k,l,i)
+  split(GbarT_P_, {V, V, O, O}, {{Gbar_, e_ijab, {2, 3, 0, 1}, 1.0}});
+  split(Gbar_P_, e_ijab, {{Gbar_, e_ijab, I, 1.0}});
+  split(Gbar_akcd_P_, {V, O, V, V}, {{Gbar_akcd_, {V, O, V, V}, I, 1.0}});
+  split(P8T_P_, {O, V, O, O}, {{P8_, {V, O, O, O}, {3, 0, 1, 2}, 1.0}});
+  // GiabcT(k,a,c,d) = Giabc(k,a,d,c), the A operand of the k-intermediate's T1 term
+  split(GiabcT_P_, {O, V, V, V}, {{Giabc_, {O, V, V, V}, {0, 1, 3, 2}, 1.0}});
   MPQC_HIP_CHECK(hipStreamSynchronize(stream_));
   planes_ready_ = true;
 }
@@ -699,9 +689,20 @@ void CCSDContext::residual() {
   const std::array<int, 4> I{0, 1, 2, 3};
 
   // tau("a,b,i,j") = t2("a,b,i,j") + t1("a,i") t1("b,j")            ccsd.h:389
-  build_tau(tau_, t2_, t1_, o, v, stream_);
+  timed("r:tau", 0.0, [&] { build_tau(tau_, t2_, t1_, o, v, stream_); });
   // amplitude layouts used by several GEMMs
   perm_add(t1T_, {1, 1, O, V}, 0.0, {{t1_, {1, 1, V, O}, {0, 1, 3, 2}, 1.0}});  // t1T(k,d) = t1(d,k)
+  const bool sm = split_mode();
+  if (sm) {
+    // 16-bit planes of the amplitude operands (the integral planes are split once per solve)
+    if (!planes_ready_) prepare_planes();
+    split(tau_P_, e_abij, {{tau_, e_abij, I, 1.0}});
+    // tau^T(i,j,c,d) = tau(c,d,i,j): the B^T operand of every tau contraction
+    split(tauT_P_, {O, O, V, V}, {{tau_, e_abij, {2, 3, 0, 1}, 1.0}});
+    split(t1T_P_, {1, 1, O, V}, {{t1_, {1, 1, V, O}, {0, 1, 3, 2}, 1.0}});
+    // Y2^T(b,j,k,c) = t2(c,b,j,k)
+    split(Y2t_P_, {V, O, O, V}, {{t2_, e_abij, {1, 2, 3, 0}, 1.0}});
+  }
 
   // ===================== r1: ccsd_r1_r2.h:70-124 =====================
   // :83-84  r1(a,i) = Fia(i,a) - 2 (Fia(k,c) t1(c,i)) t1(a,k)
@@ -710,15 +711,23 @@ void CCSDContext::residual() {
   gemm(false, false, v, o, o, -2.0, t1_, o, Xoo_, o, 1.0, r1_, o, 0, 0, 0, 1, "r1:t1_fia_t1");
   // :89-91  h_ac = F_vv = Fab(a,c) - gbar(k,l,c,d) tau(a,d,k,l); FAB = h_ac; r1 += h_ac t1
   //         (ccsd_intermediates.h:71-77)
-  perm_add(tau_akld_, {V, O, O, V}, 0.0, {{tau_, e_abij, {0, 2, 3, 1}, 1.0}});
   MPQC_HIP_CHECK(hipMemcpyAsync(FAB_, Fab_, vv * sizeof(double), hipMemcpyDeviceToDevice, stream_));
-  gemm(false, true, v, v, o * o * v, -1.0, tau_akld_, oo * V, Gbar_cKLD_, oo * V, 1.0, FAB_, v, 0, 0, 0, 1, "r1:F_vv");
+  if (sm) {  // tau(a,(d,k,l)) . gbar^T(c,(d,k,l))
+    gemmP(v, v, o * o * v, -1.0, tau_P_, oo * V, GbarT_P_, oo * V, 1.0, FAB_, v, "r1:F_vv");
+  } else {
+    perm_add(tau_akld_, {V, O, O, V}, 0.0, {{tau_, e_abij, {0, 2, 3, 1}, 1.0}});
+    gemm(false, true, v, v, o * o * v, -1.0, tau_akld_, oo * V, Gbar_cKLD_, oo * V, 1.0, FAB_, v, 0, 0, 0, 1, "r1:F_vv");
+  }
   gemm(false, false, v, o, v, 1.0, FAB_, v, t1_, o, 1.0, r1_, o, 0, 0, 0, 1, "r1:FAB_t1");
   // :95-97  h_ki = F_oo = Fij(k,i) + gbar(k,l,a,b) tau(a,b,i,l); FIJ = h_ki; r1 -= t1 h_ki
   //         (ccsd_intermediates.h:57-63)
-  perm_add(tau_ilab_, {O, O, V, V}, 0.0, {{tau_, e_abij, {2, 3, 0, 1}, 1.0}});
   MPQC_HIP_CHECK(hipMemcpyAsync(FIJ_, Fij_, oo * sizeof(double), hipMemcpyDeviceToDevice, stream_));
-  gemm(false, true, o, o, o * v * v, 1.0, Gbar_, O * vv, tau_ilab_, O * vv, 1.0, FIJ_, o, 0, 0, 0, 1, "r1:F_oo");
+  if (sm) {  // gbar(k,(l,a,b)) . tau^T(i,(l,a,b))
+    gemmP(o, o, o * v * v, 1.0, Gbar_P_, O * vv, tauT_P_, O * vv, 1.0, FIJ_, o, "r1:F_oo");
+  } else {
+    perm_add(tau_ilab_, {O, O, V, V}, 0.0, {{tau_, e_abij, {2, 3, 0, 1}, 1.0}});
+    gemm(false, true, o, o, o * v * v, 1.0, Gbar_, O * vv, tau_ilab_, O * vv, 1.0, FIJ_, o, 0, 0, 0, 1, "r1:F_oo");
+  }
   gemm(false, false, v, o, o, -1.0, t1_, o, FIJ_, o, 1.0, r1_, o, 0, 0, 0, 1, "r1:t1_FIJ");
   // :101-104 h_kc = F_ov = Fia(k,c) + gbar(k,l,c,d) t1(d,l)   (ccsd_intermediates.h:89-95)
   //          r1 += h_kc (2 t2(c,a,k,i) - t2(c,a,i,k) + t1(c,i) t1(a,k))
@@ -733,11 +742,21 @@ void CCSDContext::residual() {
   // :107-108 r1 += (2 Gijab(k,i,c,a) - Giajb(k,a,i,c)) t1(c,k)
   gemm(false, false, v * o, 1, v * o, 1.0, P6_, ov, t1_, 1, 1.0, r1_, 1, 0, 0, 0, 1, "r1:g_t1");
   // :110-114 r1 += g_iabc_bar(k,a,c,d) tau(c,d,k,i)
-  perm_add(tau_kcdi_, {O, V, V, O}, 0.0, {{tau_, e_abij, {2, 0, 1, 3}, 1.0}});
-  gemm(false, false, v, o, o * v * v, 1.0, Gbar_akcd_, O * vv, tau_kcdi_, o, 1.0, r1_, o, 0, 0, 0, 1, "r1:giabc_tau");
+  if (sm) {  // g_iabc_bar(a,(k,c,d)) . tau_ikcd(i,(k,c,d)),  tau_ikcd(i,k,c,d) = tau(c,d,k,i)
+    split(tau_ikcd_P_, {O, O, V, V}, {{tau_, e_abij, {3, 2, 0, 1}, 1.0}});
+    gemmP(v, o, o * v * v, 1.0, Gbar_akcd_P_, O * vv, tau_ikcd_P_, O * vv, 1.0, r1_, o, "r1:giabc_tau");
+  } else {
+    perm_add(tau_kcdi_, {O, V, V, O}, 0.0, {{tau_, e_abij, {2, 0, 1, 3}, 1.0}});
+    gemm(false, false, v, o, o * v * v, 1.0, Gbar_akcd_, O * vv, tau_kcdi_, o, 1.0, r1_, o, 0, 0, 0, 1, "r1:giabc_tau");
+  }
   // :120-121 r1 -= (2 Gijka(l,k,i,c) - Gijka(k,l,i,c)) tau(c,a,k,l)
-  perm_add(tau_ackl_, {V, V, O, O}, 0.0, {{tau_, e_abij, {1, 0, 2, 3}, 1.0}});
-  gemm(false, false, v, o, o * o * v, -1.0, tau_ackl_, oo * V, P8_, o, 1.0, r1_, o, 0, 0, 0, 1, "r1:gijka_tau");
+  if (sm) {  // tau_ackl(a,(c,k,l)) . P8T(i,(c,k,l)),  tau_ackl(a,c,k,l) = tau(c,a,k,l)
+    split(tau_ackl_P_, e_abij, {{tau_, e_abij, {1, 0, 2, 3}, 1.0}});
+    gemmP(v, o, o * o * v, -1.0, tau_ackl_P_, oo * V, P8T_P_, oo * V, 1.0, r1_, o, "r1:gijka_tau");
+  } else {
+    perm_add(tau_ackl_, {V, V, O, O}, 0.0, {{tau_, e_abij, {1, 0, 2, 3}, 1.0}});
+    gemm(false, false, v, o, o * o * v, -1.0, tau_ackl_, oo * V, P8_, o, 1.0, r1_, o, 0, 0, 0, 1, "r1:gijka_tau");
+  }
 
   // ===================== r2: ccsd_r1_r2.h:198-293 =====================
   // :207-208 Rd = (Giabc(i,c,a,b) - Giajb(k,b,i,c) t1(a,k)) t1(c,j)
@@ -762,7 +781,12 @@ void CCSDContext::residual() {
   gemm(false, false, v, v, o, fov_sign_, t1_, o, Fia_, v, 1.0, gac_, v, 0, 0, 0, 1, "r2:gac_t1_fia");
   gemm(false, false, v * v, 1, o * v, 1.0, P14_, ov, t1T_, 1, 1.0, gac_, 1, 0, 0, 0, 1, "r2:gac_g_t1");
   // :226 Rd += g_ac(a,c) t2(c,b,i,j) - g_ki(k,i) t2(a,b,k,j)
-  gemm(false, false, v, v * o * o, v, 1.0, gac_, v, t2_, V * oo, 1.0, Rd_, V * oo, 0, 0, 0, 1, "r2:g_ac_t2");
+  if (sm) {  // g_ac(a,c) . Y2^T((b,i,j),c)
+    split(gac_P_, {1, 1, V, V}, {{gac_, {1, 1, V, V}, I, 1.0}});
+    gemmP(v, v * o * o, v, 1.0, gac_P_, v, Y2t_P_, v, 1.0, Rd_, V * oo, "r2:g_ac_t2");
+  } else {
+    gemm(false, false, v, v * o * o, v, 1.0, gac_, v, t2_, V * oo, 1.0, Rd_, V * oo, 0, 0, 0, 1, "r2:g_ac_t2");
+  }
   perm_add(t2_abjk_, {V, V, O, O}, 0.0, {{t2_, e_abij, {0, 1, 3, 2}, 1.0}});
   gemm(false, false, v * v * o, o, o, 1.0, t2_abjk_, o, gki_, o, 0.0, X16_, o, 0, 0, 0, 1, "r2:t2_gki");
   perm_add(Rd_, {V, V, O, O}, 1.0, {{X16_, e_abij, {0, 1, 3, 2}, -1.0}});
@@ -778,13 +802,16 @@ void CCSDContext::residual() {
   MPQC_HIP_CHECK(hipMemcpyAsync(K_, K0_, ov * ov * sizeof(double), hipMemcpyDeviceToDevice, stream_));
   gemm(false, false, v, o * o * v, o, -1.0, t1_, o, X18_, oo * V, 1.0, J_, oo * V, 0, 0, 0, 1, "r2:j_gijka_t1");
   gemm(false, false, v, o * o * v, o, -1.0, t1_, o, X19_, oo * V, 1.0, K_, oo * V, 0, 0, 0, 1, "r2:k_gijka_t1");
-  gemm(false, false, o * v * v, o, v, 1.0, Giabc_, v, t1_, o, 0.0, Zj_, o, 0, 0, 0, 1, "r2:j_giabc_t1");  // Zj(k,a,c,i)
-  gemm(true, false, v, o, v, 1.0, Giabc_, v, t1_, o, 0.0, Zk_, o, vv, 0, V * O, o * v, "r2:k_giabc_t1");  // Zk(k,a,c,i)
+  if (sm) {
+    gemmP(o * v * v, o, v, 1.0, Giabc_P_, v, t1T_P_, v, 0.0, Zj_, o, "r2:j_giabc_t1");
+    gemmP(o * v * v, o, v, 1.0, GiabcT_P_, v, t1T_P_, v, 0.0, Zk_, o, "r2:k_giabc_t1");
+  } else {
+    gemm(false, false, o * v * v, o, v, 1.0, Giabc_, v, t1_, o, 0.0, Zj_, o, 0, 0, 0, 1, "r2:j_giabc_t1");  // Zj(k,a,c,i)
+    gemm(true, false, v, o, v, 1.0, Giabc_, v, t1_, o, 0.0, Zk_, o, vv, 0, V * O, o * v, "r2:k_giabc_t1");  // Zk(k,a,c,i)
+  }
   perm_add(J_, {V, O, O, V}, 1.0, {{Zj_, {O, V, V, O}, {1, 3, 0, 2}, 1.0}});
   perm_add(K_, {V, O, O, V}, 1.0, {{Zk_, {O, V, V, O}, {1, 3, 0, 2}, 1.0}});
-  const bool sm = split_mode();
   if (sm) {
-    if (!planes_ready_) prepare_planes();
     const std::array<long, 4> e_aild{V, O, O, V};
     split(TP_P_, e_aild, {{TP_, e_aild, I, 1.0}});
     split(t2P_P_, e_aild, {{t2P_, e_aild, I, 1.0}});
@@ -805,7 +832,6 @@ void CCSDContext::residual() {
     split(Y1t_P_, {V, O, O, V}, {{t2_, e_abij, {1, 3, 2, 0}, 2.0}, {t2_, e_abij, {0, 3, 2, 1}, -1.0}});
     gemmP(o * v, o * v, o * v, 0.5, Xjk_P_, ov, Y1t_P_, ov, 0.0, RR_, ov, "r2:ring");
     split(K_P_, e_aikc, {{K_, e_aikc, I, 1.0}});
-    split(Y2t_P_, {V, O, O, V}, {{t2_, e_abij, {1, 2, 3, 0}, 1.0}});
     gemmP(o * v, o * v, o * v, 1.0, K_P_, ov, Y2t_P_, ov, 0.0, tmpR_, ov, "r2:crossed_ring");
   } else {
     perm_add(Xjk_, {V, O, O, V}, 0.0, {{J_, {V, O, O, V}, I, 2.0}, {K_, {V, O, O, V}, I, -1.0}});
@@ -836,9 +862,6 @@ void CCSDContext::residual() {
            {{Gijkl_, {O, O, O, O}, I, 1.0}, {Y23_, {O, O, O, O}, I, 1.0},
             {Y23_, {O, O, O, O}, {1, 0, 3, 2}, 1.0}});
   if (sm) {
-    // tau^T(i,j,c,d) = tau_ilab, the B^T operand of every tau-contraction; tau itself for hh
-    split(tauT_P_, {O, O, V, V}, {{tau_ilab_, {O, O, V, V}, I, 1.0}});
-    split(tau_P_, e_abij, {{tau_, e_abij, I, 1.0}});
     gemmP(o * o, o * o, v * v, 1.0, Gijab_P_, vv, tauT_P_, vv, 1.0, W_, oo, "r2:W_oooo");
     // W^T(i,j,k,l) = W(k,l,i,j)
     split(WT_P_, {O, O, O, O}, {{W_, {O, O, O, O}, {2, 3, 0, 1}, 1.0}});
@@ -866,10 +889,14 @@ void CCSDContext::energy_error(double* E, double* err) {
   if (!prepared_) prepare();
   const long n1 = (long)o_ * v_, n2 = n1 * n1;
   // error = sqrt(|r1|^2 + |r2|^2) / (size(r1) + size(r2))   qc/cc/solvers.h:60-63
-  dot(r1_, r1_, n1, scalars_ + 0, partials_, stream_);
-  dot(r2_, r2_, n2, scalars_ + 1, partials_, stream_);
+  timed("en:norms", 0.0, [&] {
+    dot(r1_, r1_, n1, scalars_ + 0, partials_, stream_);
+    dot(r2_, r2_, n2, scalars_ + 1, partials_, stream_);
+  });
   // E = 2 <f_ai + r1, t1> + <g_abij + r2, 2 tau(a,b,i,j) - tau(b,a,i,j)>   ccsd.h:377-379
-  ccsd_energy(f_ai_, r1_, t1_, g_abij_, r2_, tau_, o_, v_, scalars_ + 2, partials_, stream_);
+  timed("en:energy", 0.0, [&] {
+    ccsd_energy(f_ai_, r1_, t1_, g_abij_, r2_, tau_, o_, v_, scalars_ + 2, partials_, stream_);
+  });
   double h[3];
   MPQC_HIP_CHECK(hipMemcpyAsync(h, scalars_, 3 * sizeof(double), hipMemcpyDeviceToHost, stream_));
   MPQC_HIP_CHECK(hipStreamSynchronize(stream_));
@@ -910,14 +937,18 @@ void CCSDContext::update() {
     for (int s : err_slots_) used[s] = true;
     while (used[eslot]) ++eslot;
   }
-  MPQC_HIP_CHECK(hipMemcpyAsync(hist_e1_[eslot], r1_, n1 * sizeof(double), hipMemcpyDeviceToDevice, stream_));
-  MPQC_HIP_CHECK(hipMemcpyAsync(hist_e2_[eslot], r2_, n2 * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+  timed("up:store_r", 0.0, [&] {
+    MPQC_HIP_CHECK(hipMemcpyAsync(hist_e1_[eslot], r1_, n1 * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    MPQC_HIP_CHECK(hipMemcpyAsync(hist_e2_[eslot], r2_, n2 * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+  });
   err_slots_.push_back(eslot);
   const int nvec = (int)err_slots_.size();
   {
     std::vector<const double*> x1, x2;
     for (int s : err_slots_) { x1.push_back(hist_e1_[s]); x2.push_back(hist_e2_[s]); }
-    multi_dot2(x1.data(), x2.data(), nvec, r1_, r2_, n1, n2, scalars_ + 16, partials_, stream_);
+    timed("up:diis_dots", 0.0, [&] {
+      multi_dot2(x1.data(), x2.data(), nvec, r1_, r2_, n1, n2, scalars_ + 16, partials_, stream_);
+    });
     double h[16];
     MPQC_HIP_CHECK(hipMemcpyAsync(h, scalars_ + 16, nvec * sizeof(double), hipMemcpyDeviceToHost, stream_));
     MPQC_HIP_CHECK(hipStreamSynchronize(stream_));
@@ -925,7 +956,7 @@ void CCSDContext::update() {
     for (int i = 0; i < nvec; ++i) B_[(size_t)i * m + (nvec - 1)] = B_[(size_t)(nvec - 1) * m + i] = h[i];
   }
   // --- Jacobi: t += r / D
-  jacobi_update(t1_, t2_, r1_, r2_, eo_, ev_, o_, v_, stream_);
+  timed("up:jacobi", 0.0, [&] { jacobi_update(t1_, t2_, r1_, r2_, eo_, ev_, o_, v_, stream_); });
   // --- push x = t into its history
   if ((int)x_slots_.size() == n_diis_) x_slots_.pop_front();
   int xslot = 0;
@@ -934,8 +965,10 @@ void CCSDContext::update() {
     for (int s : x_slots_) used[s] = true;
     while (used[xslot]) ++xslot;
   }
-  MPQC_HIP_CHECK(hipMemcpyAsync(hist_x1_[xslot], t1_, n1 * sizeof(double), hipMemcpyDeviceToDevice, stream_));
-  MPQC_HIP_CHECK(hipMemcpyAsync(hist_x2_[xslot], t2_, n2 * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+  timed("up:store_t", 0.0, [&] {
+    MPQC_HIP_CHECK(hipMemcpyAsync(hist_x1_[xslot], t1_, n1 * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    MPQC_HIP_CHECK(hipMemcpyAsync(hist_x2_[xslot], t2_, n2 * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+  });
   x_slots_.push_back(xslot);
   const bool do_extrap =
       diis_iter_ > diis_start_ && ((diis_iter_ - diis_start_) % diis_ngroup_) < diis_group_nstart_;
@@ -975,11 +1008,13 @@ void CCSDContext::update() {
       }
       ++k;
     }
-    lincomb(t1_, x1.data(), c.data(), (int)c.size(), n1, stream_);
-    lincomb(t2_, x2.data(), c.data(), (int)c.size(), n2, stream_);
+    timed("up:extrapolate", 0.0, [&] {
+      lincomb(t1_, x1.data(), c.data(), (int)c.size(), n1, stream_);
+      lincomb(t2_, x2.data(), c.data(), (int)c.size(), n2, stream_);
+    });
   }
   // tau("a,b,i,j") = t2 + t1 t1                                     ccsd.h:389
-  build_tau(tau_, t2_, t1_, o_, v_, stream_);
+  timed("up:tau", 0.0, [&] { build_tau(tau_, t2_, t1_, o_, v_, stream_); });
   MPQC_HIP_CHECK(hipStreamSynchronize(stream_));
 }
 

@@ -87,7 +87,9 @@ class CCSDContext {
     std::array<int, 4> perm;
     double alpha;
   };
-  void perm_add(double* out, std::array<long, 4> n, double beta, std::initializer_list<Src> srcs);
+  // untagged calls are timed under "perm@<source line of the call>"
+  void perm_add(double* out, std::array<long, 4> n, double beta, std::initializer_list<Src> srcs,
+                const char* tag = nullptr, int line = __builtin_LINE());
   // validated terms of a permuted linear combination
   int make_terms(std::array<long, 4> n, std::initializer_list<Src> srcs, TTerm* terms);
 
@@ -95,7 +97,8 @@ class CCSDContext {
   bool split_mode() const;
   PlaneKind plane_kind() const;
   Planes alloc_planes(long n);
-  void split(const Planes& out, std::array<long, 4> n, std::initializer_list<Src> srcs);
+  void split(const Planes& out, std::array<long, 4> n, std::initializer_list<Src> srcs,
+             const char* tag = nullptr, int line = __builtin_LINE());
   void gemmP(int M, int N, int K, double alpha, const Planes& A, long lda, const Planes& B, long ldb,
              double beta, double* C, long ldc, const char* tag);
   void prepare_planes();
@@ -104,7 +107,9 @@ class CCSDContext {
   size_t pws_bytes_ = 0;
   std::vector<std::pair<uint16_t*, size_t>> plane_allocs_;
   Planes Gabcd_P_, Giabc_P_, Gijab_P_, GAt_P_, GBt_P_, GCt_P_;
-  Planes tau_P_, tauT_P_, TP_P_, t2P_P_, Xjk_P_, K_P_, Y1t_P_, Y2t_P_, WT_P_;
+  Planes GbarT_P_, Gbar_P_, Gbar_akcd_P_, P8T_P_, GiabcT_P_;
+  Planes tau_P_, tauT_P_, TP_P_, t2P_P_, Xjk_P_, K_P_, Y1t_P_, Y2t_P_, WT_P_, t1T_P_, gac_P_;
+  Planes tau_ikcd_P_, tau_ackl_P_;
   void check_finite(const double* x, long n, const char* what);
   double scalar_from_device(const double* d);
 
@@ -161,6 +166,22 @@ class CCSDContext {
   std::vector<KernelTime> ktimes_;
   std::vector<hipEvent_t> event_pool_;
   void resolve_events();
+  int timing_slot(const char* tag);
+  hipEvent_t take_event();
+  // runs f() between two HIP events on the context stream when kernel timing is on
+  template <class F>
+  void timed(const char* tag, double flops, F&& f) {
+    if (!kernel_timing_) {
+      f();
+      return;
+    }
+    const int slot = timing_slot(tag);
+    hipEvent_t a = take_event(), b = take_event();
+    MPQC_HIP_CHECK(hipEventRecord(a, stream_));
+    f();
+    MPQC_HIP_CHECK(hipEventRecord(b, stream_));
+    pending_.push_back({slot, a, b, flops});
+  }
 };
 
 }  // namespace mpqc_hip

@@ -352,34 +352,58 @@ __global__ void splitk_reduce_kernel(const double* __restrict__ W, double* __res
   }
 }
 
+// Same sum for small outputs (M*N*batch below kWideReduceMax): a wave owns 16
+// consecutive outputs and its four 16-lane groups add every fourth partial
+// (coalesced 128-byte rows), then the groups are combined in a fixed order.
+constexpr long kWideReduceMax = 1L << 16;
+__global__ void splitk_reduce_wide_kernel(const double* __restrict__ W, double* __restrict__ C,
+                                          long ldc, long sC, int M, int N, int nsplit, int batch0,
+                                          int nb, double beta) {
+  const long MN = (long)M * N;
+  const int lane = threadIdx.x & 63;
+  const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const long chunks = (MN + 15) / 16;
+  if (wave >= chunks * nb) return;
+  const int bz = (int)(wave / chunks);
+  const long e = (wave - (long)bz * chunks) * 16 + (lane & 15);
+  const int g = lane >> 4;
+  double acc = 0.0;
+  if (e < MN) {
+    const double* w = W + (long)bz * nsplit * MN + e;
+    for (int s = g; s < nsplit; s += 4) acc += w[(long)s * MN];
+  }
+  // groups (0,1) and (2,3), then the two pairs
+  acc += __shfl_xor(acc, 16);
+  acc += __shfl_xor(acc, 32);
+  if (g == 0 && e < MN) {
+    const int m = (int)(e / N), n = (int)(e - (long)m * N);
+    double* c = C + (long)(batch0 + bz) * sC + (long)m * ldc + n;
+    *c = (beta == 0.0) ? acc : acc + beta * (*c);
+  }
+}
+
 template <int BM, int BN>
 void launch(const GemmDesc& d, GemmPrecision prec, const GemmWorkspace& ws, hipStream_t stream) {
   const int tiles_m = (int)ceil_div(d.M, BM), tiles_n = (int)ceil_div(d.N, BN);
   const long tiles = (long)tiles_m * tiles_n;
-  // split-K: aim for >= ~2 workgroups per CU (256 CUs) when the output is small
+  // split-K: aim for ~4 workgroups per CU (256 CUs) when the output is small,
+  // with at least 4 K-slices of work per split
   int nsplit = 1;
-  const long want = 512;
+  const long want = 1024;
   if (tiles * d.batch < want && d.K >= 8 * kBK) {
     nsplit = (int)std::min<long>(ceil_div(want, tiles * d.batch), d.K / (4 * kBK));
-    nsplit = std::max(1, std::min(nsplit, 64));
-  }
-  int kchunk = (int)(ceil_div(ceil_div(d.K, nsplit), kBK) * kBK);
-  if (nsplit > 1) {
-    nsplit = (int)ceil_div(d.K, kchunk);
-    // workspace capacity limits how many batches go in one launch
+    nsplit = std::max(1, std::min(nsplit, 512));
   }
   const size_t slice = (size_t)d.M * d.N * sizeof(double);
+  // the workspace must hold nsplit partial outputs of at least one batch entry
+  if (nsplit > 1 && ws.bytes / slice < (size_t)nsplit)
+    nsplit = (int)std::max<size_t>(1, ws.bytes / slice);
+  int kchunk = (int)(ceil_div(ceil_div(d.K, nsplit), kBK) * kBK);
+  if (nsplit > 1) nsplit = (int)ceil_div(d.K, kchunk);
+  if (nsplit == 1) kchunk = (int)(ceil_div(d.K, kBK) * kBK);
   int max_batch_per_launch = 65535 / nsplit;
-  if (nsplit > 1) {
-    const size_t cap = ws.bytes / (slice * nsplit);
-    if (cap == 0) {  // not enough workspace: fall back to no split
-      nsplit = 1;
-      kchunk = (int)(ceil_div(d.K, kBK) * kBK);
-      max_batch_per_launch = 65535;
-    } else {
-      max_batch_per_launch = (int)std::min<size_t>(max_batch_per_launch, cap);
-    }
-  }
+  if (nsplit > 1)  // workspace capacity limits how many batches go in one launch
+    max_batch_per_launch = (int)std::min<size_t>(max_batch_per_launch, ws.bytes / (slice * nsplit));
   if (d.K == 0) kchunk = kBK;
   for (int b0 = 0; b0 < d.batch; b0 += max_batch_per_launch) {
     const int nb = std::min(max_batch_per_launch, d.batch - b0);
@@ -403,17 +427,28 @@ void launch(const GemmDesc& d, GemmPrecision prec, const GemmWorkspace& ws, hipS
       else hipLaunchKernelGGL((dgemm_f64_kernel<BM, BN, true, true>), grid, dim3(kThreads), 0, stream, p);
       check_launch("dgemm_f64");
     }
-    if (nsplit > 1) {
-      const long MN = (long)d.M * d.N;
-      dim3 g2((unsigned)std::min<long>(ceil_div(MN, 256), 4096), (unsigned)nb);
-      hipLaunchKernelGGL(splitk_reduce_kernel, g2, dim3(256), 0, stream, ws.ptr, d.C, d.ldc, d.sC,
-                         d.M, d.N, nsplit, b0, d.beta);
-      check_launch("splitk_reduce");
-    }
+    if (nsplit > 1) splitk_reduce(ws.ptr, d.C, d.ldc, d.sC, d.M, d.N, nsplit, b0, nb, d.beta, stream);
   }
 }
 
 }  // namespace
+
+void splitk_reduce(const double* W, double* C, long ldc, long sC, int M, int N, int nsplit, int batch0,
+                   int nb, double beta, hipStream_t stream) {
+  const long MN = (long)M * N;
+  if (nb > 65535) throw std::invalid_argument("splitk_reduce: too many batches");
+  if (MN * nb <= kWideReduceMax) {
+    const long waves = ceil_div(MN, 16) * nb;
+    hipLaunchKernelGGL(splitk_reduce_wide_kernel, dim3((unsigned)ceil_div(waves, 4)), dim3(256), 0,
+                       stream, W, C, ldc, sC, M, N, nsplit, batch0, nb, beta);
+    check_launch("splitk_reduce_wide");
+  } else {
+    dim3 g2((unsigned)std::min<long>(ceil_div(MN, 256), 4096), (unsigned)nb);
+    hipLaunchKernelGGL(splitk_reduce_kernel, g2, dim3(256), 0, stream, W, C, ldc, sC, M, N, nsplit,
+                       batch0, beta);
+    check_launch("splitk_reduce");
+  }
+}
 
 void gemm(const GemmDesc& d, GemmPrecision prec, const GemmWorkspace& ws, hipStream_t stream) {
   if (d.M <= 0 || d.N <= 0 || d.batch <= 0) return;
@@ -424,10 +459,17 @@ void gemm(const GemmDesc& d, GemmPrecision prec, const GemmWorkspace& ws, hipStr
     gemm_split(d, prec, ws, stream);
     return;
   }
-  if (d.M <= 64 || d.N <= 64)
-    launch<64, 64>(d, prec, ws, stream);
-  else
+  // 128-wide tiles unless 64-wide ones pad the output much less
+  bool wide = d.M > 64 && d.N > 64;
+  if (wide) {
+    const double a128 = (double)ceil_div(d.M, 128) * ceil_div(d.N, 128) * 128 * 128;
+    const double a64 = (double)ceil_div(d.M, 64) * ceil_div(d.N, 64) * 64 * 64;
+    wide = a128 <= 1.25 * a64;
+  }
+  if (wide)
     launch<128, 128>(d, prec, ws, stream);
+  else
+    launch<64, 64>(d, prec, ws, stream);
 }
 
 }  // namespace mpqc_hip

@@ -47,6 +47,11 @@ struct GemmWorkspace {
 // split-K partials are summed in a fixed order by a separate pass.
 void gemm(const GemmDesc& d, GemmPrecision prec, const GemmWorkspace& ws, hipStream_t stream);
 
+// C[b] = beta*C[b] + sum_s W[(b - batch0)*nsplit + s]  for batches batch0..batch0+nb-1, with the
+// M x N partials W dense; fixed summation order (bitwise reproducible).
+void splitk_reduce(const double* W, double* C, long ldc, long sC, int M, int N, int nsplit, int batch0,
+                   int nb, double beta, hipStream_t stream);
+
 // Split-precision paths (BF16X3, F16X3, BF16X6); called by gemm().
 void gemm_split(const GemmDesc& d, GemmPrecision prec, const GemmWorkspace& ws, hipStream_t stream);
 

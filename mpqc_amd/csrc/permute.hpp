@@ -1,0 +1,101 @@
+// Permuted linear combinations  out[i0,i1,i2,i3] = sum_t alpha_t in_t[...]  as
+// HBM-streaming kernels, shared by the FP64 layout changes (tadd) and the
+// 16-bit plane producer (split_to_planes).
+//
+// Two launch shapes:
+//  * linear: one thread per output element along the flattened (i2,i3) plane,
+//    grid.y = i1, grid.z = i0.  Used when every input is contiguous along the
+//    output's inner index i3 (or none is contiguous along any other output
+//    index), so both reads and writes are wave-contiguous.
+//  * tiled: a workgroup owns a 32 x 32 tile over output dims (q, 3), where q
+//    is the output index along which some input is contiguous (a transposed
+//    input).  Such inputs are read 32 contiguous elements along q, staged in
+//    LDS, and consumed along i3; other inputs are read directly.  The two
+//    remaining output dims index grid.y / grid.z.
+#pragma once
+
+#include "tensor_ops.hpp"
+
+namespace mpqc_hip {
+
+constexpr int kPT = 32;          // tile edge of the tiled shape
+constexpr int kPThreads = 256;   // threads per workgroup (32 x 8)
+
+// One input with strides resolved against the launch geometry.
+struct PTerm {
+  const double* p;
+  long s3, sq, sr0, sr1;  // strides along output dims 3, q, r0, r1
+  double alpha;
+};
+
+struct PermGeom {
+  long n3, nq, nr0, nr1;  // extents of output dims 3, q, r0, r1
+  long o3, oq, or0, or1;  // output strides of those dims (o3 = 1)
+  int tiles3, tilesq;     // tiled shape: tiles along dims 3 and q
+  bool tiled;
+  unsigned lds_mask;      // bit t: input t goes through the LDS tile
+  int nterms;
+  PTerm t[kMaxTerms];
+};
+
+// Host: geometry (and launch grid) for output extents n and the given inputs.
+PermGeom make_perm_geom(const long n[4], const TTerm* terms, int nterms);
+dim3 perm_grid(const PermGeom& g);
+
+// Device: evaluates every output element this workgroup owns and hands
+// (linear output offset, sum) to store().  Called by every thread of the
+// workgroup (the tiled shape synchronises through LDS).
+template <class Store>
+__device__ inline void permute_block(const PermGeom& g, Store&& store) {
+  if (!g.tiled) {
+    // linear shape: q = 2, r1 = 1, r0 = 0; the host keeps the (q,3) plane below 2^31
+    const int plane = (int)(g.nq * g.n3);
+    const int e = (int)blockIdx.x * kPThreads + (int)threadIdx.x;
+    if (e >= plane) return;
+    const long ir1 = blockIdx.y, ir0 = blockIdx.z;
+    const int n3 = (int)g.n3;
+    const int iq = e / n3, i3 = e - iq * n3;
+    double acc = 0.0;
+    for (int t = 0; t < g.nterms; ++t) {
+      const PTerm& T = g.t[t];
+      acc += T.alpha * T.p[ir0 * T.sr0 + ir1 * T.sr1 + (long)iq * T.sq + (long)i3 * T.s3];
+    }
+    store(ir0 * g.or0 + ir1 * g.or1 + e, acc);
+    return;
+  }
+  __shared__ double tile[kPT][kPT + 1];
+  const int tx = threadIdx.x & (kPT - 1), ty = threadIdx.x / kPT;  // ty < 8
+  const int t3 = blockIdx.x % g.tiles3, tq = blockIdx.x / g.tiles3;
+  const long ir1 = blockIdx.y, ir0 = blockIdx.z;
+  const long b3 = (long)t3 * kPT, bq = (long)tq * kPT;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int t = 0; t < g.nterms; ++t) {
+    const PTerm& T = g.t[t];
+    const double* base = T.p + ir0 * T.sr0 + ir1 * T.sr1;
+    if ((g.lds_mask >> t) & 1u) {
+      // lanes run along q, where this input is contiguous
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const long iq = bq + tx, j3 = b3 + ty + 8 * k;
+        tile[ty + 8 * k][tx] = (iq < g.nq && j3 < g.n3) ? base[iq * T.sq + j3 * T.s3] : 0.0;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < 4; ++k) acc[k] += T.alpha * tile[tx][ty + 8 * k];
+      __syncthreads();
+    } else {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const long iq = bq + ty + 8 * k, i3 = b3 + tx;
+        if (iq < g.nq && i3 < g.n3) acc[k] += T.alpha * base[iq * T.sq + i3 * T.s3];
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const long iq = bq + ty + 8 * k, i3 = b3 + tx;
+    if (iq < g.nq && i3 < g.n3) store(ir0 * g.or0 + ir1 * g.or1 + iq * g.oq + i3, acc[k]);
+  }
+}
+
+}  // namespace mpqc_hip

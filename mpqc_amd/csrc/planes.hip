@@ -11,6 +11,8 @@
 // workgroups fit per CU).  Workgroups are mapped to tiles XCD by XCD
 // (consecutive tiles on the same XCD share operand rows in that XCD's L2).
 #include "planes.hpp"
+#include "gemm.hpp"
+#include "permute.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -30,35 +32,24 @@ typedef _Float16 f16x8_t __attribute__((ext_vector_type(8)));
 typedef unsigned short u16x8_t __attribute__((ext_vector_type(8)));
 
 // ---------------------------------------------------------------- producer
-struct SplitArgs {
-  uint16_t* p[3];
-  double* scale;
-  long n[4];
-  int nterms;
-  TTerm t[kMaxTerms];
-  int kind;
+__device__ inline uint16_t f16_bits(float f) { return __builtin_bit_cast(uint16_t, (_Float16)f); }
+__device__ inline double f16_val(uint16_t u) { return (double)(float)__builtin_bit_cast(_Float16, u); }
+__device__ inline uint16_t bf16_bits(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
+__device__ inline double bf16_val(uint16_t u) { return (double)(float)__builtin_bit_cast(__bf16, u); }
+
+// max |x| over each distinct input tensor (contiguous reads): partials[src * gridDim.x + block]
+struct AbsMaxArgs {
+  const double* p[kMaxTerms];
+  long n[kMaxTerms];
 };
 
-__device__ inline double eval_term_sum(const SplitArgs& a, long e) {
-  const long i3 = e % a.n[3];
-  long r = e / a.n[3];
-  const long i2 = r % a.n[2];
-  r /= a.n[2];
-  const long i1 = r % a.n[1];
-  const long i0 = r / a.n[1];
-  double v = 0.0;
-  for (int t = 0; t < a.nterms; ++t) {
-    const TTerm& tt = a.t[t];
-    v += tt.alpha * tt.p[i0 * tt.s[0] + i1 * tt.s[1] + i2 * tt.s[2] + i3 * tt.s[3]];
-  }
-  return v;
-}
-
-__global__ __launch_bounds__(256) void split_max_stage1(SplitArgs a, double* partials) {
-  const long total = a.n[0] * a.n[1] * a.n[2] * a.n[3];
+__global__ __launch_bounds__(256) void absmax_stage1(AbsMaxArgs a, double* partials) {
+  const int src = blockIdx.y;
+  const double* __restrict__ x = a.p[src];
+  const long n = a.n[src];
   double m = 0.0;
-  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long)gridDim.x * 256)
-    m = fmax(m, fabs(eval_term_sum(a, e)));
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < n; e += (long)gridDim.x * 256)
+    m = fmax(m, fabs(x[e]));
   __shared__ double sh[256];
   sh[threadIdx.x] = m;
   __syncthreads();
@@ -66,28 +57,41 @@ __global__ __launch_bounds__(256) void split_max_stage1(SplitArgs a, double* par
     if (threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) partials[blockIdx.x] = sh[0];
+  if (threadIdx.x == 0) partials[(long)src * gridDim.x + blockIdx.x] = sh[0];
 }
 
-// scale = 2^(14 - e) with max = f 2^e (f16); 1 for bf16
-__global__ __launch_bounds__(256) void split_max_stage2(const double* partials, int nb, int kind,
-                                                        double* scale) {
+// Plane scale from the bound  max|sum_t alpha_t x_t| <= sum_t |alpha_t| max|x_t|  (exact for
+// one input): s = 2^(14 - e) with bound = f 2^e, so every scaled value stays below 2^14 and
+// far from the fp16 overflow.  bf16 planes need no scale (s = 1).
+struct ScaleArgs {
+  int nsrc, nterms, nb, kind;
+  int src_of[kMaxTerms];
+  double absalpha[kMaxTerms];
+};
+
+__global__ __launch_bounds__(256) void scale_stage2(const double* partials, ScaleArgs a, double* scale) {
   __shared__ double sh[256];
-  double m = 0.0;
-  for (int i = threadIdx.x; i < nb; i += 256) m = fmax(m, partials[i]);
-  sh[threadIdx.x] = m;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
+  __shared__ double srcmax[kMaxTerms];
+  for (int src = 0; src < a.nsrc; ++src) {
+    double m = 0.0;
+    for (int i = threadIdx.x; i < a.nb; i += 256) m = fmax(m, partials[(long)src * a.nb + i]);
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if (threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) srcmax[src] = sh[0];
     __syncthreads();
   }
   if (threadIdx.x == 0) {
     double s = 1.0;
-    if (kind == (int)PlaneKind::F16x2) {
-      const double mx = sh[0];
-      if (mx > 0.0 && isfinite(mx)) {
+    if (a.kind == (int)PlaneKind::F16x2) {
+      double bound = 0.0;
+      for (int t = 0; t < a.nterms; ++t) bound += a.absalpha[t] * srcmax[a.src_of[t]];
+      if (bound > 0.0 && isfinite(bound)) {
         int e;
-        frexp(mx, &e);
+        frexp(bound, &e);
         s = ldexp(1.0, 14 - e);
       }
     }
@@ -95,29 +99,29 @@ __global__ __launch_bounds__(256) void split_max_stage2(const double* partials, 
   }
 }
 
-__device__ inline uint16_t f16_bits(float f) { return __builtin_bit_cast(uint16_t, (_Float16)f); }
-__device__ inline double f16_val(uint16_t u) { return (double)(float)__builtin_bit_cast(_Float16, u); }
-__device__ inline uint16_t bf16_bits(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
-__device__ inline double bf16_val(uint16_t u) { return (double)(float)__builtin_bit_cast(__bf16, u); }
+struct SplitOut {
+  uint16_t* p[3];
+  const double* scale;
+  int kind;
+};
 
-__global__ __launch_bounds__(256) void split_write(SplitArgs a) {
-  const long total = a.n[0] * a.n[1] * a.n[2] * a.n[3];
-  const double s = *a.scale;
-  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
-    const double x = eval_term_sum(a, e) * s;
-    if (a.kind == (int)PlaneKind::F16x2) {
+__global__ __launch_bounds__(kPThreads) void split_write_kernel(PermGeom g, SplitOut o) {
+  const double sc = *o.scale;
+  permute_block(g, [&](long e, double v) {
+    const double x = v * sc;
+    if (o.kind == (int)PlaneKind::F16x2) {
       const uint16_t h = f16_bits((float)x);
-      a.p[0][e] = h;
-      a.p[1][e] = f16_bits((float)(x - f16_val(h)));
+      o.p[0][e] = h;
+      o.p[1][e] = f16_bits((float)(x - f16_val(h)));
     } else {
       const uint16_t h = bf16_bits((float)x);
       const double r1 = x - bf16_val(h);
       const uint16_t m = bf16_bits((float)r1);
-      a.p[0][e] = h;
-      a.p[1][e] = m;
-      if (a.kind == (int)PlaneKind::BF16x3) a.p[2][e] = bf16_bits((float)(r1 - bf16_val(m)));
+      o.p[0][e] = h;
+      o.p[1][e] = m;
+      if (o.kind == (int)PlaneKind::BF16x3) o.p[2][e] = bf16_bits((float)(r1 - bf16_val(m)));
     }
-  }
+  });
 }
 
 // ---------------------------------------------------------------- contraction
@@ -303,34 +307,42 @@ __global__ __launch_bounds__(kThr, 2) void gemm_planes_kernel(PParams p) {
       }
 }
 
-__global__ void planes_splitk_reduce(const double* __restrict__ W, double* __restrict__ C, long ldc,
-                                     long sC, int M, int N, int nsplit, int batch0, double beta) {
-  const long MN = (long)M * N;
-  const int bz = blockIdx.y;
-  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < MN; e += (long)gridDim.x * blockDim.x) {
-    const int m = (int)(e / N), n = (int)(e - (long)(e / N) * N);
-    double acc = 0.0;
-    const double* w = W + (long)bz * nsplit * MN + e;
-    for (int s2 = 0; s2 < nsplit; ++s2) acc += w[(long)s2 * MN];
-    double* c = C + (long)(batch0 + bz) * sC + (long)m * ldc + n;
-    *c = (beta == 0.0) ? acc : acc + beta * (*c);
+// Launch plan: tile edge, split-K factor and K chunk of one contraction.
+struct PlanePlan {
+  int bm;      // square tile edge, 64 or 128
+  int nsplit;  // K chunks (each summed in FP32, chunks summed in FP64)
+  int kchunk;
+};
+
+PlanePlan plan_planes(int M, int N, int K, int batch) {
+  PlanePlan pl;
+  // 128-wide tiles unless 64-wide ones pad the output much less
+  pl.bm = (M <= 64 || N <= 64) ? 64 : 128;
+  if (pl.bm == 128) {
+    const double a128 = (double)ceil_div(M, 128) * ceil_div(N, 128) * 128 * 128;
+    const double a64 = (double)ceil_div(M, 64) * ceil_div(N, 64) * 64 * 64;
+    if (a128 > 1.25 * a64) pl.bm = 64;
   }
+  const long tiles = (long)ceil_div(M, pl.bm) * ceil_div(N, pl.bm);
+  int nsplit = 1;
+  const long want = 1024;  // about 4 workgroups per CU
+  if (tiles * batch < want && K >= 8 * kBK) {
+    nsplit = (int)std::min<long>(ceil_div(want, tiles * batch), K / (4 * kBK));
+    nsplit = std::max(1, std::min(nsplit, 512));
+  }
+  // FP32 accumulation length bounded by kMaxChunk: more chunks, summed in FP64
+  nsplit = std::max<int>(nsplit, (int)ceil_div(K, kMaxChunk));
+  pl.kchunk = (int)(ceil_div(ceil_div(K, nsplit), kBK) * kBK);
+  pl.nsplit = nsplit > 1 ? (int)ceil_div(K, pl.kchunk) : 1;
+  return pl;
 }
 
 template <int BM, int BN, int NP, bool F16>
-void launch_planes(const PlaneGemmDesc& d, double* ws, size_t ws_bytes, hipStream_t stream) {
+void launch_planes(const PlaneGemmDesc& d, const PlanePlan& pl, double* ws, size_t ws_bytes,
+                   hipStream_t stream) {
   const int tiles_m = (int)ceil_div(d.M, BM), tiles_n = (int)ceil_div(d.N, BN);
   const long tiles = (long)tiles_m * tiles_n;
-  int nsplit = 1;
-  const long want = 512;
-  if (tiles * d.batch < want && d.K >= 8 * kBK) {
-    nsplit = (int)std::min<long>(ceil_div(want, tiles * d.batch), d.K / (4 * kBK));
-    nsplit = std::max(1, std::min(nsplit, 64));
-  }
-  // FP32 accumulation length bounded by kMaxChunk: more chunks, summed in FP64
-  nsplit = std::max<int>(nsplit, (int)ceil_div(d.K, kMaxChunk));
-  int kchunk = (int)(ceil_div(ceil_div(d.K, nsplit), kBK) * kBK);
-  if (nsplit > 1) nsplit = (int)ceil_div(d.K, kchunk);
+  const int nsplit = pl.nsplit, kchunk = pl.kchunk;
   const size_t slice = (size_t)d.M * d.N * sizeof(double);
   int max_batch = 65535 / nsplit;
   if (nsplit > 1) {
@@ -359,22 +371,17 @@ void launch_planes(const PlaneGemmDesc& d, double* ws, size_t ws_bytes, hipStrea
     else
       hipLaunchKernelGGL((gemm_planes_kernel<BM, BN, NP, F16, false>), grid, dim3(kThr), 0, stream, p);
     check_launch("gemm_planes");
-    if (nsplit > 1) {
-      const long MN = (long)d.M * d.N;
-      dim3 g2((unsigned)std::min<long>(ceil_div(MN, 256), 4096), (unsigned)nb);
-      hipLaunchKernelGGL(planes_splitk_reduce, g2, dim3(256), 0, stream, ws, d.C, d.ldc, d.sC, d.M,
-                         d.N, nsplit, b0, d.beta);
-      check_launch("planes_splitk_reduce");
-    }
+    if (nsplit > 1) splitk_reduce(ws, d.C, d.ldc, d.sC, d.M, d.N, nsplit, b0, nb, d.beta, stream);
   }
 }
 
 template <int NP, bool F16>
 void dispatch_planes(const PlaneGemmDesc& d, double* ws, size_t ws_bytes, hipStream_t stream) {
-  if (d.M <= 64 || d.N <= 64)
-    launch_planes<64, 64, NP, F16>(d, ws, ws_bytes, stream);
+  const PlanePlan pl = plan_planes(d.M, d.N, d.K, d.batch);
+  if (pl.bm == 64)
+    launch_planes<64, 64, NP, F16>(d, pl, ws, ws_bytes, stream);
   else
-    launch_planes<128, 128, NP, F16>(d, ws, ws_bytes, stream);
+    launch_planes<128, 128, NP, F16>(d, pl, ws, ws_bytes, stream);
 }
 
 }  // namespace
@@ -382,29 +389,46 @@ void dispatch_planes(const PlaneGemmDesc& d, double* ws, size_t ws_bytes, hipStr
 void split_to_planes(PlaneKind kind, const Planes& out, const long n[4], const TTerm* terms,
                      int nterms, double* partials, hipStream_t stream) {
   if (nterms < 1 || nterms > kMaxTerms) throw std::invalid_argument("split_to_planes: bad term count");
-  SplitArgs a;
-  for (int q = 0; q < 3; ++q) a.p[q] = out.p[q];
-  a.scale = out.scale;
-  for (int d = 0; d < 4; ++d) a.n[d] = n[d];
-  a.nterms = nterms;
-  for (int t = 0; t < nterms; ++t) a.t[t] = terms[t];
-  a.kind = (int)kind;
   const long total = n[0] * n[1] * n[2] * n[3];
   if (total > out.n) throw std::invalid_argument("split_to_planes: planes too small");
-  const int g = (int)std::max<long>(1, std::min<long>(1024, ceil_div(total, 256)));
-  hipLaunchKernelGGL(split_max_stage1, dim3(g), dim3(256), 0, stream, a, partials);
-  hipLaunchKernelGGL(split_max_stage2, dim3(1), dim3(256), 0, stream, partials, g, (int)kind, out.scale);
-  const int g2 = (int)std::max<long>(1, std::min<long>(16384, ceil_div(total, 256)));
-  hipLaunchKernelGGL(split_write, dim3(g2), dim3(256), 0, stream, a);
+  if (total == 0) return;
+  // scale: per distinct input, max |x| (fp16 planes only)
+  ScaleArgs sa{};
+  sa.nterms = nterms;
+  sa.kind = (int)kind;
+  AbsMaxArgs am{};
+  for (int t = 0; t < nterms; ++t) {
+    int src = -1;
+    for (int u = 0; u < sa.nsrc; ++u)
+      if (am.p[u] == terms[t].p) src = u;
+    if (src < 0) {
+      if (terms[t].vol <= 0) throw std::invalid_argument("split_to_planes: input volume not set");
+      src = sa.nsrc++;
+      am.p[src] = terms[t].p;
+      am.n[src] = terms[t].vol;
+    }
+    sa.src_of[t] = src;
+    sa.absalpha[t] = std::fabs(terms[t].alpha);
+  }
+  if (kind == PlaneKind::F16x2) {
+    long nmax = 0;
+    for (int u = 0; u < sa.nsrc; ++u) nmax = std::max(nmax, am.n[u]);
+    sa.nb = (int)std::max<long>(1, std::min<long>(1024, ceil_div(nmax, 256)));
+    hipLaunchKernelGGL(absmax_stage1, dim3(sa.nb, sa.nsrc), dim3(256), 0, stream, am, partials);
+  } else {
+    sa.nsrc = 0;
+    sa.nb = 0;
+  }
+  hipLaunchKernelGGL(scale_stage2, dim3(1), dim3(256), 0, stream, partials, sa, out.scale);
+  const PermGeom g = make_perm_geom(n, terms, nterms);
+  SplitOut so{{out.p[0], out.p[1], out.p[2]}, out.scale, (int)kind};
+  hipLaunchKernelGGL(split_write_kernel, perm_grid(g), dim3(kPThreads), 0, stream, g, so);
   check_launch("split_to_planes");
 }
 
 size_t gemm_planes_workspace_bytes(int M, int N, int K, int batch) {
-  const long tiles = ceil_div(M, 128) * ceil_div(N, 128);
-  long nsplit = 1;
-  if (tiles * batch < 512 && K >= 8 * kBK) nsplit = std::min<long>(64, std::max<long>(1, std::min<long>(ceil_div(512, tiles * batch), K / (4 * kBK))));
-  nsplit = std::max<long>(nsplit, ceil_div(K, kMaxChunk)) + 1;
-  return nsplit > 2 ? (size_t)nsplit * M * N * batch * sizeof(double) : 0;
+  const PlanePlan pl = plan_planes(M, N, K, batch);
+  return pl.nsplit > 1 ? (size_t)pl.nsplit * M * N * batch * sizeof(double) : 0;
 }
 
 void gemm_planes(const PlaneGemmDesc& d, double* ws, size_t ws_bytes, hipStream_t stream) {

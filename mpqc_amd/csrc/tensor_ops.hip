@@ -2,6 +2,7 @@
 // read/write each element once with unit-stride writes and wave-contiguous
 // reads along the innermost output index.
 #include "tensor_ops.hpp"
+#include "permute.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -13,28 +14,9 @@ namespace {
 
 constexpr int kBlock = 256;
 
-struct TAddArgs {
-  double* out;
-  long n[4];
-  double beta;
-  int nterms;
-  TTerm t[kMaxTerms];
-};
-
-// grid: x over the (i2,i3) plane, y = i1, z = i0
-__global__ __launch_bounds__(kBlock) void tadd_kernel(TAddArgs a) {
-  const long plane = a.n[2] * a.n[3];
-  const long e = (long)blockIdx.x * kBlock + threadIdx.x;
-  if (e >= plane) return;
-  const long i0 = blockIdx.z, i1 = blockIdx.y;
-  const long i2 = e / a.n[3], i3 = e - i2 * a.n[3];
-  const long o = ((i0 * a.n[1] + i1) * a.n[2] + i2) * a.n[3] + i3;
-  double acc = (a.beta == 0.0) ? 0.0 : a.beta * a.out[o];
-  for (int t = 0; t < a.nterms; ++t) {
-    const TTerm& tt = a.t[t];
-    acc += tt.alpha * tt.p[i0 * tt.s[0] + i1 * tt.s[1] + i2 * tt.s[2] + i3 * tt.s[3]];
-  }
-  a.out[o] = acc;
+__global__ __launch_bounds__(kPThreads) void tadd_kernel(PermGeom g, double* __restrict__ out,
+                                                       double beta) {
+  permute_block(g, [&](long o, double v) { out[o] = (beta == 0.0) ? v : beta * out[o] + v; });
 }
 
 // Element-wise kernels use 1-D grid-stride loops with 32-bit indices (the
@@ -248,20 +230,55 @@ void permuted_strides(const long in_extents[4], const int perm[4], long out_s[4]
   for (int d = 0; d < 4; ++d) out_s[d] = st[perm[d]];
 }
 
+PermGeom make_perm_geom(const long n[4], const TTerm* terms, int nterms) {
+  if (nterms < 1 || nterms > kMaxTerms) throw std::invalid_argument("permutation: bad term count");
+  PermGeom g{};
+  long os[4];
+  os[3] = 1;
+  for (int d = 2; d >= 0; --d) os[d] = os[d + 1] * n[d + 1];
+  // q: an output dim along which some input that is not contiguous along dim 3 is contiguous
+  int q = -1;
+  if (n[3] > 1)
+    for (int t = 0; t < nterms && q < 0; ++t)
+      if (terms[t].s[3] != 1)
+        for (int d = 2; d >= 0; --d)
+          if (terms[t].s[d] == 1 && n[d] > 1) {
+            q = d;
+            break;
+          }
+  g.tiled = q >= 0;
+  if (!g.tiled) q = 2;
+  int r[2], nr = 0;
+  for (int d = 0; d < 3; ++d)
+    if (d != q) r[nr++] = d;
+  g.n3 = n[3]; g.nq = n[q]; g.nr0 = n[r[0]]; g.nr1 = n[r[1]];
+  g.o3 = 1; g.oq = os[q]; g.or0 = os[r[0]]; g.or1 = os[r[1]];
+  g.tiles3 = (int)ceil_div(n[3], kPT);
+  g.tilesq = (int)ceil_div(n[q], kPT);
+  g.nterms = nterms;
+  g.lds_mask = 0;
+  for (int t = 0; t < nterms; ++t) {
+    const TTerm& T = terms[t];
+    g.t[t] = PTerm{T.p, T.s[3], T.s[q], T.s[r[0]], T.s[r[1]], T.alpha};
+    if (g.tiled && T.s[3] != 1 && T.s[q] == 1) g.lds_mask |= 1u << t;
+  }
+  if (!g.tiled && g.nq * g.n3 >= (1L << 31)) throw std::invalid_argument("permutation plane too large");
+  return g;
+}
+
+dim3 perm_grid(const PermGeom& g) {
+  check_grid_yz(g.nr1, g.nr0);
+  const long x = g.tiled ? (long)g.tiles3 * g.tilesq : ceil_div(g.nq * g.n3, kPThreads);
+  if (x >= (1L << 31)) throw std::invalid_argument("permutation grid too large");
+  return dim3((unsigned)x, (unsigned)g.nr1, (unsigned)g.nr0);
+}
+
 void tadd(double* out, const long n[4], double beta, const TTerm* terms, int nterms,
           hipStream_t stream) {
   if (nterms > kMaxTerms) throw std::invalid_argument("tadd: too many terms");
-  TAddArgs a;
-  a.out = out;
-  for (int d = 0; d < 4; ++d) a.n[d] = n[d];
-  a.beta = beta;
-  a.nterms = nterms;
-  for (int t = 0; t < nterms; ++t) a.t[t] = terms[t];
-  const long plane = n[2] * n[3];
-  if (plane == 0 || n[0] == 0 || n[1] == 0) return;
-  check_grid_yz(n[1], n[0]);
-  dim3 grid((unsigned)ceil_div(plane, kBlock), (unsigned)n[1], (unsigned)n[0]);
-  hipLaunchKernelGGL(tadd_kernel, grid, dim3(kBlock), 0, stream, a);
+  if (n[0] * n[1] * n[2] * n[3] == 0) return;
+  const PermGeom g = make_perm_geom(n, terms, nterms);
+  hipLaunchKernelGGL(tadd_kernel, perm_grid(g), dim3(kPThreads), 0, stream, g, out, beta);
   check_launch("tadd");
 }
 

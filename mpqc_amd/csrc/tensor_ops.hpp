@@ -16,6 +16,7 @@ struct TTerm {
   const double* p = nullptr;
   long s[4] = {0, 0, 0, 0};
   double alpha = 1.0;
+  long vol = 0;  // element count of the input tensor (used for range bounds)
 };
 
 // out[i0,i1,i2,i3] (dense row-major, extents n) = beta*out + sum_t alpha_t * in_t[...]
