@@ -37,6 +37,13 @@
  *                                 (math/external/tiledarray/tiledarray_impl.cpp:36-47,
  *                                 TA's ContractReduce over TA::Tensor<double>)
  *
+ * Adjoint entry points (no counterpart in the reference, which has no Lambda
+ * equations and no CCSD gradients): mpqc_ccsd_jacobian_transpose,
+ * mpqc_ccsd_lambda_solve, mpqc_ccsd_energy_gradient, mpqc_ccsd_get_lambda,
+ * mpqc_ccsd_get_integral_gradient.  They differentiate the converged energy of
+ * ccsd.h:376-380 (at R = 0: E = 2 <f_ai, t1> + <g_abij, 2 tau - tau(b,a,i,j)>)
+ * through the residual of ccsd_r1_r2.h:70-293.
+ *
  * Conventions: row-major FP64 buffers in the reference's index orders: t1 "a,i",
  * t2/r2 "a,b,i,j", Fia "i,a", Fij "i,j", Fab "a,b", Gijab "i,j,a,b" = <ij|ab>,
  * Gijkl "i,j,k,l", Giajb "i,a,j,b", Giabc "i,a,b,c", Gijka "i,j,k,a",
@@ -66,7 +73,8 @@ enum {
   MPQC_CCSD_ERR_STATE = 3,     /* call out of order (ProgrammingError in the reference) */
   MPQC_CCSD_ERR_NONFINITE = 4, /* non-finite residual/amplitudes (SIGFPE in the reference) */
   MPQC_CCSD_ERR_DIIS = 5,      /* DIIS system poorly conditioned (TA::DIIS domain_error) */
-  MPQC_CCSD_ERR_NOMEM = 6      /* device allocation failed */
+  MPQC_CCSD_ERR_NOMEM = 6,     /* device allocation failed */
+  MPQC_CCSD_ERR_NOCONV = 7     /* an iterative solve did not reach its precision */
 };
 
 /* Contraction precisions (the Jacobi update, DIIS and energy are always FP64):
@@ -154,6 +162,24 @@ int mpqc_ccsd_get_trace(mpqc_ccsd_ctx* ctx, int max_rows, int* n_rows, int* iter
  * reference gets from its per-phase timers (ccsd.h:392-394,442-444,463-465). */
 int mpqc_ccsd_get_kernel_times(mpqc_ccsd_ctx* ctx, int max_rows, int* n_rows, char* names,
                                int64_t* calls, double* ms, double* flops);
+
+/* ---- adjoint (no counterpart in the reference) ----
+ * x = (dR/dt)^T y at the context's current amplitudes: y1 "a,i", y2 "a,b,i,j" in,
+ * x1, x2 out (same layouts).  Runs one forward residual first. */
+int mpqc_ccsd_jacobian_transpose(mpqc_ccsd_ctx* ctx, const double* y1, const double* y2, double* x1,
+                                 double* x2, int layout, int memspace);
+/* Solves (dR/dt)^T lambda = -dE/dt at the current (converged) amplitudes, to
+ * sqrt(|res1|^2 + |res2|^2) / (size1 + size2) <= target_precision (the amplitude
+ * equations' error measure, chemistry/qc/cc/solvers.h:60-63). */
+int mpqc_ccsd_lambda_solve(mpqc_ccsd_ctx* ctx, int max_iter, double target_precision, int verbose,
+                           int* n_iter);
+int mpqc_ccsd_get_lambda(mpqc_ccsd_ctx* ctx, double* l1, double* l2, int layout, int memspace);
+/* dE/d(integral block) for all nine blocks of cc::Integrals (after lambda_solve);
+ * read each with mpqc_ccsd_get_integral_gradient (same index order and layout
+ * as mpqc_ccsd_set_integral). */
+int mpqc_ccsd_energy_gradient(mpqc_ccsd_ctx* ctx);
+int mpqc_ccsd_get_integral_gradient(mpqc_ccsd_ctx* ctx, int which, double* out, int layout,
+                                    int memspace);
 
 /* Device-pointer kernel entry points (stream may be NULL for the null stream). */
 int mpqc_ccsd_pp_ladder(int o, int v, const double* gabcd_dev, const double* tau_dev,

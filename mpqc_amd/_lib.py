@@ -12,7 +12,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libmpqc_ccsd.so")
 
 OK = 0
-ERR_ARG, ERR_HIP, ERR_STATE, ERR_NONFINITE, ERR_DIIS, ERR_NOMEM = 1, 2, 3, 4, 5, 6
+ERR_ARG, ERR_HIP, ERR_STATE, ERR_NONFINITE, ERR_DIIS, ERR_NOMEM, ERR_NOCONV = 1, 2, 3, 4, 5, 6, 7
 PREC_F64, PREC_F32, PREC_BF16X3, PREC_F16X3, PREC_BF16X6 = 0, 1, 2, 3, 4
 LAYOUT_DENSE, LAYOUT_TA_TILES = 0, 1
 MEM_HOST, MEM_DEVICE = 0, 1
@@ -29,7 +29,8 @@ EXPORTS = [
     "mpqc_ccsd_get_residuals", "mpqc_ccsd_get_dressed_fock", "mpqc_ccsd_init_guess",
     "mpqc_ccsd_residual", "mpqc_ccsd_energy_error", "mpqc_ccsd_update", "mpqc_ccsd_solve",
     "mpqc_ccsd_get_trace", "mpqc_ccsd_pp_ladder", "mpqc_ccsd_gemm",
-    "mpqc_ccsd_get_kernel_times",
+    "mpqc_ccsd_get_kernel_times", "mpqc_ccsd_jacobian_transpose", "mpqc_ccsd_lambda_solve",
+    "mpqc_ccsd_get_lambda", "mpqc_ccsd_energy_gradient", "mpqc_ccsd_get_integral_gradient",
 ]
 
 _lib = None
@@ -78,6 +79,11 @@ def load():
         "mpqc_ccsd_get_trace": (I, [P, I, pI, P, P, P, P, P]),
         "mpqc_ccsd_pp_ladder": (I, [I, I, P, P, P, D, D, I, P]),
         "mpqc_ccsd_get_kernel_times": (I, [P, I, pI, P, P, P, P]),
+        "mpqc_ccsd_jacobian_transpose": (I, [P, P, P, P, P, I, I]),
+        "mpqc_ccsd_lambda_solve": (I, [P, I, D, I, pI]),
+        "mpqc_ccsd_get_lambda": (I, [P, P, P, I, I]),
+        "mpqc_ccsd_energy_gradient": (I, [P]),
+        "mpqc_ccsd_get_integral_gradient": (I, [P, I, P, I, I]),
         "mpqc_ccsd_gemm": (I, [I, I, I, I, I, D, P, I64, I64, P, I64, I64, D, P, I64, I64, I, I, P]),
     }
     for name, (res, args) in sig.items():

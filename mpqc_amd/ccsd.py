@@ -257,6 +257,44 @@ class CCSD:
         self.n_iter = n.value
         return e.value
 
+    # -- adjoint (no counterpart in the reference)
+    def jacobian_transpose(self, y1, y2):
+        """(dR/dt)^T y at the current amplitudes; returns (x1, x2)."""
+        a = np.ascontiguousarray(y1, dtype=np.float64)
+        b = np.ascontiguousarray(y2, dtype=np.float64)
+        o, v = self.o, self.v
+        x1 = np.empty((v, o))
+        x2 = np.empty((v, v, o, o))
+        self._check(self.lib.mpqc_ccsd_jacobian_transpose(
+            self.ctx, _dptr(a), _dptr(b), _dptr(x1), _dptr(x2), L.LAYOUT_DENSE, L.MEM_HOST))
+        return x1, x2
+
+    def lambda_solve(self, target_precision: float = 1e-10, max_iter: int = 100,
+                     verbose: bool = False) -> int:
+        """Solves (dR/dt)^T lambda = -dE/dt at the converged amplitudes."""
+        n = ctypes.c_int()
+        self._check(self.lib.mpqc_ccsd_lambda_solve(self.ctx, int(max_iter), float(target_precision),
+                                                    int(verbose), ctypes.byref(n)))
+        return n.value
+
+    def get_lambda(self):
+        return self._get2(self.lib.mpqc_ccsd_get_lambda, "dense")
+
+    def energy_gradient(self):
+        """dE/d(integral block) for all nine blocks; returns {name: array}."""
+        self._check(self.lib.mpqc_ccsd_energy_gradient(self.ctx))
+        out = {}
+        o, v = self.o, self.v
+        shapes = {"Fia": (o, v), "Fij": (o, o), "Fab": (v, v), "Gijab": (o, o, v, v),
+                  "Gijkl": (o, o, o, o), "Giajb": (o, v, o, v), "Giabc": (o, v, v, v),
+                  "Gijka": (o, o, o, v), "Gabcd": (v, v, v, v)}
+        for name, idx in L.INTEGRAL_IDS.items():
+            g = np.empty(shapes[name])
+            self._check(self.lib.mpqc_ccsd_get_integral_gradient(self.ctx, idx, _dptr(g),
+                                                                 L.LAYOUT_DENSE, L.MEM_HOST))
+            out[name] = g
+        return out
+
     def trace(self) -> List[TraceRow]:
         n = ctypes.c_int()
         self._check(self.lib.mpqc_ccsd_get_trace(self.ctx, 0, ctypes.byref(n), None, None, None,

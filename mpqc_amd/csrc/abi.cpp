@@ -29,6 +29,9 @@ int status_of(const std::exception& e) {
     return w.find("DIIS") != std::string::npos ? MPQC_CCSD_ERR_DIIS : MPQC_CCSD_ERR_ARG;
   }
   if (dynamic_cast<const std::range_error*>(&e)) return MPQC_CCSD_ERR_NONFINITE;
+  if (dynamic_cast<const std::runtime_error*>(&e) &&
+      std::string(e.what()).find("did not converge") != std::string::npos)
+    return MPQC_CCSD_ERR_NOCONV;
   if (dynamic_cast<const std::logic_error*>(&e) && !dynamic_cast<const std::invalid_argument*>(&e))
     return MPQC_CCSD_ERR_STATE;
   return MPQC_CCSD_ERR_ARG;
@@ -214,6 +217,38 @@ int mpqc_ccsd_get_trace(mpqc_ccsd_ctx* ctx, int max_rows, int* n_rows, int* iter
       if (seconds) seconds[k] = t[k].seconds;
     }
     if (n_rows) *n_rows = (int)t.size();
+  });
+}
+
+int mpqc_ccsd_jacobian_transpose(mpqc_ccsd_ctx* ctx, const double* y1, const double* y2, double* x1,
+                                 double* x2, int layout, int memspace) {
+  return guarded(ctx, [&](mpqc_hip::CCSDContext& c) {
+    if (!y1 || !y2 || !x1 || !x2) throw std::invalid_argument("null buffer");
+    c.jacobian_transpose_io(y1, y2, x1, x2, layout, memspace);
+  });
+}
+
+int mpqc_ccsd_lambda_solve(mpqc_ccsd_ctx* ctx, int max_iter, double target_precision, int verbose,
+                           int* n_iter) {
+  return guarded(ctx, [&](mpqc_hip::CCSDContext& c) {
+    if (!(target_precision > 0.0)) throw std::invalid_argument("target_precision must be > 0");
+    const int n = c.lambda_solve(max_iter, target_precision, verbose);
+    if (n_iter) *n_iter = n;
+  });
+}
+
+int mpqc_ccsd_get_lambda(mpqc_ccsd_ctx* ctx, double* l1, double* l2, int layout, int memspace) {
+  return guarded(ctx, [&](mpqc_hip::CCSDContext& c) { c.get_lambda(l1, l2, layout, memspace); });
+}
+
+int mpqc_ccsd_energy_gradient(mpqc_ccsd_ctx* ctx) {
+  return guarded(ctx, [&](mpqc_hip::CCSDContext& c) { c.energy_gradient(); });
+}
+
+int mpqc_ccsd_get_integral_gradient(mpqc_ccsd_ctx* ctx, int which, double* out, int layout,
+                                    int memspace) {
+  return guarded(ctx, [&](mpqc_hip::CCSDContext& c) {
+    c.get_integral_gradient(which, out, layout, memspace);
   });
 }
 

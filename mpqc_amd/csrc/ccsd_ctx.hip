@@ -18,8 +18,8 @@
 namespace mpqc_hip {
 
 namespace {
-
 constexpr int kOcc = 0, kVir = 1, kNone = -1;
+}  // namespace
 
 // Column-pivoted Householder QR solve of a small dense system (row-major A).
 // Returns |det A|; x gets the rank-truncated solution (components of zero
@@ -87,8 +87,6 @@ double colpiv_qr_solve(int n, std::vector<double> A, std::vector<double> b, std:
   return absdet;
 }
 
-}  // namespace
-
 // ---------------------------------------------------------------- lifecycle
 CCSDContext::CCSDContext(int o, int v, const int64_t* occ_hm, int n_occ_tiles, const int64_t* vir_hm,
                          int n_vir_tiles, int precision)
@@ -110,6 +108,14 @@ void CCSDContext::release() {
   pending_.clear();
   for (auto e : event_pool_) hipEventDestroy(e);
   event_pool_.clear();
+  for (double* p : adj_) if (p) hipFree(p);
+  adj_.clear();
+  if (adj_ws_) hipFree(adj_ws_);
+  adj_ws_ = nullptr;
+  adj_ws_n_ = 0;
+  if (adj_scratch_) hipFree(adj_scratch_);
+  adj_scratch_ = nullptr;
+  adj_scratch_n_ = 0;
   for (double* p : allocations_) hipFree(p);
   allocations_.clear();
   alloc_sizes_.clear();
@@ -418,8 +424,52 @@ void CCSDContext::gemm(bool tA, bool tB, int M, int N, int K, double alpha, cons
   check_access(A, (tA ? (long)(K - 1) * lda + (M - 1) : (long)(M - 1) * lda + (K - 1)) + bm * sA, "gemm A");
   check_access(B, (tB ? (long)(N - 1) * ldb + (K - 1) : (long)(K - 1) * ldb + (N - 1)) + bm * sB, "gemm B");
   check_access(C, (long)(M - 1) * ldc + (N - 1) + bm * sC, "gemm C");
+  if (tape_) {
+    TapeOp op;
+    op.kind = TapeOp::kGemm;
+    op.g = d;
+    op.prec = prec;
+    op.tag = tag ? tag : "gemm";
+    tape_->push_back(op);
+    if (!tape_execute_) return;
+  }
   timed(tag ? tag : "gemm", 2.0 * M * (double)N * K * batch,
         [&] { mpqc_hip::gemm(d, prec, ws_, stream_); });
+}
+
+void CCSDContext::copy_d2d(double* dst, const double* src, long n) {
+  check_access(dst, n - 1, "copy destination");
+  check_access(src, n - 1, "copy source");
+  if (tape_) {
+    TapeOp op;
+    op.kind = TapeOp::kCopy;
+    op.dst = dst; op.src = src; op.n = n;
+    tape_->push_back(op);
+    if (!tape_execute_) return;
+  }
+  MPQC_HIP_CHECK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+}
+
+void CCSDContext::op_tau() {
+  if (tape_) {
+    TapeOp op;
+    op.kind = TapeOp::kTau;
+    op.dst = tau_; op.src = t2_; op.src2 = t1_;
+    tape_->push_back(op);
+    if (!tape_execute_) return;
+  }
+  timed("r:tau", 0.0, [&] { build_tau(tau_, t2_, t1_, o_, v_, stream_); });
+}
+
+void CCSDContext::op_TP() {
+  if (tape_) {
+    TapeOp op;
+    op.kind = TapeOp::kTP;
+    op.dst = TP_; op.src = t2_; op.src2 = t1_;
+    tape_->push_back(op);
+    if (!tape_execute_) return;
+  }
+  timed("r:TP", 0.0, [&] { build_TP(TP_, t2_, t1_, o_, v_, stream_); });
 }
 
 int CCSDContext::timing_slot(const char* tag) {
@@ -500,6 +550,16 @@ void CCSDContext::perm_add(double* out, std::array<long, 4> n, double beta,
   check_access(out, n[0] * n[1] * n[2] * n[3] - 1, "perm_add out");
   const int nt = make_terms(n, srcs, terms);
   long nn[4] = {n[0], n[1], n[2], n[3]};
+  if (tape_) {
+    TapeOp op;
+    op.kind = TapeOp::kTAdd;
+    op.dst = out; op.ext = n; op.beta = beta; op.nterms = nt;
+    int k = 0;
+    for (const Src& sc : srcs) op.terms[k++] = sc;
+    op.tag = tag ? std::string(tag) : "perm@" + std::to_string(line);
+    tape_->push_back(op);
+    if (!tape_execute_) return;
+  }
   std::string t;
   if (kernel_timing_) t = tag ? std::string(tag) : "perm@" + std::to_string(line);
   timed(t.c_str(), 0.0, [&] { tadd(out, nn, beta, terms, nt, stream_); });
@@ -624,6 +684,9 @@ void CCSDContext::prepare() {
   const std::array<long, 4> ovv_ext_ijab{O, O, V, V}, e_iajb{O, V, O, V}, e_iabc{O, V, V, V},
       e_ijka{O, O, O, V};
   const std::array<int, 4> I{0, 1, 2, 3};
+  prepare_tape_.clear();
+  std::vector<TapeOp>* saved_tape = tape_;
+  tape_ = &prepare_tape_;
   // f_ai("a,i") = Fia("i,a")                                     ccsd.h:331-332
   perm_add(f_ai_, {1, 1, V, O}, 0.0, {{Fia_, {1, 1, O, V}, {0, 1, 3, 2}, 1.0}});
   // g_abij("a,b,i,j") = Gijab("i,j,a,b")                          ccsd.h:336-337
@@ -662,6 +725,7 @@ void CCSDContext::prepare() {
   perm_add(GB_, {O, V, O, V}, 0.0,
            {{Gijab_, ovv_ext_ijab, {1, 3, 0, 2}, 2.0}, {Gijab_, ovv_ext_ijab, {1, 2, 0, 3}, -1.0}});
   perm_add(GC_, {O, V, O, V}, 0.0, {{Gijab_, ovv_ext_ijab, {1, 2, 0, 3}, 1.0}});
+  tape_ = saved_tape;
   MPQC_HIP_CHECK(hipStreamSynchronize(stream_));
   prepared_ = true;
   planes_ready_ = false;
@@ -689,7 +753,7 @@ void CCSDContext::residual() {
   const std::array<int, 4> I{0, 1, 2, 3};
 
   // tau("a,b,i,j") = t2("a,b,i,j") + t1("a,i") t1("b,j")            ccsd.h:389
-  timed("r:tau", 0.0, [&] { build_tau(tau_, t2_, t1_, o, v, stream_); });
+  op_tau();
   // amplitude layouts used by several GEMMs
   perm_add(t1T_, {1, 1, O, V}, 0.0, {{t1_, {1, 1, V, O}, {0, 1, 3, 2}, 1.0}});  // t1T(k,d) = t1(d,k)
   const bool sm = split_mode();
@@ -707,11 +771,11 @@ void CCSDContext::residual() {
   // ===================== r1: ccsd_r1_r2.h:70-124 =====================
   // :83-84  r1(a,i) = Fia(i,a) - 2 (Fia(k,c) t1(c,i)) t1(a,k)
   gemm(false, false, o, o, v, 1.0, Fia_, v, t1_, o, 0.0, Xoo_, o, 0, 0, 0, 1, "r1:fia_t1");
-  MPQC_HIP_CHECK(hipMemcpyAsync(r1_, f_ai_, ov * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+  copy_d2d(r1_, f_ai_, ov);
   gemm(false, false, v, o, o, -2.0, t1_, o, Xoo_, o, 1.0, r1_, o, 0, 0, 0, 1, "r1:t1_fia_t1");
   // :89-91  h_ac = F_vv = Fab(a,c) - gbar(k,l,c,d) tau(a,d,k,l); FAB = h_ac; r1 += h_ac t1
   //         (ccsd_intermediates.h:71-77)
-  MPQC_HIP_CHECK(hipMemcpyAsync(FAB_, Fab_, vv * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+  copy_d2d(FAB_, Fab_, vv);
   if (sm) {  // tau(a,(d,k,l)) . gbar^T(c,(d,k,l))
     gemmP(v, v, o * o * v, -1.0, tau_P_, oo * V, GbarT_P_, oo * V, 1.0, FAB_, v, "r1:F_vv");
   } else {
@@ -721,7 +785,7 @@ void CCSDContext::residual() {
   gemm(false, false, v, o, v, 1.0, FAB_, v, t1_, o, 1.0, r1_, o, 0, 0, 0, 1, "r1:FAB_t1");
   // :95-97  h_ki = F_oo = Fij(k,i) + gbar(k,l,a,b) tau(a,b,i,l); FIJ = h_ki; r1 -= t1 h_ki
   //         (ccsd_intermediates.h:57-63)
-  MPQC_HIP_CHECK(hipMemcpyAsync(FIJ_, Fij_, oo * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+  copy_d2d(FIJ_, Fij_, oo);
   if (sm) {  // gbar(k,(l,a,b)) . tau^T(i,(l,a,b))
     gemmP(o, o, o * v * v, 1.0, Gbar_P_, O * vv, tauT_P_, O * vv, 1.0, FIJ_, o, "r1:F_oo");
   } else {
@@ -731,7 +795,7 @@ void CCSDContext::residual() {
   gemm(false, false, v, o, o, -1.0, t1_, o, FIJ_, o, 1.0, r1_, o, 0, 0, 0, 1, "r1:t1_FIJ");
   // :101-104 h_kc = F_ov = Fia(k,c) + gbar(k,l,c,d) t1(d,l)   (ccsd_intermediates.h:89-95)
   //          r1 += h_kc (2 t2(c,a,k,i) - t2(c,a,i,k) + t1(c,i) t1(a,k))
-  MPQC_HIP_CHECK(hipMemcpyAsync(hkc_, Fia_, ov * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+  copy_d2d(hkc_, Fia_, ov);
   gemm(false, false, o * v, 1, o * v, 1.0, Gbar_kcld_, ov, t1T_, 1, 1.0, hkc_, 1, 0, 0, 0, 1, "r1:F_ov");
   perm_add(hT_, {1, 1, V, O}, 0.0, {{hkc_, {1, 1, O, V}, {0, 1, 3, 2}, 1.0}});  // hT(c,k) = h(k,c)
   perm_add(Y5_, {V, O, V, O}, 0.0,
@@ -772,12 +836,12 @@ void CCSDContext::residual() {
   gemm(false, true, v, o * o, o, -1.0, t1_, o, S_, o, 1.0, Rd_, oo, 0, O * oo, V * oo, v, "r2:t1_S");
   // :218-220 g_ki = F_OO(FIJ, Fia, g_ijka_bar, t1) = FIJ + Fia t1 + gbar(k,l,i,c) t1(c,l)
   //          (ccsd_intermediates.h:105-112)
-  MPQC_HIP_CHECK(hipMemcpyAsync(gki_, FIJ_, oo * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+  copy_d2d(gki_, FIJ_, oo);
   gemm(false, false, o, o, v, 1.0, Fia_, v, t1_, o, 1.0, gki_, o, 0, 0, 0, 1, "r2:gki_fia_t1");
   gemm(false, false, o * o, 1, o * v, 1.0, P13_, ov, t1T_, 1, 1.0, gki_, 1, 0, 0, 0, 1, "r2:gki_g_t1");
   // :222-224 g_ac = F_VV(FAB, Fia, g_iabc_bar, t1) = FAB + f_ov(k,c) t1(a,k) + gbar(k,a,d,c) t1(d,k)
   //          (ccsd_intermediates.h:141-148; fov_sign = +1 is the reference as written)
-  MPQC_HIP_CHECK(hipMemcpyAsync(gac_, FAB_, vv * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+  copy_d2d(gac_, FAB_, vv);
   gemm(false, false, v, v, o, fov_sign_, t1_, o, Fia_, v, 1.0, gac_, v, 0, 0, 0, 1, "r2:gac_t1_fia");
   gemm(false, false, v * v, 1, o * v, 1.0, P14_, ov, t1T_, 1, 1.0, gac_, 1, 0, 0, 0, 1, "r2:gac_g_t1");
   // :226 Rd += g_ac(a,c) t2(c,b,i,j) - g_ki(k,i) t2(a,b,k,j)
@@ -791,15 +855,15 @@ void CCSDContext::residual() {
   gemm(false, false, v * v * o, o, o, 1.0, t2_abjk_, o, gki_, o, 0.0, X16_, o, 0, 0, 0, 1, "r2:t2_gki");
   perm_add(Rd_, {V, V, O, O}, 1.0, {{X16_, e_abij, {0, 1, 3, 2}, -1.0}});
   // :236 T(d,b,i,l) = 0.5 t2(d,b,i,l) + t1(d,i) t1(b,l), as TP(a,i,l,d) = T(d,a,i,l)
-  build_TP(TP_, t2_, t1_, o, v, stream_);
+  op_TP();
   perm_add(t2P_, {V, O, O, V}, 0.0, {{t2_, e_abij, {0, 2, 3, 1}, 1.0}});  // t2P(a,i,l,d) = t2(a,d,i,l)
   // :238-243 j(a,k,i,c) stored as J(a,i,k,c):
   //   Gijab(i,k,a,c) - Gijka(l,k,i,c) t1(a,l) + Giabc(k,a,c,d) t1(d,i)
   //   - Gijab(k,l,c,d) T(d,a,i,l) + 0.5 (2 Gijab(k,l,c,d) - Gijab(k,l,d,c)) t2(a,d,i,l)
   // :245-248 k(k,a,i,c) stored as K(a,i,k,c):
   //   Giajb(k,a,i,c) - Gijka(k,l,i,c) t1(a,l) + Giabc(k,a,d,c) t1(d,i) - Gijab(k,l,d,c) T(d,a,i,l)
-  MPQC_HIP_CHECK(hipMemcpyAsync(J_, J0_, ov * ov * sizeof(double), hipMemcpyDeviceToDevice, stream_));
-  MPQC_HIP_CHECK(hipMemcpyAsync(K_, K0_, ov * ov * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+  copy_d2d(J_, J0_, ov * ov);
+  copy_d2d(K_, K0_, ov * ov);
   gemm(false, false, v, o * o * v, o, -1.0, t1_, o, X18_, oo * V, 1.0, J_, oo * V, 0, 0, 0, 1, "r2:j_gijka_t1");
   gemm(false, false, v, o * o * v, o, -1.0, t1_, o, X19_, oo * V, 1.0, K_, oo * V, 0, 0, 0, 1, "r2:k_gijka_t1");
   if (sm) {

@@ -24,6 +24,10 @@ struct TraceRow {
   double dE, error, energy, seconds;
 };
 
+// Column-pivoted Householder QR solve of a small dense system (row-major A);
+// returns |det A|, x gets the rank-truncated solution.
+double colpiv_qr_solve(int n, std::vector<double> A, std::vector<double> b, std::vector<double>& x);
+
 class CCSDContext {
  public:
   CCSDContext(int o, int v, const int64_t* occ_hm, int n_occ_tiles, const int64_t* vir_hm,
@@ -48,6 +52,20 @@ class CCSDContext {
   void update();
   double solve(int max_iter, double target_precision, int verbose, int* n_iter);
   const std::vector<TraceRow>& trace() const { return trace_; }
+
+  // ---- adjoint of the converged energy (no counterpart in the reference).
+  // With the amplitudes t converged (R(t) = 0), lambda solves (dR/dt)^T lambda = -dE/dt
+  // and dE/d(theta) = dE/d(theta)|_t + lambda^T dR/d(theta) for every integral block.
+  // Energy functional: E = 2 <f_ai, t1> + <g_abij, 2 tau(a,b,i,j) - tau(b,a,i,j)>.
+  // Jacobian-transpose product at the current amplitudes (device buffers, dense layout).
+  void jacobian_transpose(const double* y1, const double* y2, double* x1, double* x2);
+  // the same with host or device buffers in dense or tile layout
+  void jacobian_transpose_io(const double* y1, const double* y2, double* x1, double* x2, int layout,
+                             int memspace);
+  int lambda_solve(int max_iter, double target_precision, int verbose);
+  void energy_gradient();
+  void get_lambda(double* l1, double* l2, int layout, int memspace);
+  void get_integral_gradient(int which, double* out, int layout, int memspace);
 
   // Per-contraction timing with HIP events on the context stream (enabled by
   // the "kernel_timing" option).  Tags name the reference expression.
@@ -92,6 +110,54 @@ class CCSDContext {
                 const char* tag = nullptr, int line = __builtin_LINE());
   // validated terms of a permuted linear combination
   int make_terms(std::array<long, 4> n, std::initializer_list<Src> srcs, TTerm* terms);
+  void copy_d2d(double* dst, const double* src, long n);
+  void op_tau();  // tau = t2 + t1 t1  (ccsd.h:389)
+  void op_TP();   // the T intermediate of ccsd_r1_r2.h:236
+
+  // ---- op tape: integral preparation and the residual as a list of primitive
+  // ops (copies, permuted linear combinations, contractions, tau/T builds).
+  // Replayed in reverse by the adjoint; grouped across contexts by the batch.
+  struct TapeOp {
+    enum Kind : int { kCopy = 0, kTAdd = 1, kGemm = 2, kTau = 3, kTP = 4 };
+    Kind kind = kCopy;
+    double* dst = nullptr;         // kCopy / kTau / kTP output, kTAdd output
+    const double* src = nullptr;   // kCopy source; kTau / kTP: t2
+    const double* src2 = nullptr;  // kTau / kTP: t1
+    long n = 0;                    // kCopy element count
+    std::array<long, 4> ext{};     // kTAdd output extents
+    double beta = 0.0;             // kTAdd
+    int nterms = 0;                // kTAdd
+    Src terms[kMaxTerms];          // kTAdd
+    GemmDesc g;                    // kGemm
+    GemmPrecision prec = GemmPrecision::F64;
+    std::string tag;
+  };
+  // adjoint state
+  std::vector<double*> adj_;      // adjoint buffer of allocation k (nullptr until used)
+  std::vector<TapeOp> res_tape_;  // residual recorded at the amplitudes of the last sweep
+  double *lam1_ = nullptr, *lam2_ = nullptr, *gt1_ = nullptr, *gt2_ = nullptr;
+  double* adj_ws_ = nullptr;      // partial sums of shared-operand contraction adjoints
+  size_t adj_ws_n_ = 0;
+  double* adj_scratch_ = nullptr;  // permuted copies inside tau / TP adjoints
+  long adj_scratch_n_ = 0;
+  std::vector<double*> lam_hist_;  // Lambda DIIS history (owned allocations)
+  double* adj_scratch(long n);
+  double* adj_hist(int k, long n);
+  bool lambda_ready_ = false, gradient_ready_ = false;
+  int alloc_index(const double* p) const;
+  double* adj(const double* p);
+  bool is_integral(const double* p) const;
+  void zero_adjoints();
+  void record_residual();
+  void check_tape(const std::vector<TapeOp>& tape) const;
+  void reverse_sweep(const std::vector<TapeOp>& tape, bool integrals);
+  void reverse_op(const TapeOp& op, bool integrals);
+  void energy_t_gradient(double* g1, double* g2);
+  std::vector<TapeOp>* tape_ = nullptr;  // ops are appended here when set
+  bool tape_execute_ = true;             // false: record without launching
+  std::vector<TapeOp> prepare_tape_;     // recorded by prepare()
+  friend class CCSDBatch;
+  friend struct Adjoint;
 
   // ---- mixed precision: 16-bit planes of the operands of the large contractions
   bool split_mode() const;

@@ -6,9 +6,10 @@
 // are k-contiguous in HBM and in LDS, so a slice row is copied with 16-byte
 // loads and every MFMA fragment (8 consecutive k of one row) is one 16-byte
 // LDS read.  The matrix cores accumulate in FP32 over K chunks of at most
-// kMaxChunk; chunks are summed in FP64 by the split-K pass, which bounds the
-// FP32 accumulation length without FP64 registers in the kernel (two
-// workgroups fit per CU).  Workgroups are mapped to tiles XCD by XCD
+// kMaxChunk; after each chunk the workgroup adds its FP32 sums into the FP64
+// output (or its split-K partial), which bounds the FP32 accumulation length
+// without FP64 registers in the main loop (two workgroups fit per CU).  Split-K
+// is used only for parallelism on small outputs.  Workgroups are mapped to tiles XCD by XCD
 // (consecutive tiles on the same XCD share operand rows in that XCD's L2).
 #include "planes.hpp"
 #include "gemm.hpp"
@@ -255,7 +256,35 @@ __global__ __launch_bounds__(kThr, 2) void gemm_planes_kernel(PParams p) {
 #pragma unroll
     for (int j = 0; j < CB; ++j) facc[i][j] = float4_t{0.f, 0.f, 0.f, 0.f};
 
-  int buf = 0;
+  // FP32 sums of at most kMaxChunk values of k are added into the FP64 output
+  // (the first flush applies beta to C, or starts the split-K partial)
+  const double unscale = p.alpha / ((*p.sa) * (*p.sb));
+  double* __restrict__ C = p.C + (long)b * p.sC;
+  double* __restrict__ Wz = p.W + (long)z * p.M * p.N;
+  bool first_flush = true;
+  auto flush = [&]() {
+#pragma unroll
+    for (int i = 0; i < RB; ++i)
+#pragma unroll
+      for (int j = 0; j < CB; ++j) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int gm = m0 + wm * WM + i * 16 + lk * 4 + r;
+          const int gn = n0 + wn * WN + j * 16 + lr;
+          if (gm < p.M && gn < p.N) {
+            const double v = unscale * (double)facc[i][j][r];
+            double* c = (p.nsplit == 1) ? C + (long)gm * p.ldc + gn : Wz + (long)gm * p.N + gn;
+            if (!first_flush) *c += v;
+            else if (p.nsplit > 1 || p.beta == 0.0) *c = v;
+            else *c = v + p.beta * (*c);
+          }
+        }
+        facc[i][j] = float4_t{0.f, 0.f, 0.f, 0.f};
+      }
+    first_flush = false;
+  };
+
+  int buf = 0, since_flush = 0;
   if (k_begin < k_end) {
     load_regs(k_begin);
     store_lds(0);
@@ -279,32 +308,15 @@ __global__ __launch_bounds__(kThr, 2) void gemm_planes_kernel(PParams p) {
 #pragma unroll
       for (int j = 0; j < CB; ++j) mma3<NP, F16>(fa, fb[j], facc[i][j]);
     }
+    if (++since_flush == kMaxChunk / kBK && has_next) {
+      flush();
+      since_flush = 0;
+    }
     if (has_next) store_lds(buf ^ 1);
     __syncthreads();
     buf ^= 1;
   }
-
-  const double unscale = p.alpha / ((*p.sa) * (*p.sb));
-  double* __restrict__ C = p.C + (long)b * p.sC;
-  double* __restrict__ Wz = p.W + (long)z * p.M * p.N;
-#pragma unroll
-  for (int i = 0; i < RB; ++i)
-#pragma unroll
-    for (int j = 0; j < CB; ++j)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int gm = m0 + wm * WM + i * 16 + lk * 4 + r;
-        const int gn = n0 + wn * WN + j * 16 + lr;
-        if (gm < p.M && gn < p.N) {
-          const double v = unscale * (double)facc[i][j][r];
-          if (p.nsplit == 1) {
-            double* c = C + (long)gm * p.ldc + gn;
-            *c = (p.beta == 0.0) ? v : v + p.beta * (*c);
-          } else {
-            Wz[(long)gm * p.N + gn] = v;
-          }
-        }
-      }
+  flush();  // also covers an empty K range (writes beta C or a zero partial)
 }
 
 // Launch plan: tile edge, split-K factor and K chunk of one contraction.
@@ -330,8 +342,6 @@ PlanePlan plan_planes(int M, int N, int K, int batch) {
     nsplit = (int)std::min<long>(ceil_div(want, tiles * batch), K / (4 * kBK));
     nsplit = std::max(1, std::min(nsplit, 512));
   }
-  // FP32 accumulation length bounded by kMaxChunk: more chunks, summed in FP64
-  nsplit = std::max<int>(nsplit, (int)ceil_div(K, kMaxChunk));
   pl.kchunk = (int)(ceil_div(ceil_div(K, nsplit), kBK) * kBK);
   pl.nsplit = nsplit > 1 ? (int)ceil_div(K, pl.kchunk) : 1;
   return pl;

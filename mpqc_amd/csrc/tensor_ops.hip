@@ -164,6 +164,21 @@ struct LincombArgs {
   int nvec;
 };
 
+__global__ __launch_bounds__(kBlock) void scale_matrix_kernel(double* __restrict__ C, long total, int M,
+                                                               int N, long sC, long ldc, double beta) {
+  for (long e = (long)blockIdx.x * kBlock + threadIdx.x; e < total; e += (long)gridDim.x * kBlock) {
+    const long bm = e / N;
+    const long n = e - bm * N;
+    const long b = bm / M, m = bm - b * M;
+    double* c = C + b * sC + m * ldc + n;
+    *c = (beta == 0.0) ? 0.0 : beta * *c;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void fill_kernel(double* __restrict__ x, long n, double value) {
+  for (long e = (long)blockIdx.x * kBlock + threadIdx.x; e < n; e += (long)gridDim.x * kBlock) x[e] = value;
+}
+
 __global__ __launch_bounds__(kBlock) void lincomb_kernel(double* __restrict__ y, LincombArgs a, long n) {
   for (long e = (long)blockIdx.x * kBlock + threadIdx.x; e < n; e += (long)gridDim.x * kBlock) {
     double acc = 0.0;
@@ -341,6 +356,20 @@ void multi_dot2(const double* const* xs1, const double* const* xs2, int nvec, co
   hipLaunchKernelGGL(multidot_stage1, dim3(g), dim3(kBlock), 0, s, a, partials);
   hipLaunchKernelGGL(reduce_stage2, dim3(nvec), dim3(kBlock), 0, s, partials, g, out, 1.0);
   check_launch("multi_dot2");
+}
+
+void scale_matrix(double* C, int batch, int M, int N, long sC, long ldc, double beta, hipStream_t s) {
+  const long total = (long)batch * M * N;
+  if (total <= 0 || beta == 1.0) return;
+  hipLaunchKernelGGL(scale_matrix_kernel, dim3(elem_grid(total)), dim3(kBlock), 0, s, C, total, M, N, sC,
+                     ldc, beta);
+  check_launch("scale_matrix");
+}
+
+void fill(double* x, long n, double value, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(fill_kernel, dim3(elem_grid(n)), dim3(kBlock), 0, s, x, n, value);
+  check_launch("fill");
 }
 
 void lincomb(double* y, const double* const* xs, const double* c, int nvec, long n, hipStream_t s) {

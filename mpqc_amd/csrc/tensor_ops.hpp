@@ -54,6 +54,13 @@ void ccsd_energy(const double* f_ai, const double* r1, const double* t1, const d
 void multi_dot2(const double* const* xs1, const double* const* xs2, int nvec, const double* y1,
                 const double* y2, long n1, long n2, double* out, double* partials, hipStream_t s);
 
+// C_b[m][n] *= beta for b < batch, m < M, n < N (C_b = C + b*sC, row stride ldc);
+// beta = 0 stores zeros
+void scale_matrix(double* C, int batch, int M, int N, long sC, long ldc, double beta, hipStream_t s);
+
+// x[0..n) = value
+void fill(double* x, long n, double value, hipStream_t s);
+
 // y = sum_k c_k x_k, k < nvec (fixed summation order)
 void lincomb(double* y, const double* const* xs, const double* c, int nvec, long n, hipStream_t s);
 
