@@ -44,7 +44,8 @@ def _args():
     p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--workload", default="ccsd-h2o8",
-                   choices=["ccsd-h2o8", "ccsd-h2o6", "ccsd-h20", "pp-ladder", "copy"])
+                   choices=["ccsd-h2o8", "ccsd-h2o6", "ccsd-h20", "pp-ladder", "copy",
+                            "batched-hchains", "adjoint-h2o6"])
     p.add_argument("--forms", default="f64,f16x3",
                    help="contraction precisions to run for the CCSD workloads (first = reference)")
     p.add_argument("--max-iter", type=int, default=40)
@@ -54,6 +55,10 @@ def _args():
     p.add_argument("--o", type=int, default=32)
     p.add_argument("--v", type=int, default=256)
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--batch-count", type=int, default=256, help="chains in the batched workload")
+    p.add_argument("--batch-mem-gib", type=float, default=150.0,
+                   help="HBM budget per group of chains in the batched workload")
+    p.add_argument("--batch-precision", default="f64", choices=["f64", "f32"])
     p.add_argument("--cpu-sample-rows", type=int, default=8192)
     return p.parse_args()
 
@@ -355,11 +360,230 @@ def bench_copy(args):
             "bytes_written_per_step": n * 8, "steps": args.steps, "warmup": args.warmup}
 
 
+def _chain_sizes(count):
+    """(o, v) of the seeded H-chain batch: n even uniform in [8, 32], spacing
+    U[1.6, 2.0] bohr, cc-pVDZ (5 functions per H): o = n/2, v = 4.5 n."""
+    from mpqc_amd.chem.molecule import h_chain_batch
+    return [(len(m.symbols) // 2, 9 * len(m.symbols) // 2) for m in h_chain_batch(count=count, seed=3)]
+
+
+def _context_doubles(o, v):
+    """Device doubles one residual context holds (integrals, derived layouts,
+    amplitudes, intermediates)."""
+    ov, oo, vv = o * v, o * o, v * v
+    return vv * vv + 3 * ov * vv + 35 * ov * ov + 10 * oo * ov + 3 * oo * oo + 64 * max(vv, ov, oo * oo, 4096)
+
+
+def _synthetic_blocks(o, v, gen):
+    """Seeded random integral blocks of the reference shapes on the device
+    (Fock diagonals e_occ in [-1.5, -0.3], e_vir in [0.1, 3.0])."""
+    import torch
+    dev = "cuda"
+    shapes = {"Fia": (o, v), "Fij": (o, o), "Fab": (v, v), "Gijab": (o, o, v, v),
+              "Gijkl": (o, o, o, o), "Giajb": (o, v, o, v), "Giabc": (o, v, v, v),
+              "Gijka": (o, o, o, v), "Gabcd": (v, v, v, v)}
+    out = {}
+    for name, shp in shapes.items():
+        out[name] = 0.01 * torch.randn(shp, dtype=torch.float64, device=dev, generator=gen)
+    eo = -0.3 - 1.2 * torch.rand(o, dtype=torch.float64, device=dev, generator=gen)
+    ev = 0.1 + 2.9 * torch.rand(v, dtype=torch.float64, device=dev, generator=gen)
+    out["Fij"] += torch.diag(eo)
+    out["Fab"] += torch.diag(ev)
+    return out
+
+
+def bench_batched(args):
+    """One residual for each of the seeded H chains, all molecules of a group in
+    grouped launches (one hipGraph), groups sized to the HBM budget."""
+    import ctypes
+    import torch
+    from mpqc_amd import _lib as L
+    from mpqc_amd.batch import CCSDBatch
+    from oracle.ccsd import ccsd_flops
+    sizes = _chain_sizes(args.batch_count)
+    budget = args.batch_mem_gib * (1 << 30) / 8
+    order = sorted(range(len(sizes)), key=lambda k: sizes[k])
+    groups, cur, cur_mem = [], [], 0.0
+    for k in order:
+        need = _context_doubles(*sizes[k])
+        if cur and cur_mem + need > budget:
+            groups.append(cur)
+            cur, cur_mem = [], 0.0
+        cur.append(k)
+        cur_mem += need
+    groups.append(cur)
+    log(f"batched: {len(sizes)} chains (o {min(s[0] for s in sizes)}-{max(s[0] for s in sizes)}, "
+        f"v {min(s[1] for s in sizes)}-{max(s[1] for s in sizes)}) in {len(groups)} groups")
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(3)
+    total_ms, total_flops, tune_s = 0.0, 0.0, 0.0
+    stage_totals = {}
+    for gi, grp in enumerate(groups):
+        os_ = [sizes[k][0] for k in grp]
+        vs_ = [sizes[k][1] for k in grp]
+        with CCSDBatch(os_, vs_, precision=args.batch_precision) as B:
+            for j, k in enumerate(grp):
+                o, v = sizes[k]
+                blocks = _synthetic_blocks(o, v, gen)
+                for name, idx in L.INTEGRAL_IDS.items():
+                    t = blocks[name].contiguous()
+                    B._check(B.lib.mpqc_ccsd_batch_set_integral(B.h, j, idx, ctypes.c_void_p(t.data_ptr()),
+                                                                L.LAYOUT_DENSE, L.MEM_DEVICE))
+                t1 = 0.01 * torch.randn((v, o), dtype=torch.float64, device="cuda", generator=gen)
+                t2 = 0.01 * torch.randn((v, v, o, o), dtype=torch.float64, device="cuda", generator=gen)
+                B._check(B.lib.mpqc_ccsd_batch_set_amplitudes(B.h, j, ctypes.c_void_p(t1.data_ptr()),
+                                                              ctypes.c_void_p(t2.data_ptr()),
+                                                              L.LAYOUT_DENSE, L.MEM_DEVICE))
+                del blocks, t1, t2
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            B.tune(reps=1)
+            tune_s += time.perf_counter() - t0
+            ms = B.time(reps=args.steps)
+            B.time_stages(reps=1)
+            for st in B.stages():
+                agg = stage_totals.setdefault(st["tag"], {"ms": 0.0, "flops": 0.0, "kind": st["kind"]})
+                agg["ms"] += st["ms"]
+                agg["flops"] += st["flops"]
+            fl = sum(ccsd_flops(o, v) for o, v in zip(os_, vs_))
+            log(f"group {gi}: {len(grp)} chains, {ms:.2f} ms per batched residual, "
+                f"{fl / (ms * 1e-3) / 1e12:.2f} TFLOP/s")
+            total_ms += ms
+            total_flops += fl
+        torch.cuda.empty_cache()
+    tflops = total_flops / (total_ms * 1e-3) / 1e12
+    pp = stage_totals.get("r2:pp_ladder", {"ms": float("nan"), "flops": float("nan")})
+    pp_tf = pp["flops"] / (pp["ms"] * 1e-3) / 1e12
+    top = sorted(stage_totals.items(), key=lambda kv: -kv[1]["ms"])[:12]
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline_batched(sizes)
+    peak = PEAK[args.batch_precision]
+    line = {
+        "metric": "batched CCSD residuals: wall time + TFLOP/s (frac MFMA peak)",
+        "value": round(tflops, 3), "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": 1, "ms_per_step": round(total_ms, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": args.batch_precision,
+        "data": "synthetic integrals of the chains' shapes (torch CUDA generator seed 3); "
+                "chain sizes from the seeded generator (numpy PCG64 seed 3)",
+        "config": {"workload": f"{len(sizes)} H chains n=8..32 cc-pVDZ (o=n/2, v=4.5n), one residual each",
+                   "groups": len(groups), "algorithmic_flops": total_flops, "parallelism": "single-gpu",
+                   "graph": "one hipGraph per group", "autotune_s": round(tune_s, 2)},
+        "roofline": {"bound": "mfma", "achieved": round(pp_tf, 3), "peak": peak, "unit": "TFLOP/s",
+                     "frac": round(pp_tf / peak, 4), "traffic": None,
+                     "kernel": "grouped pp-ladder contraction over all chains"},
+        "stages_top": {k: {"ms": round(v_["ms"], 3), "kind": v_["kind"],
+                           "tflops": round(v_["flops"] / max(v_["ms"], 1e-9) / 1e9, 2)}
+                       for k, v_ in top},
+        "cpu_baseline": cpu,
+    }
+    return line
+
+
+def cpu_baseline_batched(sizes, budget_s=10.0):
+    """Oracle residuals (numpy/OpenBLAS) of the smallest chains of the batch, as
+    many as fit in about budget_s seconds."""
+    import numpy as np
+    from oracle import ccsd as O
+    rng = np.random.default_rng(3)
+    done, flops, t_used = 0, 0.0, 0.0
+    for o, v in sorted(sizes):
+        blocks = {"Fia": (o, v), "Fij": (o, o), "Fab": (v, v), "Gijab": (o, o, v, v), "Gijkl": (o, o, o, o),
+                  "Giajb": (o, v, o, v), "Giabc": (o, v, v, v), "Gijka": (o, o, o, v), "Gabcd": (v, v, v, v)}
+        b = {k: 0.01 * rng.normal(size=s) for k, s in blocks.items()}
+        ints = O.Integrals(**b)
+        t1 = 0.01 * rng.normal(size=(v, o))
+        t2 = 0.01 * rng.normal(size=(v, v, o, o))
+        t0 = time.perf_counter()
+        tau = O.make_tau(t1, t2)
+        O.cs_ccsd_r1(t1, t2, tau, ints)
+        O.cs_ccsd_r2(t1, t2, tau, ints)
+        t_used += time.perf_counter() - t0
+        flops += O.ccsd_flops(o, v)
+        done += 1
+        if t_used > budget_s:
+            break
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": flops / t_used / 1e12, "unit": "TFLOP/s", "cores": threads, "kind": "port",
+            "sample": f"oracle residuals of the {done} smallest chains ({flops:.3e} FLOPs in {t_used:.1f} s)"}
+
+
+def bench_adjoint(args):
+    """(H2O)6 cc-pVDZ: converged CCSD, Lambda equations and dE/dF for a field
+    along z (f + F mu, F = 1e-4 a.u., fixed orbitals), checked against a
+    four-point central difference of the converged energy."""
+    import numpy as np
+    import torch
+    from mpqc_amd.adjoint import ccsd_energy, mo_dipole_blocks
+    from mpqc_amd.ccsd import CCSD, Integrals
+    from mpqc_amd.chem.molecule import water_cluster
+    from mpqc_amd.chem.scf import rhf
+    from mpqc_amd.chem.mo import ccsd_integrals
+    from oracle.ccsd import ccsd_flops
+    mol = water_cluster((3, 2, 1), seed=5)
+    t0 = time.perf_counter()
+    scf = rhf(mol, "cc-pVDZ")
+    inp = ccsd_integrals(scf, frozen_core=False)
+    mu = mo_dipole_blocks(scf, axis=2)
+    log(f"adjoint (H2O)6: o={inp.o} v={inp.v}, set-up {time.perf_counter() - t0:.1f}s")
+    F0, prec = 1e-4, 1e-12
+
+    def blocks_at(Fv):
+        b = dict(inp.blocks)
+        for n in ("Fia", "Fij", "Fab"):
+            b[n] = inp.blocks[n] + Fv * mu[n]
+        return b
+
+    forms = {}
+    for form in ("f64", "f32"):
+        F = torch.tensor(F0, dtype=torch.float64, requires_grad=True)
+        tb = {n: torch.from_numpy(np.array(a)) for n, a in inp.blocks.items()}
+        for n in ("Fia", "Fij", "Fab"):
+            tb[n] = tb[n] + F * torch.from_numpy(mu[n])
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        E = ccsd_energy(tb, inp.eps_occ, inp.eps_vir, precision=form, target_precision=prec,
+                        lambda_precision=1e-13, max_iter=100)
+        t2 = time.perf_counter()
+        E.backward()
+        t3 = time.perf_counter()
+        grad = F.grad.item()
+
+        def e_at(Fv):
+            with CCSD(inp.o, inp.v, precision=form, max_iter=100) as c:
+                c.set_integrals(Integrals(**blocks_at(Fv)))
+                c.set_orbital_energies(inp.eps_occ, inp.eps_vir)
+                return c.compute_ccsd(target_precision=prec)
+
+        h = 5e-4
+        fd = (8 * (e_at(F0 + h) - e_at(F0 - h)) - (e_at(F0 + 2 * h) - e_at(F0 - 2 * h))) / (12 * h)
+        forms[form] = {"E_corr": E.item(), "dE_dF_adjoint": grad, "dE_dF_central_difference": fd,
+                       "rel_err": abs(grad - fd) / abs(fd), "solve_s": round(t2 - t1, 3),
+                       "lambda_and_gradient_s": round(t3 - t2, 3)}
+        log(f"{form}: dE/dF adjoint {grad:.10e} difference {fd:.10e} rel {abs(grad - fd) / abs(fd):.2e}; "
+            f"solve {t2 - t1:.2f}s, Lambda + gradient {t3 - t2:.2f}s")
+    f = forms["f64"]
+    line = {
+        "metric": "adjoint dE/dF vs central differences (relative error) and Lambda time",
+        "value": f["rel_err"], "unit": "relative error", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": round(1e3 * f["lambda_and_gradient_s"], 3), "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "(H2O)6 seeded geometry, RHF + AO->MO on the host, field along z",
+        "config": {"workload": "(H2O)6 cc-pVDZ o=30 v=114, F_z = 1e-4 a.u.", "o": inp.o, "v": inp.v,
+                   "algorithmic_flops_per_iteration": ccsd_flops(inp.o, inp.v),
+                   "parallelism": "single-gpu"},
+        "forms": forms,
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    return line
+
+
 def main():
     args = _args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is "
-                                   "designed but not built (see DESIGN.md)", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this library runs on one GPU; sharding across GPUs is "
+                                   "designed but not built", "n_gpus": args.gpus}))
         sys.exit(2)
     import torch
     if not torch.cuda.is_available():
@@ -369,6 +593,10 @@ def main():
         line = bench_copy(args)
     elif args.workload == "pp-ladder":
         line = bench_pp_ladder(args)
+    elif args.workload == "batched-hchains":
+        line = bench_batched(args)
+    elif args.workload == "adjoint-h2o6":
+        line = bench_adjoint(args)
     else:
         line = bench_ccsd(args)
     print(json.dumps(line), flush=True)

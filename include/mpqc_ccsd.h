@@ -181,6 +181,34 @@ int mpqc_ccsd_energy_gradient(mpqc_ccsd_ctx* ctx);
 int mpqc_ccsd_get_integral_gradient(mpqc_ccsd_ctx* ctx, int which, double* out, int layout,
                                     int memspace);
 
+/* ---- batched residuals of many independent molecules ----
+ * One context per molecule (o[k], v[k]; one tile per mode, so the tile layout is
+ * the dense layout), residuals cc::compute_cs_ccsd_r1 + r2 (ccsd_r1_r2.h:70-293)
+ * of all molecules in grouped launches captured in one hipGraph.  The reference
+ * evaluates molecules one at a time; there is no batched counterpart. */
+typedef struct mpqc_ccsd_batch mpqc_ccsd_batch;
+int mpqc_ccsd_batch_create(int n_mol, const int* o, const int* v, int precision,
+                           mpqc_ccsd_batch** out);
+int mpqc_ccsd_batch_destroy(mpqc_ccsd_batch* b);
+const char* mpqc_ccsd_batch_last_error(const mpqc_ccsd_batch* b);
+int mpqc_ccsd_batch_set_integral(mpqc_ccsd_batch* b, int mol, int which, const double* data,
+                                 int layout, int memspace);
+int mpqc_ccsd_batch_set_amplitudes(mpqc_ccsd_batch* b, int mol, const double* t1, const double* t2,
+                                   int layout, int memspace);
+int mpqc_ccsd_batch_residual(mpqc_ccsd_batch* b);
+int mpqc_ccsd_batch_get_residuals(mpqc_ccsd_batch* b, int mol, double* r1, double* r2, int layout,
+                                  int memspace);
+/* Autotuner: times each candidate tile edge / split-K target of every grouped
+ * contraction on this batch, keeps the fastest, re-captures the graph; *ms gets
+ * the batched residual time afterwards. */
+int mpqc_ccsd_batch_tune(mpqc_ccsd_batch* b, int reps, int verbose, double* ms);
+int mpqc_ccsd_batch_time(mpqc_ccsd_batch* b, int reps, double* ms);
+int mpqc_ccsd_batch_time_stages(mpqc_ccsd_batch* b, int reps);
+/* Per grouped launch: tag (64 bytes each), op kind, tile edge, split-K target,
+ * last measured ms, contraction FLOPs. */
+int mpqc_ccsd_batch_get_stages(mpqc_ccsd_batch* b, int max_rows, int* n_rows, char* tags, int* kinds,
+                               int* bm, int64_t* target, double* ms, double* flops);
+
 /* Device-pointer kernel entry points (stream may be NULL for the null stream). */
 int mpqc_ccsd_pp_ladder(int o, int v, const double* gabcd_dev, const double* tau_dev,
                         double* r2_dev, double alpha, double beta, int precision, void* stream);

@@ -31,6 +31,10 @@ EXPORTS = [
     "mpqc_ccsd_get_trace", "mpqc_ccsd_pp_ladder", "mpqc_ccsd_gemm",
     "mpqc_ccsd_get_kernel_times", "mpqc_ccsd_jacobian_transpose", "mpqc_ccsd_lambda_solve",
     "mpqc_ccsd_get_lambda", "mpqc_ccsd_energy_gradient", "mpqc_ccsd_get_integral_gradient",
+    "mpqc_ccsd_batch_create", "mpqc_ccsd_batch_destroy", "mpqc_ccsd_batch_last_error",
+    "mpqc_ccsd_batch_set_integral", "mpqc_ccsd_batch_set_amplitudes", "mpqc_ccsd_batch_residual",
+    "mpqc_ccsd_batch_get_residuals", "mpqc_ccsd_batch_tune", "mpqc_ccsd_batch_time",
+    "mpqc_ccsd_batch_time_stages", "mpqc_ccsd_batch_get_stages",
 ]
 
 _lib = None
@@ -84,6 +88,17 @@ def load():
         "mpqc_ccsd_get_lambda": (I, [P, P, P, I, I]),
         "mpqc_ccsd_energy_gradient": (I, [P]),
         "mpqc_ccsd_get_integral_gradient": (I, [P, I, P, I, I]),
+        "mpqc_ccsd_batch_create": (I, [I, pI, pI, I, ctypes.POINTER(P)]),
+        "mpqc_ccsd_batch_destroy": (I, [P]),
+        "mpqc_ccsd_batch_last_error": (ctypes.c_char_p, [P]),
+        "mpqc_ccsd_batch_set_integral": (I, [P, I, I, P, I, I]),
+        "mpqc_ccsd_batch_set_amplitudes": (I, [P, I, P, P, I, I]),
+        "mpqc_ccsd_batch_residual": (I, [P]),
+        "mpqc_ccsd_batch_get_residuals": (I, [P, I, P, P, I, I]),
+        "mpqc_ccsd_batch_tune": (I, [P, I, I, pD]),
+        "mpqc_ccsd_batch_time": (I, [P, I, pD]),
+        "mpqc_ccsd_batch_time_stages": (I, [P, I]),
+        "mpqc_ccsd_batch_get_stages": (I, [P, I, pI, P, P, P, P, P, P]),
         "mpqc_ccsd_gemm": (I, [I, I, I, I, I, D, P, I64, I64, P, I64, I64, D, P, I64, I64, I, I, P]),
     }
     for name, (res, args) in sig.items():

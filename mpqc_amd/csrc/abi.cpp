@@ -10,7 +10,13 @@
 #include <string>
 
 #include "../../include/mpqc_ccsd.h"
+#include "batch.hpp"
 #include "ccsd_ctx.hpp"
+
+struct mpqc_ccsd_batch {
+  mpqc_hip::CCSDBatch* impl = nullptr;
+  std::string err;
+};
 
 struct mpqc_ccsd_ctx {
   mpqc_hip::CCSDContext* impl = nullptr;
@@ -51,6 +57,24 @@ int guarded(mpqc_ccsd_ctx* ctx, F&& f) {
     return MPQC_CCSD_ERR_NOMEM;
   } catch (const std::exception& e) {
     ctx->err = e.what();
+    return status_of(e);
+  }
+}
+
+template <typename F>
+int guarded_batch(mpqc_ccsd_batch* b, F&& f) {
+  if (b == nullptr || b->impl == nullptr) {
+    g_last_error = "null batch";
+    return MPQC_CCSD_ERR_ARG;
+  }
+  try {
+    f(*b->impl);
+    return MPQC_CCSD_OK;
+  } catch (const std::bad_alloc&) {
+    b->err = "device or host allocation failed";
+    return MPQC_CCSD_ERR_NOMEM;
+  } catch (const std::exception& e) {
+    b->err = e.what();
     return status_of(e);
   }
 }
@@ -249,6 +273,104 @@ int mpqc_ccsd_get_integral_gradient(mpqc_ccsd_ctx* ctx, int which, double* out, 
                                     int memspace) {
   return guarded(ctx, [&](mpqc_hip::CCSDContext& c) {
     c.get_integral_gradient(which, out, layout, memspace);
+  });
+}
+
+int mpqc_ccsd_batch_create(int n_mol, const int* o, const int* v, int precision,
+                           mpqc_ccsd_batch** out) {
+  if (out == nullptr) {
+    g_last_error = "null output pointer";
+    return MPQC_CCSD_ERR_ARG;
+  }
+  *out = nullptr;
+  return guarded_free([&] {
+    auto* b = new mpqc_ccsd_batch;
+    try {
+      b->impl = new mpqc_hip::CCSDBatch(n_mol, o, v, precision);
+    } catch (...) {
+      delete b;
+      throw;
+    }
+    *out = b;
+  });
+}
+
+int mpqc_ccsd_batch_destroy(mpqc_ccsd_batch* b) {
+  if (b == nullptr) return MPQC_CCSD_OK;
+  delete b->impl;
+  delete b;
+  return MPQC_CCSD_OK;
+}
+
+const char* mpqc_ccsd_batch_last_error(const mpqc_ccsd_batch* b) {
+  if (b == nullptr) return g_last_error.c_str();
+  return b->err.c_str();
+}
+
+int mpqc_ccsd_batch_set_integral(mpqc_ccsd_batch* b, int mol, int which, const double* data,
+                                 int layout, int memspace) {
+  return guarded_batch(b, [&](mpqc_hip::CCSDBatch& x) {
+    x.mol(mol).set_integral(which, data, layout, memspace);
+  });
+}
+
+int mpqc_ccsd_batch_set_amplitudes(mpqc_ccsd_batch* b, int mol, const double* t1, const double* t2,
+                                   int layout, int memspace) {
+  return guarded_batch(b, [&](mpqc_hip::CCSDBatch& x) {
+    if (!t1 || !t2) throw std::invalid_argument("null amplitude buffer");
+    x.mol(mol).set_amplitudes(t1, t2, layout, memspace);
+  });
+}
+
+int mpqc_ccsd_batch_residual(mpqc_ccsd_batch* b) {
+  return guarded_batch(b, [&](mpqc_hip::CCSDBatch& x) { x.residual(); });
+}
+
+int mpqc_ccsd_batch_get_residuals(mpqc_ccsd_batch* b, int mol, double* r1, double* r2, int layout,
+                                  int memspace) {
+  return guarded_batch(b, [&](mpqc_hip::CCSDBatch& x) {
+    x.mol(mol).get_residuals(r1, r2, layout, memspace);
+  });
+}
+
+int mpqc_ccsd_batch_tune(mpqc_ccsd_batch* b, int reps, int verbose, double* ms) {
+  return guarded_batch(b, [&](mpqc_hip::CCSDBatch& x) {
+    const double t = x.tune(reps, verbose);
+    if (ms) *ms = t;
+  });
+}
+
+int mpqc_ccsd_batch_time(mpqc_ccsd_batch* b, int reps, double* ms) {
+  return guarded_batch(b, [&](mpqc_hip::CCSDBatch& x) {
+    const double t = x.time(reps);
+    if (ms) *ms = t;
+  });
+}
+
+int mpqc_ccsd_batch_time_stages(mpqc_ccsd_batch* b, int reps) {
+  return guarded_batch(b, [&](mpqc_hip::CCSDBatch& x) {
+    if (reps < 1) throw std::invalid_argument("reps must be positive");
+    x.time_stages(reps);
+  });
+}
+
+int mpqc_ccsd_batch_get_stages(mpqc_ccsd_batch* b, int max_rows, int* n_rows, char* tags, int* kinds,
+                               int* bm, int64_t* target, double* ms, double* flops) {
+  return guarded_batch(b, [&](mpqc_hip::CCSDBatch& x) {
+    const auto st = x.stages();
+    const int n = (int)std::min<size_t>(st.size(), (size_t)std::max(0, max_rows));
+    for (int k = 0; k < n; ++k) {
+      if (tags) {
+        std::memset(tags + 64 * k, 0, 64);
+        std::strncpy(tags + 64 * k, st[k].tag.c_str(), 63);
+      }
+      if (kinds) kinds[k] = st[k].kind;
+      if (bm) bm[k] = st[k].bm;
+      if (target) target[k] = st[k].target;
+      if (ms) ms[k] = st[k].ms;
+      if (flops) flops[k] = st[k].flops;
+    }
+    if (n_rows) *n_rows = (int)st.size();
   });
 }
 

@@ -89,10 +89,10 @@ double colpiv_qr_solve(int n, std::vector<double> A, std::vector<double> b, std:
 
 // ---------------------------------------------------------------- lifecycle
 CCSDContext::CCSDContext(int o, int v, const int64_t* occ_hm, int n_occ_tiles, const int64_t* vir_hm,
-                         int n_vir_tiles, int precision)
+                         int n_vir_tiles, int precision, bool with_solver)
     : o_(o), v_(v) {
   try {
-    construct(occ_hm, n_occ_tiles, vir_hm, n_vir_tiles, precision);
+    construct(occ_hm, n_occ_tiles, vir_hm, n_vir_tiles, precision, with_solver);
   } catch (...) {
     release();
     throw;
@@ -130,7 +130,7 @@ void CCSDContext::release() {
 }
 
 void CCSDContext::construct(const int64_t* occ_hm, int n_occ_tiles, const int64_t* vir_hm,
-                            int n_vir_tiles, int precision) {
+                            int n_vir_tiles, int precision, bool with_solver) {
   const int o = o_, v = v_;
   if (o <= 0 || v <= 0) throw std::invalid_argument("o and v must be positive");
   if ((long)v * v * o * o >= (1L << 31)) throw std::invalid_argument("v^2 o^2 must be < 2^31");
@@ -176,7 +176,7 @@ void CCSDContext::construct(const int64_t* occ_hm, int n_occ_tiles, const int64_
   ws_.bytes = ws_elems * sizeof(double);
   partials_ = alloc(kReducePartials);
   scalars_ = alloc(64);
-  solver_init(1, 8, 0.0, 1, 1);
+  if (with_solver) solver_init(1, 8, 0.0, 1, 1);
   MPQC_HIP_CHECK(hipMemsetAsync(t1_, 0, ov * sizeof(double), stream_));
   MPQC_HIP_CHECK(hipMemsetAsync(t2_, 0, oovv * sizeof(double), stream_));
   MPQC_HIP_CHECK(hipStreamSynchronize(stream_));

@@ -30,8 +30,9 @@ double colpiv_qr_solve(int n, std::vector<double> A, std::vector<double> b, std:
 
 class CCSDContext {
  public:
+  // with_solver = false skips the DIIS history (contexts that only evaluate residuals)
   CCSDContext(int o, int v, const int64_t* occ_hm, int n_occ_tiles, const int64_t* vir_hm,
-              int n_vir_tiles, int precision);
+              int n_vir_tiles, int precision, bool with_solver = true);
   ~CCSDContext();
 
   int o() const { return o_; }
@@ -93,7 +94,7 @@ class CCSDContext {
   // issues is checked before launch.
   void check_access(const double* p, long max_off, const char* what) const;
   void construct(const int64_t* occ_hm, int n_occ_tiles, const int64_t* vir_hm, int n_vir_tiles,
-                 int precision);
+                 int precision, bool with_solver);
   void release();
   void prepare();
   void gemm(bool tA, bool tB, int M, int N, int K, double alpha, const double* A, long lda,

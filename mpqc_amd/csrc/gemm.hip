@@ -32,6 +32,7 @@ struct KParams {
   int M, N, K;
   int kchunk;
   int nsplit;
+  int nsplit_batch;  // batch entries of a grouped problem (reduce pass)
   int batch0;  // first batch index of this launch
   int tiles_m, tiles_n;
   int per_xcd;  // tiles per XCD (workgroup w runs on XCD w % 8)
@@ -49,8 +50,11 @@ __device__ inline int xcd_tile(const KParams& p) {
   return t < p.tiles_m * p.tiles_n ? t : -1;
 }
 
+// One BM x BN output tile `tile` (row-major tile index) of batch/split slice z
+// (z = batch * nsplit + split, batch relative to p.batch0).  Called by all 256
+// threads of a workgroup.
 template <int BM, int BN, bool TA, bool TB>
-__global__ __launch_bounds__(kThreads, 2) void dgemm_f64_kernel(KParams p) {
+__device__ __forceinline__ void dgemm_f64_tile(const KParams& p, int tile, int z) {
   constexpr int PAD = 1;
   constexpr int LDA_S = BM + PAD;
   constexpr int LDB_S = BN + PAD;
@@ -68,10 +72,7 @@ __global__ __launch_bounds__(kThreads, 2) void dgemm_f64_kernel(KParams p) {
   const int wm = wave >> 1, wn = wave & 1;
   const int lr = lane & 15, lk = lane >> 4;
 
-  const int tile = xcd_tile(p);
-  if (tile < 0) return;
   const int tm = tile / p.tiles_n, tn = tile - tm * p.tiles_n;
-  const int z = blockIdx.y;
   const int b = p.batch0 + z / p.nsplit;
   const int s = z - (z / p.nsplit) * p.nsplit;
   const int m0 = tm * BM, n0 = tn * BN;
@@ -188,6 +189,13 @@ __global__ __launch_bounds__(kThreads, 2) void dgemm_f64_kernel(KParams p) {
   }
 }
 
+template <int BM, int BN, bool TA, bool TB>
+__global__ __launch_bounds__(kThreads, 2) void dgemm_f64_kernel(KParams p) {
+  const int tile = xcd_tile(p);
+  if (tile < 0) return;
+  dgemm_f64_tile<BM, BN, TA, TB>(p, tile, blockIdx.y);
+}
+
 // FP32 path: operands are rounded to FP32 when staged into LDS and multiplied
 // on v_mfma_f32_16x16x4_f32 (same operand fragments as the FP64 instruction;
 // accumulator layout C[m = 4*(l/16) + r][n = l%16]).  Partial sums are kept in
@@ -195,7 +203,7 @@ __global__ __launch_bounds__(kThreads, 2) void dgemm_f64_kernel(KParams p) {
 // FP64 accumulators, so long contractions (K = v^2) lose no precision to FP32
 // accumulation.
 template <int BM, int BN, bool TA, bool TB>
-__global__ __launch_bounds__(kThreads, 2) void sgemm_f32acc64_kernel(KParams p) {
+__device__ __forceinline__ void sgemm_f32acc64_tile(const KParams& p, int tile, int z) {
   constexpr int PAD = 1;
   constexpr int LDA_S = BM + PAD;
   constexpr int LDB_S = BN + PAD;
@@ -214,10 +222,7 @@ __global__ __launch_bounds__(kThreads, 2) void sgemm_f32acc64_kernel(KParams p) 
   const int wm = wave >> 1, wn = wave & 1;
   const int lr = lane & 15, lk = lane >> 4;
 
-  const int tile = xcd_tile(p);
-  if (tile < 0) return;
   const int tm = tile / p.tiles_n, tn = tile - tm * p.tiles_n;
-  const int z = blockIdx.y;
   const int b = p.batch0 + z / p.nsplit;
   const int s = z - (z / p.nsplit) * p.nsplit;
   const int m0 = tm * BM, n0 = tn * BN;
@@ -337,6 +342,13 @@ __global__ __launch_bounds__(kThreads, 2) void sgemm_f32acc64_kernel(KParams p) 
       }
 }
 
+template <int BM, int BN, bool TA, bool TB>
+__global__ __launch_bounds__(kThreads, 2) void sgemm_f32acc64_kernel(KParams p) {
+  const int tile = xcd_tile(p);
+  if (tile < 0) return;
+  sgemm_f32acc64_tile<BM, BN, TA, TB>(p, tile, blockIdx.y);
+}
+
 // C[b] = beta*C[b] + sum_s W[b*nsplit + s]   (fixed summation order)
 __global__ void splitk_reduce_kernel(const double* __restrict__ W, double* __restrict__ C, long ldc,
                                      long sC, int M, int N, int nsplit, int batch0, double beta) {
@@ -431,7 +443,186 @@ void launch(const GemmDesc& d, GemmPrecision prec, const GemmWorkspace& ws, hipS
   }
 }
 
+// ---------------------------------------------------------------- grouped
+struct GProblem {
+  KParams p;
+  long first;   // first contraction work item
+  long rfirst;  // first reduce chunk (problems with nsplit > 1)
+};
+
+__device__ inline int find_problem(const GProblem* __restrict__ probs, int nprob, long it, bool reduce) {
+  int lo = 0, hi = nprob - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if ((reduce ? probs[mid].rfirst : probs[mid].first) <= it) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <int BM, int BN, bool TA, bool TB, bool F32>
+__global__ __launch_bounds__(kThreads, 2) void grouped_gemm_kernel(const GProblem* __restrict__ probs,
+                                                                   int nprob, long items) {
+  for (long it = blockIdx.x; it < items; it += gridDim.x) {
+    const int q = find_problem(probs, nprob, it, false);
+    const KParams& p = probs[q].p;
+    const long local = it - probs[q].first;
+    const int ntiles = p.tiles_m * p.tiles_n;
+    const int z = (int)(local / ntiles);
+    const int tile = (int)(local - (long)z * ntiles);
+    if (F32) sgemm_f32acc64_tile<BM, BN, TA, TB>(p, tile, z);
+    else dgemm_f64_tile<BM, BN, TA, TB>(p, tile, z);
+    __syncthreads();  // the next item reuses the LDS tiles
+  }
+}
+
+// C[b] = beta C[b] + sum_s W[b*nsplit + s] for every problem with nsplit > 1;
+// a chunk is 256 consecutive elements of one problem's batch x M x N output.
+__global__ __launch_bounds__(256) void grouped_reduce_kernel(const GProblem* __restrict__ probs,
+                                                             int nprob, long chunks) {
+  for (long ch = blockIdx.x; ch < chunks; ch += gridDim.x) {
+    const int q = find_problem(probs, nprob, ch, true);
+    const KParams& p = probs[q].p;
+    const long MN = (long)p.M * p.N;
+    const long e = (ch - probs[q].rfirst) * 256 + threadIdx.x;
+    if (e >= MN * (long)p.nsplit_batch) continue;
+    const long b = e / MN, mn = e - b * MN;
+    const double* w = p.W + b * p.nsplit * MN + mn;
+    double acc = 0.0;
+    for (int s = 0; s < p.nsplit; ++s) acc += w[(long)s * MN];
+    const long m = mn / p.N, n = mn - m * p.N;
+    double* c = p.C + b * p.sC + m * p.ldc + n;
+    *c = (p.beta == 0.0) ? acc : acc + p.beta * (*c);
+  }
+}
+
+template <int BM>
+void launch_grouped_bm(const GroupedGemmPlan& pl, hipStream_t stream) {
+  const GProblem* d = (const GProblem*)pl.dev;
+  const dim3 grid((unsigned)pl.grid);
+#define MPQC_GROUPED(TA, TB)                                                                       \
+  if (pl.prec == GemmPrecision::F32)                                                              \
+    hipLaunchKernelGGL((grouped_gemm_kernel<BM, BM, TA, TB, true>), grid, dim3(kThreads), 0, stream, \
+                       d, pl.nprob, pl.items);                                                     \
+  else                                                                                            \
+    hipLaunchKernelGGL((grouped_gemm_kernel<BM, BM, TA, TB, false>), grid, dim3(kThreads), 0, stream, \
+                       d, pl.nprob, pl.items);
+  if (!pl.transA && !pl.transB) { MPQC_GROUPED(false, false) }
+  else if (!pl.transA && pl.transB) { MPQC_GROUPED(false, true) }
+  else if (pl.transA && !pl.transB) { MPQC_GROUPED(true, false) }
+  else { MPQC_GROUPED(true, true) }
+#undef MPQC_GROUPED
+  check_launch("grouped_gemm");
+}
+
+struct GroupedSplit {
+  int nsplit, kchunk;
+};
+
+// split-K per problem: enough items for about target_items in total, at least
+// four K slices per split
+std::vector<GroupedSplit> grouped_splits(const std::vector<GemmDesc>& probs, int bm, long target) {
+  long tiles_total = 0;
+  for (const GemmDesc& d : probs)
+    tiles_total += (long)ceil_div(d.M, bm) * ceil_div(d.N, bm) * d.batch;
+  const long factor = tiles_total >= target ? 1 : ceil_div(target, std::max(1L, tiles_total));
+  std::vector<GroupedSplit> out;
+  for (const GemmDesc& d : probs) {
+    int nsplit = (int)std::max(1L, std::min<long>(std::min<long>(factor, 512), d.K / (4 * kBK)));
+    int kchunk = (int)(ceil_div(ceil_div(d.K, nsplit), kBK) * kBK);
+    nsplit = (int)ceil_div(d.K, kchunk);
+    if (nsplit <= 1) {
+      nsplit = 1;
+      kchunk = (int)(ceil_div(d.K, kBK) * kBK);
+    }
+    out.push_back({nsplit, kchunk});
+  }
+  return out;
+}
+
 }  // namespace
+
+size_t grouped_gemm_workspace(const std::vector<GemmDesc>& probs, int bm, long target_items) {
+  const auto sp = grouped_splits(probs, bm, target_items);
+  size_t n = 0;
+  for (size_t q = 0; q < probs.size(); ++q)
+    if (sp[q].nsplit > 1) n += (size_t)sp[q].nsplit * probs[q].batch * probs[q].M * probs[q].N;
+  return n;
+}
+
+GroupedGemmPlan plan_grouped_gemm(const std::vector<GemmDesc>& probs, GemmPrecision prec, int bm,
+                                  long target_items, double* ws, size_t ws_doubles,
+                                  hipStream_t stream) {
+  if (probs.empty()) throw std::invalid_argument("grouped GEMM without problems");
+  if (bm != 64 && bm != 128) throw std::invalid_argument("grouped GEMM tile must be 64 or 128");
+  if (prec != GemmPrecision::F64 && prec != GemmPrecision::F32)
+    throw std::invalid_argument("grouped GEMM runs in F64 or F32");
+  GroupedGemmPlan pl;
+  pl.bm = bm;
+  pl.prec = prec;
+  pl.transA = probs[0].transA;
+  pl.transB = probs[0].transB;
+  pl.nprob = (int)probs.size();
+  const auto sp = grouped_splits(probs, bm, target_items);
+  std::vector<GProblem> host(probs.size());
+  size_t woff = 0;
+  for (size_t q = 0; q < probs.size(); ++q) {
+    const GemmDesc& d = probs[q];
+    if (d.transA != pl.transA || d.transB != pl.transB)
+      throw std::invalid_argument("grouped GEMM problems differ in transposes");
+    if (d.M <= 0 || d.N <= 0 || d.K <= 0 || d.batch <= 0)
+      throw std::invalid_argument("grouped GEMM problem with an empty extent");
+    if (d.batch > 65535) throw std::invalid_argument("grouped GEMM batch too large");
+    KParams& p = host[q].p;
+    p.A = d.A; p.B = d.B; p.C = d.C;
+    p.lda = d.lda; p.ldb = d.ldb; p.ldc = d.ldc; p.sA = d.sA; p.sB = d.sB; p.sC = d.sC;
+    p.M = d.M; p.N = d.N; p.K = d.K;
+    p.kchunk = sp[q].kchunk;
+    p.nsplit = sp[q].nsplit;
+    p.nsplit_batch = d.batch;
+    p.batch0 = 0;
+    p.tiles_m = (int)ceil_div(d.M, bm);
+    p.tiles_n = (int)ceil_div(d.N, bm);
+    p.per_xcd = 0;
+    p.alpha = d.alpha;
+    p.beta = d.beta;
+    p.W = nullptr;
+    host[q].first = pl.items;
+    host[q].rfirst = pl.reduce_items;
+    pl.items += (long)p.tiles_m * p.tiles_n * d.batch * p.nsplit;
+    if (p.nsplit > 1) {
+      const size_t need = (size_t)p.nsplit * d.batch * d.M * d.N;
+      if (woff + need > ws_doubles) throw std::invalid_argument("grouped GEMM workspace too small");
+      p.W = ws + woff;
+      woff += need;
+      pl.reduce_items += ceil_div((long)d.batch * d.M * d.N, 256);
+    }
+  }
+  pl.ws_doubles = woff;
+  pl.grid = (int)std::min<long>(pl.items, 4096);
+  MPQC_HIP_CHECK(hipMalloc(&pl.dev, host.size() * sizeof(GProblem)));
+  MPQC_HIP_CHECK(hipMemcpyAsync(pl.dev, host.data(), host.size() * sizeof(GProblem),
+                                hipMemcpyHostToDevice, stream));
+  MPQC_HIP_CHECK(hipStreamSynchronize(stream));
+  return pl;
+}
+
+void launch_grouped_gemm(const GroupedGemmPlan& pl, hipStream_t stream) {
+  if (pl.items == 0) return;
+  if (pl.bm == 128) launch_grouped_bm<128>(pl, stream);
+  else launch_grouped_bm<64>(pl, stream);
+  if (pl.reduce_items > 0) {
+    const unsigned g = (unsigned)std::min<long>(pl.reduce_items, 65536);
+    hipLaunchKernelGGL(grouped_reduce_kernel, dim3(g), dim3(256), 0, stream, (const GProblem*)pl.dev,
+                       pl.nprob, pl.reduce_items);
+    check_launch("grouped_reduce");
+  }
+}
+
+void free_grouped_gemm(GroupedGemmPlan& pl) {
+  if (pl.dev) (void)hipFree(pl.dev);
+  pl.dev = nullptr;
+}
 
 void splitk_reduce(const double* W, double* C, long ldc, long sC, int M, int N, int nsplit, int batch0,
                    int nb, double beta, hipStream_t stream) {

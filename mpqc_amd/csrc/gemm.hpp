@@ -12,6 +12,8 @@
 // operands have been brought into GEMM layout by the permutation kernels.
 #pragma once
 
+#include <vector>
+
 #include "common.hpp"
 
 namespace mpqc_hip {
@@ -51,6 +53,32 @@ void gemm(const GemmDesc& d, GemmPrecision prec, const GemmWorkspace& ws, hipStr
 // M x N partials W dense; fixed summation order (bitwise reproducible).
 void splitk_reduce(const double* W, double* C, long ldc, long sC, int M, int N, int nsplit, int batch0,
                    int nb, double beta, hipStream_t stream);
+
+// ---- grouped contractions: many independent problems with the same transposes
+// and precision (F64 or F32) in one persistent launch.  Work items are
+// (problem, batch x split slice, output tile); a workgroup walks the item list
+// with stride gridDim.x and finds each item's problem by binary search over the
+// problems' first items.  Problems whose split-K factor is above one leave
+// partial sums in the workspace, summed by a grouped reduce launch.
+struct GroupedGemmPlan {
+  int bm = 64;             // square tile edge (64 or 128)
+  bool transA = false, transB = false;
+  GemmPrecision prec = GemmPrecision::F64;
+  int nprob = 0;
+  long items = 0;          // total work items of the contraction launch
+  long reduce_items = 0;   // total 256-element chunks of the reduce launch
+  int grid = 0;            // persistent workgroups
+  void* dev = nullptr;     // device array of problem records
+  size_t ws_doubles = 0;   // split-K workspace used
+};
+// Workspace (doubles) the plan for (bm, target_items) needs.
+size_t grouped_gemm_workspace(const std::vector<GemmDesc>& probs, int bm, long target_items);
+// Builds the plan and uploads the problem records (synchronous on `stream`).
+GroupedGemmPlan plan_grouped_gemm(const std::vector<GemmDesc>& probs, GemmPrecision prec, int bm,
+                                  long target_items, double* ws, size_t ws_doubles,
+                                  hipStream_t stream);
+void launch_grouped_gemm(const GroupedGemmPlan& plan, hipStream_t stream);
+void free_grouped_gemm(GroupedGemmPlan& plan);
 
 // Split-precision paths (BF16X3, F16X3, BF16X6); called by gemm().
 void gemm_split(const GemmDesc& d, GemmPrecision prec, const GemmWorkspace& ws, hipStream_t stream);

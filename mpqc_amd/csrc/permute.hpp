@@ -42,17 +42,18 @@ struct PermGeom {
 PermGeom make_perm_geom(const long n[4], const TTerm* terms, int nterms);
 dim3 perm_grid(const PermGeom& g);
 
-// Device: evaluates every output element this workgroup owns and hands
-// (linear output offset, sum) to store().  Called by every thread of the
-// workgroup (the tiled shape synchronises through LDS).
+// Device: evaluates every output element of block (bx, by, bz) of the launch
+// grid and hands (linear output offset, sum) to store().  Called by every
+// thread of the workgroup (the tiled shape synchronises through LDS).
 template <class Store>
-__device__ inline void permute_block(const PermGeom& g, Store&& store) {
+__device__ inline void permute_block(const PermGeom& g, unsigned bx, unsigned by, unsigned bz,
+                                     Store&& store) {
   if (!g.tiled) {
     // linear shape: q = 2, r1 = 1, r0 = 0; the host keeps the (q,3) plane below 2^31
     const int plane = (int)(g.nq * g.n3);
-    const int e = (int)blockIdx.x * kPThreads + (int)threadIdx.x;
+    const int e = (int)bx * kPThreads + (int)threadIdx.x;
     if (e >= plane) return;
-    const long ir1 = blockIdx.y, ir0 = blockIdx.z;
+    const long ir1 = by, ir0 = bz;
     const int n3 = (int)g.n3;
     const int iq = e / n3, i3 = e - iq * n3;
     double acc = 0.0;
@@ -65,8 +66,8 @@ __device__ inline void permute_block(const PermGeom& g, Store&& store) {
   }
   __shared__ double tile[kPT][kPT + 1];
   const int tx = threadIdx.x & (kPT - 1), ty = threadIdx.x / kPT;  // ty < 8
-  const int t3 = blockIdx.x % g.tiles3, tq = blockIdx.x / g.tiles3;
-  const long ir1 = blockIdx.y, ir0 = blockIdx.z;
+  const int t3 = (int)bx % g.tiles3, tq = (int)bx / g.tiles3;
+  const long ir1 = by, ir0 = bz;
   const long b3 = (long)t3 * kPT, bq = (long)tq * kPT;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   for (int t = 0; t < g.nterms; ++t) {

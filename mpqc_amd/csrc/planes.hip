@@ -108,7 +108,7 @@ struct SplitOut {
 
 __global__ __launch_bounds__(kPThreads) void split_write_kernel(PermGeom g, SplitOut o) {
   const double sc = *o.scale;
-  permute_block(g, [&](long e, double v) {
+  permute_block(g, blockIdx.x, blockIdx.y, blockIdx.z, [&](long e, double v) {
     const double x = v * sc;
     if (o.kind == (int)PlaneKind::F16x2) {
       const uint16_t h = f16_bits((float)x);

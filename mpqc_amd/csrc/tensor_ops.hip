@@ -16,7 +16,7 @@ constexpr int kBlock = 256;
 
 __global__ __launch_bounds__(kPThreads) void tadd_kernel(PermGeom g, double* __restrict__ out,
                                                        double beta) {
-  permute_block(g, [&](long o, double v) { out[o] = (beta == 0.0) ? v : beta * out[o] + v; });
+  permute_block(g, blockIdx.x, blockIdx.y, blockIdx.z, [&](long o, double v) { out[o] = (beta == 0.0) ? v : beta * out[o] + v; });
 }
 
 // Element-wise kernels use 1-D grid-stride loops with 32-bit indices (the
@@ -236,7 +236,144 @@ void check_grid_yz(long y, long z) {
   if (y > 65535 || z > 65535) throw std::invalid_argument("tensor kernel: extent exceeds grid limit");
 }
 
+// ---------------------------------------------------------------- grouped
+struct GPerm {
+  PermGeom g;
+  double* out;
+  double beta;
+  long first;          // first workgroup of this problem
+  unsigned gx, gy;     // launch grid of the problem (gz implied)
+};
+
+struct GEw {
+  int kind;
+  double* out;
+  const double* a;
+  const double* b;
+  long n;
+  int o, v;
+  long first;
+};
+
+constexpr int kEwPerBlock = kBlock * 8;  // elements per workgroup of the element-wise kinds
+
+template <class P>
+__device__ inline int find_first(const P* __restrict__ probs, int nprob, long it) {
+  int lo = 0, hi = nprob - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (probs[mid].first <= it) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(kPThreads) void grouped_tadd_kernel(const GPerm* __restrict__ probs, int nprob) {
+  const long it = (long)blockIdx.y * gridDim.x + blockIdx.x;
+  const int q = find_first(probs, nprob, it);
+  const GPerm& P = probs[q];
+  const long local = it - P.first;
+  const unsigned bx = (unsigned)(local % P.gx);
+  const long rest = local / P.gx;
+  const unsigned by = (unsigned)(rest % P.gy), bz = (unsigned)(rest / P.gy);
+  if (bz >= (unsigned)P.g.nr0) return;  // grid padding past the last problem
+  double* out = P.out;
+  const double beta = P.beta;
+  permute_block(P.g, bx, by, bz, [&](long o, double v) { out[o] = (beta == 0.0) ? v : beta * out[o] + v; });
+}
+
+__global__ __launch_bounds__(kBlock) void grouped_ew_kernel(const GEw* __restrict__ probs, int nprob) {
+  const long it = (long)blockIdx.y * gridDim.x + blockIdx.x;
+  const int q = find_first(probs, nprob, it);
+  const GEw& P = probs[q];
+  const long base = (it - P.first) * kEwPerBlock;
+  for (int k = 0; k < kEwPerBlock / kBlock; ++k) {
+    const long e = base + (long)k * kBlock + threadIdx.x;
+    if (e >= P.n) return;
+    if (P.kind == 0) {
+      P.out[e] = P.a[e];
+    } else if (P.kind == 1) {  // tau(a,b,i,j) = t2 + t1(a,i) t1(b,j)
+      const int o = P.o, v = P.v, oo = o * o;
+      const int ee = (int)e, ab = ee / oo, ij = ee - ab * oo;
+      const int a = ab / v, b = ab - a * v, i = ij / o, j = ij - i * o;
+      P.out[e] = P.a[e] + P.b[a * o + i] * P.b[b * o + j];
+    } else {  // TP(a,i,l,d) = 0.5 t2(d,a,i,l) + t1(d,i) t1(a,l)
+      const int o = P.o, v = P.v, ov = o * v;
+      const int ee = (int)e, ai = ee / ov, ld = ee - ai * ov;
+      const int a = ai / o, i = ai - a * o, l = ld / v, d = ld - l * v;
+      P.out[e] = 0.5 * P.a[((d * v + a) * o + i) * o + l] + P.b[d * o + i] * P.b[a * o + l];
+    }
+  }
+}
+
+dim3 grouped_grid(long items) {
+  if (items <= 0) throw std::invalid_argument("grouped launch without work");
+  const long gx = std::min<long>(items, 1L << 20);
+  const long gy = ceil_div(items, gx);
+  if (gy > 65535) throw std::invalid_argument("grouped launch too large");
+  return dim3((unsigned)gx, (unsigned)gy);
+}
+
 }  // namespace
+
+GroupedPlan plan_grouped_tadd(const std::vector<GroupedTAddDesc>& probs, hipStream_t s) {
+  if (probs.empty()) throw std::invalid_argument("grouped permutation without problems");
+  std::vector<GPerm> host(probs.size());
+  GroupedPlan pl;
+  pl.kind = 0;
+  pl.nprob = (int)probs.size();
+  for (size_t q = 0; q < probs.size(); ++q) {
+    const GroupedTAddDesc& d = probs[q];
+    host[q].g = make_perm_geom(d.n, d.terms, d.nterms);
+    const dim3 gr = perm_grid(host[q].g);
+    host[q].out = d.out;
+    host[q].beta = d.beta;
+    host[q].first = pl.items;
+    host[q].gx = gr.x;
+    host[q].gy = gr.y;
+    pl.items += (long)gr.x * gr.y * gr.z;
+  }
+  MPQC_HIP_CHECK(hipMalloc(&pl.dev, host.size() * sizeof(GPerm)));
+  MPQC_HIP_CHECK(hipMemcpyAsync(pl.dev, host.data(), host.size() * sizeof(GPerm), hipMemcpyHostToDevice, s));
+  MPQC_HIP_CHECK(hipStreamSynchronize(s));
+  return pl;
+}
+
+GroupedPlan plan_grouped_ew(const std::vector<GroupedEwDesc>& probs, hipStream_t s) {
+  if (probs.empty()) throw std::invalid_argument("grouped element-wise launch without problems");
+  std::vector<GEw> host(probs.size());
+  GroupedPlan pl;
+  pl.kind = 1;
+  pl.nprob = (int)probs.size();
+  for (size_t q = 0; q < probs.size(); ++q) {
+    const GroupedEwDesc& d = probs[q];
+    if (d.kind < 0 || d.kind > 2) throw std::invalid_argument("unknown element-wise kind");
+    if (d.kind != 0 && (long)d.v * d.v * d.o * d.o != d.n)
+      throw std::invalid_argument("element-wise extents do not match");
+    if (d.n >= (1L << 31)) throw std::invalid_argument("element-wise problem too large");
+    host[q] = GEw{d.kind, d.out, d.a, d.b, d.n, d.o, d.v, pl.items};
+    pl.items += ceil_div(d.n, kEwPerBlock);
+  }
+  MPQC_HIP_CHECK(hipMalloc(&pl.dev, host.size() * sizeof(GEw)));
+  MPQC_HIP_CHECK(hipMemcpyAsync(pl.dev, host.data(), host.size() * sizeof(GEw), hipMemcpyHostToDevice, s));
+  MPQC_HIP_CHECK(hipStreamSynchronize(s));
+  return pl;
+}
+
+void launch_grouped(const GroupedPlan& pl, hipStream_t s) {
+  if (pl.items == 0) return;
+  const dim3 grid = grouped_grid(pl.items);
+  if (pl.kind == 0)
+    hipLaunchKernelGGL(grouped_tadd_kernel, grid, dim3(kPThreads), 0, s, (const GPerm*)pl.dev, pl.nprob);
+  else
+    hipLaunchKernelGGL(grouped_ew_kernel, grid, dim3(kBlock), 0, s, (const GEw*)pl.dev, pl.nprob);
+  check_launch("grouped");
+}
+
+void free_grouped(GroupedPlan& pl) {
+  if (pl.dev) (void)hipFree(pl.dev);
+  pl.dev = nullptr;
+}
 
 void permuted_strides(const long in_extents[4], const int perm[4], long out_s[4]) {
   long st[4];

@@ -4,6 +4,8 @@
 // and deterministic reductions.
 #pragma once
 
+#include <vector>
+
 #include "common.hpp"
 
 namespace mpqc_hip {
@@ -79,6 +81,37 @@ void tiles_to_dense(double* dense, const double* tiled, const long n[4], const l
                     hipStream_t s);
 void dense_to_tiles(double* tiled, const double* dense, const long n[4], const long* modeinfo,
                     hipStream_t s);
+
+// ---- grouped forms (one launch over many independent problems)
+// Permuted linear combinations: problem q writes out_q (extents n_q) =
+// beta_q out_q + sum_t terms_q[t].
+struct GroupedTAddDesc {
+  double* out;
+  long n[4];
+  double beta;
+  int nterms;
+  TTerm terms[kMaxTerms];
+};
+// Element-wise builds: kind 0 copy (out = a, n elements), 1 tau (out = tau from
+// t2 = a, t1 = b), 2 TP (out = TP from t2 = a, t1 = b) with extents o, v.
+struct GroupedEwDesc {
+  int kind;
+  double* out;
+  const double* a;
+  const double* b;
+  long n;
+  int o, v;
+};
+struct GroupedPlan {
+  void* dev = nullptr;  // device array of problem records
+  int nprob = 0;
+  long items = 0;       // workgroups of the launch
+  int kind = 0;         // 0 permutation, 1 element-wise
+};
+GroupedPlan plan_grouped_tadd(const std::vector<GroupedTAddDesc>& probs, hipStream_t s);
+GroupedPlan plan_grouped_ew(const std::vector<GroupedEwDesc>& probs, hipStream_t s);
+void launch_grouped(const GroupedPlan& plan, hipStream_t s);
+void free_grouped(GroupedPlan& plan);
 
 // Count of non-finite values in x[0..n) -> out[0] (as double).
 void count_nonfinite(const double* x, long n, double* out, double* partials, hipStream_t s);
