@@ -25,6 +25,9 @@ from .ccsd import CCSD, Integrals
 
 NAMES = list(L.INTEGRAL_IDS)
 
+# iterations of the last forward solve and Lambda solve (diagnostics)
+last_info = {}
+
 
 class CCSDEnergy(torch.autograd.Function):
     """E_corr(Fia, Fij, Fab, Gijab, Gijkl, Giajb, Giabc, Gijka, Gabcd)."""
@@ -38,6 +41,7 @@ class CCSDEnergy(torch.autograd.Function):
             c.set_orbital_energies(np.asarray(eps_occ, dtype=np.float64),
                                    np.asarray(eps_vir, dtype=np.float64))
             e = c.compute_ccsd(target_precision=opts.get("target_precision", 1e-11))
+            last_info["ccsd_iterations"] = c.n_iter
         except Exception:
             c.close()
             raise
@@ -50,8 +54,9 @@ class CCSDEnergy(torch.autograd.Function):
     def backward(ctx, grad_e):
         c = ctx.c
         try:
-            c.lambda_solve(target_precision=ctx.opts.get("lambda_precision", 1e-12),
-                           max_iter=ctx.opts.get("lambda_max_iter", 200))
+            last_info["lambda_iterations"] = c.lambda_solve(
+                target_precision=ctx.opts.get("lambda_precision", 1e-12),
+                max_iter=ctx.opts.get("lambda_max_iter", 200))
             g = c.energy_gradient()
         finally:
             c.close()

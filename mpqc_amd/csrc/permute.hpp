@@ -7,19 +7,22 @@
 //    grid.y = i1, grid.z = i0.  Used when every input is contiguous along the
 //    output's inner index i3 (or none is contiguous along any other output
 //    index), so both reads and writes are wave-contiguous.
-//  * tiled: a workgroup owns a 32 x 32 tile over output dims (q, 3), where q
+//  * tiled: a workgroup owns a T3 x TQ tile over output dims (3, q), where q
 //    is the output index along which some input is contiguous (a transposed
-//    input).  Such inputs are read 32 contiguous elements along q, staged in
-//    LDS, and consumed along i3; other inputs are read directly.  The two
-//    remaining output dims index grid.y / grid.z.
+//    input).  T3, TQ <= 64 split the extents into equal parts (no padding
+//    waste at o = 40 or v = 152) with T3 * TQ <= 2048.  Transposed inputs are
+//    read TQ contiguous elements along q, staged in LDS, and consumed along
+//    i3; other inputs are read directly.  The two remaining output dims index
+//    grid.y / grid.z.
 #pragma once
 
 #include "tensor_ops.hpp"
 
 namespace mpqc_hip {
 
-constexpr int kPT = 32;          // tile edge of the tiled shape
-constexpr int kPThreads = 256;   // threads per workgroup (32 x 8)
+constexpr int kPT = 64;          // largest tile edge of the tiled shape
+constexpr int kPTEl = 2048;      // largest tile (elements)
+constexpr int kPThreads = 256;   // threads per workgroup
 
 // One input with strides resolved against the launch geometry.
 struct PTerm {
@@ -32,6 +35,8 @@ struct PermGeom {
   long n3, nq, nr0, nr1;  // extents of output dims 3, q, r0, r1
   long o3, oq, or0, or1;  // output strides of those dims (o3 = 1)
   int tiles3, tilesq;     // tiled shape: tiles along dims 3 and q
+  int t3, tq;             // tile extents along dims 3 and q
+  float inv_t3, inv_tq;   // 1 / t3, 1 / tq (exact quotients for indices < kPTEl)
   bool tiled;
   unsigned lds_mask;      // bit t: input t goes through the LDS tile
   int nterms;
@@ -64,38 +69,59 @@ __device__ inline void permute_block(const PermGeom& g, unsigned bx, unsigned by
     store(ir0 * g.or0 + ir1 * g.or1 + e, acc);
     return;
   }
-  __shared__ double tile[kPT][kPT + 1];
-  const int tx = threadIdx.x & (kPT - 1), ty = threadIdx.x / kPT;  // ty < 8
-  const int t3 = (int)bx % g.tiles3, tq = (int)bx / g.tiles3;
+  __shared__ double tile[kPT * (kPT + 1)];
+  const int T3 = g.t3, TQ = g.tq, nel = T3 * TQ;
+  const int t3i = (int)bx % g.tiles3, tqi = (int)bx / g.tiles3;
   const long ir1 = by, ir0 = bz;
-  const long b3 = (long)t3 * kPT, bq = (long)tq * kPT;
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  const long b3 = (long)t3i * T3, bq = (long)tqi * TQ;
+  constexpr int kPer = kPTEl / kPThreads;
+  double acc[kPer];
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) acc[k] = 0.0;
   for (int t = 0; t < g.nterms; ++t) {
     const PTerm& T = g.t[t];
     const double* base = T.p + ir0 * T.sr0 + ir1 * T.sr1;
     if ((g.lds_mask >> t) & 1u) {
       // lanes run along q, where this input is contiguous
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const long iq = bq + tx, j3 = b3 + ty + 8 * k;
-        tile[ty + 8 * k][tx] = (iq < g.nq && j3 < g.n3) ? base[iq * T.sq + j3 * T.s3] : 0.0;
+      for (int k = 0; k < kPer; ++k) {
+        const int e = (int)threadIdx.x + k * kPThreads;
+        if (e < nel) {
+          const int j3 = (int)(((float)e + 0.5f) * g.inv_tq), ql = e - j3 * TQ;
+          const long iq = bq + ql, i3 = b3 + j3;
+          tile[j3 * (kPT + 1) + ql] = (iq < g.nq && i3 < g.n3) ? base[iq * T.sq + i3 * T.s3] : 0.0;
+        }
       }
       __syncthreads();
 #pragma unroll
-      for (int k = 0; k < 4; ++k) acc[k] += T.alpha * tile[tx][ty + 8 * k];
+      for (int k = 0; k < kPer; ++k) {
+        const int e = (int)threadIdx.x + k * kPThreads;
+        if (e < nel) {
+          const int ql = (int)(((float)e + 0.5f) * g.inv_t3), j3 = e - ql * T3;
+          acc[k] += T.alpha * tile[j3 * (kPT + 1) + ql];
+        }
+      }
       __syncthreads();
     } else {
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const long iq = bq + ty + 8 * k, i3 = b3 + tx;
-        if (iq < g.nq && i3 < g.n3) acc[k] += T.alpha * base[iq * T.sq + i3 * T.s3];
+      for (int k = 0; k < kPer; ++k) {
+        const int e = (int)threadIdx.x + k * kPThreads;
+        if (e < nel) {
+          const int ql = (int)(((float)e + 0.5f) * g.inv_t3), j3 = e - ql * T3;
+          const long iq = bq + ql, i3 = b3 + j3;
+          if (iq < g.nq && i3 < g.n3) acc[k] += T.alpha * base[iq * T.sq + i3 * T.s3];
+        }
       }
     }
   }
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const long iq = bq + ty + 8 * k, i3 = b3 + tx;
-    if (iq < g.nq && i3 < g.n3) store(ir0 * g.or0 + ir1 * g.or1 + iq * g.oq + i3, acc[k]);
+  for (int k = 0; k < kPer; ++k) {
+    const int e = (int)threadIdx.x + k * kPThreads;
+    if (e < nel) {
+      const int ql = (int)(((float)e + 0.5f) * g.inv_t3), j3 = e - ql * T3;
+      const long iq = bq + ql, i3 = b3 + j3;
+      if (iq < g.nq && i3 < g.n3) store(ir0 * g.or0 + ir1 * g.or1 + iq * g.oq + i3, acc[k]);
+    }
   }
 }
 

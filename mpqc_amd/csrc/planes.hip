@@ -6,10 +6,12 @@
 // are k-contiguous in HBM and in LDS, so a slice row is copied with 16-byte
 // loads and every MFMA fragment (8 consecutive k of one row) is one 16-byte
 // LDS read.  The matrix cores accumulate in FP32 over K chunks of at most
-// kMaxChunk; after each chunk the workgroup adds its FP32 sums into the FP64
-// output (or its split-K partial), which bounds the FP32 accumulation length
-// without FP64 registers in the main loop (two workgroups fit per CU).  Split-K
-// is used only for parallelism on small outputs.  Workgroups are mapped to tiles XCD by XCD
+// kMaxChunk: longer contractions are split into chunks (split-K) whose FP64
+// partial sums are added in a fixed order by a separate pass, which bounds the
+// FP32 accumulation length without FP64 registers in the main loop (two
+// workgroups fit per CU).  A workgroup whose K range is longer than kMaxChunk
+// (not produced by the planner) would fold its FP32 sums into the FP64 output
+// after every chunk.  Workgroups are mapped to tiles XCD by XCD
 // (consecutive tiles on the same XCD share operand rows in that XCD's L2).
 #include "planes.hpp"
 #include "gemm.hpp"
@@ -342,6 +344,10 @@ PlanePlan plan_planes(int M, int N, int K, int batch) {
     nsplit = (int)std::min<long>(ceil_div(want, tiles * batch), K / (4 * kBK));
     nsplit = std::max(1, std::min(nsplit, 512));
   }
+  // K chunks beyond kMaxChunk go to separate splits as well: streaming partial
+  // sums to the workspace measured faster than the in-kernel FP64 flush, whose
+  // read-modify-write of the C tile stalls the K loop
+  nsplit = std::max<int>(nsplit, (int)ceil_div(K, kMaxChunk));
   pl.kchunk = (int)(ceil_div(ceil_div(K, nsplit), kBK) * kBK);
   pl.nsplit = nsplit > 1 ? (int)ceil_div(K, pl.kchunk) : 1;
   return pl;

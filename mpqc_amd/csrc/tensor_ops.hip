@@ -405,8 +405,14 @@ PermGeom make_perm_geom(const long n[4], const TTerm* terms, int nterms) {
     if (d != q) r[nr++] = d;
   g.n3 = n[3]; g.nq = n[q]; g.nr0 = n[r[0]]; g.nr1 = n[r[1]];
   g.o3 = 1; g.oq = os[q]; g.or0 = os[r[0]]; g.or1 = os[r[1]];
-  g.tiles3 = (int)ceil_div(n[3], kPT);
-  g.tilesq = (int)ceil_div(n[q], kPT);
+  // tile extents: equal parts of at most kPT along 3, then along q with t3 * tq <= kPTEl
+  g.t3 = (int)ceil_div(n[3], ceil_div(n[3], kPT));
+  const long tq_max = std::min<long>(kPT, std::max<long>(1, kPTEl / g.t3));
+  g.tq = (int)ceil_div(n[q], ceil_div(n[q], tq_max));
+  g.tiles3 = (int)ceil_div(n[3], g.t3);
+  g.tilesq = (int)ceil_div(n[q], g.tq);
+  g.inv_t3 = 1.0f / (float)g.t3;
+  g.inv_tq = 1.0f / (float)g.tq;
   g.nterms = nterms;
   g.lds_mask = 0;
   for (int t = 0; t < nterms; ++t) {
