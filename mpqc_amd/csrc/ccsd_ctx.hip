@@ -461,6 +461,19 @@ void CCSDContext::op_tau() {
   timed("r:tau", 0.0, [&] { build_tau(tau_, t2_, t1_, o_, v_, stream_); });
 }
 
+TProd CCSDContext::tp_product() const {
+  TProd pr;
+  const long O = o_, V = v_;
+  pr.x = t1_;  // t1(d,i): d*o + i over (a,i,l,d)
+  pr.sx[0] = 0; pr.sx[1] = 1; pr.sx[2] = 0; pr.sx[3] = O;
+  pr.nx = O * V;
+  pr.y = t1_;  // t1(a,l): a*o + l
+  pr.sy[0] = O; pr.sy[1] = 0; pr.sy[2] = 1; pr.sy[3] = 0;
+  pr.ny = O * V;
+  pr.alpha = 1.0;
+  return pr;
+}
+
 void CCSDContext::op_TP() {
   if (tape_) {
     TapeOp op;
@@ -469,7 +482,18 @@ void CCSDContext::op_TP() {
     tape_->push_back(op);
     if (!tape_execute_) return;
   }
-  timed("r:TP", 0.0, [&] { build_TP(TP_, t2_, t1_, o_, v_, stream_); });
+  // TP(a,i,l,d) = 0.5 t2(d,a,i,l) + t1(d,i) t1(a,l) through the tiled permute
+  const long O = o_, V = v_;
+  TTerm t;
+  t.p = t2_;
+  long ext[4] = {V, V, O, O};
+  int perm[4] = {1, 2, 3, 0};
+  permuted_strides(ext, perm, t.s);
+  t.alpha = 0.5;
+  t.vol = V * V * O * O;
+  const TProd pr = tp_product();
+  long n[4] = {V, O, O, V};
+  timed("r:TP", 0.0, [&] { tadd(TP_, n, 0.0, &t, 1, stream_, &pr); });
 }
 
 int CCSDContext::timing_slot(const char* tag) {
@@ -593,13 +617,14 @@ Planes CCSDContext::alloc_planes(long n) {
 }
 
 void CCSDContext::split(const Planes& out, std::array<long, 4> n, std::initializer_list<Src> srcs,
-                        const char* tag, int line) {
+                        const TProd* prod, const char* tag, int line) {
   TTerm terms[kMaxTerms];
   const int nt = make_terms(n, srcs, terms);
   long nn[4] = {n[0], n[1], n[2], n[3]};
   std::string t;
   if (kernel_timing_) t = tag ? std::string(tag) : "split@" + std::to_string(line);
-  timed(t.c_str(), 0.0, [&] { split_to_planes(plane_kind(), out, nn, terms, nt, partials_, stream_); });
+  timed(t.c_str(), 0.0,
+        [&] { split_to_planes(plane_kind(), out, nn, terms, nt, partials_, stream_, prod, &amax_); });
 }
 
 void CCSDContext::gemmP(int M, int N, int K, double alpha, const Planes& A, long lda, const Planes& B,
@@ -759,7 +784,12 @@ void CCSDContext::residual() {
   const bool sm = split_mode();
   if (sm) {
     // 16-bit planes of the amplitude operands (the integral planes are split once per solve)
+    if (!amax_.dev) amax_.dev = alloc(AbsMaxCache::kSlots);
+    amax_.clear();
     if (!planes_ready_) prepare_planes();
+    amax_.ptr[0] = t1_;
+    amax_.ptr[1] = t2_;
+    amax_.ptr[2] = tau_;
     split(tau_P_, e_abij, {{tau_, e_abij, I, 1.0}});
     // tau^T(i,j,c,d) = tau(c,d,i,j): the B^T operand of every tau contraction
     split(tauT_P_, {O, O, V, V}, {{tau_, e_abij, {2, 3, 0, 1}, 1.0}});
@@ -855,8 +885,10 @@ void CCSDContext::residual() {
   gemm(false, false, v * v * o, o, o, 1.0, t2_abjk_, o, gki_, o, 0.0, X16_, o, 0, 0, 0, 1, "r2:t2_gki");
   perm_add(Rd_, {V, V, O, O}, 1.0, {{X16_, e_abij, {0, 1, 3, 2}, -1.0}});
   // :236 T(d,b,i,l) = 0.5 t2(d,b,i,l) + t1(d,i) t1(b,l), as TP(a,i,l,d) = T(d,a,i,l)
-  op_TP();
-  perm_add(t2P_, {V, O, O, V}, 0.0, {{t2_, e_abij, {0, 2, 3, 1}, 1.0}});  // t2P(a,i,l,d) = t2(a,d,i,l)
+  if (!sm) {
+    op_TP();
+    perm_add(t2P_, {V, O, O, V}, 0.0, {{t2_, e_abij, {0, 2, 3, 1}, 1.0}});  // t2P(a,i,l,d) = t2(a,d,i,l)
+  }
   // :238-243 j(a,k,i,c) stored as J(a,i,k,c):
   //   Gijab(i,k,a,c) - Gijka(l,k,i,c) t1(a,l) + Giabc(k,a,c,d) t1(d,i)
   //   - Gijab(k,l,c,d) T(d,a,i,l) + 0.5 (2 Gijab(k,l,c,d) - Gijab(k,l,d,c)) t2(a,d,i,l)
@@ -877,8 +909,10 @@ void CCSDContext::residual() {
   perm_add(K_, {V, O, O, V}, 1.0, {{Zk_, {O, V, V, O}, {1, 3, 0, 2}, 1.0}});
   if (sm) {
     const std::array<long, 4> e_aild{V, O, O, V};
-    split(TP_P_, e_aild, {{TP_, e_aild, I, 1.0}});
-    split(t2P_P_, e_aild, {{t2P_, e_aild, I, 1.0}});
+    // planes of TP and t2P straight from t2 (and t1)
+    const TProd pr = tp_product();
+    split(TP_P_, e_aild, {{t2_, e_abij, {1, 2, 3, 0}, 0.5}}, &pr);
+    split(t2P_P_, e_aild, {{t2_, e_abij, {0, 2, 3, 1}, 1.0}});
     gemmP(o * v, o * v, o * v, -1.0, TP_P_, ov, GAt_P_, ov, 1.0, J_, ov, "r2:j_gT");
     gemmP(o * v, o * v, o * v, 0.5, t2P_P_, ov, GBt_P_, ov, 1.0, J_, ov, "r2:j_gbar_t2");
     gemmP(o * v, o * v, o * v, -1.0, TP_P_, ov, GCt_P_, ov, 1.0, K_, ov, "r2:k_gT");

@@ -165,7 +165,9 @@ class CCSDContext {
   PlaneKind plane_kind() const;
   Planes alloc_planes(long n);
   void split(const Planes& out, std::array<long, 4> n, std::initializer_list<Src> srcs,
-             const char* tag = nullptr, int line = __builtin_LINE());
+             const TProd* prod = nullptr, const char* tag = nullptr, int line = __builtin_LINE());
+  TProd tp_product() const;  // t1(d,i) t1(a,l) over TP's (a,i,l,d) index order
+  AbsMaxCache amax_;         // max |t1|, |t2|, |tau| shared by one residual's splits
   void gemmP(int M, int N, int K, double alpha, const Planes& A, long lda, const Planes& B, long ldb,
              double beta, double* C, long ldc, const char* tag);
   void prepare_planes();

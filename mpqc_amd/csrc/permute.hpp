@@ -41,10 +41,16 @@ struct PermGeom {
   unsigned lds_mask;      // bit t: input t goes through the LDS tile
   int nterms;
   PTerm t[kMaxTerms];
+  // optional product term alpha * x[.] * y[.] with strides along dims 3, q, r0, r1
+  bool has_prod;
+  const double* px;
+  const double* py;
+  long px3, pxq, pxr0, pxr1, py3, pyq, pyr0, pyr1;
+  double palpha;
 };
 
 // Host: geometry (and launch grid) for output extents n and the given inputs.
-PermGeom make_perm_geom(const long n[4], const TTerm* terms, int nterms);
+PermGeom make_perm_geom(const long n[4], const TTerm* terms, int nterms, const TProd* prod = nullptr);
 dim3 perm_grid(const PermGeom& g);
 
 // Device: evaluates every output element of block (bx, by, bz) of the launch
@@ -66,6 +72,9 @@ __device__ inline void permute_block(const PermGeom& g, unsigned bx, unsigned by
       const PTerm& T = g.t[t];
       acc += T.alpha * T.p[ir0 * T.sr0 + ir1 * T.sr1 + (long)iq * T.sq + (long)i3 * T.s3];
     }
+    if (g.has_prod)
+      acc += g.palpha * g.px[ir0 * g.pxr0 + ir1 * g.pxr1 + (long)iq * g.pxq + (long)i3 * g.px3] *
+             g.py[ir0 * g.pyr0 + ir1 * g.pyr1 + (long)iq * g.pyq + (long)i3 * g.py3];
     store(ir0 * g.or0 + ir1 * g.or1 + e, acc);
     return;
   }
@@ -120,7 +129,13 @@ __device__ inline void permute_block(const PermGeom& g, unsigned bx, unsigned by
     if (e < nel) {
       const int ql = (int)(((float)e + 0.5f) * g.inv_t3), j3 = e - ql * T3;
       const long iq = bq + ql, i3 = b3 + j3;
-      if (iq < g.nq && i3 < g.n3) store(ir0 * g.or0 + ir1 * g.or1 + iq * g.oq + i3, acc[k]);
+      if (iq < g.nq && i3 < g.n3) {
+        double v = acc[k];
+        if (g.has_prod)
+          v += g.palpha * g.px[ir0 * g.pxr0 + ir1 * g.pxr1 + iq * g.pxq + i3 * g.px3] *
+               g.py[ir0 * g.pyr0 + ir1 * g.pyr1 + iq * g.pyq + i3 * g.py3];
+        store(ir0 * g.or0 + ir1 * g.or1 + iq * g.oq + i3, v);
+      }
     }
   }
 }

@@ -42,8 +42,8 @@ __device__ inline double bf16_val(uint16_t u) { return (double)(float)__builtin_
 
 // max |x| over each distinct input tensor (contiguous reads): partials[src * gridDim.x + block]
 struct AbsMaxArgs {
-  const double* p[kMaxTerms];
-  long n[kMaxTerms];
+  const double* p[kMaxTerms + 2];
+  long n[kMaxTerms + 2];
 };
 
 __global__ __launch_bounds__(256) void absmax_stage1(AbsMaxArgs a, double* partials) {
@@ -70,12 +70,21 @@ struct ScaleArgs {
   int nsrc, nterms, nb, kind;
   int src_of[kMaxTerms];
   double absalpha[kMaxTerms];
+  int prod_x, prod_y;  // sources of the product term's factors (-1: none)
+  double prod_absalpha;
+  const double* cached[kMaxTerms + 2];  // max |x| already known (no partials)
+  double* record[kMaxTerms + 2];        // where to record a computed max
 };
 
 __global__ __launch_bounds__(256) void scale_stage2(const double* partials, ScaleArgs a, double* scale) {
   __shared__ double sh[256];
-  __shared__ double srcmax[kMaxTerms];
+  __shared__ double srcmax[kMaxTerms + 2];
   for (int src = 0; src < a.nsrc; ++src) {
+    if (a.cached[src]) {
+      if (threadIdx.x == 0) srcmax[src] = *a.cached[src];
+      __syncthreads();
+      continue;
+    }
     double m = 0.0;
     for (int i = threadIdx.x; i < a.nb; i += 256) m = fmax(m, partials[(long)src * a.nb + i]);
     sh[threadIdx.x] = m;
@@ -84,7 +93,10 @@ __global__ __launch_bounds__(256) void scale_stage2(const double* partials, Scal
       if (threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
       __syncthreads();
     }
-    if (threadIdx.x == 0) srcmax[src] = sh[0];
+    if (threadIdx.x == 0) {
+      srcmax[src] = sh[0];
+      if (a.record[src]) *a.record[src] = sh[0];
+    }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
@@ -92,6 +104,7 @@ __global__ __launch_bounds__(256) void scale_stage2(const double* partials, Scal
     if (a.kind == (int)PlaneKind::F16x2) {
       double bound = 0.0;
       for (int t = 0; t < a.nterms; ++t) bound += a.absalpha[t] * srcmax[a.src_of[t]];
+      if (a.prod_x >= 0) bound += a.prod_absalpha * srcmax[a.prod_x] * srcmax[a.prod_y];
       if (bound > 0.0 && isfinite(bound)) {
         int e;
         frexp(bound, &e);
@@ -161,8 +174,14 @@ __device__ inline void mma3(const u16x8_t* a, const u16x8_t* b, float4_t& c) {
   mf(a[0], b[0]);
 }
 
+// Occupancy per tile size: the 128 x 128 tile fits two workgroups per CU; the
+// 256-row tiles use the whole LDS and up to 512 registers per lane (the MFMA
+// accumulators of a 128 x 128 wave tile live in AGPRs), one workgroup per CU.
+template <int BM, int BN>
+constexpr int plane_occupancy() { return BM * BN >= 256 * 128 ? 1 : 2; }
+
 template <int BM, int BN, int NP, bool F16, bool VEC>
-__global__ __launch_bounds__(kThr, 2) void gemm_planes_kernel(PParams p) {
+__global__ __launch_bounds__(kThr, (plane_occupancy<BM, BN>())) void gemm_planes_kernel(PParams p) {
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int RB = WM / 16, CB = WN / 16;
   constexpr int ACH = BM * (kBK / 8) / kThr;  // 16-byte chunks per thread per plane (A)
@@ -323,12 +342,14 @@ __global__ __launch_bounds__(kThr, 2) void gemm_planes_kernel(PParams p) {
 
 // Launch plan: tile edge, split-K factor and K chunk of one contraction.
 struct PlanePlan {
-  int bm;      // square tile edge, 64 or 128
+  int bm, bn;  // tile: 64 x 64, 128 x 128, 256 x 128 or 256 x 256
   int nsplit;  // K chunks (each summed in FP32, chunks summed in FP64)
   int kchunk;
 };
 
-PlanePlan plan_planes(int M, int N, int K, int batch) {
+int g_tile_override_m = 0, g_tile_override_n = 0;
+
+PlanePlan plan_planes(int M, int N, int K, int batch, int force_bm = 0, int force_bn = 0) {
   PlanePlan pl;
   // 128-wide tiles unless 64-wide ones pad the output much less
   pl.bm = (M <= 64 || N <= 64) ? 64 : 128;
@@ -337,7 +358,16 @@ PlanePlan plan_planes(int M, int N, int K, int batch) {
     const double a64 = (double)ceil_div(M, 64) * ceil_div(N, 64) * 64 * 64;
     if (a128 > 1.25 * a64) pl.bm = 64;
   }
-  const long tiles = (long)ceil_div(M, pl.bm) * ceil_div(N, pl.bm);
+  pl.bn = pl.bm;
+  if (g_tile_override_m) {
+    pl.bm = g_tile_override_m;
+    pl.bn = g_tile_override_n;
+  }
+  if (force_bm) {
+    pl.bm = force_bm;
+    pl.bn = force_bn;
+  }
+  const long tiles = (long)ceil_div(M, pl.bm) * ceil_div(N, pl.bn);
   int nsplit = 1;
   const long want = 1024;  // about 4 workgroups per CU
   if (tiles * batch < want && K >= 8 * kBK) {
@@ -394,16 +424,21 @@ void launch_planes(const PlaneGemmDesc& d, const PlanePlan& pl, double* ws, size
 template <int NP, bool F16>
 void dispatch_planes(const PlaneGemmDesc& d, double* ws, size_t ws_bytes, hipStream_t stream) {
   const PlanePlan pl = plan_planes(d.M, d.N, d.K, d.batch);
-  if (pl.bm == 64)
+  if (pl.bm == 64) {
     launch_planes<64, 64, NP, F16>(d, pl, ws, ws_bytes, stream);
-  else
-    launch_planes<128, 128, NP, F16>(d, pl, ws, ws_bytes, stream);
+  } else if (pl.bm == 128 || NP == 3) {  // three planes of a 256-row tile exceed the LDS
+    launch_planes<128, 128, NP, F16>(d, pl.bm == 128 ? pl : plan_planes(d.M, d.N, d.K, d.batch, 128, 128),
+                                     ws, ws_bytes, stream);
+  } else {
+    launch_planes<(NP == 3 ? 128 : 256), 128, NP, F16>(d, pl, ws, ws_bytes, stream);
+  }
 }
 
 }  // namespace
 
 void split_to_planes(PlaneKind kind, const Planes& out, const long n[4], const TTerm* terms,
-                     int nterms, double* partials, hipStream_t stream) {
+                     int nterms, double* partials, hipStream_t stream, const TProd* prod,
+                     AbsMaxCache* cache) {
   if (nterms < 1 || nterms > kMaxTerms) throw std::invalid_argument("split_to_planes: bad term count");
   const long total = n[0] * n[1] * n[2] * n[3];
   if (total > out.n) throw std::invalid_argument("split_to_planes: planes too small");
@@ -412,21 +447,41 @@ void split_to_planes(PlaneKind kind, const Planes& out, const long n[4], const T
   ScaleArgs sa{};
   sa.nterms = nterms;
   sa.kind = (int)kind;
+  sa.prod_x = sa.prod_y = -1;
   AbsMaxArgs am{};
-  for (int t = 0; t < nterms; ++t) {
-    int src = -1;
+  auto source = [&](const double* p, long vol) {
     for (int u = 0; u < sa.nsrc; ++u)
-      if (am.p[u] == terms[t].p) src = u;
-    if (src < 0) {
-      if (terms[t].vol <= 0) throw std::invalid_argument("split_to_planes: input volume not set");
-      src = sa.nsrc++;
-      am.p[src] = terms[t].p;
-      am.n[src] = terms[t].vol;
-    }
-    sa.src_of[t] = src;
+      if (am.p[u] == p) return u;
+    if (vol <= 0) throw std::invalid_argument("split_to_planes: input volume not set");
+    const int u = sa.nsrc++;
+    am.p[u] = p;
+    am.n[u] = vol;
+    return u;
+  };
+  for (int t = 0; t < nterms; ++t) {
+    sa.src_of[t] = source(terms[t].p, terms[t].vol);
     sa.absalpha[t] = std::fabs(terms[t].alpha);
   }
+  if (prod) {
+    sa.prod_x = source(prod->x, prod->nx);
+    sa.prod_y = source(prod->y, prod->ny);
+    sa.prod_absalpha = std::fabs(prod->alpha);
+  }
   if (kind == PlaneKind::F16x2) {
+    // inputs whose max is cached skip the max pass (their stage-1 blocks see n = 0)
+    if (cache && cache->dev) {
+      for (int u = 0; u < sa.nsrc; ++u)
+        for (int k = 0; k < AbsMaxCache::kSlots; ++k)
+          if (cache->ptr[k] == am.p[u]) {
+            if (cache->valid[k]) {
+              sa.cached[u] = cache->dev + k;
+              am.n[u] = 0;
+            } else {
+              sa.record[u] = cache->dev + k;
+              cache->valid[k] = true;
+            }
+          }
+    }
     long nmax = 0;
     for (int u = 0; u < sa.nsrc; ++u) nmax = std::max(nmax, am.n[u]);
     sa.nb = (int)std::max<long>(1, std::min<long>(1024, ceil_div(nmax, 256)));
@@ -436,10 +491,18 @@ void split_to_planes(PlaneKind kind, const Planes& out, const long n[4], const T
     sa.nb = 0;
   }
   hipLaunchKernelGGL(scale_stage2, dim3(1), dim3(256), 0, stream, partials, sa, out.scale);
-  const PermGeom g = make_perm_geom(n, terms, nterms);
+  const PermGeom g = make_perm_geom(n, terms, nterms, prod);
   SplitOut so{{out.p[0], out.p[1], out.p[2]}, out.scale, (int)kind};
   hipLaunchKernelGGL(split_write_kernel, perm_grid(g), dim3(kPThreads), 0, stream, g, so);
   check_launch("split_to_planes");
+}
+
+void set_plane_tile_override(int bm, int bn) {
+  if (!((bm == 0 && bn == 0) || (bm == 64 && bn == 64) || (bm == 128 && bn == 128) ||
+        (bm == 256 && bn == 128)))
+    throw std::invalid_argument("unsupported plane GEMM tile");
+  g_tile_override_m = bm;
+  g_tile_override_n = bn;
 }
 
 size_t gemm_planes_workspace_bytes(int M, int N, int K, int batch) {

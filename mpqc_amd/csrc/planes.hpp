@@ -30,8 +30,19 @@ struct Planes {
 // Writes planes of  out[i0,i1,i2,i3] = sum_t alpha_t in_t[...]  (permuted linear
 // combination, as tadd) with extents n; computes the scale on the device first
 // (max |out| over the tensor).  `partials` needs kReducePartials doubles.
+// Device-resident max |x| of inputs that several splits read (valid while the
+// inputs are unchanged): a split finds an input here, reuses its max and
+// skips that input's max pass, or records the max it computes.
+struct AbsMaxCache {
+  static constexpr int kSlots = 8;
+  const double* ptr[kSlots] = {};
+  double* dev = nullptr;  // kSlots doubles
+  bool valid[kSlots] = {};
+  void clear() { for (bool& v : valid) v = false; }
+};
 void split_to_planes(PlaneKind kind, const Planes& out, const long n[4], const TTerm* terms,
-                     int nterms, double* partials, hipStream_t stream);
+                     int nterms, double* partials, hipStream_t stream, const TProd* prod = nullptr,
+                     AbsMaxCache* cache = nullptr);
 
 // C[m,n] (FP64) = alpha * sum_k A[m,k] B[n,k] / (sA sB) + beta C, batched, with A and
 // B given as planes in k-contiguous layout (A: row m at A + m*lda, B: row n at
@@ -49,6 +60,9 @@ struct PlaneGemmDesc {
   int batch = 1;
 };
 void gemm_planes(const PlaneGemmDesc& d, double* ws, size_t ws_bytes, hipStream_t stream);
+// Forces the contraction tile (bm x bn; 0, 0 restores the planner's choice).
+// Used by measurements that compare tile shapes.
+void set_plane_tile_override(int bm, int bn);
 // Workspace (bytes) gemm_planes needs for its split-K partial sums at this shape.
 size_t gemm_planes_workspace_bytes(int M, int N, int K, int batch);
 

@@ -382,7 +382,7 @@ void permuted_strides(const long in_extents[4], const int perm[4], long out_s[4]
   for (int d = 0; d < 4; ++d) out_s[d] = st[perm[d]];
 }
 
-PermGeom make_perm_geom(const long n[4], const TTerm* terms, int nterms) {
+PermGeom make_perm_geom(const long n[4], const TTerm* terms, int nterms, const TProd* prod) {
   if (nterms < 1 || nterms > kMaxTerms) throw std::invalid_argument("permutation: bad term count");
   PermGeom g{};
   long os[4];
@@ -421,6 +421,19 @@ PermGeom make_perm_geom(const long n[4], const TTerm* terms, int nterms) {
     if (g.tiled && T.s[3] != 1 && T.s[q] == 1) g.lds_mask |= 1u << t;
   }
   if (!g.tiled && g.nq * g.n3 >= (1L << 31)) throw std::invalid_argument("permutation plane too large");
+  g.has_prod = prod != nullptr;
+  if (prod) {
+    long mx = 0, my = 0;
+    for (int d = 0; d < 4; ++d) {
+      mx += (n[d] - 1) * prod->sx[d];
+      my += (n[d] - 1) * prod->sy[d];
+    }
+    if (!prod->x || !prod->y || mx >= prod->nx || my >= prod->ny)
+      throw std::invalid_argument("product term reads past its inputs");
+    g.px = prod->x; g.py = prod->y; g.palpha = prod->alpha;
+    g.px3 = prod->sx[3]; g.pxq = prod->sx[q]; g.pxr0 = prod->sx[r[0]]; g.pxr1 = prod->sx[r[1]];
+    g.py3 = prod->sy[3]; g.pyq = prod->sy[q]; g.pyr0 = prod->sy[r[0]]; g.pyr1 = prod->sy[r[1]];
+  }
   return g;
 }
 
@@ -432,10 +445,10 @@ dim3 perm_grid(const PermGeom& g) {
 }
 
 void tadd(double* out, const long n[4], double beta, const TTerm* terms, int nterms,
-          hipStream_t stream) {
+          hipStream_t stream, const TProd* prod) {
   if (nterms > kMaxTerms) throw std::invalid_argument("tadd: too many terms");
   if (n[0] * n[1] * n[2] * n[3] == 0) return;
-  const PermGeom g = make_perm_geom(n, terms, nterms);
+  const PermGeom g = make_perm_geom(n, terms, nterms, prod);
   hipLaunchKernelGGL(tadd_kernel, perm_grid(g), dim3(kPThreads), 0, stream, g, out, beta);
   check_launch("tadd");
 }

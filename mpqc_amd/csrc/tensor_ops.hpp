@@ -21,9 +21,22 @@ struct TTerm {
   long vol = 0;  // element count of the input tensor (used for range bounds)
 };
 
+// Optional product term of a permuted combination: element (i0,i1,i2,i3) adds
+// alpha * x[sum_d i_d sx[d]] * y[sum_d i_d sy[d]] (e.g. t1(d,i) t1(a,l)).
+struct TProd {
+  const double* x = nullptr;
+  long sx[4] = {0, 0, 0, 0};
+  long nx = 0;  // element count of x
+  const double* y = nullptr;
+  long sy[4] = {0, 0, 0, 0};
+  long ny = 0;
+  double alpha = 1.0;
+};
+
 // out[i0,i1,i2,i3] (dense row-major, extents n) = beta*out + sum_t alpha_t * in_t[...]
+// (+ the product term when prod is given)
 void tadd(double* out, const long n[4], double beta, const TTerm* terms, int nterms,
-          hipStream_t stream);
+          hipStream_t stream, const TProd* prod = nullptr);
 
 // Strides of a dense row-major tensor with extents e[0..3] (missing trailing
 // extents = 1), permuted so that output axis d reads input axis perm[d].
