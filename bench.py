@@ -174,7 +174,8 @@ def bench_pp_ladder(args):
                    "parallelism": "single-gpu"},
         "roofline": {"bound": "mfma", "achieved": round(tflops, 3),
                      "peak": PEAK[args.precision], "unit": "TFLOP/s",
-                     "frac": round(tflops / PEAK[args.precision], 4), "traffic": None,
+                     "frac": round(tflops / PEAK[args.precision], 4),
+                     "traffic": profiled_traffic("pp-ladder", args.precision, "pp_ladder")[0],
                      "kernel": "dgemm_f64_kernel (pp-ladder GEMM M=v^2 N=o^2 K=v^2)"},
         "parity_max_rel_err_vs_fp64_host": max_rel,
         "cpu_baseline": cpu,
@@ -192,6 +193,21 @@ SYSTEMS = {
                  "(o=30, v=114)",
     "ccsd-h20": "linear H20 chain, 1.8 bohr spacing, cc-pVDZ, all electrons (o=10, v=90)",
 }
+
+
+def profiled_traffic(workload, form, kernel):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 FETCH_SIZE /
+    WRITE_SIZE (calibrated against a device copy), read from the summaries under
+    profiles/ (lines 'bench-traffic: workload=... form=... kernel=... bytes=...')."""
+    import glob
+    import re
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        with open(path) as fh:
+            for line in fh:
+                m = re.match(r"\s*bench-traffic: workload=(\S+) form=(\S+) kernel=(\S+) bytes=(\S+)", line)
+                if m and m.group(1) == workload and m.group(2) == form and m.group(3) == kernel:
+                    return float(m.group(4)), os.path.relpath(path, ROOT)
+    return None, None
 
 
 def log(msg):
@@ -304,6 +320,7 @@ def bench_ccsd(args):
     head = min(ok, key=lambda p: forms[p]["ms_per_iteration"])
     hf = forms[head]
     pp = hf["kernels"].get("r2:pp_ladder", {"avg_ms": float("nan"), "tflops": float("nan")})
+    traffic, traffic_src = profiled_traffic(args.workload, head, "r2:pp_ladder")
     log("CPU baseline (oracle residual)" if not args.no_cpu_baseline else "no CPU baseline")
     cpu = None if args.no_cpu_baseline else cpu_baseline_ccsd(inp)
     line = {
@@ -324,7 +341,8 @@ def bench_ccsd(args):
                    "parallelism": "single-gpu"},
         "roofline": {"bound": "mfma", "achieved": round(pp["tflops"], 3),
                      "peak": PEAK[head], "unit": "TFLOP/s",
-                     "frac": round(pp["tflops"] / PEAK[head], 4), "traffic": None,
+                     "frac": round(pp["tflops"] / PEAK[head], 4), "traffic": traffic,
+                     "traffic_source": traffic_src,
                      "kernel": f"pp-ladder GEMM (M=v^2, N=o^2, K=v^2) avg {pp['avg_ms']:.3f} ms"},
         "forms": {p: {k: (round(x, 6) if isinstance(x, float) else x) for k, x in f.items()
                       if k != "kernels"} for p, f in forms.items()},

@@ -398,23 +398,39 @@ int CCSDContext::lambda_solve(int max_iter, double target_precision, int verbose
       for (int k = 0; k < nvec; ++k) B[(size_t)k * ndiis + slot] = B[(size_t)slot * ndiis + k] = d[k];
     }
     if (nvec >= 2) {
-      // min |sum c_k e_k| subject to sum c_k = 1 (bordered system, scaled by max diag)
-      const int r = nvec + 1;
-      double bmax = 0.0;
-      for (int k = 0; k < nvec; ++k) bmax = std::max(bmax, std::fabs(B[(size_t)k * ndiis + k]));
-      if (bmax <= 0.0) bmax = 1.0;
-      std::vector<double> A((size_t)r * r, 0.0), rhs(r, 0.0), c;
-      for (int i = 1; i < r; ++i) { A[i] = -1.0; A[(size_t)i * r] = -1.0; }
-      rhs[0] = -1.0;
-      for (int i = 1; i < r; ++i)
-        for (int j = 1; j < r; ++j) A[(size_t)i * r + j] = B[(size_t)(i - 1) * ndiis + (j - 1)] / bmax;
-      const double det = colpiv_qr_solve(r, A, rhs, c);
-      if (det > 1e-300) {
+      // min |sum c_k e_k| subject to sum c_k = 1 over the vectors in age order; as in the
+      // amplitude DIIS (TA::DIIS), the oldest vectors are dropped while the bordered
+      // system (normalised by its first diagonal element) is singular to 1e-15
+      std::vector<int> age(nvec);
+      for (int k = 0; k < nvec; ++k) age[k] = (slot - (nvec - 1) + k + ndiis) % ndiis;
+      const double zero_determinant = 1.0e-15, zero_norm = 1.0e-10;
+      int nskip = 0;
+      double absdet = 0.0;
+      std::vector<double> c;
+      while (true) {
+        const int r = nvec - nskip + 1;
+        std::vector<double> A((size_t)r * r, 0.0), rhs(r, 0.0);
+        for (int i = 1; i < r; ++i) { A[i] = -1.0; A[(size_t)i * r] = -1.0; }
+        rhs[0] = -1.0;
+        const double b00 = B[(size_t)age[nskip] * ndiis + age[nskip]];
+        const double norm = std::fabs(b00) > zero_norm ? 1.0 / std::fabs(b00) : 1.0;
+        for (int i = 1; i < r; ++i)
+          for (int j = 1; j < r; ++j)
+            A[(size_t)i * r + j] = B[(size_t)age[i - 1 + nskip] * ndiis + age[j - 1 + nskip]] * norm;
+        absdet = colpiv_qr_solve(r, A, rhs, c);
+        if (!(absdet < zero_determinant && nskip + 1 < nvec)) break;
+        ++nskip;
+      }
+      if (absdet >= zero_determinant) {
         std::vector<const double*> x1, x2;
         std::vector<double> cc;
-        for (int k = 0; k < nvec; ++k) { x1.push_back(hx1[k]); x2.push_back(hx2[k]); cc.push_back(c[k + 1]); }
-        lincomb(lam1_, x1.data(), cc.data(), nvec, n1, stream_);
-        lincomb(lam2_, x2.data(), cc.data(), nvec, n2, stream_);
+        for (int k = nskip; k < nvec; ++k) {
+          x1.push_back(hx1[age[k]]);
+          x2.push_back(hx2[age[k]]);
+          cc.push_back(c[k - nskip + 1]);
+        }
+        lincomb(lam1_, x1.data(), cc.data(), (int)cc.size(), n1, stream_);
+        lincomb(lam2_, x2.data(), cc.data(), (int)cc.size(), n2, stream_);
       }
     }
   }

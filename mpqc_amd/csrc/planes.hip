@@ -154,24 +154,32 @@ struct PParams {
   double alpha, beta;
 };
 
-template <int NP, bool F16>
-__device__ inline void mma3(const u16x8_t* a, const u16x8_t* b, float4_t& c) {
-  auto mf = [&](const u16x8_t& x, const u16x8_t& y) {
-    if (F16)
-      c = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8_t, x),
-                                                 __builtin_bit_cast(f16x8_t, y), c, 0, 0, 0);
-    else
-      c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, x),
-                                                  __builtin_bit_cast(bf16x8_t, y), c, 0, 0, 0);
-  };
-  if (NP == 3) {  // 6 products, small terms first
-    mf(a[1], b[1]);
-    mf(a[0], b[2]);
-    mf(a[2], b[0]);
-  }
-  mf(a[0], b[1]);
-  mf(a[1], b[0]);
-  mf(a[0], b[0]);
+template <bool F16>
+__device__ inline void mfma1(const u16x8_t& x, const u16x8_t& y, float4_t& c) {
+  if (F16)
+    c = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8_t, x),
+                                               __builtin_bit_cast(f16x8_t, y), c, 0, 0, 0);
+  else
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, x),
+                                                __builtin_bit_cast(bf16x8_t, y), c, 0, 0, 0);
+}
+
+// Plane products of one K slice in the order of increasing magnitude (small
+// terms first): lo*lo-class terms of the 3-plane form, then hi*lo, lo*hi,
+// hi*hi.  Product-major order: consecutive MFMAs write different accumulators,
+// so each dependent MFMA comes RB * CB instructions after the one it extends.
+template <int NP, int RB, int CB, bool F16>
+__device__ inline void mma_slice(const u16x8_t (&fa)[RB][NP], const u16x8_t (&fb)[CB][NP],
+                                 float4_t (&acc)[RB][CB]) {
+  // (plane of A, plane of B) of the six products; the 2-plane form uses the last three
+  constexpr int kPA[6] = {1, 0, 2, 0, 1, 0};
+  constexpr int kPB[6] = {1, 2, 0, 1, 0, 0};
+#pragma unroll
+  for (int t = (NP == 3 ? 0 : 3); t < 6; ++t)
+#pragma unroll
+    for (int i = 0; i < RB; ++i)
+#pragma unroll
+      for (int j = 0; j < CB; ++j) mfma1<F16>(fa[i][kPA[t]], fb[j][kPB[t]], acc[i][j]);
 }
 
 // Occupancy per tile size: the 128 x 128 tile fits two workgroups per CU; the
@@ -320,14 +328,25 @@ __global__ __launch_bounds__(kThr, (plane_occupancy<BM, BN>())) void gemm_planes
 #pragma unroll
       for (int q = 0; q < NP; ++q)
         fb[j][q] = *reinterpret_cast<const u16x8_t*>(&Bs[buf][q][wn * WN + j * 16 + lr][lk * 8]);
+    if constexpr (NP == 2) {
+      // all fragments of the slice first (one LDS latency per slice), then the products
+      u16x8_t fa[RB][NP];
 #pragma unroll
-    for (int i = 0; i < RB; ++i) {
-      u16x8_t fa[NP];
+      for (int i = 0; i < RB; ++i)
 #pragma unroll
-      for (int q = 0; q < NP; ++q)
-        fa[q] = *reinterpret_cast<const u16x8_t*>(&As[buf][q][wm * WM + i * 16 + lr][lk * 8]);
+        for (int q = 0; q < NP; ++q)
+          fa[i][q] = *reinterpret_cast<const u16x8_t*>(&As[buf][q][wm * WM + i * 16 + lr][lk * 8]);
+      mma_slice<NP, RB, CB, F16>(fa, fb, facc);
+    } else {
+      // three planes: one row block at a time keeps two workgroups per CU
 #pragma unroll
-      for (int j = 0; j < CB; ++j) mma3<NP, F16>(fa, fb[j], facc[i][j]);
+      for (int i = 0; i < RB; ++i) {
+        u16x8_t fa[1][NP];
+#pragma unroll
+        for (int q = 0; q < NP; ++q)
+          fa[0][q] = *reinterpret_cast<const u16x8_t*>(&As[buf][q][wm * WM + i * 16 + lr][lk * 8]);
+        mma_slice<NP, 1, CB, F16>(fa, fb, reinterpret_cast<float4_t(&)[1][CB]>(facc[i]));
+      }
     }
     if (++since_flush == kMaxChunk / kBK && has_next) {
       flush();
