@@ -656,6 +656,7 @@ void CCSDContext::prepare_planes() {
     tau_ikcd_P_ = alloc_planes(oovv); tau_ackl_P_ = alloc_planes(oovv);
     P8T_P_ = alloc_planes(O * O * O * V); GiabcT_P_ = alloc_planes(ovvv);
     t1T_P_ = alloc_planes(O * V); gac_P_ = alloc_planes(V * V);
+    P13_P_ = alloc_planes(O * O * O * V); P14_P_ = alloc_planes(ovvv);
   }
   {  // split-K workspace for the largest plane contraction shapes
     const int o = o_, v = v_;
@@ -663,7 +664,8 @@ void CCSDContext::prepare_planes() {
     const int shapes[][3] = {{v * v, o * o, v * v}, {o * v, o * o, v * v}, {o * o, o * o, v * v},
                              {v * v, o * o, o * o}, {o * v, o * v, o * v}, {v, v, o * o * v},
                              {o, o, o * v * v},     {v, o, o * v * v},     {v, o, v * o * o},
-                             {o * v * v, o, v},     {v, v * o * o, v}};
+                             {o * v * v, o, v},     {v, v * o * o, v},     {v * v, 1, o * v},
+                             {o * o, 1, o * v}};
     for (auto& sh : shapes) need = std::max(need, gemm_planes_workspace_bytes(sh[0], sh[1], sh[2], 1));
     if (need > pws_bytes_) {
       const size_t n = (need + sizeof(double) - 1) / sizeof(double);
@@ -689,6 +691,9 @@ void CCSDContext::prepare_planes() {
   split(P8T_P_, {O, V, O, O}, {{P8_, {V, O, O, O}, {3, 0, 1, 2}, 1.0}});
   // GiabcT(k,a,c,d) = Giabc(k,a,d,c), the A operand of the k-intermediate's T1 term
   split(GiabcT_P_, {O, V, V, V}, {{Giabc_, {O, V, V, V}, {0, 1, 3, 2}, 1.0}});
+  // g_ki / g_ac operands (ccsd_r1_r2.h:219,223)
+  split(P13_P_, {O, O, O, V}, {{P13_, {O, O, O, V}, I, 1.0}});
+  split(P14_P_, {V, V, O, V}, {{P14_, {V, V, O, V}, I, 1.0}});
   MPQC_HIP_CHECK(hipStreamSynchronize(stream_));
   planes_ready_ = true;
 }
@@ -868,12 +873,18 @@ void CCSDContext::residual() {
   //          (ccsd_intermediates.h:105-112)
   copy_d2d(gki_, FIJ_, oo);
   gemm(false, false, o, o, v, 1.0, Fia_, v, t1_, o, 1.0, gki_, o, 0, 0, 0, 1, "r2:gki_fia_t1");
-  gemm(false, false, o * o, 1, o * v, 1.0, P13_, ov, t1T_, 1, 1.0, gki_, 1, 0, 0, 0, 1, "r2:gki_g_t1");
+  if (sm)
+    gemmP(o * o, 1, o * v, 1.0, P13_P_, ov, t1T_P_, ov, 1.0, gki_, 1, "r2:gki_g_t1");
+  else
+    gemm(false, false, o * o, 1, o * v, 1.0, P13_, ov, t1T_, 1, 1.0, gki_, 1, 0, 0, 0, 1, "r2:gki_g_t1");
   // :222-224 g_ac = F_VV(FAB, Fia, g_iabc_bar, t1) = FAB + f_ov(k,c) t1(a,k) + gbar(k,a,d,c) t1(d,k)
   //          (ccsd_intermediates.h:141-148; fov_sign = +1 is the reference as written)
   copy_d2d(gac_, FAB_, vv);
   gemm(false, false, v, v, o, fov_sign_, t1_, o, Fia_, v, 1.0, gac_, v, 0, 0, 0, 1, "r2:gac_t1_fia");
-  gemm(false, false, v * v, 1, o * v, 1.0, P14_, ov, t1T_, 1, 1.0, gac_, 1, 0, 0, 0, 1, "r2:gac_g_t1");
+  if (sm)
+    gemmP(v * v, 1, o * v, 1.0, P14_P_, ov, t1T_P_, ov, 1.0, gac_, 1, "r2:gac_g_t1");
+  else
+    gemm(false, false, v * v, 1, o * v, 1.0, P14_, ov, t1T_, 1, 1.0, gac_, 1, 0, 0, 0, 1, "r2:gac_g_t1");
   // :226 Rd += g_ac(a,c) t2(c,b,i,j) - g_ki(k,i) t2(a,b,k,j)
   if (sm) {  // g_ac(a,c) . Y2^T((b,i,j),c)
     split(gac_P_, {1, 1, V, V}, {{gac_, {1, 1, V, V}, I, 1.0}});

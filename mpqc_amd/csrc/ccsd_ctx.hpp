@@ -176,7 +176,7 @@ class CCSDContext {
   size_t pws_bytes_ = 0;
   std::vector<std::pair<uint16_t*, size_t>> plane_allocs_;
   Planes Gabcd_P_, Giabc_P_, Gijab_P_, GAt_P_, GBt_P_, GCt_P_;
-  Planes GbarT_P_, Gbar_P_, Gbar_akcd_P_, P8T_P_, GiabcT_P_;
+  Planes GbarT_P_, Gbar_P_, Gbar_akcd_P_, P8T_P_, GiabcT_P_, P13_P_, P14_P_;
   Planes tau_P_, tauT_P_, TP_P_, t2P_P_, Xjk_P_, K_P_, Y1t_P_, Y2t_P_, WT_P_, t1T_P_, gac_P_;
   Planes tau_ikcd_P_, tau_ackl_P_;
   void check_finite(const double* x, long n, const char* what);
