@@ -151,6 +151,7 @@ struct PParams {
   long lda, ldb, ldc, sA, sB, sC;
   int M, N, K;
   int kchunk, nsplit, batch0, tiles_m, tiles_n, per_xcd;
+  int order;  // tile order: 0 XCD ranges of row-major tiles, 1 plain, 2 XCD ranges column-major
   double alpha, beta;
 };
 
@@ -188,22 +189,33 @@ __device__ inline void mma_slice(const u16x8_t (&fa)[RB][NP], const u16x8_t (&fb
 template <int BM, int BN>
 constexpr int plane_occupancy() { return BM * BN >= 256 * 128 ? 1 : 2; }
 
-template <int BM, int BN, int NP, bool F16, bool VEC>
+// BD (B direct): the B fragments of each wave are loaded from global memory
+// (L1/L2) straight into registers one K slice ahead instead of being staged
+// through LDS; this halves the LDS traffic per slice (the kernel's limit).
+template <int BM, int BN, int NP, bool F16, bool VEC, bool BD>
 __global__ __launch_bounds__(kThr, (plane_occupancy<BM, BN>())) void gemm_planes_kernel(PParams p) {
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int RB = WM / 16, CB = WN / 16;
   constexpr int ACH = BM * (kBK / 8) / kThr;  // 16-byte chunks per thread per plane (A)
-  constexpr int BCH = BN * (kBK / 8) / kThr;
+  constexpr int BCH = BD ? 0 : BN * (kBK / 8) / kThr;
+  static_assert(!BD || (VEC && NP == 2), "direct B fragments need 16-byte rows and two planes");
 
   __shared__ __attribute__((aligned(16))) uint16_t As[2][NP][BM][kRow];
-  __shared__ __attribute__((aligned(16))) uint16_t Bs[2][NP][BN][kRow];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[2][NP][BD ? 1 : BN][kRow];
 
   // XCD-aware tile order: workgroup w runs on XCD w % 8; each XCD takes a
   // contiguous range of row-major tiles
   const int w = blockIdx.x;
-  const int tile = (w % kXcd) * p.per_xcd + w / kXcd;
+  const int tile = p.order == 1 ? w : (w % kXcd) * p.per_xcd + w / kXcd;
   if (tile >= p.tiles_m * p.tiles_n) return;
-  const int tm = tile / p.tiles_n, tn = tile - tm * p.tiles_n;
+  int tm, tn;
+  if (p.order == 2) {  // column-major tiles
+    tn = tile / p.tiles_m;
+    tm = tile - tn * p.tiles_m;
+  } else {
+    tm = tile / p.tiles_n;
+    tn = tile - tm * p.tiles_n;
+  }
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -217,7 +229,21 @@ __global__ __launch_bounds__(kThr, (plane_occupancy<BM, BN>())) void gemm_planes
   const int k_begin = s * p.kchunk;
   const int k_end = min(p.K, k_begin + p.kchunk);
 
-  u16x8_t ra[NP][ACH], rb[NP][BCH];
+  u16x8_t ra[NP][ACH], rb[NP][BCH > 0 ? BCH : 1];
+  // B fragments of this wave for K slice k0 (lane: row n0 + wn*WN + 16j + lr, k = k0 + 8 lk .. +7)
+  auto load_bfrag = [&](int k0, u16x8_t (&dst)[CB][NP]) {
+#pragma unroll
+    for (int j = 0; j < CB; ++j) {
+      const int gn = n0 + wn * WN + j * 16 + lr, gk = k0 + lk * 8;
+#pragma unroll
+      for (int q = 0; q < NP; ++q) {
+        u16x8_t v = {0, 0, 0, 0, 0, 0, 0, 0};
+        if (gn < p.N && gk < k_end)
+          v = *reinterpret_cast<const u16x8_t*>(p.B[q] + (long)b * p.sB + (long)gn * p.ldb + gk);
+        dst[j][q] = v;
+      }
+    }
+  };
   auto load_regs = [&](int k0) {
 #pragma unroll
     for (int c = 0; c < ACH; ++c) {
@@ -314,20 +340,30 @@ __global__ __launch_bounds__(kThr, (plane_occupancy<BM, BN>())) void gemm_planes
   };
 
   int buf = 0, since_flush = 0;
+  u16x8_t fbd[CB][NP];  // direct B fragments of the current slice (BD)
   if (k_begin < k_end) {
     load_regs(k_begin);
     store_lds(0);
+    if constexpr (BD) load_bfrag(k_begin, fbd);
   }
   __syncthreads();
   for (int kt = k_begin; kt < k_end; kt += kBK) {
     const bool has_next = kt + kBK < k_end;
     if (has_next) load_regs(kt + kBK);
-    u16x8_t fb[CB][NP];
+    u16x8_t fb[CB][NP], fbn[CB][NP];
+    if constexpr (BD) {
 #pragma unroll
-    for (int j = 0; j < CB; ++j)
+      for (int j = 0; j < CB; ++j)
 #pragma unroll
-      for (int q = 0; q < NP; ++q)
-        fb[j][q] = *reinterpret_cast<const u16x8_t*>(&Bs[buf][q][wn * WN + j * 16 + lr][lk * 8]);
+        for (int q = 0; q < NP; ++q) fb[j][q] = fbd[j][q];
+      if (has_next) load_bfrag(kt + kBK, fbn);
+    } else {
+#pragma unroll
+      for (int j = 0; j < CB; ++j)
+#pragma unroll
+        for (int q = 0; q < NP; ++q)
+          fb[j][q] = *reinterpret_cast<const u16x8_t*>(&Bs[buf][q][wn * WN + j * 16 + lr][lk * 8]);
+    }
     if constexpr (NP == 2) {
       // all fragments of the slice first (one LDS latency per slice), then the products
       u16x8_t fa[RB][NP];
@@ -348,6 +384,13 @@ __global__ __launch_bounds__(kThr, (plane_occupancy<BM, BN>())) void gemm_planes
         mma_slice<NP, 1, CB, F16>(fa, fb, reinterpret_cast<float4_t(&)[1][CB]>(facc[i]));
       }
     }
+    if constexpr (BD) {
+      if (has_next)
+#pragma unroll
+        for (int j = 0; j < CB; ++j)
+#pragma unroll
+          for (int q = 0; q < NP; ++q) fbd[j][q] = fbn[j][q];
+    }
     if (++since_flush == kMaxChunk / kBK && has_next) {
       flush();
       since_flush = 0;
@@ -367,6 +410,8 @@ struct PlanePlan {
 };
 
 int g_tile_override_m = 0, g_tile_override_n = 0;
+bool g_bdirect = false;  // direct B fragments (selected by measurement)
+int g_tile_order = 0;    // see PParams::order
 
 PlanePlan plan_planes(int M, int N, int K, int batch, int force_bm = 0, int force_bn = 0) {
   PlanePlan pl;
@@ -428,13 +473,15 @@ void launch_planes(const PlaneGemmDesc& d, const PlanePlan& pl, double* ws, size
     p.sa = d.A.scale; p.sb = d.B.scale; p.C = d.C; p.W = ws;
     p.lda = d.lda; p.ldb = d.ldb; p.ldc = d.ldc; p.sA = d.sA; p.sB = d.sB; p.sC = d.sC;
     p.M = d.M; p.N = d.N; p.K = d.K; p.kchunk = kchunk; p.nsplit = nsplit; p.batch0 = b0;
-    p.tiles_m = tiles_m; p.tiles_n = tiles_n; p.per_xcd = per_xcd;
+    p.tiles_m = tiles_m; p.tiles_n = tiles_n; p.per_xcd = per_xcd; p.order = g_tile_order;
     p.alpha = d.alpha; p.beta = d.beta;
     dim3 grid((unsigned)(per_xcd * kXcd), (unsigned)(nb * nsplit));
-    if (vec)
-      hipLaunchKernelGGL((gemm_planes_kernel<BM, BN, NP, F16, true>), grid, dim3(kThr), 0, stream, p);
+    if (vec && NP == 2 && g_bdirect)
+      hipLaunchKernelGGL((gemm_planes_kernel<BM, BN, NP, F16, true, NP == 2>), grid, dim3(kThr), 0, stream, p);
+    else if (vec)
+      hipLaunchKernelGGL((gemm_planes_kernel<BM, BN, NP, F16, true, false>), grid, dim3(kThr), 0, stream, p);
     else
-      hipLaunchKernelGGL((gemm_planes_kernel<BM, BN, NP, F16, false>), grid, dim3(kThr), 0, stream, p);
+      hipLaunchKernelGGL((gemm_planes_kernel<BM, BN, NP, F16, false, false>), grid, dim3(kThr), 0, stream, p);
     check_launch("gemm_planes");
     if (nsplit > 1) splitk_reduce(ws, d.C, d.ldc, d.sC, d.M, d.N, nsplit, b0, nb, d.beta, stream);
   }
@@ -514,6 +561,12 @@ void split_to_planes(PlaneKind kind, const Planes& out, const long n[4], const T
   SplitOut so{{out.p[0], out.p[1], out.p[2]}, out.scale, (int)kind};
   hipLaunchKernelGGL(split_write_kernel, perm_grid(g), dim3(kPThreads), 0, stream, g, so);
   check_launch("split_to_planes");
+}
+
+void set_plane_bdirect(bool on) { g_bdirect = on; }
+void set_plane_tile_order(int order) {
+  if (order < 0 || order > 2) throw std::invalid_argument("unknown tile order");
+  g_tile_order = order;
 }
 
 void set_plane_tile_override(int bm, int bn) {

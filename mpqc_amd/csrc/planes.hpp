@@ -63,6 +63,11 @@ void gemm_planes(const PlaneGemmDesc& d, double* ws, size_t ws_bytes, hipStream_
 // Forces the contraction tile (bm x bn; 0, 0 restores the planner's choice).
 // Used by measurements that compare tile shapes.
 void set_plane_tile_override(int bm, int bn);
+// Selects the contraction variant that loads B fragments straight into registers.
+void set_plane_bdirect(bool on);
+// Tile order of the contraction grid: 0 XCD ranges of row-major tiles, 1 plain row-major,
+// 2 XCD ranges of column-major tiles.
+void set_plane_tile_order(int order);
 // Workspace (bytes) gemm_planes needs for its split-K partial sums at this shape.
 size_t gemm_planes_workspace_bytes(int M, int N, int K, int batch);
 
