@@ -361,10 +361,22 @@ int CCSDContext::lambda_solve(int max_iter, double target_precision, int verbose
     reverse_sweep(res_tape_, false);
     {
       const double* x1[2] = {adj(t1_), gt1_};
-      const double* x2[2] = {adj(t2_), gt2_};
       const double c[2] = {1.0, 1.0};
       lincomb(res1, x1, c, 2, n1, stream_);
-      lincomb(res2, x2, c, 2, n2, stream_);
+      // t2 part projected onto amplitudes with t2(a,b,i,j) = t2(b,a,j,i): the residual is
+      // symmetrised (ccsd_r1_r2.h:260), so the amplitudes stay in that subspace and the
+      // Lambda equations only need to hold there (dE/dt2 is symmetric already)
+      const long O = o_, V = v_;
+      const int I[4] = {0, 1, 2, 3}, sw[4] = {1, 0, 3, 2};
+      TTerm ts[3];
+      long e[4] = {V, V, O, O};
+      for (int k = 0; k < 3; ++k) {
+        ts[k].p = k < 2 ? adj(t2_) : gt2_;
+        permuted_strides(e, k == 1 ? sw : I, ts[k].s);
+        ts[k].alpha = k < 2 ? 0.5 : 1.0;
+        ts[k].vol = n2;
+      }
+      tadd(res2, e, 0.0, ts, 3, stream_);
     }
     dot(res1, res1, n1, scalars_ + 0, partials_, stream_);
     dot(res2, res2, n2, scalars_ + 1, partials_, stream_);
