@@ -363,20 +363,27 @@ int CCSDContext::lambda_solve(int max_iter, double target_precision, int verbose
       const double* x1[2] = {adj(t1_), gt1_};
       const double c[2] = {1.0, 1.0};
       lincomb(res1, x1, c, 2, n1, stream_);
-      // t2 part projected onto amplitudes with t2(a,b,i,j) = t2(b,a,j,i): the residual is
-      // symmetrised (ccsd_r1_r2.h:260), so the amplitudes stay in that subspace and the
-      // Lambda equations only need to hold there (dE/dt2 is symmetric already)
+      // By default the t2 part is projected onto amplitudes with t2(a,b,i,j) = t2(b,a,j,i).
+      // For integrals with the permutational symmetry of real orbitals the residual keeps
+      // t2 in that subspace (ccsd_r1_r2.h:260 symmetrises, and the ladders added after it
+      // are symmetric), so the Lambda equations only need to hold there; the
+      // antisymmetric block of the Jacobian lacks the orbital-energy diagonal, which makes
+      // the full-space solve converge slowly.  With "lambda_symmetric" = 0 the full
+      // space is solved: exact also for perturbations that break the integrals'
+      // symmetry (which take t2 out of the subspace).
       const long O = o_, V = v_;
       const int I[4] = {0, 1, 2, 3}, sw[4] = {1, 0, 3, 2};
       TTerm ts[3];
       long e[4] = {V, V, O, O};
-      for (int k = 0; k < 3; ++k) {
-        ts[k].p = k < 2 ? adj(t2_) : gt2_;
-        permuted_strides(e, k == 1 ? sw : I, ts[k].s);
-        ts[k].alpha = k < 2 ? 0.5 : 1.0;
+      const int nt = lambda_symmetric_ ? 3 : 2;
+      for (int k = 0; k < nt; ++k) {
+        const bool g = lambda_symmetric_ ? k == 2 : k == 1;
+        ts[k].p = g ? gt2_ : adj(t2_);
+        permuted_strides(e, (lambda_symmetric_ && k == 1) ? sw : I, ts[k].s);
+        ts[k].alpha = (lambda_symmetric_ && !g) ? 0.5 : 1.0;
         ts[k].vol = n2;
       }
-      tadd(res2, e, 0.0, ts, 3, stream_);
+      tadd(res2, e, 0.0, ts, nt, stream_);
     }
     dot(res1, res1, n1, scalars_ + 0, partials_, stream_);
     dot(res2, res2, n2, scalars_ + 1, partials_, stream_);

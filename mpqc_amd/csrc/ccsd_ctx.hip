@@ -212,6 +212,9 @@ void CCSDContext::set_option(const std::string& key, double value) {
   if (key == "fov_sign") {
     if (value != 1.0 && value != -1.0) throw std::invalid_argument("fov_sign must be +1 or -1");
     fov_sign_ = value;
+  } else if (key == "lambda_symmetric") {
+    lambda_symmetric_ = value != 0.0;
+    lambda_ready_ = false;
   } else if (key == "kernel_timing") {
     kernel_timing_ = value != 0.0;
   } else if (key == "precision") {

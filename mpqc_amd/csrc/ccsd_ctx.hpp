@@ -145,6 +145,7 @@ class CCSDContext {
   double* adj_scratch(long n);
   double* adj_hist(int k, long n);
   bool lambda_ready_ = false, gradient_ready_ = false;
+  bool lambda_symmetric_ = true;  // solve in the (ab)(ij)-symmetric t2 subspace
   int alloc_index(const double* p) const;
   double* adj(const double* p);
   bool is_integral(const double* p) const;

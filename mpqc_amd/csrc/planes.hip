@@ -26,7 +26,12 @@ namespace {
 
 constexpr int kThr = 256;
 constexpr int kBK = 32;
-constexpr int kRow = kBK + 8;  // LDS row in halves (16-byte aligned rows)
+// LDS rows of one K slice: 32 halves = 64 bytes = four 16-byte chunks, unpadded;
+// chunk c of row r is stored at chunk c ^ ((r >> 2) & 3).  A 16-lane phase of the
+// fragment reads (16 rows, one chunk) and of the staging writes (4 rows, 4
+// chunks) then touches 16 different 16-byte bank groups.
+constexpr int kRow = kBK;
+__device__ inline int swz(int row, int chunk) { return (chunk ^ ((row >> 2) & 3)) * 8; }
 constexpr int kXcd = 8;
 constexpr int kMaxChunk = 2048;  // longest K summed in FP32
 
@@ -294,14 +299,14 @@ __global__ __launch_bounds__(kThr, (plane_occupancy<BM, BN>())) void gemm_planes
       const int ch = tid + kThr * c;
       const int row = ch >> 2, kc = (ch & 3) * 8;
 #pragma unroll
-      for (int q = 0; q < NP; ++q) *reinterpret_cast<u16x8_t*>(&As[buf][q][row][kc]) = ra[q][c];
+      for (int q = 0; q < NP; ++q) *reinterpret_cast<u16x8_t*>(&As[buf][q][row][swz(row, kc / 8)]) = ra[q][c];
     }
 #pragma unroll
     for (int c = 0; c < BCH; ++c) {
       const int ch = tid + kThr * c;
       const int row = ch >> 2, kc = (ch & 3) * 8;
 #pragma unroll
-      for (int q = 0; q < NP; ++q) *reinterpret_cast<u16x8_t*>(&Bs[buf][q][row][kc]) = rb[q][c];
+      for (int q = 0; q < NP; ++q) *reinterpret_cast<u16x8_t*>(&Bs[buf][q][row][swz(row, kc / 8)]) = rb[q][c];
     }
   };
 
@@ -362,7 +367,7 @@ __global__ __launch_bounds__(kThr, (plane_occupancy<BM, BN>())) void gemm_planes
       for (int j = 0; j < CB; ++j)
 #pragma unroll
         for (int q = 0; q < NP; ++q)
-          fb[j][q] = *reinterpret_cast<const u16x8_t*>(&Bs[buf][q][wn * WN + j * 16 + lr][lk * 8]);
+          fb[j][q] = *reinterpret_cast<const u16x8_t*>(&Bs[buf][q][wn * WN + j * 16 + lr][swz(lr, lk)]);
     }
     if constexpr (NP == 2) {
       // all fragments of the slice first (one LDS latency per slice), then the products
@@ -371,7 +376,7 @@ __global__ __launch_bounds__(kThr, (plane_occupancy<BM, BN>())) void gemm_planes
       for (int i = 0; i < RB; ++i)
 #pragma unroll
         for (int q = 0; q < NP; ++q)
-          fa[i][q] = *reinterpret_cast<const u16x8_t*>(&As[buf][q][wm * WM + i * 16 + lr][lk * 8]);
+          fa[i][q] = *reinterpret_cast<const u16x8_t*>(&As[buf][q][wm * WM + i * 16 + lr][swz(lr, lk)]);
       mma_slice<NP, RB, CB, F16>(fa, fb, facc);
     } else {
       // three planes: one row block at a time keeps two workgroups per CU
@@ -380,7 +385,7 @@ __global__ __launch_bounds__(kThr, (plane_occupancy<BM, BN>())) void gemm_planes
         u16x8_t fa[1][NP];
 #pragma unroll
         for (int q = 0; q < NP; ++q)
-          fa[0][q] = *reinterpret_cast<const u16x8_t*>(&As[buf][q][wm * WM + i * 16 + lr][lk * 8]);
+          fa[0][q] = *reinterpret_cast<const u16x8_t*>(&As[buf][q][wm * WM + i * 16 + lr][swz(lr, lk)]);
         mma_slice<NP, 1, CB, F16>(fa, fb, reinterpret_cast<float4_t(&)[1][CB]>(facc[i]));
       }
     }

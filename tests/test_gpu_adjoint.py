@@ -102,30 +102,80 @@ def test_field_gradient_matches_central_differences(idx):
     assert abs(dEdF - fd) <= 1e-6 * abs(fd)
 
 
-def test_integral_gradients_match_directional_differences():
+def _symmetric_mo_perturbation(o, v, rng):
+    """Blocks of a random perturbation of the MO two-electron integrals with the
+    8-fold permutational symmetry of real orbitals, (pq|rs) in chemists'
+    notation, cut into the reference's physicist blocks <pq|rs> = (pr|qs), plus
+    symmetric Fock blocks."""
+    n = o + v
+    X = rng.normal(size=(n, n, n, n))
+    X = X + X.transpose(1, 0, 2, 3)
+    X = X + X.transpose(0, 1, 3, 2)
+    X = X + X.transpose(2, 3, 0, 1)
+    occ, vir = slice(0, o), slice(o, n)
+
+    def phys(A, B, C, D):
+        return np.ascontiguousarray(X[A, C, B, D].transpose(0, 2, 1, 3))
+
+    F = rng.normal(size=(n, n))
+    F = F + F.T
+    return {"Gijab": phys(occ, occ, vir, vir), "Gijkl": phys(occ, occ, occ, occ),
+            "Giajb": phys(occ, vir, occ, vir), "Giabc": phys(occ, vir, vir, vir),
+            "Gijka": phys(occ, occ, occ, vir), "Gabcd": phys(vir, vir, vir, vir),
+            "Fia": F[occ, vir], "Fij": F[occ, occ], "Fab": F[vir, vir]}
+
+
+def _gradient(inp, symmetric=True):
     from mpqc_amd.ccsd import CCSD, Integrals
-    case = FIX["cases"][0]
-    _, inp = _inputs(case)
     with CCSD(inp.o, inp.v, precision="f64", max_iter=200) as c:
+        c.set_option("lambda_symmetric", 1 if symmetric else 0)
         c.set_integrals(Integrals(**inp.blocks))
         c.set_orbital_energies(inp.eps_occ, inp.eps_vir)
         c.compute_ccsd(target_precision=PREC)
-        c.lambda_solve(target_precision=1e-14)
-        grad = c.energy_gradient()
+        n = c.lambda_solve(target_precision=1e-14, max_iter=400)
+        return c.energy_gradient(), n
+
+
+def test_integral_gradients_match_symmetric_perturbations():
+    """All nine blocks at once along a perturbation of the MO integrals that keeps
+    their permutational symmetry (the physical case: fields, orbital changes)."""
+    case = FIX["cases"][0]
+    _, inp = _inputs(case)
+    grad, n = _gradient(inp)
     rng = np.random.default_rng(5)
-    h = 5e-4
-    for name in ("Gabcd", "Giabc", "Gijab", "Gijkl", "Giajb", "Gijka", "Fia", "Fij", "Fab"):
-        # unit direction mixing the adjoint gradient and a random direction
-        g = grad[name]
-        r = rng.normal(size=g.shape)
-        X = (g / np.linalg.norm(g) + r / np.linalg.norm(r)) / np.sqrt(2.0)
+    dX = _symmetric_mo_perturbation(inp.o, inp.v, rng)
+    norm = np.sqrt(sum(np.vdot(x, x) for x in dX.values()))
+    dX = {k: x / norm for k, x in dX.items()}
 
-        def e_at(s):
-            b = dict(inp.blocks)
-            b[name] = inp.blocks[name] + s * X
-            return _energy(b, inp.eps_occ, inp.eps_vir)
+    def e_at(s):
+        return _energy({k: inp.blocks[k] + s * dX[k] for k in inp.blocks}, inp.eps_occ, inp.eps_vir)
 
-        fd = _central_difference(e_at, 0.0, h)
-        an = float(np.vdot(g, X))
-        print(f"{name}: |grad| {np.linalg.norm(g):.4e} adjoint {an:.10e} central difference {fd:.10e}")
-        assert abs(an - fd) <= 1e-6 * abs(fd)
+    fd = _central_difference(e_at, 0.0, 5e-4)
+    an = sum(float(np.vdot(grad[k], dX[k])) for k in dX)
+    print(f"symmetric perturbation ({n} Lambda iterations): adjoint {an:.10e} central difference {fd:.10e}")
+    for k in dX:
+        print(f"  {k}: <grad, dX> = {float(np.vdot(grad[k], dX[k])):.6e}")
+    assert abs(an - fd) <= 1e-6 * abs(fd)
+
+
+@pytest.mark.parametrize("name", ["Gabcd", "Gijkl", "Gijab", "Giabc"])
+def test_full_space_gradients_match_arbitrary_perturbations(name):
+    """With the full-space Lambda solve the block gradients hold along arbitrary
+    (symmetry-breaking) directions too."""
+    case = FIX["cases"][0]
+    _, inp = _inputs(case)
+    grad, n = _gradient(inp, symmetric=False)
+    rng = np.random.default_rng(6)
+    g = grad[name]
+    r = rng.normal(size=g.shape)
+    X = (g / np.linalg.norm(g) + r / np.linalg.norm(r)) / np.sqrt(2.0)
+
+    def e_at(s):
+        b = dict(inp.blocks)
+        b[name] = inp.blocks[name] + s * X
+        return _energy(b, inp.eps_occ, inp.eps_vir)
+
+    fd = _central_difference(e_at, 0.0, 5e-4)
+    an = float(np.vdot(g, X))
+    print(f"{name} ({n} Lambda iterations): adjoint {an:.10e} central difference {fd:.10e}")
+    assert abs(an - fd) <= 1e-6 * abs(fd)
