@@ -54,8 +54,9 @@ class CCSDEnergy(torch.autograd.Function):
     def backward(ctx, grad_e):
         c = ctx.c
         try:
+            # the error measure divides |r| by the element count, so the default is tight
             last_info["lambda_iterations"] = c.lambda_solve(
-                target_precision=ctx.opts.get("lambda_precision", 1e-12),
+                target_precision=ctx.opts.get("lambda_precision", 1e-15),
                 max_iter=ctx.opts.get("lambda_max_iter", 200))
             g = c.energy_gradient()
         finally:
