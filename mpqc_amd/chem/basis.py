@@ -28,6 +28,11 @@ def _load():
         lib.chem_one_electron.restype = ctypes.c_int
         lib.chem_one_electron.argtypes = [ctypes.c_int, P, P, P, P, P, P, P, ctypes.c_int,
                                           ctypes.c_int, P, P, P, P, P, P, P, P, P]
+        lib.chem_eri3.restype = ctypes.c_int
+        lib.chem_eri3.argtypes = [ctypes.c_int, P, P, P, P, P, P, P, ctypes.c_int,
+                                  ctypes.c_int, P, P, P, P, P, P, P, ctypes.c_int, P]
+        lib.chem_eri2.restype = ctypes.c_int
+        lib.chem_eri2.argtypes = [ctypes.c_int, P, P, P, P, P, P, P, ctypes.c_int, P]
         lib.chem_jk.restype = ctypes.c_int
         lib.chem_jk.argtypes = [ctypes.c_int, P, P, P, P]
         lib.chem_eri.restype = ctypes.c_int
@@ -112,6 +117,27 @@ class BasisSet:
                           self._p(out))
         if st != 0:
             raise RuntimeError("ERI evaluation failed")
+        return out
+
+    def eri3(self, aux: "BasisSet") -> np.ndarray:
+        """Three-centre Coulomb integrals (P|mu nu), P over ``aux``: (naux, nbf, nbf)."""
+        lib = _load()
+        out = np.empty((aux.nbf, self.nbf, self.nbf))
+        a, b = self._arrays(), aux._arrays()
+        st = lib.chem_eri3(len(self.shells), *[self._p(x) for x in a], self.nbf, len(aux.shells),
+                           *[self._p(x) for x in b], aux.nbf, self._p(out))
+        if st != 0:
+            raise RuntimeError("three-centre integrals failed")
+        return out
+
+    def eri2(self) -> np.ndarray:
+        """Two-centre Coulomb metric (P|Q) over this basis: (nbf, nbf)."""
+        lib = _load()
+        out = np.empty((self.nbf, self.nbf))
+        st = lib.chem_eri2(len(self.shells), *[self._p(x) for x in self._arrays()], self.nbf,
+                           self._p(out))
+        if st != 0:
+            raise RuntimeError("two-centre integrals failed")
         return out
 
 

@@ -296,6 +296,125 @@ double comp_norm(int i, int j, int k) {
   return std::sqrt(dfact(2 * l - 1) / (dfact(2 * i - 1) * dfact(2 * j - 1) * dfact(2 * k - 1)));
 }
 
+
+// Cartesian (ab|cd) of one shell quartet (bra pair a b, ket pair c d), component-normalised.
+void quartet_cart(const std::vector<Shell>& sh, const ShellPair& bra, const ShellPair& ket,
+                std::vector<double>& cart) {
+  const Shell& A = sh[bra.a];
+  const Shell& B = sh[bra.b];
+  const Shell& C = sh[ket.a];
+  const Shell& D = sh[ket.b];
+  int ca[10][3], cb[10][3], cc[10][3], cd[10][3];
+  cart_list(A.l, ca); cart_list(B.l, cb); cart_list(C.l, cc); cart_list(D.l, cd);
+  const int nca = ncart(A.l), ncb = ncart(B.l), ncc = ncart(C.l), ncd = ncart(D.l);
+  const int Lab = A.l + B.l, Lcd = C.l + D.l, L = Lab + Lcd;
+  const int nket = ncc * ncd;
+  // W[t][u][v][ket cart pair]
+  const int dimH = Lab + 1;
+  std::vector<double> W((size_t)dimH * dimH * dimH * nket, 0.0);
+  cart.assign((size_t)nca * ncb * nket, 0.0);
+  RTab R;
+  for (const PrimPair& P : bra.pp) {
+    std::fill(W.begin(), W.end(), 0.0);
+    for (const PrimPair& Q : ket.pp) {
+      const double p = P.p, q = Q.p;
+      const double alpha = p * q / (p + q);
+      hermite_R(L, alpha, P.P[0] - Q.P[0], P.P[1] - Q.P[1], P.P[2] - Q.P[2], R);
+      const double pref = 2.0 * std::pow(kPi, 2.5) / (p * q * std::sqrt(p + q));
+      for (int z = 0; z < ncc; ++z)
+        for (int w = 0; w < ncd; ++w) {
+          const int i2 = cc[z][0], j2 = cc[z][1], k2 = cc[z][2];
+          const int i3 = cd[w][0], j3 = cd[w][1], k3 = cd[w][2];
+          const int kp = z * ncd + w;
+          for (int t = 0; t <= Lab; ++t)
+            for (int u = 0; u <= Lab - t; ++u)
+              for (int v = 0; v <= Lab - t - u; ++v) {
+                double s = 0.0;
+                for (int tau = 0; tau <= i2 + i3; ++tau) {
+                  const double ex = Q.Ex.v[i2][i3][tau];
+                  if (ex == 0.0) continue;
+                  for (int nu = 0; nu <= j2 + j3; ++nu) {
+                    const double exy = ex * Q.Ey.v[j2][j3][nu];
+                    if (exy == 0.0) continue;
+                    for (int phi = 0; phi <= k2 + k3; ++phi) {
+                      const double e = exy * Q.Ez.v[k2][k3][phi];
+                      const double r = R.v[t + tau][u + nu][v + phi];
+                      s += ((tau + nu + phi) & 1) ? -e * r : e * r;
+                    }
+                  }
+                }
+                W[(((size_t)t * dimH + u) * dimH + v) * nket + kp] += pref * s;
+              }
+        }
+    }
+    for (int x = 0; x < nca; ++x)
+      for (int y = 0; y < ncb; ++y) {
+        const int i0 = ca[x][0], j0 = ca[x][1], k0 = ca[x][2];
+        const int i1 = cb[y][0], j1 = cb[y][1], k1 = cb[y][2];
+        double* dst = &cart[((size_t)x * ncb + y) * nket];
+        for (int t = 0; t <= i0 + i1; ++t) {
+          const double ex = P.Ex.v[i0][i1][t];
+          if (ex == 0.0) continue;
+          for (int u = 0; u <= j0 + j1; ++u) {
+            const double exy = ex * P.Ey.v[j0][j1][u];
+            if (exy == 0.0) continue;
+            for (int v = 0; v <= k0 + k1; ++v) {
+              const double e = exy * P.Ez.v[k0][k1][v];
+              if (e == 0.0) continue;
+              const double* src = &W[(((size_t)t * dimH + u) * dimH + v) * nket];
+              for (int kp = 0; kp < nket; ++kp) dst[kp] += e * src[kp];
+            }
+          }
+        }
+      }
+  }
+  // Cartesian component normalisation
+  for (int x = 0; x < nca; ++x)
+    for (int y = 0; y < ncb; ++y)
+      for (int z = 0; z < ncc; ++z)
+        for (int w = 0; w < ncd; ++w)
+          cart[(((size_t)x * ncb + y) * ncc + z) * ncd + w] *=
+              comp_norm(ca[x][0], ca[x][1], ca[x][2]) * comp_norm(cb[y][0], cb[y][1], cb[y][2]) *
+              comp_norm(cc[z][0], cc[z][1], cc[z][2]) * comp_norm(cd[w][0], cd[w][1], cd[w][2]);
+}
+
+// transform a Cartesian quartet to the output basis: out4[na][nb][nc][nd]
+void quartet_to_output(const Shell* const S4[4], const std::vector<double>& cart, std::vector<double>& res) {
+  int nc[4], nf[4];
+  std::vector<double> M[4];
+  for (int k = 0; k < 4; ++k) {
+    nc[k] = ncart(S4[k]->l);
+    nf[k] = nfunc(*S4[k]);
+    if (S4[k]->pure) sph_matrix(S4[k]->l, M[k]);
+  }
+  std::vector<double> cur(cart), nxt;
+  int dims[4] = {nc[0], nc[1], nc[2], nc[3]};
+  for (int k = 0; k < 4; ++k) {
+    if (!S4[k]->pure) continue;
+    int nd[4] = {dims[0], dims[1], dims[2], dims[3]};
+    nd[k] = nf[k];
+    nxt.assign((size_t)nd[0] * nd[1] * nd[2] * nd[3], 0.0);
+    for (int a = 0; a < nd[0]; ++a)
+      for (int b = 0; b < nd[1]; ++b)
+        for (int c = 0; c < nd[2]; ++c)
+          for (int d = 0; d < nd[3]; ++d) {
+            int idx[4] = {a, b, c, d};
+            const int f = idx[k];
+            double s = 0.0;
+            for (int x = 0; x < nc[k]; ++x) {
+              const double m = M[k][f * nc[k] + x];
+              if (m == 0.0) continue;
+              int j[4] = {a, b, c, d};
+              j[k] = x;
+              s += m * cur[(((size_t)j[0] * dims[1] + j[1]) * dims[2] + j[2]) * dims[3] + j[3]];
+            }
+            nxt[(((size_t)a * nd[1] + b) * nd[2] + c) * nd[3] + d] = s;
+          }
+    cur.swap(nxt);
+    for (int q = 0; q < 4; ++q) dims[q] = nd[q];
+  }
+  res.swap(cur);
+}
 }  // namespace
 
 extern "C" {
@@ -474,120 +593,11 @@ int chem_eri(int nshell, const int* l, const int* pure, const int* nprim, const 
     std::vector<double> schwarz(np, 0.0);
 
     auto quartet = [&](const ShellPair& bra, const ShellPair& ket, std::vector<double>& cart) {
-      const Shell& A = sh[bra.a];
-      const Shell& B = sh[bra.b];
-      const Shell& C = sh[ket.a];
-      const Shell& D = sh[ket.b];
-      int ca[10][3], cb[10][3], cc[10][3], cd[10][3];
-      cart_list(A.l, ca); cart_list(B.l, cb); cart_list(C.l, cc); cart_list(D.l, cd);
-      const int nca = ncart(A.l), ncb = ncart(B.l), ncc = ncart(C.l), ncd = ncart(D.l);
-      const int Lab = A.l + B.l, Lcd = C.l + D.l, L = Lab + Lcd;
-      const int nket = ncc * ncd;
-      // W[t][u][v][ket cart pair]
-      const int dimH = Lab + 1;
-      std::vector<double> W((size_t)dimH * dimH * dimH * nket, 0.0);
-      cart.assign((size_t)nca * ncb * nket, 0.0);
-      RTab R;
-      for (const PrimPair& P : bra.pp) {
-        std::fill(W.begin(), W.end(), 0.0);
-        for (const PrimPair& Q : ket.pp) {
-          const double p = P.p, q = Q.p;
-          const double alpha = p * q / (p + q);
-          hermite_R(L, alpha, P.P[0] - Q.P[0], P.P[1] - Q.P[1], P.P[2] - Q.P[2], R);
-          const double pref = 2.0 * std::pow(kPi, 2.5) / (p * q * std::sqrt(p + q));
-          for (int z = 0; z < ncc; ++z)
-            for (int w = 0; w < ncd; ++w) {
-              const int i2 = cc[z][0], j2 = cc[z][1], k2 = cc[z][2];
-              const int i3 = cd[w][0], j3 = cd[w][1], k3 = cd[w][2];
-              const int kp = z * ncd + w;
-              for (int t = 0; t <= Lab; ++t)
-                for (int u = 0; u <= Lab - t; ++u)
-                  for (int v = 0; v <= Lab - t - u; ++v) {
-                    double s = 0.0;
-                    for (int tau = 0; tau <= i2 + i3; ++tau) {
-                      const double ex = Q.Ex.v[i2][i3][tau];
-                      if (ex == 0.0) continue;
-                      for (int nu = 0; nu <= j2 + j3; ++nu) {
-                        const double exy = ex * Q.Ey.v[j2][j3][nu];
-                        if (exy == 0.0) continue;
-                        for (int phi = 0; phi <= k2 + k3; ++phi) {
-                          const double e = exy * Q.Ez.v[k2][k3][phi];
-                          const double r = R.v[t + tau][u + nu][v + phi];
-                          s += ((tau + nu + phi) & 1) ? -e * r : e * r;
-                        }
-                      }
-                    }
-                    W[(((size_t)t * dimH + u) * dimH + v) * nket + kp] += pref * s;
-                  }
-            }
-        }
-        for (int x = 0; x < nca; ++x)
-          for (int y = 0; y < ncb; ++y) {
-            const int i0 = ca[x][0], j0 = ca[x][1], k0 = ca[x][2];
-            const int i1 = cb[y][0], j1 = cb[y][1], k1 = cb[y][2];
-            double* dst = &cart[((size_t)x * ncb + y) * nket];
-            for (int t = 0; t <= i0 + i1; ++t) {
-              const double ex = P.Ex.v[i0][i1][t];
-              if (ex == 0.0) continue;
-              for (int u = 0; u <= j0 + j1; ++u) {
-                const double exy = ex * P.Ey.v[j0][j1][u];
-                if (exy == 0.0) continue;
-                for (int v = 0; v <= k0 + k1; ++v) {
-                  const double e = exy * P.Ez.v[k0][k1][v];
-                  if (e == 0.0) continue;
-                  const double* src = &W[(((size_t)t * dimH + u) * dimH + v) * nket];
-                  for (int kp = 0; kp < nket; ++kp) dst[kp] += e * src[kp];
-                }
-              }
-            }
-          }
-      }
-      // Cartesian component normalisation
-      for (int x = 0; x < nca; ++x)
-        for (int y = 0; y < ncb; ++y)
-          for (int z = 0; z < ncc; ++z)
-            for (int w = 0; w < ncd; ++w)
-              cart[(((size_t)x * ncb + y) * ncc + z) * ncd + w] *=
-                  comp_norm(ca[x][0], ca[x][1], ca[x][2]) * comp_norm(cb[y][0], cb[y][1], cb[y][2]) *
-                  comp_norm(cc[z][0], cc[z][1], cc[z][2]) * comp_norm(cd[w][0], cd[w][1], cd[w][2]);
+      quartet_cart(sh, bra, ket, cart);
     };
 
-    // transform a Cartesian quartet to the output basis: out4[na][nb][nc][nd]
-    auto to_output = [&](const Shell* S4[4], const std::vector<double>& cart, std::vector<double>& res) {
-      int nc[4], nf[4];
-      std::vector<double> M[4];
-      for (int k = 0; k < 4; ++k) {
-        nc[k] = ncart(S4[k]->l);
-        nf[k] = nfunc(*S4[k]);
-        if (S4[k]->pure) sph_matrix(S4[k]->l, M[k]);
-      }
-      std::vector<double> cur(cart), nxt;
-      int dims[4] = {nc[0], nc[1], nc[2], nc[3]};
-      for (int k = 0; k < 4; ++k) {
-        if (!S4[k]->pure) continue;
-        int nd[4] = {dims[0], dims[1], dims[2], dims[3]};
-        nd[k] = nf[k];
-        nxt.assign((size_t)nd[0] * nd[1] * nd[2] * nd[3], 0.0);
-        for (int a = 0; a < nd[0]; ++a)
-          for (int b = 0; b < nd[1]; ++b)
-            for (int c = 0; c < nd[2]; ++c)
-              for (int d = 0; d < nd[3]; ++d) {
-                int idx[4] = {a, b, c, d};
-                const int f = idx[k];
-                double s = 0.0;
-                for (int x = 0; x < nc[k]; ++x) {
-                  const double m = M[k][f * nc[k] + x];
-                  if (m == 0.0) continue;
-                  int j[4] = {a, b, c, d};
-                  j[k] = x;
-                  s += m * cur[(((size_t)j[0] * dims[1] + j[1]) * dims[2] + j[2]) * dims[3] + j[3]];
-                }
-                nxt[(((size_t)a * nd[1] + b) * nd[2] + c) * nd[3] + d] = s;
-              }
-        cur.swap(nxt);
-        for (int q = 0; q < 4; ++q) dims[q] = nd[q];
-      }
-      res.swap(cur);
+    auto to_output = [](const Shell* const S4[4], const std::vector<double>& cart, std::vector<double>& res) {
+      quartet_to_output(S4, cart, res);
     };
 
     // Schwarz factors sqrt(max |(ab|ab)|)
@@ -631,6 +641,105 @@ int chem_eri(int nshell, const int* l, const int* pure, const int* nprim, const 
                 out[((k * n + l2) * n + j) * n + i] = val;
                 out[((l2 * n + k) * n + j) * n + i] = val;
               }
+      }
+    }
+    return 0;
+  } catch (...) {
+    return 1;
+  }
+}
+
+}  // extern "C"
+
+extern "C" {
+// Density-fitting Coulomb integrals: three-centre (P|mu nu) over auxiliary
+// shells P and orbital shells mu, nu, written to out[naux][nbf][nbf], and
+// two-centre (P|Q) written to out[naux][naux].  The missing fourth function is
+// the constant 1 (an s primitive of exponent 0 with coefficient 1), so each
+// block is the quartet (P 1|mu nu) or (P 1|Q 1) of the four-centre code.
+int chem_eri3(int nshell, const int* l, const int* pure, const int* nprim, const double* exps,
+              const double* coefs, const double* centers, const int* bf, int nbf, int nashell,
+              const int* al, const int* apure, const int* anprim, const double* aexps,
+              const double* acoefs, const double* acenters, const int* abf, int naux, double* out) {
+  try {
+    init_boys();
+    auto sh = make_shells(nshell, l, pure, nprim, exps, coefs, centers, bf);
+    auto ash = make_shells(nashell, al, apure, anprim, aexps, acoefs, acenters, abf);
+    const int a0 = (int)sh.size();
+    sh.insert(sh.end(), ash.begin(), ash.end());
+    Shell unit;
+    unit.l = 0; unit.pure = false; unit.a = {0.0}; unit.c = {1.0};
+    unit.R[0] = unit.R[1] = unit.R[2] = 0.0; unit.bf = 0;
+    const int iu = (int)sh.size();
+    sh.push_back(unit);
+    std::vector<ShellPair> aux(nashell), orb;
+    for (int P = 0; P < nashell; ++P) {
+      sh[iu].R[0] = sh[a0 + P].R[0]; sh[iu].R[1] = sh[a0 + P].R[1]; sh[iu].R[2] = sh[a0 + P].R[2];
+      make_pair(sh[a0 + P], sh[iu], a0 + P, iu, aux[P]);
+    }
+    for (int a = 0; a < nshell; ++a)
+      for (int b = 0; b <= a; ++b) {
+        ShellPair sp;
+        make_pair(sh[a], sh[b], a, b, sp);
+        orb.push_back(std::move(sp));
+      }
+    const size_t n = (size_t)nbf;
+    const long nwork = (long)nashell * (long)orb.size();
+#pragma omp parallel for schedule(dynamic, 4)
+    for (long w = 0; w < nwork; ++w) {
+      const int P = (int)(w / (long)orb.size());
+      const ShellPair& ket = orb[w % (long)orb.size()];
+      std::vector<double> cart, res;
+      quartet_cart(sh, aux[P], ket, cart);
+      const Shell* S4[4] = {&sh[a0 + P], &sh[iu], &sh[ket.a], &sh[ket.b]};
+      quartet_to_output(S4, cart, res);
+      const int np = nfunc(*S4[0]), nc = nfunc(*S4[2]), nd = nfunc(*S4[3]);
+      for (int x = 0; x < np; ++x)
+        for (int c = 0; c < nc; ++c)
+          for (int d = 0; d < nd; ++d) {
+            const double val = res[((size_t)x * nc + c) * nd + d];
+            const size_t q = S4[0]->bf + x, i = S4[2]->bf + c, j = S4[3]->bf + d;
+            out[(q * n + i) * n + j] = val;
+            out[(q * n + j) * n + i] = val;
+          }
+    }
+    (void)naux;
+    return 0;
+  } catch (...) {
+    return 1;
+  }
+}
+
+int chem_eri2(int nashell, const int* al, const int* apure, const int* anprim, const double* aexps,
+              const double* acoefs, const double* acenters, const int* abf, int naux, double* out) {
+  try {
+    init_boys();
+    auto sh = make_shells(nashell, al, apure, anprim, aexps, acoefs, acenters, abf);
+    Shell unit;
+    unit.l = 0; unit.pure = false; unit.a = {0.0}; unit.c = {1.0}; unit.bf = 0;
+    const int iu = (int)sh.size();
+    sh.push_back(unit);
+    std::vector<ShellPair> aux(nashell);
+    for (int P = 0; P < nashell; ++P) {
+      for (int d = 0; d < 3; ++d) sh[iu].R[d] = sh[P].R[d];
+      make_pair(sh[P], sh[iu], P, iu, aux[P]);
+    }
+    const size_t n = (size_t)naux;
+#pragma omp parallel for schedule(dynamic)
+    for (int P = 0; P < nashell; ++P) {
+      std::vector<double> cart, res;
+      for (int Q = 0; Q <= P; ++Q) {
+        quartet_cart(sh, aux[P], aux[Q], cart);
+        const Shell* S4[4] = {&sh[P], &sh[iu], &sh[Q], &sh[iu]};
+        quartet_to_output(S4, cart, res);
+        const int np = nfunc(sh[P]), nq = nfunc(sh[Q]);
+        for (int x = 0; x < np; ++x)
+          for (int y = 0; y < nq; ++y) {
+            const double val = res[(size_t)x * nq + y];
+            const size_t i = sh[P].bf + x, j = sh[Q].bf + y;
+            out[i * n + j] = val;
+            out[j * n + i] = val;
+          }
       }
     }
     return 0;

@@ -7,6 +7,10 @@ Data taken: the h2o.xyz geometry, RHF energies, CCSD iteration tables
   tests/validation/reference/outputs/h2o-eom-ccsd-direct-631g.out  (6-31G, frozen core)
   tests/validation/reference/outputs/h2o-eom-ip-ccsd-pvdz.out      (cc-pVDZ, all electron)
   tests/validation/reference/outputs/h2o-eom-ea-ccsd-pvdz.out      (cc-pVDZ, frozen core)
+and, under "df_cases", the density-fitted run
+  tests/validation/reference/outputs/h2o-ccsd-631g-pvdz.out        (RI-RHF 6-31G with the
+      cc-pVDZ fitting basis, CCSD method "df", reduced_abcd_memory, frozen core,
+      precision 1e-11)
 """
 import json
 import os
@@ -48,6 +52,16 @@ for fname, basis, fc, o, v, prec, scale, note in cases:
                          "basis": basis, "frozen_core": fc, "o": o, "v": v, "e_rhf": e_scf,
                          "e_ccsd": e, "precision": prec, "precision_scale_for_current_rule": scale,
                          "note": note, "trace": rows})
+out["df_cases"] = []
+for fname, basis, dfbs, fc, o, v, prec in [("h2o-ccsd-631g-pvdz.out", "6-31G", "cc-pVDZ", True, 4, 8,
+                                            1e-11)]:
+    rev, e_scf, rows, e = parse(fname)
+    out["df_cases"].append({"source": f"tests/validation/reference/outputs/{fname}", "revision": rev,
+                            "basis": basis, "df_basis": dfbs, "method": "df",
+                            "reduced_abcd_memory": True, "frozen_core": fc, "o": o, "v": v,
+                            "e_rhf": e_scf, "e_ccsd": e, "precision": prec,
+                            "precision_scale_for_current_rule": 1.0,
+                            "note": "RI-RHF reference; current convergence rule", "trace": rows})
 with open(os.path.join(HERE, "h2o_ccsd.json"), "w") as f:
     json.dump(out, f, indent=1)
-print("wrote", len(out["cases"]), "cases")
+print("wrote", len(out["cases"]), "cases and", len(out["df_cases"]), "DF cases")
