@@ -11,6 +11,11 @@
  *   mpqc_ccsd_set_integral        cc::Integrals<Array> members Fia..Gabcd
  *                                 (chemistry/qc/lcao/cc/ccsd_r1_r2.h:24-56), filled at
  *                                 ccsd.h:281-305
+ *   mpqc_ccsd_set_df_basis        the density-fitting basis of the "df" method: its size
+ *                                 and the TiledRange1 of the fitting index of
+ *                                 (K|G|p q)[inv_sqr] (ccsd.h:93-99, 481-493); selects
+ *                                 compute_cs_ccsd_r1_df / compute_cs_ccsd_r2_df
+ *                                 (ccsd_r1_r2.h:136-184, 309-520)
  *   mpqc_ccsd_set_orbital_energies JacobiDIISSolver(kv, f_ii, f_aa)
  *                                 (chemistry/qc/lcao/cc/solvers.h:1140-1146; values from
  *                                 make_diagonal_fpq, chemistry/qc/lcao/scf/mo_build.h:20-26)
@@ -101,7 +106,12 @@ enum {
   MPQC_INT_GIAJB = 5,
   MPQC_INT_GIABC = 6,
   MPQC_INT_GIJKA = 7,
-  MPQC_INT_GABCD = 8
+  MPQC_INT_GABCD = 8,
+  /* three-index density-fitting factors X(K,p,q) = (K|G|p q)[inv_sqr]
+   * (ccsd.h:481-493), index orders "K,a,i", "K,i,j", "K,a,b" */
+  MPQC_INT_XAI = 9,
+  MPQC_INT_XIJ = 10,
+  MPQC_INT_XAB = 11
 };
 
 /* Library version string. */
@@ -120,13 +130,24 @@ const char* mpqc_ccsd_last_error(const mpqc_ccsd_ctx* ctx);
 
 /* Numeric options: "fov_sign" (+1 reference F_VV as written, -1 exact CCSD for
  * non-Brillouin references), "precision" (MPQC_CCSD_PREC_*), "kernel_timing"
- * (1 = time every contraction with HIP events on the context stream). */
+ * (1 = time every contraction with HIP events on the context stream),
+ * "method" (0 standard, 1 df; ccsd.h:94-99), "reduced_abcd_memory" (1 default:
+ * the df method builds (ab|cd) lazily from Xab instead of reading Gabcd and
+ * Giabc, ccsd.h:102,292), "df_block_bytes" (device memory for the lazily built
+ * (ab|cd) blocks, default 4 GiB; set before the first residual). */
 int mpqc_ccsd_set_option(mpqc_ccsd_ctx* ctx, const char* key, double value);
 
 int mpqc_ccsd_set_integral(mpqc_ccsd_ctx* ctx, int which, const double* data, int layout,
                            int memspace);
 int mpqc_ccsd_set_orbital_energies(mpqc_ccsd_ctx* ctx, const double* eps_occ_act,
                                    const double* eps_vir);
+/* Declares naux fitting functions tiled by aux_hashmarks (NULL: one tile) and
+ * switches the context to the "df" residual.  Then MPQC_INT_XAI/XIJ/XAB are set
+ * with mpqc_ccsd_set_integral (the fitting index uses these hashmarks in
+ * MPQC_LAYOUT_TA_TILES buffers); Giabc and Gabcd are needed only with
+ * reduced_abcd_memory = 0.  The df residual runs in MPQC_CCSD_PREC_F64 and F32. */
+int mpqc_ccsd_set_df_basis(mpqc_ccsd_ctx* ctx, int naux, const int64_t* aux_hashmarks,
+                           int n_aux_tiles);
 int mpqc_ccsd_solver_init(mpqc_ccsd_ctx* ctx, int diis_start, int n_diis, double diis_damp,
                           int diis_ngroup, int diis_group_nstart);
 

@@ -17,6 +17,8 @@ PREC_F64, PREC_F32, PREC_BF16X3, PREC_F16X3, PREC_BF16X6 = 0, 1, 2, 3, 4
 LAYOUT_DENSE, LAYOUT_TA_TILES = 0, 1
 MEM_HOST, MEM_DEVICE = 0, 1
 INT_FIA, INT_FIJ, INT_FAB, INT_GIJAB, INT_GIJKL, INT_GIAJB, INT_GIABC, INT_GIJKA, INT_GABCD = range(9)
+INT_XAI, INT_XIJ, INT_XAB = 9, 10, 11
+DF_INTEGRAL_IDS = {"Xai": INT_XAI, "Xij": INT_XIJ, "Xab": INT_XAB}
 INTEGRAL_IDS = {"Fia": INT_FIA, "Fij": INT_FIJ, "Fab": INT_FAB, "Gijab": INT_GIJAB,
                 "Gijkl": INT_GIJKL, "Giajb": INT_GIAJB, "Giabc": INT_GIABC,
                 "Gijka": INT_GIJKA, "Gabcd": INT_GABCD}
@@ -24,7 +26,8 @@ INTEGRAL_IDS = {"Fia": INT_FIA, "Fij": INT_FIJ, "Fab": INT_FAB, "Gijab": INT_GIJ
 # every symbol declared in include/mpqc_ccsd.h
 EXPORTS = [
     "mpqc_ccsd_version", "mpqc_ccsd_ctx_create", "mpqc_ccsd_ctx_destroy", "mpqc_ccsd_last_error",
-    "mpqc_ccsd_set_option", "mpqc_ccsd_set_integral", "mpqc_ccsd_set_orbital_energies",
+    "mpqc_ccsd_set_option", "mpqc_ccsd_set_integral", "mpqc_ccsd_set_df_basis",
+    "mpqc_ccsd_set_orbital_energies",
     "mpqc_ccsd_solver_init", "mpqc_ccsd_set_amplitudes", "mpqc_ccsd_get_amplitudes",
     "mpqc_ccsd_get_residuals", "mpqc_ccsd_get_dressed_fock", "mpqc_ccsd_init_guess",
     "mpqc_ccsd_residual", "mpqc_ccsd_energy_error", "mpqc_ccsd_update", "mpqc_ccsd_solve",
@@ -70,6 +73,7 @@ def load():
         "mpqc_ccsd_set_option": (I, [P, ctypes.c_char_p, D]),
         "mpqc_ccsd_set_integral": (I, [P, I, P, I, I]),
         "mpqc_ccsd_set_orbital_energies": (I, [P, pD, pD]),
+        "mpqc_ccsd_set_df_basis": (I, [P, I, pI64, I]),
         "mpqc_ccsd_solver_init": (I, [P, I, I, D, I, I]),
         "mpqc_ccsd_set_amplitudes": (I, [P, P, P, I, I]),
         "mpqc_ccsd_get_amplitudes": (I, [P, P, P, I, I]),

@@ -160,17 +160,28 @@ class CCSD:
 
     def _modes(self, name):
         O, V = self.occ_hm, self.vir_hm
+        K = getattr(self, "aux_hm", None)
         return {"Fia": [O, V], "Fij": [O, O], "Fab": [V, V], "Gijab": [O, O, V, V],
                 "Gijkl": [O, O, O, O], "Giajb": [O, V, O, V], "Giabc": [O, V, V, V],
                 "Gijka": [O, O, O, V], "Gabcd": [V, V, V, V], "t1": [V, O],
-                "t2": [V, V, O, O]}[name]
+                "t2": [V, V, O, O], "Xai": [K, V, O], "Xij": [K, O, O], "Xab": [K, V, V]}[name]
+
+    def set_df_basis(self, naux: int, aux_block_size: int = 0):
+        """Density fitting with ``naux`` fitting functions: selects the "df"
+        residual (ccsd.h:93-99).  ``aux_block_size`` tiles the fitting index
+        of TA-layout buffers (0: one tile)."""
+        self.naux = int(naux)
+        self.aux_hm = compute_trange1(self.naux, aux_block_size) if aux_block_size else [0, self.naux]
+        hm = (ctypes.c_int64 * len(self.aux_hm))(*self.aux_hm)
+        self._check(self.lib.mpqc_ccsd_set_df_basis(self.ctx, self.naux, hm, len(self.aux_hm) - 1))
 
     def set_integral(self, name: str, arr: np.ndarray, layout: str = "dense"):
         a = np.ascontiguousarray(arr, dtype=np.float64)
         if layout == "ta":
             a = to_ta_tiles(a, self._modes(name))
+        ids = {**L.INTEGRAL_IDS, **L.DF_INTEGRAL_IDS}
         self._check(self.lib.mpqc_ccsd_set_integral(
-            self.ctx, L.INTEGRAL_IDS[name], a.ctypes.data_as(ctypes.c_void_p),
+            self.ctx, ids[name], a.ctypes.data_as(ctypes.c_void_p),
             L.LAYOUT_TA_TILES if layout == "ta" else L.LAYOUT_DENSE, L.MEM_HOST))
 
     def set_integral_device(self, name: str, dev_ptr: int):
@@ -179,8 +190,13 @@ class CCSD:
                                                     L.MEM_DEVICE))
 
     def set_integrals(self, ints, layout: str = "dense"):
-        for name in L.INTEGRAL_IDS:
-            self.set_integral(name, getattr(ints, name), layout)
+        """All blocks present in ``ints`` (an Integrals, or a dict of the
+        cc::Integrals member names, e.g. the df set without Giabc / Gabcd)."""
+        get = ints.get if isinstance(ints, dict) else (lambda k: getattr(ints, k, None))
+        for name in list(L.INTEGRAL_IDS) + list(L.DF_INTEGRAL_IDS):
+            a = get(name)
+            if a is not None:
+                self.set_integral(name, a, layout)
 
     def set_orbital_energies(self, eps_occ_act, eps_vir):
         eo = np.ascontiguousarray(eps_occ_act, dtype=np.float64)

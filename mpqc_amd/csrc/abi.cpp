@@ -170,6 +170,11 @@ int mpqc_ccsd_set_integral(mpqc_ccsd_ctx* ctx, int which, const double* data, in
   return guarded(ctx, [&](mpqc_hip::CCSDContext& c) { c.set_integral(which, data, layout, memspace); });
 }
 
+int mpqc_ccsd_set_df_basis(mpqc_ccsd_ctx* ctx, int naux, const int64_t* aux_hashmarks,
+                           int n_aux_tiles) {
+  return guarded(ctx, [&](mpqc_hip::CCSDContext& c) { c.set_df_basis(naux, aux_hashmarks, n_aux_tiles); });
+}
+
 int mpqc_ccsd_set_orbital_energies(mpqc_ccsd_ctx* ctx, const double* eps_occ_act,
                                    const double* eps_vir) {
   return guarded(ctx, [&](mpqc_hip::CCSDContext& c) { c.set_orbital_energies(eps_occ_act, eps_vir); });

@@ -92,6 +92,7 @@ void CCSDContext::zero_adjoints() {
 
 void CCSDContext::record_residual() {
   if (split_mode()) throw std::invalid_argument("the adjoint runs in the f64 and f32 forms");
+  if (df_) throw std::invalid_argument("the adjoint runs on the standard method");
   res_tape_.clear();
   tape_ = &res_tape_;
   tape_execute_ = true;

@@ -18,7 +18,7 @@
 namespace mpqc_hip {
 
 namespace {
-constexpr int kOcc = 0, kVir = 1, kNone = -1;
+constexpr int kOcc = 0, kVir = 1, kAux = 2, kNone = -1;
 }  // namespace
 
 // Column-pivoted Householder QR solve of a small dense system (row-major A).
@@ -217,6 +217,18 @@ void CCSDContext::set_option(const std::string& key, double value) {
     lambda_ready_ = false;
   } else if (key == "kernel_timing") {
     kernel_timing_ = value != 0.0;
+  } else if (key == "method") {  // 0 standard, 1 df (ccsd.h:94-99)
+    if (value != 0.0 && value != 1.0) throw std::invalid_argument("method must be 0 (standard) or 1 (df)");
+    if (value == 1.0 && naux_ == 0) throw std::invalid_argument("method df needs set_df_basis first");
+    df_ = value == 1.0;
+    prepared_ = false;
+  } else if (key == "reduced_abcd_memory") {  // ccsd.h:102, used at :292
+    reduced_abcd_ = value != 0.0;
+    prepared_ = false;
+  } else if (key == "df_block_bytes") {
+    if (!(value >= 1 << 20)) throw std::invalid_argument("df_block_bytes must be at least 1 MiB");
+    df_block_bytes_ = value;
+    df_block_ = 0;
   } else if (key == "precision") {
     const int p = (int)value;
     if (p < 0 || p > 4 || p != value) throw std::invalid_argument("unknown precision");
@@ -240,7 +252,7 @@ void CCSDContext::upload(double* dst, const double* src, const std::array<int, 4
   long n[4];
   long total = 1;
   for (int d = 0; d < 4; ++d) {
-    n[d] = modes[d] == kOcc ? o_ : modes[d] == kVir ? v_ : 1;
+    n[d] = modes[d] == kOcc ? o_ : modes[d] == kVir ? v_ : modes[d] == kAux ? naux_ : 1;
     total *= n[d];
   }
   if (memspace != 0 && memspace != 1) throw std::invalid_argument("memspace must be host or device");
@@ -269,6 +281,7 @@ void CCSDContext::upload(double* dst, const double* src, const std::array<int, 4
   for (int d = 0; d < 4; ++d) {
     if (modes[d] == kOcc) { hms[d] = occ_hm_.data(); nts[d] = (int)occ_hm_.size() - 1; }
     else if (modes[d] == kVir) { hms[d] = vir_hm_.data(); nts[d] = (int)vir_hm_.size() - 1; }
+    else if (modes[d] == kAux) { hms[d] = aux_hm_.data(); nts[d] = (int)aux_hm_.size() - 1; }
     else { hms[d] = nullptr; nts[d] = 1; }
   }
   std::vector<long> mi(2 * (n[0] + n[1] + n[2] + n[3]));
@@ -290,7 +303,7 @@ void CCSDContext::download(double* dst, const double* src, const std::array<int,
   long n[4];
   long total = 1;
   for (int d = 0; d < 4; ++d) {
-    n[d] = modes[d] == kOcc ? o_ : modes[d] == kVir ? v_ : 1;
+    n[d] = modes[d] == kOcc ? o_ : modes[d] == kVir ? v_ : modes[d] == kAux ? naux_ : 1;
     total *= n[d];
   }
   if (memspace != 0 && memspace != 1) throw std::invalid_argument("memspace must be host or device");
@@ -306,6 +319,7 @@ void CCSDContext::download(double* dst, const double* src, const std::array<int,
   for (int d = 0; d < 4; ++d) {
     if (modes[d] == kOcc) { hms[d] = occ_hm_.data(); nts[d] = (int)occ_hm_.size() - 1; }
     else if (modes[d] == kVir) { hms[d] = vir_hm_.data(); nts[d] = (int)vir_hm_.size() - 1; }
+    else if (modes[d] == kAux) { hms[d] = aux_hm_.data(); nts[d] = (int)aux_hm_.size() - 1; }
     else { hms[d] = nullptr; nts[d] = 1; }
   }
   std::vector<long> mi(2 * (n[0] + n[1] + n[2] + n[3]));
@@ -349,6 +363,14 @@ void CCSDContext::set_integral(int which, const double* data, int layout, int me
       if (!Gabcd_) Gabcd_ = alloc((size_t)v_ * v_ * v_ * v_);
       upload(Gabcd_, data, {V, V, V, V}, layout, memspace);
       break;
+    case 9: case 10: case 11: {  // X(K,a,i), X(K,i,j), X(K,a,b)  (ccsd.h:481-493)
+      if (naux_ == 0) throw std::logic_error("three-index integrals need set_df_basis first");
+      const int A = kAux;
+      if (which == 9) upload(Xai_, data, {X, A, V, O}, layout, memspace);
+      else if (which == 10) upload(Xij_, data, {X, A, O, O}, layout, memspace);
+      else upload(Xab_, data, {X, A, V, V}, layout, memspace);
+      break;
+    }
     default: throw std::invalid_argument("unknown integral id");
   }
   have_ints_ |= 1u << which;
@@ -711,8 +733,14 @@ double CCSDContext::scalar_from_device(const double* d) {
 // ---------------------------------------------------------------- set-up
 // Integral layouts consumed by the GEMMs (constant for a solve).
 void CCSDContext::prepare() {
-  if ((have_ints_ & 0x1ff) != 0x1ff)
-    throw std::logic_error("all of Fia, Fij, Fab, Gijab, Gijkl, Giajb, Giabc, Gijka, Gabcd must be set");
+  // standard: all nine blocks; df (ccsd.h:281-305): no Giabc / Gabcd under
+  // reduced_abcd_memory, plus Xai, Xij, Xab
+  const unsigned need = !df_ ? 0x1ffu : reduced_abcd_ ? 0xebfu : 0xfffu;
+  if ((have_ints_ & need) != need)
+    throw std::logic_error(!df_ ? "all of Fia, Fij, Fab, Gijab, Gijkl, Giajb, Giabc, Gijka, Gabcd must be set"
+                           : reduced_abcd_ ? "method df needs Fia, Fij, Fab, Gijab, Gijkl, Giajb, Gijka, Xai, Xij, Xab"
+                                           : "method df without reduced_abcd_memory also needs Giabc and Gabcd");
+  const bool have_iabc = !df_ || !reduced_abcd_;
   const long O = o_, V = v_;
   const std::array<long, 4> ovv_ext_ijab{O, O, V, V}, e_iajb{O, V, O, V}, e_iabc{O, V, V, V},
       e_ijka{O, O, O, V};
@@ -733,8 +761,9 @@ void CCSDContext::prepare() {
   perm_add(P6_, {V, O, V, O}, 0.0,
            {{Gijab_, ovv_ext_ijab, {3, 1, 2, 0}, 2.0}, {Giajb_, e_iajb, {1, 2, 3, 0}, -1.0}});
   // g_iabc_bar(k,a,c,d) = 2 Giabc(k,a,c,d) - Giabc(k,a,d,c) as (a,k,c,d)   ccsd_r1_r2.h:112-113
-  perm_add(Gbar_akcd_, {V, O, V, V}, 0.0,
-           {{Giabc_, e_iabc, {1, 0, 2, 3}, 2.0}, {Giabc_, e_iabc, {1, 0, 3, 2}, -1.0}});
+  if (have_iabc && !df_)
+    perm_add(Gbar_akcd_, {V, O, V, V}, 0.0,
+             {{Giabc_, e_iabc, {1, 0, 2, 3}, 2.0}, {Giabc_, e_iabc, {1, 0, 3, 2}, -1.0}});
   // P8(c,k,l,i) = 2 Gijka(l,k,i,c) - Gijka(k,l,i,c)                 ccsd_r1_r2.h:120-121
   perm_add(P8_, {V, O, O, O}, 0.0,
            {{Gijka_, e_ijka, {3, 1, 0, 2}, 2.0}, {Gijka_, e_ijka, {3, 0, 1, 2}, -1.0}});
@@ -744,8 +773,9 @@ void CCSDContext::prepare() {
   perm_add(P13_, {O, O, O, V}, 0.0,
            {{Gijka_, e_ijka, {0, 2, 1, 3}, 2.0}, {Gijka_, e_ijka, {1, 2, 0, 3}, -1.0}});
   // P14(a,c,k,d) = g_iabc_bar(k,a,d,c) = 2 Giabc(k,a,d,c) - Giabc(k,a,c,d)   ccsd_r1_r2.h:223
-  perm_add(P14_, {V, V, O, V}, 0.0,
-           {{Giabc_, e_iabc, {1, 3, 0, 2}, 2.0}, {Giabc_, e_iabc, {1, 2, 0, 3}, -1.0}});
+  if (have_iabc && !df_)
+    perm_add(P14_, {V, V, O, V}, 0.0,
+             {{Giabc_, e_iabc, {1, 3, 0, 2}, 2.0}, {Giabc_, e_iabc, {1, 2, 0, 3}, -1.0}});
   // J0(a,i,k,c) = Gijab(i,k,a,c);  K0(a,i,k,c) = Giajb(k,a,i,c)     ccsd_r1_r2.h:239,245
   perm_add(J0_, {V, O, O, V}, 0.0, {{Gijab_, ovv_ext_ijab, {2, 0, 1, 3}, 1.0}});
   perm_add(K0_, {V, O, O, V}, 0.0, {{Giajb_, e_iajb, {1, 2, 0, 3}, 1.0}});
@@ -758,6 +788,7 @@ void CCSDContext::prepare() {
   perm_add(GB_, {O, V, O, V}, 0.0,
            {{Gijab_, ovv_ext_ijab, {1, 3, 0, 2}, 2.0}, {Gijab_, ovv_ext_ijab, {1, 2, 0, 3}, -1.0}});
   perm_add(GC_, {O, V, O, V}, 0.0, {{Gijab_, ovv_ext_ijab, {1, 2, 0, 3}, 1.0}});
+  if (df_) prepare_df();
   tape_ = saved_tape;
   MPQC_HIP_CHECK(hipStreamSynchronize(stream_));
   prepared_ = true;
@@ -790,6 +821,9 @@ void CCSDContext::residual() {
   // amplitude layouts used by several GEMMs
   perm_add(t1T_, {1, 1, O, V}, 0.0, {{t1_, {1, 1, V, O}, {0, 1, 3, 2}, 1.0}});  // t1T(k,d) = t1(d,k)
   const bool sm = split_mode();
+  if (df_ && sm) throw std::logic_error("the df residual runs in the f64 and f32 forms");
+  const long Q = naux_;
+  if (df_) df_r1_intermediates();
   if (sm) {
     // 16-bit planes of the amplitude operands (the integral planes are split once per solve)
     if (!amax_.dev) amax_.dev = alloc(AbsMaxCache::kSlots);
@@ -814,7 +848,9 @@ void CCSDContext::residual() {
   // :89-91  h_ac = F_vv = Fab(a,c) - gbar(k,l,c,d) tau(a,d,k,l); FAB = h_ac; r1 += h_ac t1
   //         (ccsd_intermediates.h:71-77)
   copy_d2d(FAB_, Fab_, vv);
-  if (sm) {  // tau(a,(d,k,l)) . gbar^T(c,(d,k,l))
+  if (df_) {  // df :155: h_ac = Fab(a,c) - X_ai_tau(K,a,l) Xai(K,c,l):  XtP(a,(K,l)) . XaiP(c,(K,l))^T
+    gemm(false, true, v, v, (int)(Q * O), -1.0, XtP_, Q * O, XaiP_, Q * O, 1.0, FAB_, v, 0, 0, 0, 1, "r1:df_F_vv");
+  } else if (sm) {  // tau(a,(d,k,l)) . gbar^T(c,(d,k,l))
     gemmP(v, v, o * o * v, -1.0, tau_P_, oo * V, GbarT_P_, oo * V, 1.0, FAB_, v, "r1:F_vv");
   } else {
     perm_add(tau_akld_, {V, O, O, V}, 0.0, {{tau_, e_abij, {0, 2, 3, 1}, 1.0}});
@@ -824,7 +860,9 @@ void CCSDContext::residual() {
   // :95-97  h_ki = F_oo = Fij(k,i) + gbar(k,l,a,b) tau(a,b,i,l); FIJ = h_ki; r1 -= t1 h_ki
   //         (ccsd_intermediates.h:57-63)
   copy_d2d(FIJ_, Fij_, oo);
-  if (sm) {  // gbar(k,(l,a,b)) . tau^T(i,(l,a,b))
+  if (df_) {  // df :161: h_ki = Fij(k,i) + X_ai_tau(K,c,i) Xai(K,c,k):  Xai((K,c),k)^T . Xt((K,c),i)
+    gemm(true, false, o, o, (int)(Q * V), 1.0, Xai_, o, Xt_, o, 1.0, FIJ_, o, 0, 0, 0, 1, "r1:df_F_oo");
+  } else if (sm) {  // gbar(k,(l,a,b)) . tau^T(i,(l,a,b))
     gemmP(o, o, o * v * v, 1.0, Gbar_P_, O * vv, tauT_P_, O * vv, 1.0, FIJ_, o, "r1:F_oo");
   } else {
     perm_add(tau_ilab_, {O, O, V, V}, 0.0, {{tau_, e_abij, {2, 3, 0, 1}, 1.0}});
@@ -844,7 +882,12 @@ void CCSDContext::residual() {
   // :107-108 r1 += (2 Gijab(k,i,c,a) - Giajb(k,a,i,c)) t1(c,k)
   gemm(false, false, v * o, 1, v * o, 1.0, P6_, ov, t1_, 1, 1.0, r1_, 1, 0, 0, 0, 1, "r1:g_t1");
   // :110-114 r1 += g_iabc_bar(k,a,c,d) tau(c,d,k,i)
-  if (sm) {  // g_iabc_bar(a,(k,c,d)) . tau_ikcd(i,(k,c,d)),  tau_ikcd(i,k,c,d) = tau(c,d,k,i)
+  if (df_) {
+    // df :177 r1 += X_ai_tau(K,c,i) Xab(K,a,c):  XabP(a,(K,c)) . Xt((K,c),i)
+    gemm(false, false, v, o, (int)(Q * V), 1.0, XabP_, Q * V, Xt_, o, 1.0, r1_, o, 0, 0, 0, 1, "r1:df_Xt_Xab");
+    // df :179 r1 -= X_ai_tau(K,a,l) Xij(K,l,i):  XtP(a,(K,l)) . Xij((K,l),i)
+    gemm(false, false, v, o, (int)(Q * O), -1.0, XtP_, Q * O, Xij_, o, 1.0, r1_, o, 0, 0, 0, 1, "r1:df_Xt_Xij");
+  } else if (sm) {  // g_iabc_bar(a,(k,c,d)) . tau_ikcd(i,(k,c,d)),  tau_ikcd(i,k,c,d) = tau(c,d,k,i)
     split(tau_ikcd_P_, {O, O, V, V}, {{tau_, e_abij, {3, 2, 0, 1}, 1.0}});
     gemmP(v, o, o * v * v, 1.0, Gbar_akcd_P_, O * vv, tau_ikcd_P_, O * vv, 1.0, r1_, o, "r1:giabc_tau");
   } else {
@@ -852,7 +895,9 @@ void CCSDContext::residual() {
     gemm(false, false, v, o, o * v * v, 1.0, Gbar_akcd_, O * vv, tau_kcdi_, o, 1.0, r1_, o, 0, 0, 0, 1, "r1:giabc_tau");
   }
   // :120-121 r1 -= (2 Gijka(l,k,i,c) - Gijka(k,l,i,c)) tau(c,a,k,l)
-  if (sm) {  // tau_ackl(a,(c,k,l)) . P8T(i,(c,k,l)),  tau_ackl(a,c,k,l) = tau(c,a,k,l)
+  if (df_) {
+    // (replaced by the X_ai_tau Xij term above)
+  } else if (sm) {  // tau_ackl(a,(c,k,l)) . P8T(i,(c,k,l)),  tau_ackl(a,c,k,l) = tau(c,a,k,l)
     split(tau_ackl_P_, e_abij, {{tau_, e_abij, {1, 0, 2, 3}, 1.0}});
     gemmP(v, o, o * o * v, -1.0, tau_ackl_P_, oo * V, P8T_P_, oo * V, 1.0, r1_, o, "r1:gijka_tau");
   } else {
@@ -861,12 +906,23 @@ void CCSDContext::residual() {
   }
 
   // ===================== r2: ccsd_r1_r2.h:198-293 =====================
+  if (df_) {
+    // df :317 X_ab_t1(K,a,i) = Xab(K,a,b) t1(b,i)
+    gemm(false, false, (int)(Q * V), o, v, 1.0, Xab_, v, t1_, o, 0.0, Xabt1_, o, 0, 0, 0, 1, "r2:df_Xab_t1");
+    perm_add(Xabt1P_, {1, V, Q, O}, 0.0, {{Xabt1_, {1, Q, V, O}, {0, 2, 1, 3}, 1.0}});
+    // df :321-322 Rd(a,b,i,j) = X_ab_t1(K,b,j) (Xai(K,a,i) - Xij(K,k,i) t1(a,k))
+    copy_d2d(Zdf_, Xai_, Q * ov);
+    gemm(false, false, v, o, o, -1.0, t1_, o, Xij_, o, 1.0, Zdf_, o, 0, oo, ov, (int)Q, "r2:df_Xij_t1");
+    gemm(true, false, v * o, v * o, (int)Q, 1.0, Zdf_, ov, Xabt1_, ov, 0.0, Rdf_, ov, 0, 0, 0, 1, "r2:df_Z_Xabt1");
+    perm_add(Rd_, {V, V, O, O}, 0.0, {{Rdf_, {V, O, V, O}, {0, 2, 1, 3}, 1.0}});
+  } else {
   // :207-208 Rd = (Giabc(i,c,a,b) - Giajb(k,b,i,c) t1(a,k)) t1(c,j)
   //   first term, batched over i: Rd[(a,b), i, j] = sum_c Giabc[i][c,(a,b)] t1[c,j]
   gemm(true, false, v * v, o, v, 1.0, Giabc_, vv, t1_, o, 0.0, Rd_, oo, vv * V, 0, O, o, "r2:giabc_t1");
   //   second term as t1(a,k) * (Giajb(k,b,i,c) t1(c,j))
   gemm(false, false, o * o * v, o, v, 1.0, Giajb_, v, t1_, o, 0.0, Q_, o, 0, 0, 0, 1, "r2:giajb_t1");
   gemm(false, false, v, v * o * o, o, -1.0, t1_, o, Q_, V * oo, 1.0, Rd_, V * oo, 0, 0, 0, 1, "r2:t1_giajb_t1");
+  }
   // :210-211 Rd -= (Gijka(j,i,k,a) + Gijab(i,k,a,c) t1(c,j)) t1(b,k)
   gemm(false, false, o * o * v, o, v, 1.0, Gijab_, v, t1_, o, 0.0, M12_, o, 0, 0, 0, 1, "r2:gijab_t1");
   perm_add(S_, {V, O, O, O}, 0.0,
@@ -883,8 +939,15 @@ void CCSDContext::residual() {
   // :222-224 g_ac = F_VV(FAB, Fia, g_iabc_bar, t1) = FAB + f_ov(k,c) t1(a,k) + gbar(k,a,d,c) t1(d,k)
   //          (ccsd_intermediates.h:141-148; fov_sign = +1 is the reference as written)
   copy_d2d(gac_, FAB_, vv);
-  gemm(false, false, v, v, o, fov_sign_, t1_, o, Fia_, v, 1.0, gac_, v, 0, 0, 0, 1, "r2:gac_t1_fia");
-  if (sm)
+  // (the df form, :333, has - f(k,c) t1(a,k))
+  gemm(false, false, v, v, o, df_ ? -1.0 : fov_sign_, t1_, o, Fia_, v, 1.0, gac_, v, 0, 0, 0, 1, "r2:gac_t1_fia");
+  if (df_) {
+    // df :334 + 2 (Xai(K,d,k) t1(d,k)) Xab(K,a,c)
+    gemm(false, false, (int)Q, 1, v * o, 1.0, Xai_, ov, t1_, 1, 0.0, sK_, 1, 0, 0, 0, 1, "r2:df_Xai_t1");
+    gemm(true, false, v * v, 1, (int)Q, 2.0, Xab_, vv, sK_, 1, 1.0, gac_, 1, 0, 0, 0, 1, "r2:df_gac_sXab");
+    // df :335 - X_ab_t1(K,a,k) Xai(K,c,k):  Xabt1P(a,(K,k)) . XaiP(c,(K,k))^T
+    gemm(false, true, v, v, (int)(Q * O), -1.0, Xabt1P_, Q * O, XaiP_, Q * O, 1.0, gac_, v, 0, 0, 0, 1, "r2:df_gac_Xabt1_Xai");
+  } else if (sm)
     gemmP(v * v, 1, o * v, 1.0, P14_P_, ov, t1T_P_, ov, 1.0, gac_, 1, "r2:gac_g_t1");
   else
     gemm(false, false, v * v, 1, o * v, 1.0, P14_, ov, t1T_, 1, 1.0, gac_, 1, 0, 0, 0, 1, "r2:gac_g_t1");
@@ -912,6 +975,17 @@ void CCSDContext::residual() {
   copy_d2d(K_, K0_, ov * ov);
   gemm(false, false, v, o * o * v, o, -1.0, t1_, o, X18_, oo * V, 1.0, J_, oo * V, 0, 0, 0, 1, "r2:j_gijka_t1");
   gemm(false, false, v, o * o * v, o, -1.0, t1_, o, X19_, oo * V, 1.0, K_, oo * V, 0, 0, 0, 1, "r2:k_gijka_t1");
+  if (df_) {
+    // df :353-354 j += X_ab_t1(K,a,i) Xai(K,c,k) - Xai(x,d,l) T(d,a,i,l) Xai(x,c,k)
+    //   = U(K,(a,i)) Xia(K,(k,c)),  U = X_ab_t1 - Xia(x,(l,d)) TP((a,i),(l,d))^T
+    copy_d2d(Udf_, Xabt1_, Q * ov);
+    gemm(false, true, (int)Q, v * o, o * v, -1.0, Xia_, ov, TP_, ov, 1.0, Udf_, ov, 0, 0, 0, 1, "r2:df_j_XT");
+    gemm(true, false, v * o, o * v, (int)Q, 1.0, Udf_, ov, Xia_, ov, 1.0, J_, ov, 0, 0, 0, 1, "r2:df_j_U_Xia");
+    // df :360 k += (Xai(K,d,k) t1(d,i)) Xab(K,a,c):  V1(K,k,i), then Zk(a,c,k,i) = Xab^T . V1
+    gemm(true, false, o, o, v, 1.0, Xai_, o, t1_, o, 0.0, V1_, o, ov, 0, oo, (int)Q, "r2:df_k_Xai_t1");
+    gemm(true, false, v * v, o * o, (int)Q, 1.0, Xab_, vv, V1_, oo, 0.0, Zk_, oo, 0, 0, 0, 1, "r2:df_k_V1_Xab");
+    perm_add(K_, {V, O, O, V}, 1.0, {{Zk_, {V, V, O, O}, {0, 3, 2, 1}, 1.0}});
+  } else {
   if (sm) {
     gemmP(o * v * v, o, v, 1.0, Giabc_P_, v, t1T_P_, v, 0.0, Zj_, o, "r2:j_giabc_t1");
     gemmP(o * v * v, o, v, 1.0, GiabcT_P_, v, t1T_P_, v, 0.0, Zk_, o, "r2:k_giabc_t1");
@@ -921,6 +995,7 @@ void CCSDContext::residual() {
   }
   perm_add(J_, {V, O, O, V}, 1.0, {{Zj_, {O, V, V, O}, {1, 3, 0, 2}, 1.0}});
   perm_add(K_, {V, O, O, V}, 1.0, {{Zk_, {O, V, V, O}, {1, 3, 0, 2}, 1.0}});
+  }
   if (sm) {
     const std::array<long, 4> e_aild{V, O, O, V};
     // planes of TP and t2P straight from t2 (and t1)
@@ -931,7 +1006,8 @@ void CCSDContext::residual() {
     gemmP(o * v, o * v, o * v, 0.5, t2P_P_, ov, GBt_P_, ov, 1.0, J_, ov, "r2:j_gbar_t2");
     gemmP(o * v, o * v, o * v, -1.0, TP_P_, ov, GCt_P_, ov, 1.0, K_, ov, "r2:k_gT");
   } else {
-    gemm(false, false, o * v, o * v, o * v, -1.0, TP_, ov, GA_, ov, 1.0, J_, ov, 0, 0, 0, 1, "r2:j_gT");
+    if (!df_)  // (df: the X T X form above)
+      gemm(false, false, o * v, o * v, o * v, -1.0, TP_, ov, GA_, ov, 1.0, J_, ov, 0, 0, 0, 1, "r2:j_gT");
     gemm(false, false, o * v, o * v, o * v, 0.5, t2P_, ov, GB_, ov, 1.0, J_, ov, 0, 0, 0, 1, "r2:j_gbar_t2");
     gemm(false, false, o * v, o * v, o * v, -1.0, TP_, ov, GC_, ov, 1.0, K_, ov, 0, 0, 0, 1, "r2:k_gT");
   }
@@ -985,6 +1061,10 @@ void CCSDContext::residual() {
   } else {
     gemm(false, false, o * o, o * o, v * v, 1.0, Gijab_, vv, tau_, oo, 1.0, W_, oo, 0, 0, 0, 1, "r2:W_oooo");
     gemm(false, false, v * v, o * o, o * o, 1.0, tau_, oo, W_, oo, 1.0, r2_, oo, 0, 0, 0, 1, "r2:hh_ladder");
+    if (df_ && reduced_abcd_) {
+      df_pp_ladder();  // df :497-508
+      return;
+    }
     // :275 b(a,b,i,j) = tau(c,d,i,j) Gabcd(a,b,c,d)  -- the particle-particle ladder
     gemm(false, false, v * v, o * o, v * v, 1.0, Gabcd_, vv, tau_, oo, 1.0, r2_, oo, 0, 0, 0, 1, "r2:pp_ladder");
     // :277 tmp(k,a,i,j) = Giabc(k,a,c,d) tau(c,d,i,j)

@@ -40,6 +40,11 @@ class CCSDContext {
 
   void set_option(const std::string& key, double value);
   void set_integral(int which, const double* data, int layout, int memspace);
+  // density fitting: naux fitting functions tiled by aux_hm (nullptr: one tile);
+  // selects the "df" residual (ccsd.h:93-99: method defaults to df when a
+  // fitting basis is given)
+  void set_df_basis(int naux, const int64_t* aux_hm, int n_aux_tiles);
+  bool df() const { return df_; }
   void set_orbital_energies(const double* eo, const double* ev);
   void solver_init(int diis_start, int n_diis, double damp, int ngroup, int ngroup_nstart);
   void set_amplitudes(const double* t1, const double* t2, int layout, int memspace);
@@ -113,6 +118,11 @@ class CCSDContext {
   int make_terms(std::array<long, 4> n, std::initializer_list<Src> srcs, TTerm* terms);
   void copy_d2d(double* dst, const double* src, long n);
   void op_tau();  // tau = t2 + t1 t1  (ccsd.h:389)
+  // ---- density-fitted residual pieces (ccsd_r1_r2.h:136-184, 309-520)
+  void prepare_df();                    // layouts of the three-index factors
+  void df_r1_intermediates();           // X_ai_tau and its (a,K,l) layout
+  void df_pp_ladder();                  // lazy (ab|cd) ladder of reduced_abcd_memory
+  void ensure_df_blocks();
   void op_TP();   // the T intermediate of ccsd_r1_r2.h:236
 
   // ---- op tape: integral preparation and the residual as a list of primitive
@@ -190,6 +200,18 @@ class CCSDContext {
   std::vector<int64_t> occ_hm_, vir_hm_;
   std::vector<double*> allocations_;
   std::vector<size_t> alloc_sizes_;
+
+  // density fitting (method "df"): three-index factors X(K,p,q) and their layouts
+  bool df_ = false, reduced_abcd_ = true;
+  int naux_ = 0;
+  std::vector<int64_t> aux_hm_;
+  double *Xai_ = nullptr, *Xij_ = nullptr, *Xab_ = nullptr;
+  double *XaiP_ = nullptr, *XabP_ = nullptr, *Xia_ = nullptr;
+  double *Tbar_ = nullptr, *Xt_ = nullptr, *XtP_ = nullptr, *Xabt1_ = nullptr, *Xabt1P_ = nullptr,
+         *Zdf_ = nullptr, *Udf_ = nullptr, *sK_ = nullptr, *V1_ = nullptr, *Ydf_ = nullptr,
+         *Rdf_ = nullptr, *bdf_ = nullptr, *Gblk_ = nullptr, *Gp_ = nullptr;
+  int df_block_ = 0;            // virtual rows a per (ab|cd) block
+  double df_block_bytes_ = 4.0 * (1 << 30);  // budget of the two (ab|cd) block buffers
 
   // integrals
   double *Fia_ = nullptr, *Fij_ = nullptr, *Fab_ = nullptr, *Gijab_ = nullptr, *Gijkl_ = nullptr,

@@ -1,0 +1,111 @@
+"""Density-fitted CCSD on the GPU (method "df", through the C-ABI) against the
+DF oracle (oracle/ccsd_df.py: ccsd_r1_r2.h:136-184, 309-520) and the
+reference's h2o-ccsd-631g-pvdz trace.
+
+Bar: FP64 residuals 1e-13 relative to their largest term, energies 2e-12 Eh
+on the golden trace; FP32 contractions 2e-6 relative.
+"""
+import numpy as np
+import pytest
+
+from oracle import ccsd as O
+from oracle import ccsd_df as D
+from test_golden import FIX, check_trace
+from test_oracle_df import _df_inputs, _random_factors
+
+pytestmark = pytest.mark.gpu
+
+
+def _df_ints(o, v, F, Xai, Xij, Xab, with_abcd=False):
+    g = D.four_index_from_factors(Xai, Xij, Xab)
+    extra = dict(Giabc=g["Giabc"], Gabcd=g["Gabcd"]) if with_abcd else {}
+    return D.DFIntegrals(Fia=F[:o, o:].copy(), Fij=F[:o, :o].copy(), Fab=F[o:, o:].copy(),
+                         Gijab=g["Gijab"], Gijkl=g["Gijkl"], Giajb=g["Giajb"], Gijka=g["Gijka"],
+                         Xai=Xai, Xij=Xij, Xab=Xab, **extra)
+
+
+def _blocks(ints: D.DFIntegrals):
+    names = ["Fia", "Fij", "Fab", "Gijab", "Gijkl", "Giajb", "Gijka", "Xai", "Xij", "Xab"]
+    b = {k: getattr(ints, k) for k in names}
+    if ints.Gabcd is not None:
+        b.update(Giabc=ints.Giabc, Gabcd=ints.Gabcd)
+    return b
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.mark.parametrize("o,v,naux,block,layout,reduced,block_bytes,prec,tol", [
+    (3, 6, 11, 24, "dense", True, 0, "f64", 1e-13),
+    (4, 13, 29, 4, "ta", True, 0, "f64", 1e-13),
+    (2, 30, 45, 7, "ta", True, 1 << 20, "f64", 1e-13),   # 15 blocks of the lazy (ab|cd)
+    (5, 19, 37, 24, "dense", False, 0, "f64", 1e-13),    # Gabcd / Giabc given
+    (6, 21, 50, 5, "ta", True, 0, "f32", 2e-6),
+])
+def test_df_residual_matches_oracle(o, v, naux, block, layout, reduced, block_bytes, prec, tol):
+    from mpqc_amd.ccsd import CCSD
+    F, Xai, Xij, Xab, t1, t2 = _random_factors(o, v, naux, seed=o * 100 + v + naux)
+    ints = _df_ints(o, v, F, Xai, Xij, Xab, with_abcd=not reduced)
+    tau = O.make_tau(t1, t2)
+    r1_ref = D.cs_ccsd_r1_df(t1, t2, tau, ints)
+    r2_ref = D.cs_ccsd_r2_df(t1, t2, tau, ints)
+    print(f"o={o} v={v} naux={naux} layout={layout} reduced={reduced} prec={prec}", flush=True)
+    with CCSD(o, v, precision=prec, occ_block_size=block, unocc_block_size=block) as c:
+        c.set_df_basis(naux, aux_block_size=block if layout == "ta" else 0)
+        if not reduced:
+            c.set_option("reduced_abcd_memory", 0)
+        if block_bytes:
+            c.set_option("df_block_bytes", block_bytes)
+        c.set_integrals(_blocks(ints), layout=layout)
+        c.set_orbital_energies(np.diag(F)[:o], np.diag(F)[o:])
+        r1, r2 = c.compute_residuals(t1, t2)
+        FIJ, FAB = c.get_dressed_fock()
+    e1, e2 = _rel(r1, r1_ref), _rel(r2, r2_ref)
+    print(f"  r1 {e1:.2e}  r2 {e2:.2e}")
+    assert e1 < tol and e2 < tol
+    assert np.abs(FIJ - ints.FIJ).max() < 1e3 * tol and np.abs(FAB - ints.FAB).max() < 1e3 * tol
+
+
+@pytest.mark.parametrize("idx", range(len(FIX["df_cases"])))
+def test_gpu_df_ccsd_trace_matches_reference(idx):
+    from mpqc_amd.ccsd import CCSD
+    case = FIX["df_cases"][idx]
+    _, inp = _df_inputs(case)
+    tp = case["precision"] * case["precision_scale_for_current_rule"]
+    with CCSD(inp.o, inp.v, occ_block_size=2, unocc_block_size=2) as c:
+        c.set_df_basis(inp.naux, aux_block_size=4)
+        c.set_integrals(inp.df_blocks(), layout="ta")
+        c.set_orbital_energies(inp.base.eps_occ, inp.base.eps_vir)
+        E = c.compute_ccsd(target_precision=tp)
+        rows = c.trace()
+    print(f"GPU DF-CCSD {E:.15f} reference {case['e_ccsd']:.15f}")
+    check_trace(rows, case, 2e-12)
+    assert len(rows) == len(case["trace"])
+    assert abs(E - case["e_ccsd"]) < 2e-12
+
+
+def test_df_rejects_misuse():
+    from mpqc_amd import _lib as L
+    from mpqc_amd.ccsd import CCSD
+    with CCSD(3, 6) as c:
+        with pytest.raises(L.CCSDError):
+            c.set_integral("Xai", np.zeros((4, 6, 3)))   # before set_df_basis
+        with pytest.raises(L.CCSDError):
+            c.set_option("method", 1)                    # df without a fitting basis
+    F, Xai, Xij, Xab, t1, t2 = _random_factors(3, 6, 8, seed=5)
+    ints = _df_ints(3, 6, F, Xai, Xij, Xab)
+    with CCSD(3, 6, precision="f16x3") as c:
+        c.set_df_basis(8)
+        c.set_integrals(_blocks(ints))
+        c.set_orbital_energies(np.diag(F)[:3], np.diag(F)[3:])
+        with pytest.raises(L.CCSDError):
+            c.compute_residuals(t1, t2)
+    with CCSD(3, 6) as c:
+        c.set_df_basis(8)
+        b = _blocks(ints)
+        del b["Xab"]
+        c.set_integrals(b)
+        c.set_orbital_energies(np.diag(F)[:3], np.diag(F)[3:])
+        with pytest.raises(L.CCSDError):                 # Xab missing
+            c.compute_residuals(t1, t2)
