@@ -16,7 +16,8 @@ baseline (one oracle residual on the host cores).
 Other workloads: pp-ladder (R[a,b,i,j] += sum_cd tau[c,d,i,j] V[a,b,c,d],
 o=32, v=256, FP64, ccsd_r1_r2.h:275; V ~ N(0, 0.05) symmetrised to the 8-fold
 (ac|bd) symmetry, tau ~ N(0, 1e-2) / (e_c + e_d - e_i - e_j)); ccsd-h2o6
-(o=30, v=114); ccsd-h20 (H20 chain, o=10, v=90); copy (counter calibration).
+(o=30, v=114); ccsd-h20 (H20 chain, o=10, v=90); copy (counter calibration);
+batched-hchains; adjoint-h2o6; df-h2o8 (density-fitted residual, method df).
 """
 from __future__ import annotations
 
@@ -45,7 +46,7 @@ def _args():
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--workload", default="ccsd-h2o8",
                    choices=["ccsd-h2o8", "ccsd-h2o6", "ccsd-h20", "pp-ladder", "copy",
-                            "batched-hchains", "adjoint-h2o6"])
+                            "batched-hchains", "adjoint-h2o6", "df-h2o8"])
     p.add_argument("--forms", default="f64,f16x3",
                    help="contraction precisions to run for the CCSD workloads (first = reference)")
     p.add_argument("--max-iter", type=int, default=40)
@@ -258,6 +259,41 @@ def cpu_baseline_ccsd(inp):
                       f"{flops:.3e} algorithmic FLOPs"}
 
 
+def _timed_iterations(c, args, label):
+    """W untimed, then K timed iterations from the initial guess; one iteration
+    = Jacobi/DIIS update + tau + residual + energy/error (the last call
+    synchronises).  Then the same iterations again with per-contraction HIP-event
+    timing.  Returns (ms per iteration, kernel times per iteration)."""
+    c.init_guess()
+    c.energy_error()
+    for _ in range(args.warmup):
+        c.update()
+        c.residual()
+        c.energy_error()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        c.update()
+        c.residual()
+        c.energy_error()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    c.set_option("kernel_timing", 1)
+    c.init_guess()
+    c.energy_error()
+    for _ in range(args.warmup + args.steps):
+        c.update()
+        c.residual()
+        c.energy_error()
+    kt = c.kernel_times()
+    c.set_option("kernel_timing", 0)
+    n_it = args.warmup + args.steps
+    tot = sum(x["ms"] for x in kt.values()) / n_it
+    log(f"{label}: {ms:.2f} ms/iteration; event-timed phases sum to {tot:.2f} ms/iteration")
+    for k, x in sorted(kt.items(), key=lambda kv: -kv[1]["ms"])[:40]:
+        log(f"   {k:24s} {x['ms'] / n_it:8.3f} ms/it  {x['calls'] / n_it:5.1f} calls/it"
+            + (f"  {x['flops'] / max(1e-12, x['ms'] * 1e-3) / 1e12:7.2f} TF" if x["flops"] else ""))
+    return ms, kt
+
+
 def bench_ccsd(args):
     import numpy as np
     from mpqc_amd.ccsd import CCSD, Integrals
@@ -275,36 +311,8 @@ def bench_ccsd(args):
         E = c.compute_ccsd(target_precision=1e-8)
         rows = c.trace()
         log(f"{prec}: E_corr={E:.12f} after {len(rows)} iterations")
-        # timed iterations: W untimed, then K timed; one iteration = Jacobi/DIIS
-        # update + tau + residual + energy/error (the last call synchronises)
-        c.init_guess()
-        c.energy_error()
-        for _ in range(args.warmup):
-            c.update()
-            c.residual()
-            c.energy_error()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            c.update()
-            c.residual()
-            c.energy_error()
-        ms = (time.perf_counter() - t0) * 1e3 / args.steps
-        # the same iterations again with per-contraction HIP-event timing
-        c.set_option("kernel_timing", 1)
-        c.init_guess()
-        c.energy_error()
-        for _ in range(args.warmup + args.steps):
-            c.update()
-            c.residual()
-            c.energy_error()
-        kt = c.kernel_times()
+        ms, kt = _timed_iterations(c, args, prec)
         c.close()
-        n_it = args.warmup + args.steps
-        tot = sum(x["ms"] for x in kt.values()) / n_it
-        log(f"{prec}: {ms:.2f} ms/iteration; event-timed phases sum to {tot:.2f} ms/iteration")
-        for k, x in sorted(kt.items(), key=lambda kv: -kv[1]["ms"])[:40]:
-            log(f"   {k:24s} {x['ms'] / n_it:8.3f} ms/it  {x['calls'] / n_it:5.1f} calls/it"
-                + (f"  {x['flops'] / max(1e-12, x['ms'] * 1e-3) / 1e12:7.2f} TF" if x["flops"] else ""))
         forms[prec] = {"E_corr": E, "iterations": len(rows), "ms_per_iteration": ms,
                        "tflops": flops / (ms * 1e-3) / 1e12,
                        "solve_iteration_ms": [round(r.seconds * 1e3, 3) for r in rows],
@@ -614,6 +622,103 @@ def bench_adjoint(args):
     return line
 
 
+def bench_df(args):
+    """Density-fitted CCSD (method "df", reduced_abcd_memory) on (H2O)8 with
+    the cc-pVDZ orbital basis and cc-pVDZ as the fitting basis (the pairing of
+    the reference's DF validation input, orbital basis 6-31G / fitting basis
+    cc-pVDZ, moved up one orbital basis): RI-RHF and the DF MO factors on the
+    host, then converged solves and K timed iterations in the FP64 and FP32
+    forms, beside the standard FP64 residual on the same (DF four-index)
+    integrals."""
+    import numpy as np
+    from mpqc_amd.ccsd import CCSD, Integrals
+    from mpqc_amd.chem.df import df_ccsd_integrals, ri_rhf
+    from mpqc_amd.chem.molecule import water_cluster
+    from oracle.ccsd import ccsd_flops
+    mol = water_cluster((2, 2, 2), seed=4)
+    t0 = time.perf_counter()
+    d = ri_rhf(mol, "cc-pVDZ", "cc-pVDZ")
+    t1 = time.perf_counter()
+    inp = df_ccsd_integrals(d, frozen_core=False)
+    t2 = time.perf_counter()
+    o, v, Q = inp.o, inp.v, inp.naux
+    log(f"df-h2o8: RI-RHF E={d.scf.energy:.10f} ({t1 - t0:.1f}s), DF MO factors o={o} v={v} "
+        f"naux={Q} ({t2 - t1:.1f}s)")
+    del d
+    forms = {}
+    for prec in [p for p in args.forms.split(",") if p in ("f64", "f32")] or ["f64"]:
+        c = CCSD(o, v, precision=prec, max_iter=args.max_iter)
+        c.set_df_basis(Q)
+        c.set_integrals(inp.df_blocks())
+        c.set_orbital_energies(inp.base.eps_occ, inp.base.eps_vir)
+        E = c.compute_ccsd(target_precision=1e-8)
+        n_rows = len(c.trace())
+        log(f"df {prec}: E_corr={E:.12f} after {n_rows} iterations")
+        ms, kt = _timed_iterations(c, args, f"df {prec}")
+        c.close()
+        n_it = args.warmup + args.steps
+        exec_flops = sum(x["flops"] for x in kt.values()) / n_it
+        forms[prec] = {"E_corr": E, "iterations": n_rows, "ms_per_iteration": ms,
+                       "executed_contraction_flops": exec_flops,
+                       "tflops_executed": exec_flops / (ms * 1e-3) / 1e12,
+                       "kernels": {k: {"calls": x["calls"], "avg_ms": x["ms"] / max(1, x["calls"]),
+                                       "tflops": x["flops"] / max(1e-12, x["ms"] * 1e-3) / 1e12}
+                                   for k, x in kt.items()}}
+    # the standard residual on the same integrals (Gabcd, Giabc from the same factors)
+    c = CCSD(o, v, precision="f64", max_iter=args.max_iter)
+    c.set_integrals(Integrals(**inp.base.blocks))
+    c.set_orbital_energies(inp.base.eps_occ, inp.base.eps_vir)
+    E_std = c.compute_ccsd(target_precision=1e-8)
+    ms_std, _ = _timed_iterations(c, args, "standard f64 on the DF integrals")
+    c.close()
+    ref = forms.get("f64", next(iter(forms.values())))
+    for f in forms.values():
+        f["dE_vs_f64"] = f["E_corr"] - ref["E_corr"]
+    head = min([p for p, f in forms.items() if abs(f["dE_vs_f64"]) <= 1e-6],
+               key=lambda p: forms[p]["ms_per_iteration"])
+    hf = forms[head]
+    lad = hf["kernels"].get("r2:df_pp_ladder", {"avg_ms": float("nan"), "tflops": float("nan")})
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import ccsd as O
+        from oracle import ccsd_df as D
+        ints = D.DFIntegrals(**inp.df_blocks())
+        t1a = O.jacobi_update_t1(ints.Fia.T, inp.base.eps_occ, inp.base.eps_vir)
+        t2a = O.jacobi_update_t2(ints.Gijab.transpose(2, 3, 0, 1), inp.base.eps_occ, inp.base.eps_vir)
+        tau = O.make_tau(t1a, t2a)
+        ta = time.perf_counter()
+        D.cs_ccsd_r1_df(t1a, t2a, tau, ints)
+        D.cs_ccsd_r2_df(t1a, t2a, tau, ints)
+        dt = time.perf_counter() - ta
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        cpu = {"value": dt * 1e3, "unit": "ms per residual", "cores": threads, "kind": "port",
+               "sample": f"one DF oracle residual (numpy/OpenBLAS fp64 einsum) at o={o} v={v} naux={Q}, "
+                         "first-iteration amplitudes"}
+    line = {
+        "metric": "DF-CCSD iteration wall-time (method df, reduced_abcd_memory)",
+        "value": round(hf["ms_per_iteration"], 3), "unit": "ms/iteration", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(hf["ms_per_iteration"], 3),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": DTYPE[head],
+        "data": "synthetic molecule (seeded geometry), RI-RHF + DF MO factors on the host",
+        "config": {"workload": "(H2O)8 cc-pVDZ orbitals, cc-pVDZ fitting basis, method df", "o": o,
+                   "v": v, "naux": Q, "form": head, "parallelism": "single-gpu",
+                   "standard_algorithmic_flops_per_iteration": ccsd_flops(o, v)},
+        "roofline": {"bound": "mfma", "achieved": round(lad["tflops"], 3), "peak": PEAK[head],
+                     "unit": "TFLOP/s", "frac": round(lad["tflops"] / PEAK[head], 4), "traffic": None,
+                     "kernel": f"DF pp-ladder GEMM per (ab|cd) block (M=nb v, N=o^2, K=v^2) "
+                               f"avg {lad['avg_ms']:.3f} ms"},
+        "forms": {p: {k: (round(x, 6) if isinstance(x, float) else x) for k, x in f.items()
+                      if k != "kernels"} for p, f in forms.items()},
+        "kernels": {p: f["kernels"] for p, f in forms.items()},
+        "standard_f64_same_integrals": {"E_corr": E_std, "ms_per_iteration": round(ms_std, 3),
+                                        "dE_vs_df_f64": E_std - ref["E_corr"]},
+        "setup": {"ri_rhf_s": round(t1 - t0, 2), "df_factors_s": round(t2 - t1, 2)},
+        "cpu_baseline": cpu,
+    }
+    return line
+
+
 def main():
     args = _args()
     if args.gpus != 1:
@@ -632,6 +737,8 @@ def main():
         line = bench_batched(args)
     elif args.workload == "adjoint-h2o6":
         line = bench_adjoint(args)
+    elif args.workload == "df-h2o8":
+        line = bench_df(args)
     else:
         line = bench_ccsd(args)
     print(json.dumps(line), flush=True)
