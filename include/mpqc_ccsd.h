@@ -16,6 +16,12 @@
  *                                 (K|G|p q)[inv_sqr] (ccsd.h:93-99, 481-493); selects
  *                                 compute_cs_ccsd_r1_df / compute_cs_ccsd_r2_df
  *                                 (ccsd_r1_r2.h:136-184, 309-520)
+ *   mpqc_ccsd_get_integral        reads a cc::Integrals member back (ccsd_r1_r2.h:24-56)
+ *   mpqc_ccsd_transform_ao        LCAOFactory::compute4 without density fitting
+ *                                 (chemistry/qc/lcao/factory/lcao_factory.h:409-493) for the
+ *                                 nine blocks of ccsd.h:281-305, and <p|F|q> for Fia/Fij/Fab
+ *   mpqc_ccsd_transform_df        the "[df]" blocks: (K|G|p q)[inv_sqr] (ccsd.h:481-493) and
+ *                                 compute4 with density fitting (lcao_factory.h:386-406)
  *   mpqc_ccsd_set_orbital_energies JacobiDIISSolver(kv, f_ii, f_aa)
  *                                 (chemistry/qc/lcao/cc/solvers.h:1140-1146; values from
  *                                 make_diagonal_fpq, chemistry/qc/lcao/scf/mo_build.h:20-26)
@@ -141,6 +147,24 @@ int mpqc_ccsd_set_integral(mpqc_ccsd_ctx* ctx, int which, const double* data, in
                            int memspace);
 int mpqc_ccsd_set_orbital_energies(mpqc_ccsd_ctx* ctx, const double* eps_occ_act,
                                    const double* eps_vir);
+/* Reads back an integral block held by the context (as set, or as built by the
+ * transforms below), in the layout and index order of mpqc_ccsd_set_integral. */
+int mpqc_ccsd_get_integral(mpqc_ccsd_ctx* ctx, int which, double* out, int layout, int memspace);
+/* AO -> MO on the device: fills every integral block (Fia..Gabcd) from the AO
+ * integrals (mu nu|la si) (nbf^4, chemist notation, row-major), the AO Fock matrix
+ * (nbf x nbf) and the active occupied / virtual MO coefficients (nbf x o and
+ * nbf x v, row-major: C[mu][p]).  All four in host or device memory (memspace). */
+int mpqc_ccsd_transform_ao(mpqc_ccsd_ctx* ctx, int nbf, const double* ao_eri, const double* fock_ao,
+                           const double* c_occ, const double* c_vir, int memspace);
+/* The density-fitted integral set of the "df" method on the device (after
+ * mpqc_ccsd_set_df_basis): from the three-centre integrals (K|mu nu) (naux x nbf x nbf)
+ * and a metric factor L (naux x naux, L L^T = V^-1 for the Coulomb metric V; the
+ * fitted factors are X = L^T (K|mu nu); pass NULL when the buffer is already fitted),
+ * builds Xai, Xij, Xab, the four-index blocks Gijab, Gijkl, Giajb, Gijka (and Giabc,
+ * Gabcd with reduced_abcd_memory = 0) and Fia, Fij, Fab from the AO Fock matrix. */
+int mpqc_ccsd_transform_df(mpqc_ccsd_ctx* ctx, int nbf, const double* ao_three_center, const double* metric,
+                           const double* fock_ao, const double* c_occ, const double* c_vir, int memspace);
+
 /* Declares naux fitting functions tiled by aux_hashmarks (NULL: one tile) and
  * switches the context to the "df" residual.  Then MPQC_INT_XAI/XIJ/XAB are set
  * with mpqc_ccsd_set_integral (the fitting index uses these hashmarks in

@@ -198,6 +198,50 @@ class CCSD:
             if a is not None:
                 self.set_integral(name, a, layout)
 
+    def get_integral(self, name: str, layout: str = "dense") -> np.ndarray:
+        """An integral block held by the context (as set or as transformed)."""
+        o, v, k = self.o, self.v, getattr(self, "naux", 0)
+        shapes = {"Fia": (o, v), "Fij": (o, o), "Fab": (v, v), "Gijab": (o, o, v, v),
+                  "Gijkl": (o, o, o, o), "Giajb": (o, v, o, v), "Giabc": (o, v, v, v),
+                  "Gijka": (o, o, o, v), "Gabcd": (v, v, v, v), "Xai": (k, v, o), "Xij": (k, o, o),
+                  "Xab": (k, v, v)}
+        ids = {**L.INTEGRAL_IDS, **L.DF_INTEGRAL_IDS}
+        buf = np.empty(int(np.prod(shapes[name])))
+        self._check(self.lib.mpqc_ccsd_get_integral(self.ctx, ids[name], buf.ctypes.data_as(ctypes.c_void_p),
+                                                    L.LAYOUT_TA_TILES if layout == "ta" else L.LAYOUT_DENSE,
+                                                    L.MEM_HOST))
+        if layout == "ta":
+            return from_ta_tiles(buf, shapes[name], self._modes(name))
+        return buf.reshape(shapes[name])
+
+    def transform_ao(self, eri, fock_ao, c_occ, c_vir):
+        """Every integral block from AO (mu nu|la si), the AO Fock matrix and the
+        active occupied / virtual MO coefficients, transformed on the device
+        (LCAOFactory::compute4, lcao_factory.h:378-497)."""
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (eri, fock_ao, c_occ, c_vir)]
+        nbf = arrs[1].shape[0]
+        if arrs[0].shape != (nbf,) * 4 or arrs[2].shape != (nbf, self.o) or arrs[3].shape != (nbf, self.v):
+            raise ValueError("AO integral / coefficient shapes do not match nbf, o, v")
+        self._check(self.lib.mpqc_ccsd_transform_ao(self.ctx, nbf, *[a.ctypes.data_as(ctypes.c_void_p)
+                                                                      for a in arrs], L.MEM_HOST))
+
+    def transform_df(self, three_center, metric, fock_ao, c_occ, c_vir):
+        """The df integral set from (K|mu nu) and a metric factor L (L L^T = V^-1,
+        or None when ``three_center`` is already fitted), on the device."""
+        B = np.ascontiguousarray(three_center, dtype=np.float64)
+        F = np.ascontiguousarray(fock_ao, dtype=np.float64)
+        Co = np.ascontiguousarray(c_occ, dtype=np.float64)
+        Cv = np.ascontiguousarray(c_vir, dtype=np.float64)
+        nbf = F.shape[0]
+        if B.shape != (self.naux, nbf, nbf) or Co.shape != (nbf, self.o) or Cv.shape != (nbf, self.v):
+            raise ValueError("three-centre integral / coefficient shapes do not match naux, nbf, o, v")
+        Lm = None if metric is None else np.ascontiguousarray(metric, dtype=np.float64)
+        if Lm is not None and Lm.shape != (self.naux, self.naux):
+            raise ValueError("metric factor must be naux x naux")
+        vp = lambda a: None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+        self._check(self.lib.mpqc_ccsd_transform_df(self.ctx, nbf, vp(B), vp(Lm), vp(F), vp(Co), vp(Cv),
+                                                    L.MEM_HOST))
+
     def set_orbital_energies(self, eps_occ_act, eps_vir):
         eo = np.ascontiguousarray(eps_occ_act, dtype=np.float64)
         ev = np.ascontiguousarray(eps_vir, dtype=np.float64)

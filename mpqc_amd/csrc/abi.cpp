@@ -175,6 +175,22 @@ int mpqc_ccsd_set_df_basis(mpqc_ccsd_ctx* ctx, int naux, const int64_t* aux_hash
   return guarded(ctx, [&](mpqc_hip::CCSDContext& c) { c.set_df_basis(naux, aux_hashmarks, n_aux_tiles); });
 }
 
+int mpqc_ccsd_get_integral(mpqc_ccsd_ctx* ctx, int which, double* out, int layout, int memspace) {
+  return guarded(ctx, [&](mpqc_hip::CCSDContext& c) { c.get_integral(which, out, layout, memspace); });
+}
+
+int mpqc_ccsd_transform_ao(mpqc_ccsd_ctx* ctx, int nbf, const double* ao_eri, const double* fock_ao,
+                           const double* c_occ, const double* c_vir, int memspace) {
+  return guarded(ctx, [&](mpqc_hip::CCSDContext& c) { c.transform_ao(nbf, ao_eri, fock_ao, c_occ, c_vir, memspace); });
+}
+
+int mpqc_ccsd_transform_df(mpqc_ccsd_ctx* ctx, int nbf, const double* ao_three_center, const double* metric,
+                           const double* fock_ao, const double* c_occ, const double* c_vir, int memspace) {
+  return guarded(ctx, [&](mpqc_hip::CCSDContext& c) {
+    c.transform_df(nbf, ao_three_center, metric, fock_ao, c_occ, c_vir, memspace);
+  });
+}
+
 int mpqc_ccsd_set_orbital_energies(mpqc_ccsd_ctx* ctx, const double* eps_occ_act,
                                    const double* eps_vir) {
   return guarded(ctx, [&](mpqc_hip::CCSDContext& c) { c.set_orbital_energies(eps_occ_act, eps_vir); });

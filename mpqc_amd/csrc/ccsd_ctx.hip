@@ -377,6 +377,26 @@ void CCSDContext::set_integral(int which, const double* data, int layout, int me
   prepared_ = false;
 }
 
+void CCSDContext::get_integral(int which, double* out, int layout, int memspace) {
+  if (out == nullptr) throw std::invalid_argument("null output buffer");
+  if (which < 0 || which > 11 || !((have_ints_ >> which) & 1u)) throw std::logic_error("integral block not set");
+  const int O = kOcc, V = kVir, X = kNone, A = kAux;
+  switch (which) {
+    case 0: download(out, Fia_, {X, X, O, V}, layout, memspace); break;
+    case 1: download(out, Fij_, {X, X, O, O}, layout, memspace); break;
+    case 2: download(out, Fab_, {X, X, V, V}, layout, memspace); break;
+    case 3: download(out, Gijab_, {O, O, V, V}, layout, memspace); break;
+    case 4: download(out, Gijkl_, {O, O, O, O}, layout, memspace); break;
+    case 5: download(out, Giajb_, {O, V, O, V}, layout, memspace); break;
+    case 6: download(out, Giabc_, {O, V, V, V}, layout, memspace); break;
+    case 7: download(out, Gijka_, {O, O, O, V}, layout, memspace); break;
+    case 8: download(out, Gabcd_, {V, V, V, V}, layout, memspace); break;
+    case 9: download(out, Xai_, {X, A, V, O}, layout, memspace); break;
+    case 10: download(out, Xij_, {X, A, O, O}, layout, memspace); break;
+    default: download(out, Xab_, {X, A, V, V}, layout, memspace); break;
+  }
+}
+
 void CCSDContext::set_orbital_energies(const double* eo, const double* ev) {
   if (!eo || !ev) throw std::invalid_argument("null orbital energies");
   MPQC_HIP_CHECK(hipMemcpyAsync(eo_, eo, o_ * sizeof(double), hipMemcpyHostToDevice, stream_));

@@ -40,11 +40,22 @@ class CCSDContext {
 
   void set_option(const std::string& key, double value);
   void set_integral(int which, const double* data, int layout, int memspace);
+  void get_integral(int which, double* out, int layout, int memspace);
   // density fitting: naux fitting functions tiled by aux_hm (nullptr: one tile);
   // selects the "df" residual (ccsd.h:93-99: method defaults to df when a
   // fitting basis is given)
   void set_df_basis(int naux, const int64_t* aux_hm, int n_aux_tiles);
   bool df() const { return df_; }
+  // AO -> MO on the device (LCAOFactory::compute4, lcao_factory.h:378-497): the
+  // integral blocks and Fock blocks from AO integrals (mu nu|la si) (nbf^4,
+  // chemist notation), the AO Fock matrix and the active occupied / virtual MO
+  // coefficients (nbf x o, nbf x v, row-major)
+  void transform_ao(int nbf, const double* eri, const double* fock, const double* c_occ, const double* c_vir,
+                    int memspace);
+  // the density-fitted set: three-centre (K|mu nu) (naux x nbf^2), a metric factor
+  // L (naux x naux, L L^T = V^-1; X = L^T B) or nullptr when B is already fitted
+  void transform_df(int nbf, const double* B3, const double* metric, const double* fock, const double* c_occ,
+                    const double* c_vir, int memspace);
   void set_orbital_energies(const double* eo, const double* ev);
   void solver_init(int diis_start, int n_diis, double damp, int ngroup, int ngroup_nstart);
   void set_amplitudes(const double* t1, const double* t2, int layout, int memspace);
@@ -120,6 +131,7 @@ class CCSDContext {
   void op_tau();  // tau = t2 + t1 t1  (ccsd.h:389)
   // ---- density-fitted residual pieces (ccsd_r1_r2.h:136-184, 309-520)
   void prepare_df();                    // layouts of the three-index factors
+  void transform_fock(int nbf, const double* fock, const double* Cact, int memspace);
   void df_r1_intermediates();           // X_ai_tau and its (a,K,l) layout
   void df_pp_ladder();                  // lazy (ab|cd) ladder of reduced_abcd_memory
   void ensure_df_blocks();
