@@ -16,6 +16,10 @@
  *                                 (K|G|p q)[inv_sqr] (ccsd.h:93-99, 481-493); selects
  *                                 compute_cs_ccsd_r1_df / compute_cs_ccsd_r2_df
  *                                 (ccsd_r1_r2.h:136-184, 309-520)
+ *   mpqc_ccsd_triples             the (T) correction CCSD_T::compute_ccsd_t
+ *                                 (chemistry/qc/lcao/cc/ccsd_t.h:144-176; algorithm of
+ *                                 compute_ccsd_t_straight :1127-1167 and CCSD_T_Reduce
+ *                                 :2274-2335) from the context's amplitudes
  *   mpqc_ccsd_get_integral        reads a cc::Integrals member back (ccsd_r1_r2.h:24-56)
  *   mpqc_ccsd_transform_ao        LCAOFactory::compute4 without density fitting
  *                                 (chemistry/qc/lcao/factory/lcao_factory.h:409-493) for the
@@ -147,6 +151,12 @@ int mpqc_ccsd_set_integral(mpqc_ccsd_ctx* ctx, int which, const double* data, in
                            int memspace);
 int mpqc_ccsd_set_orbital_energies(mpqc_ccsd_ctx* ctx, const double* eps_occ_act,
                                    const double* eps_vir);
+/* (T) correction from the amplitudes held by the context (normally after
+ * mpqc_ccsd_solve) and its integrals Gijab, Gijka and Giabc (built from Xai and Xab
+ * for the df method when not set) and orbital energies.  *e_t gets E(T);
+ * *seconds (may be NULL) the device time.  Option "triples_block_bytes" (default
+ * 8 GiB) bounds the device memory of the raw triples of one block. */
+int mpqc_ccsd_triples(mpqc_ccsd_ctx* ctx, double* e_t, double* seconds);
 /* Reads back an integral block held by the context (as set, or as built by the
  * transforms below), in the layout and index order of mpqc_ccsd_set_integral. */
 int mpqc_ccsd_get_integral(mpqc_ccsd_ctx* ctx, int which, double* out, int layout, int memspace);

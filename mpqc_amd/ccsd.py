@@ -198,6 +198,14 @@ class CCSD:
             if a is not None:
                 self.set_integral(name, a, layout)
 
+    def triples(self):
+        """(T) correction from the context's amplitudes (CCSD_T::compute_ccsd_t,
+        ccsd_t.h:144-176).  Returns (E(T), device seconds)."""
+        e = ctypes.c_double()
+        sec = ctypes.c_double()
+        self._check(self.lib.mpqc_ccsd_triples(self.ctx, ctypes.byref(e), ctypes.byref(sec)))
+        return e.value, sec.value
+
     def get_integral(self, name: str, layout: str = "dense") -> np.ndarray:
         """An integral block held by the context (as set or as transformed)."""
         o, v, k = self.o, self.v, getattr(self, "naux", 0)

@@ -175,6 +175,13 @@ int mpqc_ccsd_set_df_basis(mpqc_ccsd_ctx* ctx, int naux, const int64_t* aux_hash
   return guarded(ctx, [&](mpqc_hip::CCSDContext& c) { c.set_df_basis(naux, aux_hashmarks, n_aux_tiles); });
 }
 
+int mpqc_ccsd_triples(mpqc_ccsd_ctx* ctx, double* e_t, double* seconds) {
+  return guarded(ctx, [&](mpqc_hip::CCSDContext& c) {
+    if (!e_t) throw std::invalid_argument("null (T) output");
+    *e_t = c.triples(c.triples_block_bytes_, seconds);
+  });
+}
+
 int mpqc_ccsd_get_integral(mpqc_ccsd_ctx* ctx, int which, double* out, int layout, int memspace) {
   return guarded(ctx, [&](mpqc_hip::CCSDContext& c) { c.get_integral(which, out, layout, memspace); });
 }

@@ -225,6 +225,9 @@ void CCSDContext::set_option(const std::string& key, double value) {
   } else if (key == "reduced_abcd_memory") {  // ccsd.h:102, used at :292
     reduced_abcd_ = value != 0.0;
     prepared_ = false;
+  } else if (key == "triples_block_bytes") {
+    if (!(value >= 1 << 20)) throw std::invalid_argument("triples_block_bytes must be at least 1 MiB");
+    triples_block_bytes_ = value;
   } else if (key == "df_block_bytes") {
     if (!(value >= 1 << 20)) throw std::invalid_argument("df_block_bytes must be at least 1 MiB");
     df_block_bytes_ = value;

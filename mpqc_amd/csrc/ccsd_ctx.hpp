@@ -69,6 +69,11 @@ class CCSDContext {
   void update();
   double solve(int max_iter, double target_precision, int verbose, int* n_iter);
   const std::vector<TraceRow>& trace() const { return trace_; }
+  // (T) correction from the current (converged) amplitudes (CCSD_T::compute_ccsd_t,
+  // ccsd_t.h:144-176, "straight" algorithm :1127-1167); seconds (optional) gets the
+  // device time
+  double triples(double block_bytes, double* seconds);
+  double triples_block_bytes_ = 8.0 * (1 << 30);
 
   // ---- adjoint of the converged energy (no counterpart in the reference).
   // With the amplitudes t converged (R(t) = 0), lambda solves (dR/dt)^T lambda = -dE/dt
@@ -131,6 +136,7 @@ class CCSDContext {
   void op_tau();  // tau = t2 + t1 t1  (ccsd.h:389)
   // ---- density-fitted residual pieces (ccsd_r1_r2.h:136-184, 309-520)
   void prepare_df();                    // layouts of the three-index factors
+  void ensure_iabc_for_triples();       // <ia|bc> from the factors when the df set lacks it
   void transform_fock(int nbf, const double* fock, const double* Cact, int memspace);
   void df_r1_intermediates();           // X_ai_tau and its (a,K,l) layout
   void df_pp_ladder();                  // lazy (ab|cd) ladder of reduced_abcd_memory

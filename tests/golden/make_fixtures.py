@@ -11,6 +11,9 @@ and, under "df_cases", the density-fitted run
   tests/validation/reference/outputs/h2o-ccsd-631g-pvdz.out        (RI-RHF 6-31G with the
       cc-pVDZ fitting basis, CCSD method "df", reduced_abcd_memory, frozen core,
       precision 1e-11)
+and, under "triples_cases", the (T) energy of
+  tests/validation/reference/outputs/h2o-ccsd_t-631g-pvdz.out      (the same DF water,
+      CCSD(T) with approach "straight")
 """
 import json
 import os
@@ -62,6 +65,17 @@ for fname, basis, dfbs, fc, o, v, prec in [("h2o-ccsd-631g-pvdz.out", "6-31G", "
                             "e_rhf": e_scf, "e_ccsd": e, "precision": prec,
                             "precision_scale_for_current_rule": 1.0,
                             "note": "RI-RHF reference; current convergence rule", "trace": rows})
+out["triples_cases"] = []
+for fname in ["h2o-ccsd_t-631g-pvdz.out"]:
+    txt = open(os.path.join(REF, "outputs", fname)).read()
+    e_t = float(re.search(r"\(T\) Energy: (-?\d+\.\d+(?:e-?\d+)?)", txt).group(1))
+    e_cc = float(re.search(r"CCSD Energy\s+(-?\d+\.\d+)", txt).group(1))
+    out["triples_cases"].append({"source": f"tests/validation/reference/outputs/{fname}",
+                                 "revision": re.search(r"Revision (\w+)", txt).group(1),
+                                 "basis": "6-31G", "df_basis": "cc-pVDZ", "method": "df",
+                                 "frozen_core": True, "approach": "straight", "e_ccsd": e_cc,
+                                 "e_triples": e_t})
 with open(os.path.join(HERE, "h2o_ccsd.json"), "w") as f:
     json.dump(out, f, indent=1)
-print("wrote", len(out["cases"]), "cases and", len(out["df_cases"]), "DF cases")
+print("wrote", len(out["cases"]), "cases,", len(out["df_cases"]), "DF cases,", len(out["triples_cases"]),
+      "(T) cases")

@@ -206,3 +206,33 @@ def fci_two_electron(f, g):
     Hs = S.T @ H @ S
     w = np.linalg.eigvalsh(Hs)
     return float(w[0] - e_ref)
+
+
+def spinorb_triples(f, g, no, t1_ai, t2_abij):
+    """Spin-orbital (T) of the textbook equations (K. Raghavachari, G. W. Trucks,
+    J. A. Pople, M. Head-Gordon, Chem. Phys. Lett. 157, 479 (1989); written as in
+    T. D. Crawford, H. F. Schaefer, Rev. Comp. Chem. 14, 33 (2000)):
+      D t(c) = P(i/jk) P(a/bc) [t_jk^ae <ei||bc> - t_im^bc <ma||jk>]
+      D t(d) = P(i/jk) P(a/bc) t_i^a <jk||bc>
+      E(T) = 1/36 sum t(c) D (t(c) + t(d)),  P(i/jk) f = f(ijk) - f(jik) - f(kji)
+    from spin-orbital amplitudes t1[a,i], t2[a,b,i,j] (as spinorb_ccsd returns)."""
+    fs, G, O = _spin_blocks(f, g, no)
+    ns = fs.shape[0]
+    o = slice(0, O)
+    v = slice(O, ns)
+    e = np.einsum
+    t1 = t1_ai.T
+    t2 = t2_abij.transpose(2, 3, 0, 1)
+    eo, ev = np.diag(fs)[o], np.diag(fs)[v]
+    D = (eo[:, None, None, None, None, None] + eo[None, :, None, None, None, None]
+         + eo[None, None, :, None, None, None] - ev[None, None, None, :, None, None]
+         - ev[None, None, None, None, :, None] - ev[None, None, None, None, None, :])
+    X = e("ia,jkbc->ijkabc", t1, G[o, o, v, v])
+    Y = e("jkae,eibc->ijkabc", t2, G[v, o, v, v]) - e("imbc,majk->ijkabc", t2, G[o, v, o, o])
+
+    def P(Z):
+        Z = Z - Z.transpose(1, 0, 2, 3, 4, 5) - Z.transpose(2, 1, 0, 3, 4, 5)
+        return Z - Z.transpose(0, 1, 2, 4, 3, 5) - Z.transpose(0, 1, 2, 5, 4, 3)
+    tc = P(Y) / D
+    td = P(X) / D
+    return float(np.sum(tc * D * (tc + td))) / 36.0
