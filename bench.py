@@ -18,7 +18,7 @@ o=32, v=256, FP64, ccsd_r1_r2.h:275; V ~ N(0, 0.05) symmetrised to the 8-fold
 (ac|bd) symmetry, tau ~ N(0, 1e-2) / (e_c + e_d - e_i - e_j)); ccsd-h2o6
 (o=30, v=114); ccsd-h20 (H20 chain, o=10, v=90); copy (counter calibration);
 batched-hchains; adjoint-h2o6; df-h2o8 (density-fitted residual, method df);
-transform-h2o8 (AO->MO integral transform on the device).
+transform-h2o8 (AO->MO integral transform on the device); triples-h2o6 / triples-h2o8 (CCSD(T)).
 """
 from __future__ import annotations
 
@@ -47,7 +47,8 @@ def _args():
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--workload", default="ccsd-h2o8",
                    choices=["ccsd-h2o8", "ccsd-h2o6", "ccsd-h20", "pp-ladder", "copy",
-                            "batched-hchains", "adjoint-h2o6", "df-h2o8", "transform-h2o8"])
+                            "batched-hchains", "adjoint-h2o6", "df-h2o8", "transform-h2o8",
+                            "triples-h2o6", "triples-h2o8"])
     p.add_argument("--forms", default="f64,f16x3",
                    help="contraction precisions to run for the CCSD workloads (first = reference)")
     p.add_argument("--max-iter", type=int, default=40)
@@ -835,6 +836,71 @@ def bench_transform(args):
     return line
 
 
+def bench_triples(args):
+    """CCSD(T) on one GPU: (H2O)6 or (H2O)8 cc-pVDZ, FP64 CCSD converged to 1e-8,
+    then one (T) evaluation (mpqc_ccsd_triples) per step from the resident
+    amplitudes and integrals."""
+    from mpqc_amd.ccsd import CCSD, Integrals
+    from oracle.ccsd_t import ccsd_t_flops
+    name = "ccsd-h2o8" if args.workload == "triples-h2o8" else "ccsd-h2o6"
+    inp, setup = build_system(name)
+    o, v = inp.o, inp.v
+    with CCSD(o, v, max_iter=args.max_iter) as c:
+        c.set_integrals(Integrals(**inp.blocks))
+        c.set_orbital_energies(inp.eps_occ, inp.eps_vir)
+        E = c.compute_ccsd(target_precision=1e-8)
+        log(f"{name}: E_CCSD={E:.12f}; (T)")
+        for _ in range(args.warmup):
+            c.triples()
+        times, ets = [], []
+        for _ in range(args.steps):
+            t0 = time.perf_counter()
+            et, dev_s = c.triples()
+            times.append(time.perf_counter() - t0)
+            ets.append(et)
+            log(f"  (T) = {et:.12e} in {times[-1]:.3f}s (device {dev_s:.3f}s)")
+    flops = ccsd_t_flops(o, v)
+    sec = sum(times) / len(times)
+    cpu = None
+    if not args.no_cpu_baseline:
+        import numpy as np
+        from mpqc_amd.chem.mo import ccsd_integrals
+        from mpqc_amd.chem.molecule import water_cluster
+        from mpqc_amd.chem.scf import rhf
+        from oracle import ccsd as O
+        from oracle.ccsd_t import ccsd_t_straight
+        # bounded sample: the oracle's straight (T) for one water monomer of the
+        # same generator (its t3 fits in host memory), at MP2 amplitudes
+        small = ccsd_integrals(rhf(water_cluster((1, 1, 1), seed=4), "cc-pVDZ"))
+        b = small.blocks
+        t1 = O.jacobi_update_t1(b["Fia"].T, small.eps_occ, small.eps_vir)
+        t2 = O.jacobi_update_t2(b["Gijab"].transpose(2, 3, 0, 1), small.eps_occ, small.eps_vir)
+        ta = time.perf_counter()
+        ccsd_t_straight(t1, t2, b["Gijab"], b["Giabc"], b["Gijka"], small.eps_occ, small.eps_vir)
+        dt = time.perf_counter() - ta
+        sf = ccsd_t_flops(small.o, small.v)
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        cpu = {"value": sf / dt / 1e12, "unit": "TFLOP/s", "cores": threads, "kind": "port",
+               "seconds": round(dt, 3),
+               "sample": f"the oracle's straight (T) (numpy einsum, full t3 in memory) for one water "
+                         f"(o={small.o}, v={small.v}), {sf:.3e} FLOPs"}
+    line = {
+        "metric": "CCSD(T) triples time on one GPU",
+        "value": round(sec, 4), "unit": "s per (T)", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(sec * 1e3, 3), "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic molecule (seeded geometry), RHF + AO->MO on the host, converged FP64 CCSD",
+        "config": {"workload": f"(T) for {SYSTEMS[name]}", "o": o, "v": v, "triples_flops": flops,
+                   "parallelism": "single-gpu", "E_CCSD": E, "E_T": ets[-1]},
+        "roofline": {"bound": "mfma", "achieved": round(flops / sec / 1e12, 3), "peak": PEAK["f64"],
+                     "unit": "TFLOP/s", "frac": round(flops / sec / 1e12 / PEAK["f64"], 4), "traffic": None,
+                     "kernel": "whole (T): 2 o^3 v^4 + 2 o^4 v^3 GEMM FLOPs over the wall time"},
+        "setup": setup,
+        "cpu_baseline": cpu,
+    }
+    return line
+
+
 def main():
     args = _args()
     if args.gpus != 1:
@@ -857,6 +923,8 @@ def main():
         line = bench_df(args)
     elif args.workload == "transform-h2o8":
         line = bench_transform(args)
+    elif args.workload in ("triples-h2o6", "triples-h2o8"):
+        line = bench_triples(args)
     else:
         line = bench_ccsd(args)
     print(json.dumps(line), flush=True)

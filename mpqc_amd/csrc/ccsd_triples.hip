@@ -8,16 +8,21 @@
 //   v3s = <ij|ab> t1(c,k) + <jk|bc> t1(a,i) + <ik|ac> t1(b,j)
 //   E(T) = 1/3 sum (t3s + v3s)(4 t3s + t3s(abckij) + t3s(abcjki)
 //                   - 2 (t3s(abckji) + t3s(abcikj) + t3s(abcjik))) / D
-// The reference stores t3 over all (a,b,c,i,j,k); here the occupied triples
+// The reference stores t3 over all (a,b,c,i,j,k).  Here the occupied triples
 // are visited as multisets {i <= j <= k}: for a pair (j,k) and a block of i,
-// 12 batched GEMMs build the raw W of all six orderings of each triple
-// (particle term M = v^2, N = v, K = v; hole term K = o), and one kernel sums
-// the energy.  A thread of that kernel owns one virtual multiset
-// {a >= b >= c}: it loads the 36 values W[ordering t](permutation q of a,b,c)
-// once, forms t3s for every (permutation, ordering) pair from them and adds
-// the 36 energy terms of the orbit, weighted by 1/(|stabiliser of abc| *
-// |stabiliser of ijk|) so that every ordered (a,b,c,i,j,k) counts once.  No
-// t3 element is stored beyond the current block.
+// six batched GEMMs build the raw W of the six orderings of each triple.  The
+// particle and hole terms are one GEMM with K = v + o: A_i = [<id|ab>^T | t2(a,b,i,:)]
+// ((a,b) x (v+o), built once) and B_jk = [t2(:,c,j,k) ; -<jk|:c>] ((v+o) x c).
+//
+// Energy: t3s and v3s are invariant under the simultaneous permutation of
+// (a,b,c) and (i,j,k), and the coefficients (4, 1, 1, -2, -2, -2) depend only
+// on the conjugacy class of the occupied permutation, so the 36 energy terms
+// of the orbit pair {a,b,c} x {i,j,k} collapse to 6 x sum_u F(u) with
+//   T_u = t3s(x, y_u) = sum_r W_{u o r}(x_r),  F(u) = (T_u + V_u) sum_m c_m T_{u o m},
+// x = (a >= b >= c), y = (i <= j <= k), weighted by 1/(|stab x| |stab y|) so
+// that every ordered (a,b,c,i,j,k) counts once.  A workgroup owns a multiset
+// of 8^3 cubes of (a,b,c) space and stages, ordering by ordering, the six
+// permuted cubes of W_s through LDS, so W is read with coalesced rows once.
 #include <cmath>
 #include <stdexcept>
 
@@ -29,33 +34,44 @@ namespace {
 
 // S3 as position maps: 0 (0,1,2) 1 (0,2,1) 2 (1,0,2) 3 (1,2,0) 4 (2,0,1) 5 (2,1,0)
 __constant__ int kPerm[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
-// kComp[p][r] = index of the map n -> p[r[n]]
+// kComp[p][r] = index of the map n -> p[r[n]];  kInv[r] = index of r^-1
 __constant__ int kComp[6][6];
+__constant__ int kInv[6];
 // energy terms: t3s at occupied order (y'[m[0]], y'[m[1]], y'[m[2]]) with coefficient
 // (i,j,k):4  (k,i,j):1  (j,k,i):1  (k,j,i):-2  (i,k,j):-2  (j,i,k):-2
 __constant__ int kTermPerm[6] = {0, 4, 3, 5, 1, 2};
 __constant__ double kTermCoef[6] = {4.0, 1.0, 1.0, -2.0, -2.0, -2.0};
 
+constexpr int kT = 8;                        // cube edge
+constexpr int kS2 = kT + 1, kS1 = kT * kS2 + 1;  // padded LDS strides (odd: no bank conflicts)
+constexpr int kCube = kT * kS1;
 constexpr int kTThreads = 256;
 
 void upload_s3_tables(hipStream_t s) {
   static const int P[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
-  int comp[6][6];
-  for (int p = 0; p < 6; ++p)
+  auto index_of = [&](const int m[3]) {
+    for (int q = 0; q < 6; ++q)
+      if (P[q][0] == m[0] && P[q][1] == m[1] && P[q][2] == m[2]) return q;
+    return -1;
+  };
+  int comp[6][6], inv[6];
+  for (int p = 0; p < 6; ++p) {
     for (int r = 0; r < 6; ++r) {
       const int m[3] = {P[p][P[r][0]], P[p][P[r][1]], P[p][P[r][2]]};
-      int idx = -1;
-      for (int q = 0; q < 6; ++q)
-        if (P[q][0] == m[0] && P[q][1] == m[1] && P[q][2] == m[2]) idx = q;
-      comp[p][r] = idx;
+      comp[p][r] = index_of(m);
     }
+    int m[3];
+    for (int n = 0; n < 3; ++n) m[P[p][n]] = n;
+    inv[p] = index_of(m);
+  }
   MPQC_HIP_CHECK(hipMemcpyToSymbolAsync(HIP_SYMBOL(kComp), comp, sizeof(comp), 0, hipMemcpyHostToDevice, s));
+  MPQC_HIP_CHECK(hipMemcpyToSymbolAsync(HIP_SYMBOL(kInv), inv, sizeof(inv), 0, hipMemcpyHostToDevice, s));
 }
 
 __device__ inline long tet(long a) { return a * (a + 1) * (a + 2) / 6; }
 
-// virtual multiset number e -> (a >= b >= c)
-__device__ inline void unrank_abc(long e, int& a, int& b, int& c) {
+// multiset number e -> (a >= b >= c)
+__device__ inline void unrank3(long e, int& a, int& b, int& c) {
   long x = (long)cbrt(6.0 * (double)e);
   while (x > 0 && tet(x) > e) --x;
   while (tet(x + 1) <= e) ++x;
@@ -76,67 +92,88 @@ struct TriplesArgs {
   const double* eo;
   const double* ev;
   int o, v, i0, j, k;
-  long nabc;        // v (v+1) (v+2) / 6
 };
 
 __global__ __launch_bounds__(kTThreads) void triples_energy_kernel(TriplesArgs g, double* partials) {
-  const long e = (long)blockIdx.x * kTThreads + threadIdx.x;
+  __shared__ double tile[6 * kCube];
+  __shared__ double red[kTThreads];
   const int z = blockIdx.y;
-  double acc = 0.0;
-  if (e < g.nabc) {
-    int x[3];
-    unrank_abc(e, x[0], x[1], x[2]);
-    const int y[3] = {g.i0 + z, g.j, g.k};
-    const int v = g.v, o = g.o;
-    const long v3 = (long)v * v * v;
-    double L[6][6];  // L[q][t] = W_t(x_q)
+  const int v = g.v, o = g.o;
+  const long v3 = (long)v * v * v;
+  int X[3];
+  unrank3(blockIdx.x, X[0], X[1], X[2]);  // cube multiset A >= B >= C
+  const int y[3] = {g.i0 + z, g.j, g.k};
+  // this thread's two points of cube (A,B,C): local coordinates and validity
+  int l[2][3];
+  bool ok[2];
 #pragma unroll
-    for (int t = 0; t < 6; ++t) {
-      const double* Wt = g.W + t * g.wsig + (long)z * v3;
-#pragma unroll
-      for (int q = 0; q < 6; ++q)
-        L[q][t] = Wt[((long)x[kPerm[q][0]] * v + x[kPerm[q][1]]) * v + x[kPerm[q][2]]];
-    }
-    double TS[6][6];  // t3s(x_p, y_s) = sum_r L[p o r][s o r]
-#pragma unroll
-    for (int p = 0; p < 6; ++p)
-#pragma unroll
-      for (int s = 0; s < 6; ++s) {
-        double t = 0.0;
-#pragma unroll
-        for (int r = 0; r < 6; ++r) t += L[kComp[p][r]][kComp[s][r]];
-        TS[p][s] = t;
-      }
-    double sum = 0.0;
-#pragma unroll
-    for (int p = 0; p < 6; ++p) {
-      const int a0 = x[kPerm[p][0]], a1 = x[kPerm[p][1]], a2 = x[kPerm[p][2]];
-#pragma unroll
-      for (int s = 0; s < 6; ++s) {
-        const int i0 = y[kPerm[s][0]], i1 = y[kPerm[s][1]], i2 = y[kPerm[s][2]];
-        // v3s(x', y') with g_abij(a,b,i,j) = Gijab(i,j,a,b)
-        const double vs = g.G[(((long)i0 * o + i1) * v + a0) * v + a1] * g.t1[(long)a2 * o + i2] +
-                          g.G[(((long)i1 * o + i2) * v + a1) * v + a2] * g.t1[(long)a0 * o + i0] +
-                          g.G[(((long)i0 * o + i2) * v + a0) * v + a2] * g.t1[(long)a1 * o + i1];
-        double X = 0.0;
-#pragma unroll
-        for (int m = 0; m < 6; ++m) X += kTermCoef[m] * TS[p][kComp[s][kTermPerm[m]]];
-        sum += (TS[p][s] + vs) * X;
-      }
-    }
-    const int sx = (x[0] == x[1] && x[1] == x[2]) ? 6 : (x[0] == x[1] || x[1] == x[2]) ? 2 : 1;
-    const int sy = (y[0] == y[1] && y[1] == y[2]) ? 6 : (y[0] == y[1] || y[1] == y[2]) ? 2 : 1;
-    const double D = g.eo[y[0]] + g.eo[y[1]] + g.eo[y[2]] - g.ev[x[0]] - g.ev[x[1]] - g.ev[x[2]];
-    acc = sum / (3.0 * D * (double)(sx * sy));
+  for (int h = 0; h < 2; ++h) {
+    const int q = threadIdx.x + h * kTThreads;
+    l[h][0] = q / (kT * kT);
+    l[h][1] = (q / kT) % kT;
+    l[h][2] = q % kT;
+    const int a = X[0] * kT + l[h][0], b = X[1] * kT + l[h][1], c = X[2] * kT + l[h][2];
+    ok[h] = a < v && b < v && c < v && a >= b && b >= c;
   }
-  __shared__ double sh[kTThreads];
-  sh[threadIdx.x] = acc;
-  __syncthreads();
-  for (int w = kTThreads / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+  double T[2][6];
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int u = 0; u < 6; ++u) T[h][u] = 0.0;
+  for (int s = 0; s < 6; ++s) {
+    const double* Ws = g.W + s * g.wsig + (long)z * v3;
+    // the six permuted cubes (X[r0], X[r1], X[r2]) of W_s; rows of kT contiguous c
+    for (int e = threadIdx.x; e < 6 * kT * kT * kT; e += kTThreads) {
+      const int r = e / (kT * kT * kT), rem = e - r * (kT * kT * kT);
+      const int u = rem / (kT * kT), w = (rem / kT) % kT, zz = rem % kT;
+      const int ga = X[kPerm[r][0]] * kT + u, gb = X[kPerm[r][1]] * kT + w, gc = X[kPerm[r][2]] * kT + zz;
+      double val = 0.0;
+      if (ga < v && gb < v && gc < v) val = Ws[((long)ga * v + gb) * v + gc];
+      tile[r * kCube + u * kS1 + w * kS2 + zz] = val;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      if (!ok[h]) continue;
+#pragma unroll
+      for (int r = 0; r < 6; ++r) {
+        // x_r lies in cube r at local (l[r0], l[r1], l[r2]); W_s(x_r) adds to T_{s o r^-1}
+        const double val = tile[r * kCube + l[h][kPerm[r][0]] * kS1 + l[h][kPerm[r][1]] * kS2 + l[h][kPerm[r][2]]];
+        T[h][kComp[s][kInv[r]]] += val;
+      }
+    }
     __syncthreads();
   }
-  if (threadIdx.x == 0) partials[(long)blockIdx.y * gridDim.x + blockIdx.x] = sh[0];
+  double acc = 0.0;
+  const int sy = (y[0] == y[1] && y[1] == y[2]) ? 6 : (y[0] == y[1] || y[1] == y[2]) ? 2 : 1;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    if (!ok[h]) continue;
+    const int x[3] = {X[0] * kT + l[h][0], X[1] * kT + l[h][1], X[2] * kT + l[h][2]};
+    double sum = 0.0;
+#pragma unroll
+    for (int u = 0; u < 6; ++u) {
+      const int i0 = y[kPerm[u][0]], i1 = y[kPerm[u][1]], i2 = y[kPerm[u][2]];
+      // v3s(x, y_u) with g_abij(a,b,i,j) = Gijab(i,j,a,b)
+      const double vs = g.G[(((long)i0 * o + i1) * v + x[0]) * v + x[1]] * g.t1[(long)x[2] * o + i2] +
+                        g.G[(((long)i1 * o + i2) * v + x[1]) * v + x[2]] * g.t1[(long)x[0] * o + i0] +
+                        g.G[(((long)i0 * o + i2) * v + x[0]) * v + x[2]] * g.t1[(long)x[1] * o + i1];
+      double Xm = 0.0;
+#pragma unroll
+      for (int m = 0; m < 6; ++m) Xm += kTermCoef[m] * T[h][kComp[u][kTermPerm[m]]];
+      sum += (T[h][u] + vs) * Xm;
+    }
+    const int sx = (x[0] == x[1] && x[1] == x[2]) ? 6 : (x[0] == x[1] || x[1] == x[2]) ? 2 : 1;
+    const double D = g.eo[y[0]] + g.eo[y[1]] + g.eo[y[2]] - g.ev[x[0]] - g.ev[x[1]] - g.ev[x[2]];
+    acc += 2.0 * sum / (D * (double)(sx * sy));  // 6 sum_u F(u) / (3 D |stab x| |stab y|)
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = kTThreads / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partials[(long)blockIdx.y * gridDim.x + blockIdx.x] = red[0];
 }
 
 // acc[slot] = sum of partials[0..n) in a fixed order (one workgroup)
@@ -169,6 +206,17 @@ struct DevBuf {
   DevBuf& operator=(const DevBuf&) = delete;
 };
 
+// out (dense, extents out_ext) = alpha * input permuted (out axis d reads input axis perm[d])
+void permute_into(double* out, const long out_ext[4], const double* in, const long in_ext[4], const int perm[4],
+                  double alpha, hipStream_t s) {
+  TTerm t;
+  t.p = in;
+  permuted_strides(in_ext, perm, t.s);
+  t.alpha = alpha;
+  t.vol = in_ext[0] * in_ext[1] * in_ext[2] * in_ext[3];
+  tadd(out, out_ext, 0.0, &t, 1, s);
+}
+
 }  // namespace
 
 // Giabc(i,a,b,c) = <ia|bc> = (ib|ac) = sum_K Xai(K,b,i) Xab(K,a,c) for the df
@@ -182,15 +230,9 @@ void CCSDContext::ensure_iabc_for_triples() {
   // T[(i,b)][(a,c)] = Xia(K,(i,b))^T Xab(K,(a,c)) ; Giabc(i,a,b,c) = T(i,b,a,c)
   mpqc_hip::gemm(make_gemm(true, false, (int)(O * V), (int)(V * V), (int)Q, 1.0, Xia_, O * V, Xab_, V * V, 0.0,
                            T.p, V * V), GemmPrecision::F64, ws_, stream_);
-  TTerm t;
-  t.p = T.p;
-  const long in_ext[4] = {O, V, V, V};
+  const long ext[4] = {O, V, V, V};
   const int pm[4] = {0, 2, 1, 3};
-  permuted_strides(in_ext, pm, t.s);
-  t.alpha = 1.0;
-  t.vol = O * V * V * V;
-  const long out_ext[4] = {O, V, V, V};
-  tadd(Giabc_, out_ext, 0.0, &t, 1, stream_);
+  permute_into(Giabc_, ext, T.p, ext, pm, 1.0, stream_);
   MPQC_HIP_CHECK(hipStreamSynchronize(stream_));
   have_ints_ |= 1u << 6;
 }
@@ -200,9 +242,10 @@ double CCSDContext::triples(double block_bytes, double* seconds) {
   if (!prepared_) prepare();
   ensure_iabc_for_triples();
   const int o = o_, v = v_;
-  const long O = o, V = v, v3 = V * V * V, ov = O * V;
-  if (V * (V + 1) * (V + 2) / 6 >= (1L << 31) || V * V >= (1L << 31))
-    throw std::invalid_argument("v too large for (T)");
+  const long O = o, V = v, v3 = V * V * V, VK = V + O;
+  const long nB = ceil_div(V, kT);
+  const long ncube = nB * (nB + 1) * (nB + 2) / 6;
+  if (ncube >= (1L << 31) || V * V >= (1L << 31)) throw std::invalid_argument("v too large for (T)");
   long nbmax = (long)(block_bytes / (6.0 * sizeof(double) * (double)v3));
   nbmax = std::max(1L, std::min(nbmax, O));
   hipEvent_t ev0, ev1;
@@ -210,24 +253,53 @@ double CCSDContext::triples(double block_bytes, double* seconds) {
   MPQC_HIP_CHECK(hipEventCreate(&ev1));
   MPQC_HIP_CHECK(hipEventRecord(ev0, stream_));
   upload_s3_tables(stream_);
-  // t2P(j,k,d,c) = t2(d,c,j,k): the B operand of the particle term for a pair (j,k)
-  DevBuf t2P((size_t)(ov * ov)), W((size_t)(6 * nbmax * v3));
+  // GEMM operands with K = v + o:
+  //   Acat(i, (a,b), K) = [<id|ab> for K = d < v | t2(a,b,i,l) for K = v + l]
+  //   Bcat(j, k, K, c)  = [t2(d,c,j,k)           | -<jk|lc>]
+  DevBuf Acat((size_t)(O * V * V * VK)), Bcat((size_t)(O * O * VK * V)), W((size_t)(6 * nbmax * v3));
   {
+    // Acat[i][(a,b)][d] = Giabc(i,d,a,b): a permutation of Giabc (i,d,a,b) -> (i,a,b,d)
+    // into rows of length VK; Acat[i][(a,b)][v + l] = t2(a,b,i,l)
     TTerm t;
-    t.p = t2_;
-    const long in_ext[4] = {V, V, O, O};
-    const int pm[4] = {2, 3, 0, 1};
-    permuted_strides(in_ext, pm, t.s);
-    t.vol = ov * ov;
-    const long out_ext[4] = {O, O, V, V};
-    tadd(t2P.p, out_ext, 0.0, &t, 1, stream_);
+    const long e_iabc[4] = {O, V, V, V};
+    const int p1[4] = {0, 2, 3, 1};
+    t.p = Giabc_;
+    permuted_strides(e_iabc, p1, t.s);
+    t.alpha = 1.0;
+    t.vol = O * V * V * V;
+    // write through strided output: tadd writes dense rows, so permute into a dense
+    // (i,a,b,d) buffer first and interleave rows with 2D copies
+    DevBuf tmp((size_t)(O * V * V * std::max(V, O)));
+    const long ext1[4] = {O, V, V, V};
+    tadd(tmp.p, ext1, 0.0, &t, 1, stream_);
+    const size_t d = sizeof(double);
+    MPQC_HIP_CHECK(hipMemcpy2DAsync(Acat.p, VK * d, tmp.p, V * d, V * d, O * V * V, hipMemcpyDeviceToDevice,
+                                    stream_));
+    // t2(a,b,i,l) -> (i,a,b,l)
+    const long e_t2[4] = {V, V, O, O};
+    const int p2[4] = {2, 0, 1, 3};
+    const long ext2[4] = {O, V, V, O};
+    permute_into(tmp.p, ext2, t2_, e_t2, p2, 1.0, stream_);
+    MPQC_HIP_CHECK(hipMemcpy2DAsync(Acat.p + V, VK * d, tmp.p, O * d, O * d, O * V * V, hipMemcpyDeviceToDevice,
+                                    stream_));
+    // Bcat[j][k][d][c] = t2(d,c,j,k): (j,k,d,c) blocks of V rows; Bcat[j][k][v + l][c] = -Gijka(j,k,l,c)
+    DevBuf tmp2((size_t)(O * O * std::max(V, O) * V));
+    const int p3[4] = {2, 3, 0, 1};
+    const long ext3[4] = {O, O, V, V};
+    permute_into(tmp2.p, ext3, t2_, e_t2, p3, 1.0, stream_);
+    MPQC_HIP_CHECK(hipMemcpy2DAsync(Bcat.p, VK * V * d, tmp2.p, V * V * d, V * V * d, O * O,
+                                    hipMemcpyDeviceToDevice, stream_));
+    const long e_ijka[4] = {O, O, O, V};
+    const int pid[4] = {0, 1, 2, 3};
+    permute_into(tmp2.p, e_ijka, Gijka_, e_ijka, pid, -1.0, stream_);
+    MPQC_HIP_CHECK(hipMemcpy2DAsync(Bcat.p + V * V, VK * V * d, tmp2.p, O * V * d, O * V * d, O * O,
+                                    hipMemcpyDeviceToDevice, stream_));
+    MPQC_HIP_CHECK(hipStreamSynchronize(stream_));  // tmp buffers are released on return
   }
-  const long nabc = V * (V + 1) * (V + 2) / 6;
-  const int gx = (int)ceil_div(nabc, kTThreads);
   long nchunks = 0;
   for (int k = 0; k < o; ++k)
     for (int j = 0; j <= k; ++j) nchunks += ceil_div(j + 1, nbmax);
-  DevBuf partials((size_t)gx * nbmax), acc((size_t)nchunks);
+  DevBuf partials((size_t)(ncube * nbmax)), acc((size_t)nchunks);
   static const int P[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
   long slot = 0;
   for (int k = 0; k < o; ++k)
@@ -236,33 +308,25 @@ double CCSDContext::triples(double block_bytes, double* seconds) {
         const int nb = (int)std::min<long>(nbmax, j + 1 - i0);
         const int y0[3] = {i0, j, k};
         for (int s = 0; s < 6; ++s) {
-          // ordering s: (o1, o2, o3) = (y[P0], y[P1], y[P2]); position of i among them
+          // ordering s: (o1, o2, o3) = (y[P0], y[P1], y[P2]); i sits at position ipos
           const int o1 = y0[P[s][0]], o2 = y0[P[s][1]], o3 = y0[P[s][2]];
           const int ipos = P[s][0] == 0 ? 0 : P[s][1] == 0 ? 1 : 2;
-          double* Ws = W.p + (long)s * nbmax * v3;
-          // particle: W(a,b,c) = sum_d Giabc(o1,d,a,b) t2P(o2,o3,d,c)
-          GemmDesc dp = make_gemm(true, false, v * v, v, v, 1.0, Giabc_ + (long)o1 * v3, V * V,
-                                  t2P.p + ((long)o2 * O + o3) * V * V, V, 0.0, Ws, V);
-          dp.sA = ipos == 0 ? v3 : 0;
-          dp.sB = ipos == 1 ? O * V * V : ipos == 2 ? V * V : 0;
-          dp.sC = v3;
-          dp.batch = nb;
-          mpqc_hip::gemm(dp, GemmPrecision::F64, ws_, stream_);
-          // hole: W(a,b,c) -= sum_l t2(a,b,o1,l) Gijka(o2,o3,l,c)
-          GemmDesc dh = make_gemm(false, false, v * v, v, o, -1.0, t2_ + (long)o1 * O, O * O,
-                                  Gijka_ + ((long)o2 * O + o3) * O * V, V, 1.0, Ws, V);
-          dh.sA = ipos == 0 ? O : 0;
-          dh.sB = ipos == 1 ? O * O * V : ipos == 2 ? O * V : 0;
-          dh.sC = v3;
-          dh.batch = nb;
-          mpqc_hip::gemm(dh, GemmPrecision::F64, ws_, stream_);
+          // W(a,b,c) = sum_K Acat(o1,(a,b),K) Bcat(o2,o3,K,c)
+          GemmDesc dg = make_gemm(false, false, v * v, v, (int)VK, 1.0, Acat.p + (long)o1 * V * V * VK, VK,
+                                  Bcat.p + ((long)o2 * O + o3) * VK * V, V, 0.0, W.p + (long)s * nbmax * v3, V);
+          dg.sA = ipos == 0 ? V * V * VK : 0;
+          dg.sB = ipos == 1 ? O * VK * V : ipos == 2 ? VK * V : 0;
+          dg.sC = v3;
+          dg.batch = nb;
+          mpqc_hip::gemm(dg, GemmPrecision::F64, ws_, stream_);
         }
         TriplesArgs g;
         g.W = W.p; g.wsig = nbmax * v3; g.G = Gijab_; g.t1 = t1_; g.eo = eo_; g.ev = ev_;
-        g.o = o; g.v = v; g.i0 = i0; g.j = j; g.k = k; g.nabc = nabc;
-        hipLaunchKernelGGL(triples_energy_kernel, dim3(gx, nb), dim3(kTThreads), 0, stream_, g, partials.p);
+        g.o = o; g.v = v; g.i0 = i0; g.j = j; g.k = k;
+        hipLaunchKernelGGL(triples_energy_kernel, dim3((unsigned)ncube, nb), dim3(kTThreads), 0, stream_, g,
+                           partials.p);
         check_launch("triples_energy");
-        hipLaunchKernelGGL(sum_to_slot, dim3(1), dim3(256), 0, stream_, partials.p, (long)gx * nb, acc.p, slot);
+        hipLaunchKernelGGL(sum_to_slot, dim3(1), dim3(256), 0, stream_, partials.p, ncube * nb, acc.p, slot);
         check_launch("triples_sum");
         ++slot;
       }
