@@ -10,9 +10,12 @@
 //                   - 2 (t3s(abckji) + t3s(abcikj) + t3s(abcjik))) / D
 // The reference stores t3 over all (a,b,c,i,j,k).  Here the occupied triples
 // are visited as multisets {i <= j <= k}: for a pair (j,k) and a block of i,
-// six batched GEMMs build the raw W of the six orderings of each triple.  The
+// five GEMMs build the raw W of the six orderings of each triple.  The
 // particle and hole terms are one GEMM with K = v + o: A_i = [<id|ab>^T | t2(a,b,i,:)]
 // ((a,b) x (v+o), built once) and B_jk = [t2(:,c,j,k) ; -<jk|:c>] ((v+o) x c).
+// The orderings (i,j,k), (i,k,j) stack the block's i along M and the two B's
+// along N; the four orderings led by j or k share A and put the block's i
+// along N (N = nb v), so every GEMM has long M and N.
 //
 // Energy: t3s and v3s are invariant under the simultaneous permutation of
 // (a,b,c) and (i,j,k), and the coefficients (4, 1, 1, -2, -2, -2) depend only
@@ -85,8 +88,10 @@ __device__ inline void unrank3(long e, int& a, int& b, int& c) {
 }
 
 struct TriplesArgs {
-  const double* W;  // [6 orderings][nbmax][v^3]
-  long wsig;        // stride between orderings (nbmax v^3)
+  // W of ordering s for triple z of the block at (a,b,c):
+  //   Wb[s] + z * Wz[s] + (a v + b) * Wr[s] + c
+  const double* Wb[6];
+  long Wz[6], Wr[6];
   const double* G;  // Gijab (i,j,a,b)
   const double* t1; // (a,i)
   const double* eo;
@@ -99,7 +104,6 @@ __global__ __launch_bounds__(kTThreads) void triples_energy_kernel(TriplesArgs g
   __shared__ double red[kTThreads];
   const int z = blockIdx.y;
   const int v = g.v, o = g.o;
-  const long v3 = (long)v * v * v;
   int X[3];
   unrank3(blockIdx.x, X[0], X[1], X[2]);  // cube multiset A >= B >= C
   const int y[3] = {g.i0 + z, g.j, g.k};
@@ -121,14 +125,15 @@ __global__ __launch_bounds__(kTThreads) void triples_energy_kernel(TriplesArgs g
 #pragma unroll
     for (int u = 0; u < 6; ++u) T[h][u] = 0.0;
   for (int s = 0; s < 6; ++s) {
-    const double* Ws = g.W + s * g.wsig + (long)z * v3;
+    const double* Ws = g.Wb[s] + (long)z * g.Wz[s];
+    const long rs = g.Wr[s];
     // the six permuted cubes (X[r0], X[r1], X[r2]) of W_s; rows of kT contiguous c
     for (int e = threadIdx.x; e < 6 * kT * kT * kT; e += kTThreads) {
       const int r = e / (kT * kT * kT), rem = e - r * (kT * kT * kT);
       const int u = rem / (kT * kT), w = (rem / kT) % kT, zz = rem % kT;
       const int ga = X[kPerm[r][0]] * kT + u, gb = X[kPerm[r][1]] * kT + w, gc = X[kPerm[r][2]] * kT + zz;
       double val = 0.0;
-      if (ga < v && gb < v && gc < v) val = Ws[((long)ga * v + gb) * v + gc];
+      if (ga < v && gb < v && gc < v) val = Ws[((long)ga * v + gb) * rs + gc];
       tile[r * kCube + u * kS1 + w * kS2 + zz] = val;
     }
     __syncthreads();
@@ -254,36 +259,29 @@ double CCSDContext::triples(double block_bytes, double* seconds) {
   MPQC_HIP_CHECK(hipEventRecord(ev0, stream_));
   upload_s3_tables(stream_);
   // GEMM operands with K = v + o:
-  //   Acat(i, (a,b), K) = [<id|ab> for K = d < v | t2(a,b,i,l) for K = v + l]
-  //   Bcat(j, k, K, c)  = [t2(d,c,j,k)           | -<jk|lc>]
-  DevBuf Acat((size_t)(O * V * V * VK)), Bcat((size_t)(O * O * VK * V)), W((size_t)(6 * nbmax * v3));
+  //   Acat(i, (a,b), K)   = [<id|ab> for K = d < v | t2(a,b,i,l) for K = v + l]
+  //   Bcat(j, k, K, c)    = [t2(d,c,j,k)           | -<jk|lc>]
+  // and three layouts of Bcat that make the N dimension of each GEMM long:
+  //   Bt (K, j, k, c), Bt2 (K, k, j, c) = Bcat(j, k, K, c), Bp (j, k, K, 2, c) = [Bcat(j,k) | Bcat(k,j)]
+  const size_t d = sizeof(double);
+  DevBuf Acat((size_t)(O * V * V * VK)), Bt((size_t)(VK * O * O * V)), Bt2((size_t)(VK * O * O * V)),
+      Bp((size_t)(2 * O * O * VK * V)), W((size_t)(6 * nbmax * v3));
   {
-    // Acat[i][(a,b)][d] = Giabc(i,d,a,b): a permutation of Giabc (i,d,a,b) -> (i,a,b,d)
-    // into rows of length VK; Acat[i][(a,b)][v + l] = t2(a,b,i,l)
-    TTerm t;
+    DevBuf tmp((size_t)(O * V * V * std::max(V, O)));
+    // Acat[i][(a,b)][d] = Giabc(i,d,a,b); Acat[i][(a,b)][v + l] = t2(a,b,i,l)
     const long e_iabc[4] = {O, V, V, V};
     const int p1[4] = {0, 2, 3, 1};
-    t.p = Giabc_;
-    permuted_strides(e_iabc, p1, t.s);
-    t.alpha = 1.0;
-    t.vol = O * V * V * V;
-    // write through strided output: tadd writes dense rows, so permute into a dense
-    // (i,a,b,d) buffer first and interleave rows with 2D copies
-    DevBuf tmp((size_t)(O * V * V * std::max(V, O)));
-    const long ext1[4] = {O, V, V, V};
-    tadd(tmp.p, ext1, 0.0, &t, 1, stream_);
-    const size_t d = sizeof(double);
+    permute_into(tmp.p, e_iabc, Giabc_, e_iabc, p1, 1.0, stream_);
     MPQC_HIP_CHECK(hipMemcpy2DAsync(Acat.p, VK * d, tmp.p, V * d, V * d, O * V * V, hipMemcpyDeviceToDevice,
                                     stream_));
-    // t2(a,b,i,l) -> (i,a,b,l)
     const long e_t2[4] = {V, V, O, O};
     const int p2[4] = {2, 0, 1, 3};
     const long ext2[4] = {O, V, V, O};
     permute_into(tmp.p, ext2, t2_, e_t2, p2, 1.0, stream_);
     MPQC_HIP_CHECK(hipMemcpy2DAsync(Acat.p + V, VK * d, tmp.p, O * d, O * d, O * V * V, hipMemcpyDeviceToDevice,
                                     stream_));
-    // Bcat[j][k][d][c] = t2(d,c,j,k): (j,k,d,c) blocks of V rows; Bcat[j][k][v + l][c] = -Gijka(j,k,l,c)
-    DevBuf tmp2((size_t)(O * O * std::max(V, O) * V));
+    // Bcat[j][k][d][c] = t2(d,c,j,k); Bcat[j][k][v + l][c] = -Gijka(j,k,l,c)
+    DevBuf Bcat((size_t)(O * O * VK * V)), tmp2((size_t)(O * O * std::max(V, O) * V));
     const int p3[4] = {2, 3, 0, 1};
     const long ext3[4] = {O, O, V, V};
     permute_into(tmp2.p, ext3, t2_, e_t2, p3, 1.0, stream_);
@@ -294,34 +292,57 @@ double CCSDContext::triples(double block_bytes, double* seconds) {
     permute_into(tmp2.p, e_ijka, Gijka_, e_ijka, pid, -1.0, stream_);
     MPQC_HIP_CHECK(hipMemcpy2DAsync(Bcat.p + V * V, VK * V * d, tmp2.p, O * V * d, O * V * d, O * O,
                                     hipMemcpyDeviceToDevice, stream_));
-    MPQC_HIP_CHECK(hipStreamSynchronize(stream_));  // tmp buffers are released on return
+    const long e_b[4] = {O, O, VK, V};
+    const int pt[4] = {2, 0, 1, 3}, pt2[4] = {2, 1, 0, 3}, psw[4] = {1, 0, 2, 3};
+    const long e_bt[4] = {VK, O, O, V};
+    permute_into(Bt.p, e_bt, Bcat.p, e_b, pt, 1.0, stream_);
+    permute_into(Bt2.p, e_bt, Bcat.p, e_b, pt2, 1.0, stream_);
+    MPQC_HIP_CHECK(hipMemcpy2DAsync(Bp.p, 2 * V * d, Bcat.p, V * d, V * d, O * O * VK, hipMemcpyDeviceToDevice,
+                                    stream_));
+    DevBuf BcatT((size_t)(O * O * VK * V));
+    permute_into(BcatT.p, e_b, Bcat.p, e_b, psw, 1.0, stream_);  // BcatT(j,k) = Bcat(k,j)
+    MPQC_HIP_CHECK(hipMemcpy2DAsync(Bp.p + V, 2 * V * d, BcatT.p, V * d, V * d, O * O * VK,
+                                    hipMemcpyDeviceToDevice, stream_));
+    MPQC_HIP_CHECK(hipStreamSynchronize(stream_));  // temporaries are released on return
   }
+  // W buffers: orderings (i,j,k), (i,k,j) interleaved as [(i,a,b)][2][c]; the other four as
+  // [(a,b)][i][c] with row stride nbmax v
+  double* W1 = W.p;                       // 2 nbmax v^3
+  double* Wg[4];                          // (j,k,i), (j,i,k), (k,j,i), (k,i,j)
+  for (int q = 0; q < 4; ++q) Wg[q] = W.p + (2 + q) * nbmax * v3;
+  const long ldw = nbmax * V;
   long nchunks = 0;
   for (int k = 0; k < o; ++k)
     for (int j = 0; j <= k; ++j) nchunks += ceil_div(j + 1, nbmax);
   DevBuf partials((size_t)(ncube * nbmax)), acc((size_t)nchunks);
-  static const int P[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
   long slot = 0;
   for (int k = 0; k < o; ++k)
     for (int j = 0; j <= k; ++j)
       for (int i0 = 0; i0 <= j; i0 += (int)nbmax) {
         const int nb = (int)std::min<long>(nbmax, j + 1 - i0);
-        const int y0[3] = {i0, j, k};
-        for (int s = 0; s < 6; ++s) {
-          // ordering s: (o1, o2, o3) = (y[P0], y[P1], y[P2]); i sits at position ipos
-          const int o1 = y0[P[s][0]], o2 = y0[P[s][1]], o3 = y0[P[s][2]];
-          const int ipos = P[s][0] == 0 ? 0 : P[s][1] == 0 ? 1 : 2;
-          // W(a,b,c) = sum_K Acat(o1,(a,b),K) Bcat(o2,o3,K,c)
-          GemmDesc dg = make_gemm(false, false, v * v, v, (int)VK, 1.0, Acat.p + (long)o1 * V * V * VK, VK,
-                                  Bcat.p + ((long)o2 * O + o3) * VK * V, V, 0.0, W.p + (long)s * nbmax * v3, V);
-          dg.sA = ipos == 0 ? V * V * VK : 0;
-          dg.sB = ipos == 1 ? O * VK * V : ipos == 2 ? VK * V : 0;
-          dg.sC = v3;
-          dg.batch = nb;
-          mpqc_hip::gemm(dg, GemmPrecision::F64, ws_, stream_);
-        }
+        // (i,j,k) and (i,k,j): C[(i,a,b)][(t,c)] = Acat[(i,a,b)][K] Bp[j][k][K][(t,c)]
+        mpqc_hip::gemm(make_gemm(false, false, (int)(nb * V * V), (int)(2 * V), (int)VK, 1.0,
+                                 Acat.p + (long)i0 * V * V * VK, VK, Bp.p + ((long)j * O + k) * VK * 2 * V, 2 * V,
+                                 0.0, W1, 2 * V), GemmPrecision::F64, ws_, stream_);
+        // fixed first index o1 in {j, k}: C[(a,b)][(i,c)] = Acat[o1][(a,b)][K] B[K][(i,c)]
+        struct G4 { int o1; const double* B; };
+        const G4 g4[4] = {{j, Bt.p + ((long)k * O + i0) * V},    // (j,k,i): Bcat(k, i)
+                          {j, Bt2.p + ((long)k * O + i0) * V},   // (j,i,k): Bcat(i, k)
+                          {k, Bt.p + ((long)j * O + i0) * V},    // (k,j,i): Bcat(j, i)
+                          {k, Bt2.p + ((long)j * O + i0) * V}};  // (k,i,j): Bcat(i, j)
+        for (int q = 0; q < 4; ++q)
+          mpqc_hip::gemm(make_gemm(false, false, (int)(V * V), (int)(nb * V), (int)VK, 1.0,
+                                   Acat.p + (long)g4[q].o1 * V * V * VK, VK, g4[q].B, O * O * V, 0.0, Wg[q], ldw),
+                         GemmPrecision::F64, ws_, stream_);
         TriplesArgs g;
-        g.W = W.p; g.wsig = nbmax * v3; g.G = Gijab_; g.t1 = t1_; g.eo = eo_; g.ev = ev_;
+        // orderings by S3 index: 0 (i,j,k) 1 (i,k,j) 2 (j,i,k) 3 (j,k,i) 4 (k,i,j) 5 (k,j,i)
+        g.Wb[0] = W1;     g.Wz[0] = 2 * v3; g.Wr[0] = 2 * V;
+        g.Wb[1] = W1 + V; g.Wz[1] = 2 * v3; g.Wr[1] = 2 * V;
+        g.Wb[2] = Wg[1];  g.Wz[2] = V;      g.Wr[2] = ldw;
+        g.Wb[3] = Wg[0];  g.Wz[3] = V;      g.Wr[3] = ldw;
+        g.Wb[4] = Wg[3];  g.Wz[4] = V;      g.Wr[4] = ldw;
+        g.Wb[5] = Wg[2];  g.Wz[5] = V;      g.Wr[5] = ldw;
+        g.G = Gijab_; g.t1 = t1_; g.eo = eo_; g.ev = ev_;
         g.o = o; g.v = v; g.i0 = i0; g.j = j; g.k = k;
         hipLaunchKernelGGL(triples_energy_kernel, dim3((unsigned)ncube, nb), dim3(kTThreads), 0, stream_, g,
                            partials.p);
