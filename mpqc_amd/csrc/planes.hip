@@ -249,26 +249,34 @@ __global__ __launch_bounds__(kThr, (plane_occupancy<BM, BN>())) void gemm_planes
       }
     }
   };
+  // VEC loads are unconditional from clamped addresses and the out-of-range
+  // chunks are zeroed when they are stored to LDS, so no instruction needs the
+  // loaded data before the store (the loads stay in flight across the MFMAs)
+  uint32_t amask = ~0u, bmask = ~0u;
   auto load_regs = [&](int k0) {
 #pragma unroll
     for (int c = 0; c < ACH; ++c) {
       const int ch = tid + kThr * c;
       const int row = ch >> 2, kc = (ch & 3) * 8;
       const int gm = m0 + row, gk = k0 + kc;
+      if constexpr (VEC) {
+        const bool ok = gm < p.M && gk < k_end;
+        amask = ok ? (amask | (1u << c)) : (amask & ~(1u << c));
+        const long off = (long)b * p.sA + (long)min(gm, p.M - 1) * p.lda + min(gk, p.K - 8);
 #pragma unroll
-      for (int q = 0; q < NP; ++q) {
-        u16x8_t v = {0, 0, 0, 0, 0, 0, 0, 0};
-        const uint16_t* src = p.A[q] + (long)b * p.sA + (long)gm * p.lda + gk;
-        if (gm < p.M) {
-          if (VEC) {
-            if (gk < k_end) v = *reinterpret_cast<const u16x8_t*>(src);
-          } else {
+        for (int q = 0; q < NP; ++q) ra[q][c] = *reinterpret_cast<const u16x8_t*>(p.A[q] + off);
+      } else {
+#pragma unroll
+        for (int q = 0; q < NP; ++q) {
+          u16x8_t v = {0, 0, 0, 0, 0, 0, 0, 0};
+          const uint16_t* src = p.A[q] + (long)b * p.sA + (long)gm * p.lda + gk;
+          if (gm < p.M) {
 #pragma unroll
             for (int j = 0; j < 8; ++j)
               if (gk + j < k_end) v[j] = src[j];
           }
+          ra[q][c] = v;
         }
-        ra[q][c] = v;
       }
     }
 #pragma unroll
@@ -276,37 +284,46 @@ __global__ __launch_bounds__(kThr, (plane_occupancy<BM, BN>())) void gemm_planes
       const int ch = tid + kThr * c;
       const int row = ch >> 2, kc = (ch & 3) * 8;
       const int gn = n0 + row, gk = k0 + kc;
+      if constexpr (VEC) {
+        const bool ok = gn < p.N && gk < k_end;
+        bmask = ok ? (bmask | (1u << c)) : (bmask & ~(1u << c));
+        const long off = (long)b * p.sB + (long)min(gn, p.N - 1) * p.ldb + min(gk, p.K - 8);
 #pragma unroll
-      for (int q = 0; q < NP; ++q) {
-        u16x8_t v = {0, 0, 0, 0, 0, 0, 0, 0};
-        const uint16_t* src = p.B[q] + (long)b * p.sB + (long)gn * p.ldb + gk;
-        if (gn < p.N) {
-          if (VEC) {
-            if (gk < k_end) v = *reinterpret_cast<const u16x8_t*>(src);
-          } else {
+        for (int q = 0; q < NP; ++q) rb[q][c] = *reinterpret_cast<const u16x8_t*>(p.B[q] + off);
+      } else {
+#pragma unroll
+        for (int q = 0; q < NP; ++q) {
+          u16x8_t v = {0, 0, 0, 0, 0, 0, 0, 0};
+          const uint16_t* src = p.B[q] + (long)b * p.sB + (long)gn * p.ldb + gk;
+          if (gn < p.N) {
 #pragma unroll
             for (int j = 0; j < 8; ++j)
               if (gk + j < k_end) v[j] = src[j];
           }
+          rb[q][c] = v;
         }
-        rb[q][c] = v;
       }
     }
   };
   auto store_lds = [&](int buf) {
+    const u16x8_t zero = {0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
     for (int c = 0; c < ACH; ++c) {
       const int ch = tid + kThr * c;
       const int row = ch >> 2, kc = (ch & 3) * 8;
+      const bool ok = (amask >> c) & 1u;
 #pragma unroll
-      for (int q = 0; q < NP; ++q) *reinterpret_cast<u16x8_t*>(&As[buf][q][row][swz(row, kc / 8)]) = ra[q][c];
+      for (int q = 0; q < NP; ++q)
+        *reinterpret_cast<u16x8_t*>(&As[buf][q][row][swz(row, kc / 8)]) = ok ? ra[q][c] : zero;
     }
 #pragma unroll
     for (int c = 0; c < BCH; ++c) {
       const int ch = tid + kThr * c;
       const int row = ch >> 2, kc = (ch & 3) * 8;
+      const bool ok = (bmask >> c) & 1u;
 #pragma unroll
-      for (int q = 0; q < NP; ++q) *reinterpret_cast<u16x8_t*>(&Bs[buf][q][row][swz(row, kc / 8)]) = rb[q][c];
+      for (int q = 0; q < NP; ++q)
+        *reinterpret_cast<u16x8_t*>(&Bs[buf][q][row][swz(row, kc / 8)]) = ok ? rb[q][c] : zero;
     }
   };
 
@@ -377,6 +394,10 @@ __global__ __launch_bounds__(kThr, (plane_occupancy<BM, BN>())) void gemm_planes
 #pragma unroll
         for (int q = 0; q < NP; ++q)
           fa[i][q] = *reinterpret_cast<const u16x8_t*>(&As[buf][q][wm * WM + i * 16 + lr][swz(lr, lk)]);
+      // keep every fragment read of the slice ahead of the first product, so the
+      // scheduler issues them together (one LDS latency) and does not re-read
+      // fragments between products to save registers
+      __builtin_amdgcn_sched_barrier(0);
       mma_slice<NP, RB, CB, F16>(fa, fb, facc);
     } else {
       // three planes: one row block at a time keeps two workgroups per CU
