@@ -118,6 +118,70 @@ def cpu_baseline_pp(V, tau_np, rows):
             "full_kernel_seconds_extrapolated": dt * (v * v) / rows}, R
 
 
+def uneven_hashmarks(n, rng, lo=20, hi=64):
+    """TiledRange1 hashmarks with tile sizes drawn uniformly from {lo..hi}; the
+    last tile is clipped into [lo, hi] by merging or splitting with its
+    predecessor."""
+    hm = [0]
+    while n - hm[-1] > hi:
+        hm.append(hm[-1] + int(rng.integers(lo, hi + 1)))
+    if n - hm[-1] < lo and len(hm) > 1:
+        hm.pop()
+        rest = n - hm[-1]
+        if rest > hi:
+            hm.append(hm[-1] + rest // 2)
+    hm.append(n)
+    return hm
+
+
+def pp_ladder_tile_boundary(V, o, v, seed=2):
+    """The pp-ladder workload's operands arrive as TiledArray tiles of uneven
+    sizes 20-64: the seeded tiling is applied to V, the tiled buffer is handed to
+    a context through mpqc_ccsd_set_integral (device memory, TA layout), and the
+    gathered dense Gabcd is read back and compared with V.  Returns the tilings
+    and the gather time (not part of the contraction's timed region)."""
+    import ctypes
+    import numpy as np
+    import torch
+    from mpqc_amd import _lib as L
+    rng = np.random.default_rng(seed)
+    occ_hm = uneven_hashmarks(o, rng) if o > 64 else [0, o]
+    vir_hm = uneven_hashmarks(v, rng)
+    ho = (ctypes.c_int64 * len(occ_hm))(*occ_hm)
+    hv = (ctypes.c_int64 * len(vir_hm))(*vir_hm)
+    lib = L.load()
+    ctx = ctypes.c_void_p()
+    L.check(lib.mpqc_ccsd_ctx_create(o, v, ho, len(occ_hm) - 1, hv, len(vir_hm) - 1, L.PREC_F64,
+                                     ctypes.byref(ctx)), None)
+    try:
+        tiled = torch.empty_like(V)
+        # dense -> tiles through the context (set dense, read back in tile layout)
+        L.check(lib.mpqc_ccsd_set_integral(ctx, L.INT_GABCD, ctypes.c_void_p(V.data_ptr()), L.LAYOUT_DENSE,
+                                           L.MEM_DEVICE), ctx)
+        L.check(lib.mpqc_ccsd_get_integral(ctx, L.INT_GABCD, ctypes.c_void_p(tiled.data_ptr()),
+                                           L.LAYOUT_TA_TILES, L.MEM_DEVICE), ctx)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        L.check(lib.mpqc_ccsd_set_integral(ctx, L.INT_GABCD, ctypes.c_void_p(tiled.data_ptr()),
+                                           L.LAYOUT_TA_TILES, L.MEM_DEVICE), ctx)
+        gather_s = time.perf_counter() - t0
+        back = torch.empty_like(V)
+        L.check(lib.mpqc_ccsd_get_integral(ctx, L.INT_GABCD, ctypes.c_void_p(back.data_ptr()), L.LAYOUT_DENSE,
+                                           L.MEM_DEVICE), ctx)
+        torch.cuda.synchronize()
+        same = bool(torch.equal(back, V))
+        moved = not bool(torch.equal(tiled.reshape(-1)[:4096], V.reshape(-1)[:4096]))
+        del tiled, back
+    finally:
+        lib.mpqc_ccsd_ctx_destroy(ctx)
+    torch.cuda.empty_cache()
+    sizes = [b - a for a, b in zip(vir_hm[:-1], vir_hm[1:])]
+    log(f"uneven tiles: vir {sizes}; TA-tile gather of V {gather_s * 1e3:.1f} ms, round trip exact: {same}")
+    return {"occ_hashmarks": occ_hm, "vir_hashmarks": vir_hm, "vir_tile_sizes": sizes,
+            "gather_ms": round(gather_s * 1e3, 3), "round_trip_exact": same and moved,
+            "note": "tiles are gathered into the dense GEMM layout once at the boundary"}
+
+
 def bench_pp_ladder(args):
     import numpy as np
     import torch
@@ -155,6 +219,7 @@ def bench_pp_ladder(args):
     ref = Vr @ tau_np.reshape(v * v, o * o)
     got = R.reshape(v * v, o * o)[torch.from_numpy(rows).to("cuda")].cpu().numpy()
     max_rel = float(np.abs(got - ref).max() / np.abs(ref).max())
+    tiles = pp_ladder_tile_boundary(V, o, v) if args.precision == "f64" else None
     cpu = None
     if not args.no_cpu_baseline:
         cpu, _ = cpu_baseline_pp(V, tau_np, args.cpu_sample_rows)
@@ -181,6 +246,7 @@ def bench_pp_ladder(args):
                      "traffic": profiled_traffic("pp-ladder", args.precision, "pp_ladder")[0],
                      "kernel": "dgemm_f64_kernel (pp-ladder GEMM M=v^2 N=o^2 K=v^2)"},
         "parity_max_rel_err_vs_fp64_host": max_rel,
+        "ta_tiles": tiles,
         "cpu_baseline": cpu,
     }
     if cpu:
