@@ -925,6 +925,12 @@ def bench_triples(args):
             times.append(time.perf_counter() - t0)
             ets.append(et)
             log(f"  (T) = {et:.12e} in {times[-1]:.3f}s (device {dev_s:.3f}s)")
+        # the same (T) with the W GEMMs on fp16 hi+lo planes
+        c.set_option("precision", 3)
+        t0 = time.perf_counter()
+        et16, _ = c.triples()
+        sec16 = time.perf_counter() - t0
+        log(f"  f16x3 (T) = {et16:.12e} in {sec16:.3f}s (vs f64 {et16 - ets[-1]:+.2e} Eh)")
     flops = ccsd_t_flops(o, v)
     sec = sum(times) / len(times)
     cpu = None
@@ -961,6 +967,8 @@ def bench_triples(args):
         "roofline": {"bound": "mfma", "achieved": round(flops / sec / 1e12, 3), "peak": PEAK["f64"],
                      "unit": "TFLOP/s", "frac": round(flops / sec / 1e12 / PEAK["f64"], 4), "traffic": None,
                      "kernel": "whole (T): 2 o^3 v^4 + 2 o^4 v^3 GEMM FLOPs over the wall time"},
+        "f16x3": {"seconds": round(sec16, 4), "E_T": et16, "dE_T_vs_f64": et16 - ets[-1],
+                  "speedup_vs_f64": round(sec / sec16, 3)},
         "setup": setup,
         "cpu_baseline": cpu,
     }

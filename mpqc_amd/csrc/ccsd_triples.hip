@@ -23,10 +23,13 @@
 // of the orbit pair {a,b,c} x {i,j,k} collapse to 6 x sum_u F(u) with
 //   T_u = t3s(x, y_u) = sum_r W_{u o r}(x_r),  F(u) = (T_u + V_u) sum_m c_m T_{u o m},
 // x = (a >= b >= c), y = (i <= j <= k), weighted by 1/(|stab x| |stab y|) so
-// that every ordered (a,b,c,i,j,k) counts once.  A workgroup owns a multiset
+// that every ordered (a,b,c,i,j,k) counts once.  In the split-precision forms
+// the W GEMMs read 16-bit planes of A and B (split once per call) and write
+// FP64 W; the energy is FP64 throughout.  A workgroup owns a multiset
 // of 8^3 cubes of (a,b,c) space and stages, ordering by ordering, the six
 // permuted cubes of W_s through LDS, so W is read with coalesced rows once.
 #include <cmath>
+#include <memory>
 #include <stdexcept>
 
 #include "ccsd_ctx.hpp"
@@ -211,6 +214,48 @@ struct DevBuf {
   DevBuf& operator=(const DevBuf&) = delete;
 };
 
+// planes of one (T) operand, owned for the duration of the call
+struct TPlanes {
+  Planes P;
+  std::vector<void*> owned;
+  TPlanes(PlaneKind kind, long n) {
+    P.n = n;
+    for (int q = 0; q < plane_count(kind); ++q) {
+      void* ptr = nullptr;
+      if (hipMalloc(&ptr, std::max<long>(n, 8) * sizeof(uint16_t)) != hipSuccess) {
+        (void)hipGetLastError();
+        release();
+        throw std::bad_alloc();
+      }
+      owned.push_back(ptr);
+      P.p[q] = (uint16_t*)ptr;
+    }
+    void* sc = nullptr;
+    if (hipMalloc(&sc, sizeof(double)) != hipSuccess) {
+      (void)hipGetLastError();
+      release();
+      throw std::bad_alloc();
+    }
+    owned.push_back(sc);
+    P.scale = (double*)sc;
+  }
+  void release() {
+    for (void* x : owned) (void)hipFree(x);
+    owned.clear();
+  }
+  ~TPlanes() { release(); }
+  TPlanes(const TPlanes&) = delete;
+  TPlanes& operator=(const TPlanes&) = delete;
+  // the same planes starting at element `off`
+  Planes at(long off) const {
+    Planes q = P;
+    for (int k = 0; k < 3; ++k)
+      if (q.p[k]) q.p[k] += off;
+    q.n = P.n - off;
+    return q;
+  }
+};
+
 // out (dense, extents out_ext) = alpha * input permuted (out axis d reads input axis perm[d])
 void permute_into(double* out, const long out_ext[4], const double* in, const long in_ext[4], const int perm[4],
                   double alpha, hipStream_t s) {
@@ -305,6 +350,52 @@ double CCSDContext::triples(double block_bytes, double* seconds) {
                                     hipMemcpyDeviceToDevice, stream_));
     MPQC_HIP_CHECK(hipStreamSynchronize(stream_));  // temporaries are released on return
   }
+  // split-precision forms: the W GEMMs read 16-bit planes of the operands in
+  // k-contiguous layout (A as is; B transposed to [c][K] rows), split once here
+  const bool sm = split_mode();
+  const PlaneKind kind = plane_kind();
+  std::unique_ptr<TPlanes> Ap, BTp, BT2p, BPp;
+  std::unique_ptr<DevBuf> wsP;
+  size_t wsP_bytes = 0;
+  if (sm) {
+    Ap.reset(new TPlanes(kind, O * V * V * VK));
+    BTp.reset(new TPlanes(kind, O * O * V * VK));
+    BT2p.reset(new TPlanes(kind, O * O * V * VK));
+    BPp.reset(new TPlanes(kind, O * O * 2 * V * VK));
+    auto split1 = [&](const Planes& out, const long out_ext[4], const double* src, const long in_ext[4],
+                      const int perm[4]) {
+      TTerm t;
+      t.p = src;
+      permuted_strides(in_ext, perm, t.s);
+      t.alpha = 1.0;
+      t.vol = in_ext[0] * in_ext[1] * in_ext[2] * in_ext[3];
+      split_to_planes(kind, out, out_ext, &t, 1, partials_, stream_);
+    };
+    const long e_a[4] = {1, O, V * V, VK};
+    const int pid[4] = {0, 1, 2, 3};
+    split1(Ap->P, e_a, Acat.p, e_a, pid);
+    const long e_bt[4] = {VK, O, O, V}, e_bto[4] = {O, O, V, VK};
+    const int p_bt[4] = {1, 2, 3, 0}, p_bt2[4] = {2, 1, 3, 0};
+    split1(BTp->P, e_bto, Bt.p, e_bt, p_bt);    // [o2][o3][c][K]
+    split1(BT2p->P, e_bto, Bt.p, e_bt, p_bt2);  // [o3][o2][c][K]
+    const long e_bp[4] = {O, O, VK, 2 * V}, e_bpo[4] = {O, O, 2 * V, VK};
+    const int p_bp[4] = {0, 1, 3, 2};
+    split1(BPp->P, e_bpo, Bp.p, e_bp, p_bp);    // [j][k][(t,c)][K]
+    for (long nb = 1; nb <= nbmax; ++nb) {
+      wsP_bytes = std::max(wsP_bytes, gemm_planes_workspace_bytes((int)(nb * V * V), (int)(2 * V), (int)VK, 1));
+      wsP_bytes = std::max(wsP_bytes, gemm_planes_workspace_bytes((int)(V * V), (int)(nb * V), (int)VK, 1));
+    }
+    wsP.reset(new DevBuf((wsP_bytes + sizeof(double) - 1) / sizeof(double)));
+    MPQC_HIP_CHECK(hipStreamSynchronize(stream_));
+  }
+  auto plane_gemm = [&](int M, int N, const Planes& A, const Planes& B, double* C, long ldc) {
+    PlaneGemmDesc dg;
+    dg.kind = kind;
+    dg.M = M; dg.N = N; dg.K = (int)VK; dg.alpha = 1.0; dg.beta = 0.0;
+    dg.A = A; dg.lda = VK; dg.sA = 0; dg.B = B; dg.ldb = VK; dg.sB = 0;
+    dg.C = C; dg.ldc = ldc; dg.sC = 0; dg.batch = 1;
+    gemm_planes(dg, wsP->p, wsP_bytes, stream_);
+  };
   // W buffers: orderings (i,j,k), (i,k,j) interleaved as [(i,a,b)][2][c]; the other four as
   // [(a,b)][i][c] with row stride nbmax v
   double* W1 = W.p;                       // 2 nbmax v^3
@@ -321,19 +412,31 @@ double CCSDContext::triples(double block_bytes, double* seconds) {
       for (int i0 = 0; i0 <= j; i0 += (int)nbmax) {
         const int nb = (int)std::min<long>(nbmax, j + 1 - i0);
         // (i,j,k) and (i,k,j): C[(i,a,b)][(t,c)] = Acat[(i,a,b)][K] Bp[j][k][K][(t,c)]
-        mpqc_hip::gemm(make_gemm(false, false, (int)(nb * V * V), (int)(2 * V), (int)VK, 1.0,
-                                 Acat.p + (long)i0 * V * V * VK, VK, Bp.p + ((long)j * O + k) * VK * 2 * V, 2 * V,
-                                 0.0, W1, 2 * V), GemmPrecision::F64, ws_, stream_);
+        if (sm)
+          plane_gemm((int)(nb * V * V), (int)(2 * V), Ap->at((long)i0 * V * V * VK),
+                     BPp->at(((long)j * O + k) * 2 * V * VK), W1, 2 * V);
+        else
+          mpqc_hip::gemm(make_gemm(false, false, (int)(nb * V * V), (int)(2 * V), (int)VK, 1.0,
+                                   Acat.p + (long)i0 * V * V * VK, VK, Bp.p + ((long)j * O + k) * VK * 2 * V,
+                                   2 * V, 0.0, W1, 2 * V), GemmPrecision::F64, ws_, stream_);
         // fixed first index o1 in {j, k}: C[(a,b)][(i,c)] = Acat[o1][(a,b)][K] B[K][(i,c)]
         struct G4 { int o1; const double* B; };
         const G4 g4[4] = {{j, Bt.p + ((long)k * O + i0) * V},    // (j,k,i): Bcat(k, i)
                           {j, Bt2.p + ((long)k * O + i0) * V},   // (j,i,k): Bcat(i, k)
                           {k, Bt.p + ((long)j * O + i0) * V},    // (k,j,i): Bcat(j, i)
                           {k, Bt2.p + ((long)j * O + i0) * V}};  // (k,i,j): Bcat(i, j)
-        for (int q = 0; q < 4; ++q)
-          mpqc_hip::gemm(make_gemm(false, false, (int)(V * V), (int)(nb * V), (int)VK, 1.0,
-                                   Acat.p + (long)g4[q].o1 * V * V * VK, VK, g4[q].B, O * O * V, 0.0, Wg[q], ldw),
-                         GemmPrecision::F64, ws_, stream_);
+        // the same four as planes: B rows (i, c) of BTp[o2][.] / BT2p[o3][.]
+        const long pb[4] = {((long)k * O + i0) * V * VK, ((long)k * O + i0) * V * VK,
+                            ((long)j * O + i0) * V * VK, ((long)j * O + i0) * V * VK};
+        for (int q = 0; q < 4; ++q) {
+          if (sm)
+            plane_gemm((int)(V * V), (int)(nb * V), Ap->at((long)g4[q].o1 * V * V * VK),
+                       (q % 2 == 0 ? BTp : BT2p)->at(pb[q]), Wg[q], ldw);
+          else
+            mpqc_hip::gemm(make_gemm(false, false, (int)(V * V), (int)(nb * V), (int)VK, 1.0,
+                                     Acat.p + (long)g4[q].o1 * V * V * VK, VK, g4[q].B, O * O * V, 0.0, Wg[q],
+                                     ldw), GemmPrecision::F64, ws_, stream_);
+        }
         TriplesArgs g;
         // orderings by S3 index: 0 (i,j,k) 1 (i,k,j) 2 (j,i,k) 3 (j,k,i) 4 (k,i,j) 5 (k,j,i)
         g.Wb[0] = W1;     g.Wz[0] = 2 * v3; g.Wr[0] = 2 * V;

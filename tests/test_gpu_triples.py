@@ -74,3 +74,39 @@ def test_gpu_df_triples_match_reference():
     print(f"GPU DF-CCSD {E:.15f}, (T) {et:.15e}; reference (T) {tcase['e_triples']:.15e}")
     assert np.abs(giabc - inp.base.blocks["Giabc"]).max() < 1e-13
     assert abs(et - tcase["e_triples"]) < 2e-12
+
+
+@pytest.mark.parametrize("prec,tol", [("f16x3", 2e-6), ("bf16x3", 2e-4)])
+def test_triples_split_precision_forms(prec, tol):
+    """The W GEMMs on 16-bit planes: E(T) within the form's precision of the
+    FP64 oracle (random amplitudes, several i blocks) and of FP64 at converged
+    water amplitudes."""
+    from mpqc_amd.ccsd import CCSD, Integrals
+    from test_golden import _inputs
+    no, nv = 6, 20
+    f, g = random_hamiltonian(no, nv, seed=77, eri_scale=0.05)
+    b = mpqc_blocks(f, g, no)
+    rng = np.random.default_rng(78)
+    t1 = rng.normal(0, 0.05, (nv, no))
+    t2 = rng.normal(0, 0.05, (nv, nv, no, no))
+    eo, ev = np.diag(f)[:no], np.diag(f)[no:]
+    ref = T.ccsd_t_straight(t1, t2, b["Gijab"], b["Giabc"], b["Gijka"], eo, ev)
+    with CCSD(no, nv, precision=prec) as c:
+        c.set_integrals(Integrals(**b))
+        c.set_orbital_energies(eo, ev)
+        c.set_option("triples_block_bytes", 1 << 20)
+        c.set_amplitudes(t1, t2)
+        et, _ = c.triples()
+    print(f"{prec} random: {et:.12e} oracle {ref:.12e} rel {abs(et - ref) / abs(ref):.2e}")
+    assert abs(et - ref) < tol * abs(ref)
+    case = FIX["cases"][1]
+    _, inp = _inputs(case)
+    with CCSD(inp.o, inp.v) as c:
+        c.set_integrals(Integrals(**inp.blocks))
+        c.set_orbital_energies(inp.eps_occ, inp.eps_vir)
+        c.compute_ccsd(target_precision=1e-10)
+        e64, _ = c.triples()
+        c.set_option("precision", {"f16x3": 3, "bf16x3": 2}[prec])
+        esp, _ = c.triples()
+    print(f"{prec} water: {esp:.12e} f64 {e64:.12e} diff {esp - e64:.2e}")
+    assert abs(esp - e64) < tol * abs(e64)
