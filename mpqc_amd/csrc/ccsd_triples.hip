@@ -52,6 +52,7 @@ constexpr int kT = 8;                        // cube edge
 constexpr int kS2 = kT + 1, kS1 = kT * kS2 + 1;  // padded LDS strides (odd: no bank conflicts)
 constexpr int kCube = kT * kS1;
 constexpr int kTThreads = 256;
+static_assert(6 * kT * kT * kT % kTThreads == 0 && 6 * kT * kT * kT / kTThreads <= 32, "cube staging layout");
 
 void upload_s3_tables(hipStream_t s) {
   static const int P[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
@@ -127,19 +128,41 @@ __global__ __launch_bounds__(kTThreads) void triples_energy_kernel(TriplesArgs g
   for (int h = 0; h < 2; ++h)
 #pragma unroll
     for (int u = 0; u < 6; ++u) T[h][u] = 0.0;
-  for (int s = 0; s < 6; ++s) {
+  // W_s cubes: every thread owns kPerThr elements of the six permuted cubes; the
+  // loads of ordering s + 1 are issued before the products of ordering s, from
+  // clamped addresses (out-of-range elements are zeroed at the LDS store), so they
+  // stay in flight across the compute
+  constexpr int kCubeEl = kT * kT * kT, kPerThr = 6 * kCubeEl / kTThreads;
+  double pre[kPerThr];
+  uint32_t okmask = 0;
+  auto gload = [&](int s) {
     const double* Ws = g.Wb[s] + (long)z * g.Wz[s];
     const long rs = g.Wr[s];
-    // the six permuted cubes (X[r0], X[r1], X[r2]) of W_s; rows of kT contiguous c
-    for (int e = threadIdx.x; e < 6 * kT * kT * kT; e += kTThreads) {
-      const int r = e / (kT * kT * kT), rem = e - r * (kT * kT * kT);
+#pragma unroll
+    for (int k = 0; k < kPerThr; ++k) {
+      const int e = threadIdx.x + k * kTThreads;
+      const int r = e / kCubeEl, rem = e - r * kCubeEl;
       const int u = rem / (kT * kT), w = (rem / kT) % kT, zz = rem % kT;
       const int ga = X[kPerm[r][0]] * kT + u, gb = X[kPerm[r][1]] * kT + w, gc = X[kPerm[r][2]] * kT + zz;
-      double val = 0.0;
-      if (ga < v && gb < v && gc < v) val = Ws[((long)ga * v + gb) * rs + gc];
-      tile[r * kCube + u * kS1 + w * kS2 + zz] = val;
+      const bool in = ga < v && gb < v && gc < v;
+      okmask = in ? (okmask | (1u << k)) : (okmask & ~(1u << k));
+      pre[k] = Ws[((long)min(ga, v - 1) * v + min(gb, v - 1)) * rs + min(gc, v - 1)];
     }
+  };
+  auto lstore = [&]() {
+#pragma unroll
+    for (int k = 0; k < kPerThr; ++k) {
+      const int e = threadIdx.x + k * kTThreads;
+      const int r = e / kCubeEl, rem = e - r * kCubeEl;
+      const int u = rem / (kT * kT), w = (rem / kT) % kT, zz = rem % kT;
+      tile[r * kCube + u * kS1 + w * kS2 + zz] = ((okmask >> k) & 1u) ? pre[k] : 0.0;
+    }
+  };
+  gload(0);
+  for (int s = 0; s < 6; ++s) {
+    lstore();
     __syncthreads();
+    if (s + 1 < 6) gload(s + 1);
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       if (!ok[h]) continue;
