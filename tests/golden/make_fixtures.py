@@ -11,6 +11,10 @@ and, under "df_cases", the density-fitted run
   tests/validation/reference/outputs/h2o-ccsd-631g-pvdz.out        (RI-RHF 6-31G with the
       cc-pVDZ fitting basis, CCSD method "df", reduced_abcd_memory, frozen core,
       precision 1e-11)
+under "eom_cases", the converged EOM-CCSD singlet excitation energies (last Davidson
+table, 12 digits) of
+  tests/validation/reference/outputs/h2o-eom-ccsd-direct-631g.out  (6-31G, frozen core)
+  tests/validation/reference/outputs/h2o-eom-ccsd-631g-pvdz.out    (DF: 6-31G / cc-pVDZ fit)
 and, under "triples_cases", the (T) energy of
   tests/validation/reference/outputs/h2o-ccsd_t-631g-pvdz.out      (the same DF water,
       CCSD(T) with approach "straight")
@@ -75,7 +79,21 @@ for fname in ["h2o-ccsd_t-631g-pvdz.out"]:
                                  "basis": "6-31G", "df_basis": "cc-pVDZ", "method": "df",
                                  "frozen_core": True, "approach": "straight", "e_ccsd": e_cc,
                                  "e_triples": e_t})
+out["eom_cases"] = []
+for fname, basis, dfbs in [("h2o-eom-ccsd-direct-631g.out", "6-31G", None),
+                           ("h2o-eom-ccsd-631g-pvdz.out", "6-31G", "cc-pVDZ")]:
+    txt = open(os.path.join(REF, "outputs", fname)).read()
+    head = txt.split("Excitation Energy: ( Singlets )")[0]
+    last = head.split("iter=")[-1]
+    roots = [float(m.group(1)) for m in re.finditer(r"^\s+\d+\s+\S+\s+\S+\s+(\d+\.\d+)\s*$", last, re.M)]
+    table = txt.split("Excitation Energy: ( Singlets )")[1]
+    summary = [float(m.group(1)) for m in re.finditer(r"^\s+\d+\s+(\d+\.\d+)\s+\d+\.\d+\s+\d+\.\d+",
+                                                       table.split("EOM-CCSD Total Time")[0], re.M)]
+    out["eom_cases"].append({"source": f"tests/validation/reference/outputs/{fname}",
+                             "revision": re.search(r"Revision (\w+)", txt).group(1), "basis": basis,
+                             "df_basis": dfbs, "frozen_core": True, "singlets": roots,
+                             "singlets_summary": summary})
 with open(os.path.join(HERE, "h2o_ccsd.json"), "w") as f:
     json.dump(out, f, indent=1)
 print("wrote", len(out["cases"]), "cases,", len(out["df_cases"]), "DF cases,", len(out["triples_cases"]),
-      "(T) cases")
+      "(T) cases,", len(out["eom_cases"]), "EOM cases")
