@@ -20,6 +20,11 @@
  *                                 (chemistry/qc/lcao/cc/ccsd_t.h:144-176; algorithm of
  *                                 compute_ccsd_t_straight :1127-1167 and CCSD_T_Reduce
  *                                 :2274-2335) from the context's amplitudes
+ *   mpqc_ccsd_eom_ee              EOM-EE-CCSD singlet excitation energies EOM_CCSD::evaluate
+ *                                 (chemistry/qc/lcao/cc/eom/eom_ccsd.h:17, eom_ccsd_impl.h:395-478;
+ *                                 Davidson of eom_ccsd_davidson_solver :236-390 with
+ *                                 math/linalg/davidson_diag.h:128-425 and EEPred,
+ *                                 eom/eom_preconditioner.h:18-74) from the context's amplitudes
  *   mpqc_ccsd_get_integral        reads a cc::Integrals member back (ccsd_r1_r2.h:24-56)
  *   mpqc_ccsd_transform_ao        LCAOFactory::compute4 without density fitting
  *                                 (chemistry/qc/lcao/factory/lcao_factory.h:409-493) for the
@@ -157,6 +162,24 @@ int mpqc_ccsd_set_orbital_energies(mpqc_ccsd_ctx* ctx, const double* eps_occ_act
  * *seconds (may be NULL) the device time.  Option "triples_block_bytes" (default
  * 8 GiB) bounds the device memory of the raw triples of one block. */
 int mpqc_ccsd_triples(mpqc_ccsd_ctx* ctx, double* e_t, double* seconds);
+/* EOM-EE-CCSD: the n_roots lowest singlet excitation energies (Eh, ascending) at
+ * the amplitudes held by the context (normally after mpqc_ccsd_solve), written to
+ * energies[0..n_roots).  They are the eigenvalues of the Jacobian of the residual
+ * on (ab)(ij)-symmetric doubles, found by Davidson iterations (on its transpose,
+ * whose products are exact reverse sweeps of the residual).  Converged when the
+ * largest change of a root <= convergence and the largest residual norm <=
+ * 10 convergence (davidson_diag.h:139-141), or when no new vector passes the
+ * vector threshold (the roots are then returned as they stand, as the reference
+ * does); an error after max_iter iterations otherwise.  *n_iter (may be NULL) gets the
+ * iteration count; verbose prints one line per iteration.  Options
+ * "eom_max_vector" (subspace vectors per root, default 8), "eom_vector_threshold"
+ * (norm below which a new vector is dropped, default 10 convergence) and
+ * "eom_n_guess" (initial unit singles on the smallest FAB(a,a) - FIJ(i,i),
+ * default 2 n_roots).  Runs in the f64 and f32 precision forms; the df method
+ * evaluates the same Jacobian on the four-index blocks its factors assemble
+ * (Giabc and Gabcd are built on the device when not set). */
+int mpqc_ccsd_eom_ee(mpqc_ccsd_ctx* ctx, int n_roots, int max_iter, double convergence, int verbose,
+                     double* energies, int* n_iter);
 /* Reads back an integral block held by the context (as set, or as built by the
  * transforms below), in the layout and index order of mpqc_ccsd_set_integral. */
 int mpqc_ccsd_get_integral(mpqc_ccsd_ctx* ctx, int which, double* out, int layout, int memspace);

@@ -206,6 +206,19 @@ class CCSD:
         self._check(self.lib.mpqc_ccsd_triples(self.ctx, ctypes.byref(e), ctypes.byref(sec)))
         return e.value, sec.value
 
+    def eom_ee(self, n_roots: int, target_precision: float = 1e-8, max_iter: int = 100,
+               verbose: bool = False):
+        """EOM-EE-CCSD singlet excitation energies from the context's amplitudes
+        (EOM_CCSD::evaluate, eom_ccsd_impl.h:395-478, with the Davidson solver of
+        :236-390; target_precision is the reference's convergence of the roots).
+        Returns (energies ascending, iterations)."""
+        e = np.empty(int(n_roots))
+        it = ctypes.c_int()
+        self._check(self.lib.mpqc_ccsd_eom_ee(self.ctx, int(n_roots), int(max_iter), float(target_precision),
+                                              int(bool(verbose)), e.ctypes.data_as(ctypes.c_void_p),
+                                              ctypes.byref(it)))
+        return e, it.value
+
     def get_integral(self, name: str, layout: str = "dense") -> np.ndarray:
         """An integral block held by the context (as set or as transformed)."""
         o, v, k = self.o, self.v, getattr(self, "naux", 0)

@@ -182,6 +182,14 @@ int mpqc_ccsd_triples(mpqc_ccsd_ctx* ctx, double* e_t, double* seconds) {
   });
 }
 
+int mpqc_ccsd_eom_ee(mpqc_ccsd_ctx* ctx, int n_roots, int max_iter, double convergence, int verbose,
+                     double* energies, int* n_iter) {
+  return guarded(ctx, [&](mpqc_hip::CCSDContext& c) {
+    const int it = c.eom_ee(n_roots, max_iter, convergence, verbose, energies);
+    if (n_iter) *n_iter = it;
+  });
+}
+
 int mpqc_ccsd_get_integral(mpqc_ccsd_ctx* ctx, int which, double* out, int layout, int memspace) {
   return guarded(ctx, [&](mpqc_hip::CCSDContext& c) { c.get_integral(which, out, layout, memspace); });
 }

@@ -225,6 +225,16 @@ void CCSDContext::set_option(const std::string& key, double value) {
   } else if (key == "reduced_abcd_memory") {  // ccsd.h:102, used at :292
     reduced_abcd_ = value != 0.0;
     prepared_ = false;
+  } else if (key == "eom_max_vector") {  // eom_ccsd.h:33
+    if (value < 2 || value > 64 || value != (int)value)
+      throw std::invalid_argument("eom_max_vector must be an integer in [2, 64]");
+    eom_max_vector_ = (int)value;
+  } else if (key == "eom_vector_threshold") {  // eom_ccsd.h:34
+    if (!(value >= 0.0)) throw std::invalid_argument("eom_vector_threshold must be >= 0");
+    eom_vector_threshold_ = value;
+  } else if (key == "eom_n_guess") {
+    if (value < 0 || value != (int)value) throw std::invalid_argument("eom_n_guess must be a non-negative integer");
+    eom_n_guess_ = (int)value;
   } else if (key == "triples_block_bytes") {
     if (!(value >= 1 << 20)) throw std::invalid_argument("triples_block_bytes must be at least 1 MiB");
     triples_block_bytes_ = value;

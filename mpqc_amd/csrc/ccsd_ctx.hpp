@@ -74,6 +74,15 @@ class CCSDContext {
   // device time
   double triples(double block_bytes, double* seconds);
   double triples_block_bytes_ = 8.0 * (1 << 30);
+  // EOM-EE-CCSD singlet excitation energies (EOM_CCSD, eom_ccsd.h:17 with
+  // eom_ccsd_impl.h:236-390): the lowest n_roots eigenvalues of the CCSD
+  // Jacobian on (ab)(ij)-symmetric doubles at the current (converged)
+  // amplitudes, by Davidson iterations on its transpose; returns the number of
+  // iterations
+  int eom_ee(int n_roots, int max_iter, double convergence, int verbose, double* energies);
+  int eom_max_vector_ = 8;             // subspace vectors per root (eom_ccsd.h:33)
+  double eom_vector_threshold_ = 0.0;  // new-vector norm threshold; 0: 10 * convergence (:34)
+  int eom_n_guess_ = 0;                // initial singles guesses; 0: 2 n_roots
 
   // ---- adjoint of the converged energy (no counterpart in the reference).
   // With the amplitudes t converged (R(t) = 0), lambda solves (dR/dt)^T lambda = -dE/dt
@@ -183,6 +192,10 @@ class CCSDContext {
   void reverse_sweep(const std::vector<TapeOp>& tape, bool integrals);
   void reverse_op(const TapeOp& op, bool integrals);
   void energy_t_gradient(double* g1, double* g2);
+  // x = P J^T y for contiguous (t1, t2) vectors, P the projection onto
+  // t2(a,b,i,j) = t2(b,a,j,i); the residual tape must be recorded
+  void eom_apply_jt(const double* y, double* x);
+  void ensure_abcd_from_factors();  // Gabcd from the three-index factors
   std::vector<TapeOp>* tape_ = nullptr;  // ops are appended here when set
   bool tape_execute_ = true;             // false: record without launching
   std::vector<TapeOp> prepare_tape_;     // recorded by prepare()
