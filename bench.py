@@ -48,7 +48,7 @@ def _args():
     p.add_argument("--workload", default="ccsd-h2o8",
                    choices=["ccsd-h2o8", "ccsd-h2o6", "ccsd-h20", "pp-ladder", "copy",
                             "batched-hchains", "adjoint-h2o6", "df-h2o8", "transform-h2o8",
-                            "triples-h2o6", "triples-h2o8"])
+                            "triples-h2o6", "triples-h2o8", "eom-h2o6"])
     p.add_argument("--forms", default="f64,f16x3",
                    help="contraction precisions to run for the CCSD workloads (first = reference)")
     p.add_argument("--max-iter", type=int, default=40)
@@ -975,6 +975,79 @@ def bench_triples(args):
     return line
 
 
+def bench_eom(args):
+    """EOM-EE-CCSD on one GPU: (H2O)6 cc-pVDZ, FP64 CCSD converged to 1e-10, then the
+    four lowest singlets to 1e-9 (the settings of the reference's h2o-eom-ccsd
+    inputs: property precision 1e-9, CCSD at 0.1 of it, 8 vectors per root)."""
+    import numpy as np
+    from mpqc_amd.ccsd import CCSD, Integrals
+    from oracle.ccsd import ccsd_flops
+    name = "ccsd-h2o6"
+    inp, setup = build_system(name)
+    o, v = inp.o, inp.v
+    n_roots, conv = 4, 1e-9
+    with CCSD(o, v, precision="f64", max_iter=args.max_iter) as c:
+        c.set_integrals(Integrals(**inp.blocks))
+        c.set_orbital_energies(inp.eps_occ, inp.eps_vir)
+        E = c.compute_ccsd(target_precision=0.1 * conv)
+        log(f"{name}: E_CCSD={E:.12f}; EOM-EE-CCSD, {n_roots} singlets")
+        for _ in range(args.warmup):
+            c.eom_ee(n_roots, target_precision=conv, max_iter=100)
+        times, roots = [], None
+        for _ in range(args.steps):
+            t0 = time.perf_counter()
+            w, it = c.eom_ee(n_roots, target_precision=conv, max_iter=100)
+            times.append(time.perf_counter() - t0)
+            roots = w
+            nprod, psec = c.eom_stats()
+            log(f"  roots {np.array2string(w, precision=10)} in {times[-1]:.3f}s, {it} iterations, "
+                f"{nprod} J^T products, {1e3 * psec / nprod:.2f} ms each")
+    sec = sum(times) / len(times)
+    # each contraction of the residual has two adjoint contractions of the same size
+    flops_product = 2.0 * ccsd_flops(o, v)
+    per_product = psec / nprod
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import ccsd as O
+        ints = O.Integrals(**inp.blocks)
+        t1 = O.jacobi_update_t1(ints.Fia.T, inp.eps_occ, inp.eps_vir)
+        t2 = O.jacobi_update_t2(ints.Gijab.transpose(2, 3, 0, 1), inp.eps_occ, inp.eps_vir)
+        rng = np.random.default_rng(0)
+        d1, d2 = rng.normal(size=t1.shape), rng.normal(size=t2.shape)
+        h = 1e-5
+        ta = time.perf_counter()
+        for sgn in (1.0, -1.0):  # one Jacobian-vector product by central differences
+            a1, a2 = t1 + sgn * h * d1, t2 + sgn * h * d2
+            tau = O.make_tau(a1, a2)
+            O.cs_ccsd_r1(a1, a2, tau, ints)
+            O.cs_ccsd_r2(a1, a2, tau, ints)
+        dt = time.perf_counter() - ta
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        cpu = {"value": round(dt, 3), "unit": "s per Jacobian product", "cores": threads, "kind": "port",
+               "gpu_seconds_per_product": round(per_product, 5),
+               "sample": f"one Jacobian-vector product of the oracle residual by central differences "
+                         f"(two numpy/OpenBLAS fp64 residuals) at o={o} v={v}"}
+    line = {
+        "metric": "EOM-EE-CCSD time (4 singlets) on one GPU",
+        "value": round(sec, 4), "unit": "s per EOM-EE-CCSD", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(sec * 1e3, 3), "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic molecule (seeded geometry), RHF + AO->MO on the host, converged FP64 CCSD",
+        "config": {"workload": f"EOM-EE-CCSD singlets for {SYSTEMS[name]}", "o": o, "v": v,
+                   "n_roots": n_roots, "convergence": conv, "max_vector": 8, "parallelism": "single-gpu",
+                   "E_CCSD": E, "singlets": [float(x) for x in roots], "jt_products": nprod,
+                   "jt_product_ms": round(1e3 * per_product, 3), "jt_product_flops": flops_product},
+        "roofline": {"bound": "mfma", "achieved": round(flops_product / per_product / 1e12, 3),
+                     "peak": PEAK["f64"], "unit": "TFLOP/s",
+                     "frac": round(flops_product / per_product / 1e12 / PEAK["f64"], 4), "traffic": None,
+                     "kernel": "whole J^T product (reverse sweep of the residual): twice the residual's "
+                               "algorithmic FLOPs over its event-timed duration"},
+        "setup": setup,
+        "cpu_baseline": cpu,
+    }
+    return line
+
+
 def main():
     args = _args()
     if args.gpus != 1:
@@ -999,6 +1072,8 @@ def main():
         line = bench_transform(args)
     elif args.workload in ("triples-h2o6", "triples-h2o8"):
         line = bench_triples(args)
+    elif args.workload == "eom-h2o6":
+        line = bench_eom(args)
     else:
         line = bench_ccsd(args)
     print(json.dumps(line), flush=True)

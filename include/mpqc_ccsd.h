@@ -180,6 +180,10 @@ int mpqc_ccsd_triples(mpqc_ccsd_ctx* ctx, double* e_t, double* seconds);
  * (Giabc and Gabcd are built on the device when not set). */
 int mpqc_ccsd_eom_ee(mpqc_ccsd_ctx* ctx, int n_roots, int max_iter, double convergence, int verbose,
                      double* energies, int* n_iter);
+/* Products with the transposed Jacobian made by the last mpqc_ccsd_eom_ee and
+ * their device time (HIP events on the context stream).  No counterpart in the
+ * reference (its Davidson prints per-iteration times instead). */
+int mpqc_ccsd_eom_stats(mpqc_ccsd_ctx* ctx, int* n_products, double* product_seconds);
 /* Reads back an integral block held by the context (as set, or as built by the
  * transforms below), in the layout and index order of mpqc_ccsd_set_integral. */
 int mpqc_ccsd_get_integral(mpqc_ccsd_ctx* ctx, int which, double* out, int layout, int memspace);

@@ -219,6 +219,13 @@ class CCSD:
                                               ctypes.byref(it)))
         return e, it.value
 
+    def eom_stats(self):
+        """(J^T products, their device seconds) of the last eom_ee."""
+        n = ctypes.c_int()
+        sec = ctypes.c_double()
+        self._check(self.lib.mpqc_ccsd_eom_stats(self.ctx, ctypes.byref(n), ctypes.byref(sec)))
+        return n.value, sec.value
+
     def get_integral(self, name: str, layout: str = "dense") -> np.ndarray:
         """An integral block held by the context (as set or as transformed)."""
         o, v, k = self.o, self.v, getattr(self, "naux", 0)

@@ -190,6 +190,13 @@ int mpqc_ccsd_eom_ee(mpqc_ccsd_ctx* ctx, int n_roots, int max_iter, double conve
   });
 }
 
+int mpqc_ccsd_eom_stats(mpqc_ccsd_ctx* ctx, int* n_products, double* product_seconds) {
+  return guarded(ctx, [&](mpqc_hip::CCSDContext& c) {
+    if (n_products) *n_products = c.eom_products_;
+    if (product_seconds) *product_seconds = c.eom_product_seconds_;
+  });
+}
+
 int mpqc_ccsd_get_integral(mpqc_ccsd_ctx* ctx, int which, double* out, int layout, int memspace) {
   return guarded(ctx, [&](mpqc_hip::CCSDContext& c) { c.get_integral(which, out, layout, memspace); });
 }

@@ -232,6 +232,12 @@ void CCSDContext::set_option(const std::string& key, double value) {
   } else if (key == "eom_vector_threshold") {  // eom_ccsd.h:34
     if (!(value >= 0.0)) throw std::invalid_argument("eom_vector_threshold must be >= 0");
     eom_vector_threshold_ = value;
+  } else if (key == "eom_extra_roots") {
+    if (value < 0 || value > 64 || value != (int)value)
+      throw std::invalid_argument("eom_extra_roots must be an integer in [0, 64]");
+    eom_extra_roots_ = (int)value;
+  } else if (key == "eom_skip_converged") {
+    eom_skip_converged_ = value != 0.0;
   } else if (key == "eom_n_guess") {
     if (value < 0 || value != (int)value) throw std::invalid_argument("eom_n_guess must be a non-negative integer");
     eom_n_guess_ = (int)value;

@@ -83,6 +83,10 @@ class CCSDContext {
   int eom_max_vector_ = 8;             // subspace vectors per root (eom_ccsd.h:33)
   double eom_vector_threshold_ = 0.0;  // new-vector norm threshold; 0: 10 * convergence (:34)
   int eom_n_guess_ = 0;                // initial singles guesses; 0: 2 n_roots
+  int eom_extra_roots_ = 0;            // roots tracked above the requested ones
+  bool eom_skip_converged_ = true;     // no new vectors for converged roots
+  int eom_products_ = 0;               // J^T products of the last eom_ee
+  double eom_product_seconds_ = 0.0;   // their device time (HIP events on the context stream)
 
   // ---- adjoint of the converged energy (no counterpart in the reference).
   // With the amplitudes t converged (R(t) = 0), lambda solves (dR/dt)^T lambda = -dE/dt

@@ -40,16 +40,23 @@ namespace {
 
 constexpr int kEomBlock = 256;
 
-// x /= (theta - D): D = e_a - e_i (singles), e_a + e_b - e_i - e_j (doubles)
-__global__ __launch_bounds__(kEomBlock) void ee_precondition_kernel(double* __restrict__ x,
+// out = P r / (theta - D): D = e_a - e_i (singles), e_a + e_b - e_i - e_j
+// (doubles, summed in an order symmetric under (a,i) <-> (b,j)), P the
+// projection onto t2(a,b,i,j) = t2(b,a,j,i).  Out of place: the doubles read
+// their exchanged element.  Without the projection the doubles would keep a
+// round-off antisymmetric part, on which P J^T vanishes exactly: Gram-Schmidt
+// would amplify it into subspace vectors with a spurious zero root.
+__global__ __launch_bounds__(kEomBlock) void ee_precondition_kernel(double* __restrict__ out,
+                                                                    const double* __restrict__ r,
                                                                     const double* __restrict__ eo,
                                                                     const double* __restrict__ ev, int o,
                                                                     int v, double theta, long n1, long n) {
   for (long e = (long)blockIdx.x * kEomBlock + threadIdx.x; e < n; e += (long)gridDim.x * kEomBlock) {
-    double d;
+    double d, x;
     if (e < n1) {
       const long a = e / o, i = e - a * o;
       d = theta + eo[i] - ev[a];
+      x = r[e];
     } else {
       long k = e - n1;
       const long j = k % o;
@@ -57,10 +64,12 @@ __global__ __launch_bounds__(kEomBlock) void ee_precondition_kernel(double* __re
       const long i = k % o;
       k /= o;
       const long b = k % v, a = k / v;
-      d = theta - ev[a] - ev[b] + eo[i] + eo[j];
+      d = theta - (ev[a] + ev[b]) + (eo[i] + eo[j]);
+      const long sw = n1 + ((b * v + a) * o + j) * o + i;
+      x = 0.5 * (r[e] + r[sw]);
     }
     if (fabs(d) < 1e-8) d = d < 0.0 ? -1e-8 : 1e-8;
-    x[e] /= d;
+    out[e] = x / d;
   }
 }
 
@@ -155,8 +164,11 @@ int CCSDContext::eom_ee(int n_roots, int max_iter, double convergence, int verbo
   if (n_roots > n1) throw std::invalid_argument("n_roots exceeds the number of single excitations");
   const int max_vector = std::max(eom_max_vector_, 2);
   const double threshold = eom_vector_threshold_ > 0.0 ? eom_vector_threshold_ : 10.0 * convergence;
-  const int n_guess =
-      (int)std::min<long>(n1, std::max<long>(n_roots, eom_n_guess_ > 0 ? eom_n_guess_ : 2L * n_roots));
+  // tracked roots: the requested ones plus eom_extra_roots above them, which keep
+  // near-degenerate states from flipping in and out of the requested set
+  const int nt = (int)std::min<long>(n1, (long)n_roots + eom_extra_roots_);
+  const int n_guess = (int)std::min<long>(
+      n1, std::max<long>(nt, eom_n_guess_ > 0 ? eom_n_guess_ : std::max(2L * n_roots, (long)nt)));
 
   // The df residual is the standard residual with -f_ov t1 in g_ac on the
   // four-index blocks the factors assemble (ccsd_r1_r2.h:136-184 against
@@ -195,8 +207,32 @@ int CCSDContext::eom_ee(int n_roots, int max_iter, double convergence, int verbo
   MPQC_HIP_CHECK(hipMemcpyAsync(hdg.data(), dg, (O + V) * sizeof(double), hipMemcpyDeviceToHost, stream_));
   MPQC_HIP_CHECK(hipStreamSynchronize(stream_));
 
-  const int cap = std::max({n_roots * max_vector, n_guess, 3 * n_roots});
-  SlotPool pool(2 * cap + 3 * n_roots, n);
+  eom_products_ = 0;
+  eom_product_seconds_ = 0.0;
+  hipEvent_t ev0, ev1;
+  MPQC_HIP_CHECK(hipEventCreate(&ev0));
+  MPQC_HIP_CHECK(hipEventCreate(&ev1));
+  struct Events {
+    hipEvent_t a, b;
+    ~Events() {
+      (void)hipEventDestroy(a);
+      (void)hipEventDestroy(b);
+    }
+  } events{ev0, ev1};
+  // h = P J^T g, timed
+  auto product = [&](const double* g, double* h) {
+    MPQC_HIP_CHECK(hipEventRecord(ev0, stream_));
+    eom_apply_jt(g, h);
+    MPQC_HIP_CHECK(hipEventRecord(ev1, stream_));
+    MPQC_HIP_CHECK(hipEventSynchronize(ev1));
+    float ms = 0.0f;
+    MPQC_HIP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
+    eom_product_seconds_ += 1e-3 * ms;
+    ++eom_products_;
+  };
+
+  const int cap = std::max({nt * max_vector, n_guess, 3 * nt});
+  SlotPool pool(2 * cap + 3 * nt + 1, n);
   double* scal = scalars_ + 16;
 
   // out[k] = <xs[k], y>
@@ -285,7 +321,7 @@ int CCSDContext::eom_ee(int n_roots, int max_iter, double convergence, int verbo
 
   std::vector<double*> B, HB, Xcur, Xprev, R;
   std::vector<double> G;  // row-major, leading dimension cap
-  std::vector<double> eig(n_roots, 0.0), E(n_roots, 0.0), norms(n_roots, 0.0);
+  std::vector<double> eig(nt, 0.0), E(nt, 0.0), norms(nt, 0.0);
   double norm_e = 1.0, norm_r = 1.0;
   int iter = 0;
   const int grid = (int)std::max<long>(1, std::min<long>(ceil_div(n, kEomBlock), 16384));
@@ -294,7 +330,7 @@ int CCSDContext::eom_ee(int n_roots, int max_iter, double convergence, int verbo
     const int ns = (int)B.size();
     for (double* g : guess) {
       double* h = pool.get();
-      eom_apply_jt(g, h);
+      product(g, h);
       B.push_back(g);
       HB.push_back(h);
     }
@@ -326,7 +362,7 @@ int CCSDContext::eom_ee(int n_roots, int max_iter, double convergence, int verbo
     Xcur.clear();
     for (double* r : R) pool.put(r);
     R.clear();
-    for (int r = 0; r < n_roots; ++r) {
+    for (int r = 0; r < nt; ++r) {
       const std::complex<double> lam = w[order[r]];
       E[r] = lam.real();
       const std::vector<std::complex<double>> cv = small_eigenvector(nv, Gs, lam);
@@ -351,19 +387,30 @@ int CCSDContext::eom_ee(int n_roots, int max_iter, double convergence, int verbo
       norms[r] = norm(rr);
       if (!std::isfinite(norms[r])) throw std::range_error("non-finite EOM residual");
     }
-    for (int r = 0; r < n_roots; ++r) {
-      hipLaunchKernelGGL(ee_precondition_kernel, dim3(grid), dim3(kEomBlock), 0, stream_, R[r], dg, dg + O,
+    // a root whose residual and change are both within the criteria gets no new vector
+    std::vector<bool> done(nt);
+    for (int r = 0; r < nt; ++r)
+      done[r] = eom_skip_converged_ && norms[r] <= 10.0 * convergence && std::fabs(eig[r] - E[r]) <= convergence;
+    for (int r = 0; r < nt; ++r) {
+      double* p = pool.get();
+      hipLaunchKernelGGL(ee_precondition_kernel, dim3(grid), dim3(kEomBlock), 0, stream_, p, R[r], dg, dg + O,
                          (int)O, (int)V, E[r], n1, n);
       check_launch("ee_precondition");
+      pool.put(R[r]);
+      R[r] = p;
     }
-    if (nv > n_roots * (max_vector - 1)) {
+    if (nv > nt * (max_vector - 1)) {
       // collapse: the residuals, then the Ritz vectors of the previous and this iteration
       for (double* b : B) pool.put(b);
       for (double* h : HB) pool.put(h);
       B.clear();
       HB.clear();
       std::fill(G.begin(), G.end(), 0.0);
-      guess = R;
+      guess.clear();
+      for (int r = 0; r < nt; ++r) {
+        if (done[r]) pool.put(R[r]);
+        else guess.push_back(R[r]);
+      }
       R.clear();
       for (const std::vector<double*>* xs : {&Xprev, &Xcur})
         for (double* x : *xs) {
@@ -373,7 +420,11 @@ int CCSDContext::eom_ee(int n_roots, int max_iter, double convergence, int verbo
         }
       gram_schmidt({}, guess);
     } else {
-      guess = R;
+      guess.clear();
+      for (int r = 0; r < nt; ++r) {
+        if (done[r]) pool.put(R[r]);
+        else guess.push_back(R[r]);
+      }
       R.clear();
       gram_schmidt(B, guess);
     }
