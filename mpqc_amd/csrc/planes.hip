@@ -386,7 +386,23 @@ __global__ __launch_bounds__(kThr, (plane_occupancy<BM, BN>())) void gemm_planes
         for (int q = 0; q < NP; ++q)
           fb[j][q] = *reinterpret_cast<const u16x8_t*>(&Bs[buf][q][wn * WN + j * 16 + lr][swz(lr, lk)]);
     }
-    if constexpr (NP == 2) {
+    if constexpr (NP == 2 && BM >= 256) {
+      // 128-row wave tiles: the B fragments of the slice stay in registers and the A
+      // fragments stream one 16-row block at a time, read one block ahead of its products
+      u16x8_t fa[2][1][NP];
+#pragma unroll
+      for (int q = 0; q < NP; ++q)
+        fa[0][0][q] = *reinterpret_cast<const u16x8_t*>(&As[buf][q][wm * WM + lr][swz(lr, lk)]);
+#pragma unroll
+      for (int i = 0; i < RB; ++i) {
+        if (i + 1 < RB)
+#pragma unroll
+          for (int q = 0; q < NP; ++q)
+            fa[(i + 1) & 1][0][q] =
+                *reinterpret_cast<const u16x8_t*>(&As[buf][q][wm * WM + (i + 1) * 16 + lr][swz(lr, lk)]);
+        mma_slice<NP, 1, CB, F16>(fa[i & 1], fb, reinterpret_cast<float4_t(&)[1][CB]>(facc[i]));
+      }
+    } else if constexpr (NP == 2) {
       // all fragments of the slice first (one LDS latency per slice), then the products
       u16x8_t fa[RB][NP];
 #pragma unroll
@@ -417,9 +433,11 @@ __global__ __launch_bounds__(kThr, (plane_occupancy<BM, BN>())) void gemm_planes
 #pragma unroll
           for (int q = 0; q < NP; ++q) fbd[j][q] = fbn[j][q];
     }
-    if (++since_flush == kMaxChunk / kBK && has_next) {
-      flush();
-      since_flush = 0;
+    if constexpr (BM < 256) {  // (the planner keeps the 256-row tiles' K ranges within kMaxChunk)
+      if (++since_flush == kMaxChunk / kBK && has_next) {
+        flush();
+        since_flush = 0;
+      }
     }
     if (has_next) store_lds(buf ^ 1);
     __syncthreads();
@@ -518,6 +536,8 @@ void dispatch_planes(const PlaneGemmDesc& d, double* ws, size_t ws_bytes, hipStr
   const PlanePlan pl = plan_planes(d.M, d.N, d.K, d.batch);
   if (pl.bm == 64) {
     launch_planes<64, 64, NP, F16>(d, pl, ws, ws_bytes, stream);
+  } else if (pl.bm == 256 && pl.bn == 256 && NP == 2) {
+    if constexpr (NP == 2) launch_planes<256, 256, NP, F16>(d, pl, ws, ws_bytes, stream);
   } else if (pl.bm == 128 || NP == 3) {  // three planes of a 256-row tile exceed the LDS
     launch_planes<128, 128, NP, F16>(d, pl.bm == 128 ? pl : plan_planes(d.M, d.N, d.K, d.batch, 128, 128),
                                      ws, ws_bytes, stream);
@@ -597,7 +617,7 @@ void set_plane_tile_order(int order) {
 
 void set_plane_tile_override(int bm, int bn) {
   if (!((bm == 0 && bn == 0) || (bm == 64 && bn == 64) || (bm == 128 && bn == 128) ||
-        (bm == 256 && bn == 128)))
+        (bm == 256 && bn == 128) || (bm == 256 && bn == 256)))
     throw std::invalid_argument("unsupported plane GEMM tile");
   g_tile_override_m = bm;
   g_tile_override_n = bn;
