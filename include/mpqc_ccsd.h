@@ -11,6 +11,8 @@
  *   mpqc_ccsd_set_integral        cc::Integrals<Array> members Fia..Gabcd
  *                                 (chemistry/qc/lcao/cc/ccsd_r1_r2.h:24-56), filled at
  *                                 ccsd.h:281-305
+ *   mpqc_ccsd_set_direct_ao       the AO integrals and MO coefficients of the "direct" method
+ *                                 (ccsd.h:73, 246-247, 313-322; compute_u2_u11 :573-603)
  *   mpqc_ccsd_set_df_basis        the density-fitting basis of the "df" method: its size
  *                                 and the TiledRange1 of the fitting index of
  *                                 (K|G|p q)[inv_sqr] (ccsd.h:93-99, 481-493); selects
@@ -146,7 +148,7 @@ const char* mpqc_ccsd_last_error(const mpqc_ccsd_ctx* ctx);
 /* Numeric options: "fov_sign" (+1 reference F_VV as written, -1 exact CCSD for
  * non-Brillouin references), "precision" (MPQC_CCSD_PREC_*), "kernel_timing"
  * (1 = time every contraction with HIP events on the context stream),
- * "method" (0 standard, 1 df; ccsd.h:94-99), "reduced_abcd_memory" (1 default:
+ * "method" (0 standard, 1 df, 2 direct; ccsd.h:94-99), "reduced_abcd_memory" (1 default:
  * the df method builds (ab|cd) lazily from Xab instead of reading Gabcd and
  * Giabc, ccsd.h:102,292), "df_block_bytes" (device memory for the lazily built
  * (ab|cd) blocks, default 4 GiB; set before the first residual). */
@@ -209,6 +211,16 @@ int mpqc_ccsd_transform_df(mpqc_ccsd_ctx* ctx, int nbf, const double* ao_three_c
  * reduced_abcd_memory = 0.  The df residual runs in MPQC_CCSD_PREC_F64 and F32. */
 int mpqc_ccsd_set_df_basis(mpqc_ccsd_ctx* ctx, int naux, const int64_t* aux_hashmarks,
                            int n_aux_tiles);
+/* The AO integral-direct method ("direct", ccsd.h:73): the AO integrals (mu nu|la si)
+ * (nbf^4, chemist notation, row-major) and the active occupied / virtual MO coefficients
+ * (nbf x o, nbf x v, row-major C[mu][p]), host or device memory.  Replaces
+ * AOFactory::compute_direct("(mu nu|G|ka la)") and ints.Ci / ints.Ca (ccsd.h:246-247,
+ * 313-322); switches the context to the direct residual, which needs every block but
+ * Gabcd and forms U = (pq|rs) [Ca tau Ca^T] each iteration (compute_u2_u11,
+ * ccsd.h:573-603; used at ccsd_r1_r2.h:116-117, 282-286).  The AO integrals stay
+ * resident in HBM instead of being recomputed.  Runs in MPQC_CCSD_PREC_F64 and F32. */
+int mpqc_ccsd_set_direct_ao(mpqc_ccsd_ctx* ctx, int nbf, const double* ao_eri, const double* c_occ,
+                            const double* c_vir, int memspace);
 int mpqc_ccsd_solver_init(mpqc_ccsd_ctx* ctx, int diis_start, int n_diis, double diis_damp,
                           int diis_ngroup, int diis_group_nstart);
 

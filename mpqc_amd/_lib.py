@@ -28,7 +28,7 @@ EXPORTS = [
     "mpqc_ccsd_version", "mpqc_ccsd_ctx_create", "mpqc_ccsd_ctx_destroy", "mpqc_ccsd_last_error",
     "mpqc_ccsd_set_option", "mpqc_ccsd_set_integral", "mpqc_ccsd_set_df_basis",
     "mpqc_ccsd_transform_ao", "mpqc_ccsd_transform_df", "mpqc_ccsd_get_integral",
-    "mpqc_ccsd_triples", "mpqc_ccsd_eom_ee", "mpqc_ccsd_eom_stats",
+    "mpqc_ccsd_triples", "mpqc_ccsd_eom_ee", "mpqc_ccsd_eom_stats", "mpqc_ccsd_set_direct_ao",
     "mpqc_ccsd_set_orbital_energies",
     "mpqc_ccsd_solver_init", "mpqc_ccsd_set_amplitudes", "mpqc_ccsd_get_amplitudes",
     "mpqc_ccsd_get_residuals", "mpqc_ccsd_get_dressed_fock", "mpqc_ccsd_init_guess",
@@ -81,6 +81,7 @@ def load():
         "mpqc_ccsd_triples": (I, [P, pD, pD]),
         "mpqc_ccsd_eom_ee": (I, [P, I, I, D, I, P, P]),
         "mpqc_ccsd_eom_stats": (I, [P, P, P]),
+        "mpqc_ccsd_set_direct_ao": (I, [P, I, P, P, P, I]),
         "mpqc_ccsd_transform_df": (I, [P, I, P, P, P, P, P, I]),
         "mpqc_ccsd_solver_init": (I, [P, I, I, D, I, I]),
         "mpqc_ccsd_set_amplitudes": (I, [P, P, P, I, I]),

@@ -175,6 +175,20 @@ class CCSD:
         hm = (ctypes.c_int64 * len(self.aux_hm))(*self.aux_hm)
         self._check(self.lib.mpqc_ccsd_set_df_basis(self.ctx, self.naux, hm, len(self.aux_hm) - 1))
 
+    def set_direct_ao(self, ao_eri, c_occ, c_vir):
+        """AO integral-direct method (ccsd.h:73): AO integrals (mu nu|la si) and the
+        active occupied / virtual MO coefficients C[mu, p]; selects the "direct"
+        residual (U = (pq|rs) Ca tau Ca^T each iteration, ccsd.h:573-603)."""
+        eri = np.ascontiguousarray(ao_eri, dtype=np.float64)
+        ci = np.ascontiguousarray(c_occ, dtype=np.float64)
+        ca = np.ascontiguousarray(c_vir, dtype=np.float64)
+        nbf = eri.shape[0]
+        if eri.shape != (nbf,) * 4 or ci.shape != (nbf, self.o) or ca.shape != (nbf, self.v):
+            raise ValueError("set_direct_ao: shapes must be (nbf,)*4, (nbf, o), (nbf, v)")
+        vp = ctypes.c_void_p
+        self._check(self.lib.mpqc_ccsd_set_direct_ao(self.ctx, nbf, eri.ctypes.data_as(vp), ci.ctypes.data_as(vp),
+                                                     ca.ctypes.data_as(vp), L.MEM_HOST))
+
     def set_integral(self, name: str, arr: np.ndarray, layout: str = "dense"):
         a = np.ascontiguousarray(arr, dtype=np.float64)
         if layout == "ta":

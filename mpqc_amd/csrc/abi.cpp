@@ -170,6 +170,11 @@ int mpqc_ccsd_set_integral(mpqc_ccsd_ctx* ctx, int which, const double* data, in
   return guarded(ctx, [&](mpqc_hip::CCSDContext& c) { c.set_integral(which, data, layout, memspace); });
 }
 
+int mpqc_ccsd_set_direct_ao(mpqc_ccsd_ctx* ctx, int nbf, const double* ao_eri, const double* c_occ,
+                            const double* c_vir, int memspace) {
+  return guarded(ctx, [&](mpqc_hip::CCSDContext& c) { c.set_direct_ao(nbf, ao_eri, c_occ, c_vir, memspace); });
+}
+
 int mpqc_ccsd_set_df_basis(mpqc_ccsd_ctx* ctx, int naux, const int64_t* aux_hashmarks,
                            int n_aux_tiles) {
   return guarded(ctx, [&](mpqc_hip::CCSDContext& c) { c.set_df_basis(naux, aux_hashmarks, n_aux_tiles); });

@@ -62,7 +62,7 @@ double* CCSDContext::adj(const double* p) {
 }
 
 bool CCSDContext::is_integral(const double* p) const {
-  for (const double* q : {Fia_, Fij_, Fab_, Gijab_, Gijkl_, Giajb_, Giabc_, Gijka_, Gabcd_})
+  for (const double* q : {Fia_, Fij_, Fab_, Gijab_, Gijkl_, Giajb_, Giabc_, Gijka_, Gabcd_, Xao_, Ci_, Ca_})
     if (q && alloc_index(q) == alloc_index(p)) return true;
   for (const TapeOp& op : prepare_tape_)
     if (alloc_index(op.dst) == alloc_index(p)) return true;

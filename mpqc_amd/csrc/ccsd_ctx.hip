@@ -217,10 +217,13 @@ void CCSDContext::set_option(const std::string& key, double value) {
     lambda_ready_ = false;
   } else if (key == "kernel_timing") {
     kernel_timing_ = value != 0.0;
-  } else if (key == "method") {  // 0 standard, 1 df (ccsd.h:94-99)
-    if (value != 0.0 && value != 1.0) throw std::invalid_argument("method must be 0 (standard) or 1 (df)");
+  } else if (key == "method") {  // 0 standard, 1 df, 2 direct (ccsd.h:94-99)
+    if (value != 0.0 && value != 1.0 && value != 2.0)
+      throw std::invalid_argument("method must be 0 (standard), 1 (df) or 2 (direct)");
     if (value == 1.0 && naux_ == 0) throw std::invalid_argument("method df needs set_df_basis first");
+    if (value == 2.0 && !Xao_) throw std::invalid_argument("method direct needs set_direct_ao first");
     df_ = value == 1.0;
+    direct_ = value == 2.0;
     prepared_ = false;
   } else if (key == "reduced_abcd_memory") {  // ccsd.h:102, used at :292
     reduced_abcd_ = value != 0.0;
@@ -779,9 +782,10 @@ double CCSDContext::scalar_from_device(const double* d) {
 void CCSDContext::prepare() {
   // standard: all nine blocks; df (ccsd.h:281-305): no Giabc / Gabcd under
   // reduced_abcd_memory, plus Xai, Xij, Xab
-  const unsigned need = !df_ ? 0x1ffu : reduced_abcd_ ? 0xebfu : 0xfffu;
+  const unsigned need = direct_ ? 0x0ffu : !df_ ? 0x1ffu : reduced_abcd_ ? 0xebfu : 0xfffu;
   if ((have_ints_ & need) != need)
-    throw std::logic_error(!df_ ? "all of Fia, Fij, Fab, Gijab, Gijkl, Giajb, Giabc, Gijka, Gabcd must be set"
+    throw std::logic_error(direct_ ? "method direct needs Fia, Fij, Fab, Gijab, Gijkl, Giajb, Giabc, Gijka"
+                           : !df_ ? "all of Fia, Fij, Fab, Gijab, Gijkl, Giajb, Giabc, Gijka, Gabcd must be set"
                            : reduced_abcd_ ? "method df needs Fia, Fij, Fab, Gijab, Gijkl, Giajb, Gijka, Xai, Xij, Xab"
                                            : "method df without reduced_abcd_memory also needs Giabc and Gabcd");
   const bool have_iabc = !df_ || !reduced_abcd_;
@@ -866,8 +870,10 @@ void CCSDContext::residual() {
   perm_add(t1T_, {1, 1, O, V}, 0.0, {{t1_, {1, 1, V, O}, {0, 1, 3, 2}, 1.0}});  // t1T(k,d) = t1(d,k)
   const bool sm = split_mode();
   if (df_ && sm) throw std::logic_error("the df residual runs in the f64 and f32 forms");
+  if (direct_ && sm) throw std::logic_error("the direct residual runs in the f64 and f32 forms");
   const long Q = naux_;
   if (df_) df_r1_intermediates();
+  if (direct_) direct_u();  // ccsd.h:423-426: U before r1
   if (sm) {
     // 16-bit planes of the amplitude operands (the integral planes are split once per solve)
     if (!amax_.dev) amax_.dev = alloc(AbsMaxCache::kSlots);
@@ -931,6 +937,8 @@ void CCSDContext::residual() {
     gemm(false, false, v, o, (int)(Q * V), 1.0, XabP_, Q * V, Xt_, o, 1.0, r1_, o, 0, 0, 0, 1, "r1:df_Xt_Xab");
     // df :179 r1 -= X_ai_tau(K,a,l) Xij(K,l,i):  XtP(a,(K,l)) . Xij((K,l),i)
     gemm(false, false, v, o, (int)(Q * O), -1.0, XtP_, Q * O, Xij_, o, 1.0, r1_, o, 0, 0, 0, 1, "r1:df_Xt_Xij");
+  } else if (direct_) {  // :116-117
+    direct_r1();
   } else if (sm) {  // g_iabc_bar(a,(k,c,d)) . tau_ikcd(i,(k,c,d)),  tau_ikcd(i,k,c,d) = tau(c,d,k,i)
     split(tau_ikcd_P_, {O, O, V, V}, {{tau_, e_abij, {3, 2, 0, 1}, 1.0}});
     gemmP(v, o, o * v * v, 1.0, Gbar_akcd_P_, O * vv, tau_ikcd_P_, O * vv, 1.0, r1_, o, "r1:giabc_tau");
@@ -1107,6 +1115,10 @@ void CCSDContext::residual() {
     gemm(false, false, v * v, o * o, o * o, 1.0, tau_, oo, W_, oo, 1.0, r2_, oo, 0, 0, 0, 1, "r2:hh_ladder");
     if (df_ && reduced_abcd_) {
       df_pp_ladder();  // df :497-508
+      return;
+    }
+    if (direct_) {
+      direct_r2();  // :282-286
       return;
     }
     // :275 b(a,b,i,j) = tau(c,d,i,j) Gabcd(a,b,c,d)  -- the particle-particle ladder

@@ -56,6 +56,13 @@ class CCSDContext {
   // L (naux x naux, L L^T = V^-1; X = L^T B) or nullptr when B is already fitted
   void transform_df(int nbf, const double* B3, const double* metric, const double* fock, const double* c_occ,
                     const double* c_vir, int memspace);
+  // AO integral-direct method ("direct", ccsd.h:73, 246-247): the AO integrals
+  // (mu nu|la si) (nbf^4, chemist notation) and the active occupied / virtual MO
+  // coefficients (nbf x o, nbf x v, row-major) stay resident; every residual forms
+  // U(p,r,i,j) = (pq|rs) [Ca tau Ca^T](q,s,i,j) (compute_u2_u11, ccsd.h:573-603) in
+  // place of the (ab|cd) ladder and the Giabc tau terms.  Selects the method.
+  void set_direct_ao(int nbf, const double* ao_eri, const double* c_occ, const double* c_vir, int memspace);
+  bool direct() const { return direct_; }
   void set_orbital_energies(const double* eo, const double* ev);
   void solver_init(int diis_start, int n_diis, double damp, int ngroup, int ngroup_nstart);
   void set_amplitudes(const double* t1, const double* t2, int layout, int memspace);
@@ -155,6 +162,10 @@ class CCSDContext {
   void df_pp_ladder();                  // lazy (ab|cd) ladder of reduced_abcd_memory
   void ensure_df_blocks();
   void op_TP();   // the T intermediate of ccsd_r1_r2.h:236
+  // ---- AO integral-direct residual pieces
+  void direct_u();   // U at the current tau (ccsd.h:581-596)
+  void direct_r1();  // ccsd_r1_r2.h:116-117
+  void direct_r2();  // ccsd_r1_r2.h:282-286
 
   // ---- op tape: integral preparation and the residual as a list of primitive
   // ops (copies, permuted linear combinations, contractions, tau/T builds).
@@ -245,6 +256,11 @@ class CCSDContext {
   double *Tbar_ = nullptr, *Xt_ = nullptr, *XtP_ = nullptr, *Xabt1_ = nullptr, *Xabt1P_ = nullptr,
          *Zdf_ = nullptr, *Udf_ = nullptr, *sK_ = nullptr, *V1_ = nullptr, *Ydf_ = nullptr,
          *Rdf_ = nullptr, *bdf_ = nullptr, *Gblk_ = nullptr, *Gp_ = nullptr;
+  // AO integral-direct method: X((p,r),(q,s)) = (pq|rs), coefficients, U and its transforms
+  bool direct_ = false;
+  int nbf_ = 0;
+  double *Xao_ = nullptr, *Ci_ = nullptr, *Ca_ = nullptr, *Uao_ = nullptr, *TauAO_ = nullptr, *T1ao_ = nullptr,
+         *Vr_ = nullptr, *Yr_ = nullptr, *Dct_ = nullptr, *CaD_ = nullptr, *G1ao_ = nullptr, *G2ao_ = nullptr;
   int df_block_ = 0;            // virtual rows a per (ab|cd) block
   double df_block_bytes_ = 4.0 * (1 << 30);  // budget of the two (ab|cd) block buffers
 

@@ -47,6 +47,7 @@ void CCSDContext::set_df_basis(int naux, const int64_t* aux_hm, int n_aux_tiles)
       if (aux_hm_[t + 1] <= aux_hm_[t]) throw std::invalid_argument("hashmarks must increase");
   }
   df_ = true;
+  direct_ = false;
   prepared_ = false;
 }
 

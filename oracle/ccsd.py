@@ -13,8 +13,8 @@ it follows (paths relative to the MPQC source tree):
 * ``F_oo/F_vv/F_ov``       src/mpqc/chemistry/qc/lcao/cc/ccsd_intermediates.h:57-95
 * ``F_OO/F_VV``            ccsd_intermediates.h:105-112, 141-148
 * ``W_oooo``               ccsd_intermediates.h:182-193
-* ``cs_ccsd_r1``           ccsd_r1_r2.h:70-124   (standard method, ``u`` unset)
-* ``cs_ccsd_r2``           ccsd_r1_r2.h:198-293  (standard method, ``u`` unset)
+* ``cs_ccsd_r1``           ccsd_r1_r2.h:70-124   (``u`` given: the direct method's terms)
+* ``cs_ccsd_r2``           ccsd_r1_r2.h:198-293  (``u`` given: the direct method's terms)
 * ``jacobi_update_*``      src/mpqc/chemistry/qc/lcao/cc/solvers.h:165-258
 * ``error``                src/mpqc/chemistry/qc/cc/solvers.h:60-63
 * ``is_converged``         src/mpqc/chemistry/qc/cc/solvers.h:141-152
@@ -61,10 +61,13 @@ class Integrals:
     Giajb: np.ndarray
     Giabc: np.ndarray
     Gijka: np.ndarray
-    Gabcd: np.ndarray
+    Gabcd: Optional[np.ndarray]
     # effective one-body Hamiltonian, written by cs_ccsd_r1 (ccsd_r1_r2.h:90,96)
     FIJ: Optional[np.ndarray] = None
     FAB: Optional[np.ndarray] = None
+    # active occupied / virtual MO coefficients C[mu, p] of the direct method (ccsd.h:313-322)
+    Ci: Optional[np.ndarray] = None
+    Ca: Optional[np.ndarray] = None
 
     @property
     def o(self) -> int:
@@ -125,8 +128,10 @@ def make_tau(t1, t2):
     return t2 + _e("ai,bj->abij", t1, t1)
 
 
-def cs_ccsd_r1(t1, t2, tau, ints: Integrals):
-    """ccsd_r1_r2.h:70-124.  Returns r1(a,i); writes ints.FAB and ints.FIJ."""
+def cs_ccsd_r1(t1, t2, tau, ints: Integrals, u=None):
+    """ccsd_r1_r2.h:70-124.  Returns r1(a,i); writes ints.FAB and ints.FIJ.
+    ``u`` (U(p,r,i,j) of the direct method, AO indices p, r) replaces the
+    g_iabc_bar tau term (:110-118)."""
     Fia = ints.Fia
     # :83-84
     r1 = Fia.T - 2.0 * _e("kc,ci,ak->ai", Fia, t1, t1)
@@ -146,18 +151,23 @@ def cs_ccsd_r1(t1, t2, tau, ints: Integrals):
                + _e("kc,ci,ak->ai", h_kc, t1, t1))
     # :107-108
     r1 = r1 + (2.0 * _e("kica,ck->ai", ints.Gijab, t1) - _e("kaic,ck->ai", ints.Giajb, t1))
-    # :110-114
-    g_iabc_bar = 2.0 * ints.Giabc - ints.Giabc.transpose(0, 1, 3, 2)
-    r1 = r1 + _e("kacd,cdki->ai", g_iabc_bar, tau)
+    if u is None:
+        # :110-114
+        g_iabc_bar = 2.0 * ints.Giabc - ints.Giabc.transpose(0, 1, 3, 2)
+        r1 = r1 + _e("kacd,cdki->ai", g_iabc_bar, tau)
+    else:
+        # :116-117
+        r1 = r1 + _e("prki,pk,ra->ai", 2.0 * u - u.transpose(0, 1, 3, 2), ints.Ci, ints.Ca)
     # :120-121
     g_lkic_bar = 2.0 * ints.Gijka - ints.Gijka.transpose(1, 0, 2, 3)  # (l,k,i,c)
     r1 = r1 - _e("lkic,cakl->ai", g_lkic_bar, tau)
     return r1
 
 
-def cs_ccsd_r2(t1, t2, tau, ints: Integrals, fov_sign=1.0):
+def cs_ccsd_r2(t1, t2, tau, ints: Integrals, fov_sign=1.0, u=None):
     """ccsd_r1_r2.h:198-293.  Requires ints.FIJ / ints.FAB from cs_ccsd_r1.
-    ``fov_sign``: see ``F_VV``."""
+    ``fov_sign``: see ``F_VV``.  ``u`` (direct method) replaces the
+    particle-particle ladder and its Giabc terms (:282-286)."""
     G = ints
     # "permutation part" :206-212
     r2 = _e("icab,cj->abij", G.Giabc, t1) - _e("kbic,ak,cj->abij", G.Giajb, t1, t1)
@@ -194,11 +204,17 @@ def cs_ccsd_r2(t1, t2, tau, ints: Integrals, fov_sign=1.0):
     # hole-hole ladder :266-269
     a_klij = W_oooo(G.Gijkl, G.Gijka, G.Gijab, tau, t1)
     r2 = r2 + _e("klij,abkl->abij", a_klij, tau)
-    # particle-particle ladder :275-280
-    b_abij = _e("cdij,abcd->abij", tau, G.Gabcd)
-    tmp = _e("kacd,cdij->kaij", G.Giabc, tau)
-    b_abij = b_abij - _e("kaji,bk->abij", tmp, t1)
-    b_abij = b_abij - _e("kbij,ak->abij", tmp, t1)
+    if u is None:
+        # particle-particle ladder :275-280
+        b_abij = _e("cdij,abcd->abij", tau, G.Gabcd)
+        tmp = _e("kacd,cdij->kaij", G.Giabc, tau)
+        b_abij = b_abij - _e("kaji,bk->abij", tmp, t1)
+        b_abij = b_abij - _e("kbij,ak->abij", tmp, t1)
+    else:
+        # :282-286: (U Ca(r,b) - Ci(r,k) t1(b,k) U) Ca(p,a) - U Ci(p,k) Ca(r,b) t1(a,k)
+        b_abij = (_e("prij,rb,pa->abij", u, G.Ca, G.Ca)
+                  - _e("rk,bk,prij,pa->abij", G.Ci, t1, u, G.Ca)
+                  - _e("prij,pk,rb,ak->abij", u, G.Ci, G.Ca, t1))
     r2 = r2 + b_abij
     return r2
 
