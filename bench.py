@@ -48,7 +48,7 @@ def _args():
     p.add_argument("--workload", default="ccsd-h2o8",
                    choices=["ccsd-h2o8", "ccsd-h2o6", "ccsd-h20", "pp-ladder", "copy",
                             "batched-hchains", "adjoint-h2o6", "df-h2o8", "transform-h2o8",
-                            "triples-h2o6", "triples-h2o8", "eom-h2o6"])
+                            "triples-h2o6", "triples-h2o8", "eom-h2o6", "direct-h2o8"])
     p.add_argument("--forms", default="f64,f16x3",
                    help="contraction precisions to run for the CCSD workloads (first = reference)")
     p.add_argument("--max-iter", type=int, default=40)
@@ -1048,6 +1048,78 @@ def bench_eom(args):
     return line
 
 
+def bench_direct(args):
+    """AO integral-direct CCSD (method "direct") on one GPU: (H2O)8 cc-pVDZ, FP64,
+    AO integrals resident in HBM; a converged solve, then timed iterations with the
+    U = (pq|rs) [Ca tau Ca^T] GEMM per iteration, beside the standard FP64 solve."""
+    from mpqc_amd.ccsd import CCSD, Integrals
+    from mpqc_amd.chem.molecule import water_cluster
+    from mpqc_amd.chem.scf import rhf
+    from mpqc_amd.chem.mo import ccsd_integrals
+    from oracle.ccsd import ccsd_flops
+    t0 = time.perf_counter()
+    scf = rhf(water_cluster((2, 2, 2), seed=4), "cc-pVDZ")
+    inp = ccsd_integrals(scf, frozen_core=False)
+    nbf = scf.eri.shape[0]
+    Ci, Ca = scf.C[:, :scf.ndocc], scf.C[:, scf.ndocc:]
+    o, v = inp.o, inp.v
+    log(f"direct (H2O)8: nbf={nbf} o={o} v={v}, set-up {time.perf_counter() - t0:.1f}s")
+    blocks = {k: a for k, a in inp.blocks.items() if k != "Gabcd"}
+    with CCSD(o, v, precision="f64", max_iter=args.max_iter) as c:
+        c.set_integrals(blocks)
+        ta = time.perf_counter()
+        c.set_direct_ao(scf.eri, Ci, Ca)
+        upload_s = time.perf_counter() - ta
+        c.set_orbital_energies(inp.eps_occ, inp.eps_vir)
+        E = c.compute_ccsd(target_precision=1e-8)
+        n_it = len(c.trace())
+        log(f"direct: E_corr={E:.12f} after {n_it} iterations")
+        ms, kt = _timed_iterations(c, args, "direct f64")
+    with CCSD(o, v, precision="f64", max_iter=args.max_iter) as c:
+        c.set_integrals(Integrals(**inp.blocks))
+        c.set_orbital_energies(inp.eps_occ, inp.eps_vir)
+        E_std = c.compute_ccsd(target_precision=1e-8)
+        ms_std, _ = _timed_iterations(c, argparse.Namespace(steps=args.steps, warmup=args.warmup), "standard f64")
+    n_kt = args.warmup + args.steps
+    u = kt["u:ao_eri_tau"]
+    u_flops = 2.0 * nbf ** 4 * o * o
+    u_ms = u["ms"] / u["calls"]
+    cpu = None
+    if not args.no_cpu_baseline:
+        import numpy as np
+        from oracle.ccsd_direct import compute_u2_u11
+        rng = np.random.default_rng(0)
+        t1 = rng.normal(0, 0.01, (v, o))
+        t2 = rng.normal(0, 0.01, (v, v, o, o))
+        tb = time.perf_counter()
+        compute_u2_u11(t2, t1, scf.eri, Ca)
+        dt = time.perf_counter() - tb
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        cpu = {"value": round(u_flops / dt / 1e12, 3), "unit": "TFLOP/s", "cores": threads, "kind": "port",
+               "seconds": round(dt, 3),
+               "sample": f"the oracle's compute_u2_u11 (numpy einsum over the AO integrals) at the same size, "
+                         f"{u_flops:.3e} FLOPs in its U contraction"}
+    line = {
+        "metric": "direct-method CCSD iteration time on one GPU",
+        "value": round(ms, 3), "unit": "ms per iteration", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic molecule (seeded geometry), RHF + AO integrals on the host",
+        "config": {"workload": f"method direct for {SYSTEMS['ccsd-h2o8']}", "nbf": nbf, "o": o, "v": v,
+                   "parallelism": "single-gpu", "E_corr": E, "E_corr_standard": E_std,
+                   "dE_vs_standard": E - E_std, "iterations": n_it,
+                   "standard_ms_per_iteration": round(ms_std, 3), "ao_upload_s": round(upload_s, 3),
+                   "u_gemm_flops": u_flops, "algorithmic_flops_standard": ccsd_flops(o, v)},
+        "roofline": {"bound": "mfma", "achieved": round(u_flops / u_ms / 1e9, 3), "peak": PEAK["f64"],
+                     "unit": "TFLOP/s", "frac": round(u_flops / u_ms / 1e9 / PEAK["f64"], 4), "traffic": None,
+                     "kernel": f"U GEMM (M = K = nbf^2, N = o^2) avg {u_ms:.3f} ms"},
+        "kernels": {k: {"calls": x["calls"], "avg_ms": x["ms"] / max(1, x["calls"])} for k, x in kt.items()
+                    if x["ms"] / n_kt > 0.2},
+        "cpu_baseline": cpu,
+    }
+    return line
+
+
 def main():
     args = _args()
     if args.gpus != 1:
@@ -1074,6 +1146,8 @@ def main():
         line = bench_triples(args)
     elif args.workload == "eom-h2o6":
         line = bench_eom(args)
+    elif args.workload == "direct-h2o8":
+        line = bench_direct(args)
     else:
         line = bench_ccsd(args)
     print(json.dumps(line), flush=True)
