@@ -191,22 +191,27 @@ __device__ inline void mma_slice(const u16x8_t (&fa)[RB][NP], const u16x8_t (&fb
 // Occupancy per tile size: the 128 x 128 tile fits two workgroups per CU; the
 // 256-row tiles use the whole LDS and up to 512 registers per lane (the MFMA
 // accumulators of a 128 x 128 wave tile live in AGPRs), one workgroup per CU.
-template <int BM, int BN>
-constexpr int plane_occupancy() { return BM * BN >= 256 * 128 ? 1 : 2; }
+template <int BM, int BN, int NBUF>
+constexpr int plane_occupancy() { return NBUF == 1 ? 3 : BM * BN >= 256 * 128 ? 1 : 2; }
 
 // BD (B direct): the B fragments of each wave are loaded from global memory
 // (L1/L2) straight into registers one K slice ahead instead of being staged
 // through LDS; this halves the LDS traffic per slice (the kernel's limit).
-template <int BM, int BN, int NP, bool F16, bool VEC, bool BD>
-__global__ __launch_bounds__(kThr, (plane_occupancy<BM, BN>())) void gemm_planes_kernel(PParams p) {
+// NBUF: LDS stages of the K slices.  Two overlap the next slice's LDS stores with
+// the current products (64 KB at 128 x 128: two workgroups per CU); one halves the
+// LDS so that three workgroups share a CU and hide each other's load latency, at the
+// price of a second barrier per slice (its A fragments stream as in the 256-row
+// tiles, which keeps it within the registers of three waves per SIMD).
+template <int BM, int BN, int NP, bool F16, bool VEC, bool BD, int NBUF = 2>
+__global__ __launch_bounds__(kThr, (plane_occupancy<BM, BN, NBUF>())) void gemm_planes_kernel(PParams p) {
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int RB = WM / 16, CB = WN / 16;
   constexpr int ACH = BM * (kBK / 8) / kThr;  // 16-byte chunks per thread per plane (A)
   constexpr int BCH = BD ? 0 : BN * (kBK / 8) / kThr;
   static_assert(!BD || (VEC && NP == 2), "direct B fragments need 16-byte rows and two planes");
 
-  __shared__ __attribute__((aligned(16))) uint16_t As[2][NP][BM][kRow];
-  __shared__ __attribute__((aligned(16))) uint16_t Bs[2][NP][BD ? 1 : BN][kRow];
+  __shared__ __attribute__((aligned(16))) uint16_t As[NBUF][NP][BM][kRow];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[NBUF][NP][BD ? 1 : BN][kRow];
 
   // XCD-aware tile order: workgroup w runs on XCD w % 8; each XCD takes a
   // contiguous range of row-major tiles
@@ -386,7 +391,7 @@ __global__ __launch_bounds__(kThr, (plane_occupancy<BM, BN>())) void gemm_planes
         for (int q = 0; q < NP; ++q)
           fb[j][q] = *reinterpret_cast<const u16x8_t*>(&Bs[buf][q][wn * WN + j * 16 + lr][swz(lr, lk)]);
     }
-    if constexpr (NP == 2 && BM >= 256) {
+    if constexpr (NP == 2 && (BM >= 256 || NBUF == 1)) {
       // 128-row wave tiles: the B fragments of the slice stay in registers and the A
       // fragments stream one 16-row block at a time, read one block ahead of its products
       u16x8_t fa[2][1][NP];
@@ -433,15 +438,23 @@ __global__ __launch_bounds__(kThr, (plane_occupancy<BM, BN>())) void gemm_planes
 #pragma unroll
           for (int q = 0; q < NP; ++q) fbd[j][q] = fbn[j][q];
     }
-    if constexpr (BM < 256) {  // (the planner keeps the 256-row tiles' K ranges within kMaxChunk)
+    if constexpr (BM < 256 && NBUF == 2) {  // (the planner keeps every K range within kMaxChunk)
       if (++since_flush == kMaxChunk / kBK && has_next) {
         flush();
         since_flush = 0;
       }
     }
-    if (has_next) store_lds(buf ^ 1);
-    __syncthreads();
-    buf ^= 1;
+    if constexpr (NBUF == 2) {
+      if (has_next) store_lds(buf ^ 1);
+      __syncthreads();
+      buf ^= 1;
+    } else {
+      if (has_next) {
+        __syncthreads();  // every wave has read the slice
+        store_lds(0);
+      }
+      __syncthreads();
+    }
   }
   flush();  // also covers an empty K range (writes beta C or a zero partial)
 }
@@ -456,6 +469,7 @@ struct PlanePlan {
 int g_tile_override_m = 0, g_tile_override_n = 0;
 bool g_bdirect = false;  // direct B fragments (selected by measurement)
 int g_tile_order = 0;    // see PParams::order
+int g_lds_stages = 2;    // LDS stages of the 128 x 128 two-plane kernel
 
 PlanePlan plan_planes(int M, int N, int K, int batch, int force_bm = 0, int force_bn = 0) {
   PlanePlan pl;
@@ -520,7 +534,9 @@ void launch_planes(const PlaneGemmDesc& d, const PlanePlan& pl, double* ws, size
     p.tiles_m = tiles_m; p.tiles_n = tiles_n; p.per_xcd = per_xcd; p.order = g_tile_order;
     p.alpha = d.alpha; p.beta = d.beta;
     dim3 grid((unsigned)(per_xcd * kXcd), (unsigned)(nb * nsplit));
-    if (vec && NP == 2 && g_bdirect)
+    if (vec && NP == 2 && BM == 128 && BN == 128 && g_lds_stages == 1 && !g_bdirect)
+      hipLaunchKernelGGL((gemm_planes_kernel<BM, BN, NP, F16, true, false, 1>), grid, dim3(kThr), 0, stream, p);
+    else if (vec && NP == 2 && g_bdirect)
       hipLaunchKernelGGL((gemm_planes_kernel<BM, BN, NP, F16, true, NP == 2>), grid, dim3(kThr), 0, stream, p);
     else if (vec)
       hipLaunchKernelGGL((gemm_planes_kernel<BM, BN, NP, F16, true, false>), grid, dim3(kThr), 0, stream, p);
@@ -610,6 +626,10 @@ void split_to_planes(PlaneKind kind, const Planes& out, const long n[4], const T
 }
 
 void set_plane_bdirect(bool on) { g_bdirect = on; }
+void set_plane_lds_stages(int stages) {
+  if (stages != 1 && stages != 2) throw std::invalid_argument("plane LDS stages must be 1 or 2");
+  g_lds_stages = stages;
+}
 void set_plane_tile_order(int order) {
   if (order < 0 || order > 2) throw std::invalid_argument("unknown tile order");
   g_tile_order = order;

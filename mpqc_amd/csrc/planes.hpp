@@ -63,6 +63,9 @@ void gemm_planes(const PlaneGemmDesc& d, double* ws, size_t ws_bytes, hipStream_
 // Forces the contraction tile (bm x bn; 0, 0 restores the planner's choice).
 // Used by measurements that compare tile shapes.
 void set_plane_tile_override(int bm, int bn);
+// LDS stages of the 128 x 128 two-plane contraction (2: double buffered, two
+// workgroups per CU; 1: single buffered, four workgroups per CU).
+void set_plane_lds_stages(int stages);
 // Selects the contraction variant that loads B fragments straight into registers.
 void set_plane_bdirect(bool on);
 // Tile order of the contraction grid: 0 XCD ranges of row-major tiles, 1 plain row-major,

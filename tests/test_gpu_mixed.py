@@ -32,7 +32,7 @@ def test_mixed_precision_energy_h2o_pvdz(prec):
 RES_BOUNDS = {"f32": 1e-6, "f16x3": 2e-6, "bf16x6": 1e-6, "bf16x3": 1e-3}
 
 
-def _residual_errors(prec, tile=0):
+def _residual_errors(prec, tile=0, stages=2):
     from oracle import ccsd as O
     from mpqc_amd.ccsd import CCSD, Integrals
     from spinorb_ccsd import mpqc_blocks, random_hamiltonian
@@ -48,12 +48,14 @@ def _residual_errors(prec, tile=0):
     r2r = O.cs_ccsd_r2(t1, t2, tau, ints)
     with CCSD(no, nv, precision=prec) as c:
         c.set_option("plane_tile", tile)
+        c.set_option("plane_lds_stages", stages)
         try:
             c.set_integrals(Integrals(**b))
             c.set_orbital_energies(np.diag(f)[:no], np.diag(f)[no:])
             r1, r2 = c.compute_residuals(t1, t2)
         finally:
-            c.set_option("plane_tile", 0)   # process-wide setting
+            c.set_option("plane_tile", 0)   # process-wide settings
+            c.set_option("plane_lds_stages", 2)
     e1 = np.abs(r1 - r1r).max() / np.abs(r1r).max()
     e2 = np.abs(r2 - r2r).max() / np.abs(r2r).max()
     return e1, e2
@@ -69,11 +71,12 @@ def test_mixed_precision_residual(prec):
     assert max(e1, e2) > 1e-13  # reduced precision really used
 
 
-@pytest.mark.parametrize("prec,tile", [("f16x3", 256256), ("f16x3", 256128), ("bf16x3", 256256),
-                                       ("f16x3", 64064)])
-def test_plane_tiles_residual(prec, tile):
-    """Every plane contraction on a forced tile (256-row tiles stream their A
-    fragments; o=6, v=33 leaves ragged edges on every tile) against the oracle."""
-    e1, e2 = _residual_errors(prec, tile)
-    print(f"{prec} tile {tile}: rel err r1 {e1:.2e} r2 {e2:.2e}")
+@pytest.mark.parametrize("prec,tile,stages", [("f16x3", 256256, 2), ("f16x3", 256128, 2), ("bf16x3", 256256, 2),
+                                              ("f16x3", 64064, 2), ("f16x3", 128128, 1), ("bf16x3", 128128, 1)])
+def test_plane_tiles_residual(prec, tile, stages):
+    """Every plane contraction on a forced tile and LDS staging (256-row and
+    single-buffered tiles stream their A fragments; o=6, v=33 leaves ragged edges
+    on every tile) against the oracle."""
+    e1, e2 = _residual_errors(prec, tile, stages)
+    print(f"{prec} tile {tile} stages {stages}: rel err r1 {e1:.2e} r2 {e2:.2e}")
     assert e1 < RES_BOUNDS[prec] and e2 < RES_BOUNDS[prec]
