@@ -1096,7 +1096,7 @@ void CCSDContext::residual() {
               {tmpR_, e_aibj, {2, 0, 1, 3}, -1.0},
               {tmpR_, e_aibj, {2, 0, 3, 1}, -0.5},
               {tmpR_, e_aibj, {0, 2, 3, 1}, -1.0},
-              {Gijab_, {O, O, V, V}, {2, 3, 0, 1}, 1.0}});
+              {g_abij_, e_abij, I, 1.0}});  // Gijab(i,j,a,b) in the output's order (prepare())
   }
   // :266-269 a_klij = W_oooo (ccsd_intermediates.h:182-193); r2 += a_klij(k,l,i,j) tau(a,b,k,l)
   gemm(false, false, o * o * o, o, v, 1.0, Gijka_, v, t1_, o, 0.0, Y23_, o, 0, 0, 0, 1, "r2:W_gijka_t1");

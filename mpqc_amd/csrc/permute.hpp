@@ -52,6 +52,11 @@ struct PermGeom {
 // Host: geometry (and launch grid) for output extents n and the given inputs.
 PermGeom make_perm_geom(const long n[4], const TTerm* terms, int nterms, const TProd* prod = nullptr);
 dim3 perm_grid(const PermGeom& g);
+// Dynamic LDS of a launch over this geometry: the tiled shape stages T3 x (TQ + 1)
+// doubles (sized to the tile, so small tiles leave room for more workgroups per CU).
+inline size_t perm_lds_bytes(const PermGeom& g) {
+  return g.tiled ? (size_t)g.t3 * (size_t)(g.tq + 1) * sizeof(double) : 0;
+}
 
 // Device: evaluates every output element of block (bx, by, bz) of the launch
 // grid and hands (linear output offset, sum) to store().  Called by every
@@ -78,8 +83,9 @@ __device__ inline void permute_block(const PermGeom& g, unsigned bx, unsigned by
     store(ir0 * g.or0 + ir1 * g.or1 + e, acc);
     return;
   }
-  __shared__ double tile[kPT * (kPT + 1)];
-  const int T3 = g.t3, TQ = g.tq, nel = T3 * TQ;
+  extern __shared__ double perm_tile[];  // T3 x (TQ + 1), perm_lds_bytes()
+  double* tile = perm_tile;
+  const int T3 = g.t3, TQ = g.tq, nel = T3 * TQ, ld = TQ + 1;
   const int t3i = (int)bx % g.tiles3, tqi = (int)bx / g.tiles3;
   const long ir1 = by, ir0 = bz;
   const long b3 = (long)t3i * T3, bq = (long)tqi * TQ;
@@ -98,7 +104,7 @@ __device__ inline void permute_block(const PermGeom& g, unsigned bx, unsigned by
         if (e < nel) {
           const int j3 = (int)(((float)e + 0.5f) * g.inv_tq), ql = e - j3 * TQ;
           const long iq = bq + ql, i3 = b3 + j3;
-          tile[j3 * (kPT + 1) + ql] = (iq < g.nq && i3 < g.n3) ? base[iq * T.sq + i3 * T.s3] : 0.0;
+          tile[j3 * ld + ql] = (iq < g.nq && i3 < g.n3) ? base[iq * T.sq + i3 * T.s3] : 0.0;
         }
       }
       __syncthreads();
@@ -107,7 +113,7 @@ __device__ inline void permute_block(const PermGeom& g, unsigned bx, unsigned by
         const int e = (int)threadIdx.x + k * kPThreads;
         if (e < nel) {
           const int ql = (int)(((float)e + 0.5f) * g.inv_t3), j3 = e - ql * T3;
-          acc[k] += T.alpha * tile[j3 * (kPT + 1) + ql];
+          acc[k] += T.alpha * tile[j3 * ld + ql];
         }
       }
       __syncthreads();

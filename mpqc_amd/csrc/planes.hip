@@ -621,7 +621,7 @@ void split_to_planes(PlaneKind kind, const Planes& out, const long n[4], const T
   hipLaunchKernelGGL(scale_stage2, dim3(1), dim3(256), 0, stream, partials, sa, out.scale);
   const PermGeom g = make_perm_geom(n, terms, nterms, prod);
   SplitOut so{{out.p[0], out.p[1], out.p[2]}, out.scale, (int)kind};
-  hipLaunchKernelGGL(split_write_kernel, perm_grid(g), dim3(kPThreads), 0, stream, g, so);
+  hipLaunchKernelGGL(split_write_kernel, perm_grid(g), dim3(kPThreads), perm_lds_bytes(g), stream, g, so);
   check_launch("split_to_planes");
 }
 

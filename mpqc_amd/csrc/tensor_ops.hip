@@ -332,6 +332,7 @@ GroupedPlan plan_grouped_tadd(const std::vector<GroupedTAddDesc>& probs, hipStre
     host[q].gx = gr.x;
     host[q].gy = gr.y;
     pl.items += (long)gr.x * gr.y * gr.z;
+    pl.lds = std::max(pl.lds, perm_lds_bytes(host[q].g));
   }
   MPQC_HIP_CHECK(hipMalloc(&pl.dev, host.size() * sizeof(GPerm)));
   MPQC_HIP_CHECK(hipMemcpyAsync(pl.dev, host.data(), host.size() * sizeof(GPerm), hipMemcpyHostToDevice, s));
@@ -364,7 +365,7 @@ void launch_grouped(const GroupedPlan& pl, hipStream_t s) {
   if (pl.items == 0) return;
   const dim3 grid = grouped_grid(pl.items);
   if (pl.kind == 0)
-    hipLaunchKernelGGL(grouped_tadd_kernel, grid, dim3(kPThreads), 0, s, (const GPerm*)pl.dev, pl.nprob);
+    hipLaunchKernelGGL(grouped_tadd_kernel, grid, dim3(kPThreads), pl.lds, s, (const GPerm*)pl.dev, pl.nprob);
   else
     hipLaunchKernelGGL(grouped_ew_kernel, grid, dim3(kBlock), 0, s, (const GEw*)pl.dev, pl.nprob);
   check_launch("grouped");
@@ -449,7 +450,7 @@ void tadd(double* out, const long n[4], double beta, const TTerm* terms, int nte
   if (nterms > kMaxTerms) throw std::invalid_argument("tadd: too many terms");
   if (n[0] * n[1] * n[2] * n[3] == 0) return;
   const PermGeom g = make_perm_geom(n, terms, nterms, prod);
-  hipLaunchKernelGGL(tadd_kernel, perm_grid(g), dim3(kPThreads), 0, stream, g, out, beta);
+  hipLaunchKernelGGL(tadd_kernel, perm_grid(g), dim3(kPThreads), perm_lds_bytes(g), stream, g, out, beta);
   check_launch("tadd");
 }
 

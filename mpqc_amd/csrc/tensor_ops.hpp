@@ -120,6 +120,7 @@ struct GroupedPlan {
   int nprob = 0;
   long items = 0;       // workgroups of the launch
   int kind = 0;         // 0 permutation, 1 element-wise
+  size_t lds = 0;       // dynamic LDS of the launch (largest tile of the permutations)
 };
 GroupedPlan plan_grouped_tadd(const std::vector<GroupedTAddDesc>& probs, hipStream_t s);
 GroupedPlan plan_grouped_ew(const std::vector<GroupedEwDesc>& probs, hipStream_t s);
