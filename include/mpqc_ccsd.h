@@ -177,7 +177,9 @@ int mpqc_ccsd_triples(mpqc_ccsd_ctx* ctx, double* e_t, double* seconds);
  * "eom_max_vector" (subspace vectors per root, default 8), "eom_vector_threshold"
  * (norm below which a new vector is dropped, default 10 convergence) and
  * "eom_n_guess" (initial unit singles on the smallest FAB(a,a) - FIJ(i,i),
- * default 2 n_roots).  Runs in the f64 and f32 precision forms; the df method
+ * default 2 n_roots), "eom_extra_roots" (roots tracked above the requested ones, default 0),
+ * "eom_cis_diagonal" (1: singles preconditioned and guessed with the CIS singlet diagonal
+ * instead of FAB(a,a) - FIJ(i,i), default 0).  Runs in the f64 and f32 precision forms; the df method
  * evaluates the same Jacobian on the four-index blocks its factors assemble
  * (Giabc and Gabcd are built on the device when not set). */
 int mpqc_ccsd_eom_ee(mpqc_ccsd_ctx* ctx, int n_roots, int max_iter, double convergence, int verbose,

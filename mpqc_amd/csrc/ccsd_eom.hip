@@ -46,16 +46,18 @@ constexpr int kEomBlock = 256;
 // their exchanged element.  Without the projection the doubles would keep a
 // round-off antisymmetric part, on which P J^T vanishes exactly: Gram-Schmidt
 // would amplify it into subspace vectors with a spurious zero root.
+// d1 (optional, v x o): the singles' diagonal in place of e_a - e_i
 __global__ __launch_bounds__(kEomBlock) void ee_precondition_kernel(double* __restrict__ out,
                                                                     const double* __restrict__ r,
                                                                     const double* __restrict__ eo,
-                                                                    const double* __restrict__ ev, int o,
+                                                                    const double* __restrict__ ev,
+                                                                    const double* __restrict__ d1, int o,
                                                                     int v, double theta, long n1, long n) {
   for (long e = (long)blockIdx.x * kEomBlock + threadIdx.x; e < n; e += (long)gridDim.x * kEomBlock) {
     double d, x;
     if (e < n1) {
       const long a = e / o, i = e - a * o;
-      d = theta + eo[i] - ev[a];
+      d = d1 ? theta - d1[e] : theta + eo[i] - ev[a];
       x = r[e];
     } else {
       long k = e - n1;
@@ -206,6 +208,27 @@ int CCSDContext::eom_ee(int n_roots, int max_iter, double convergence, int verbo
   std::vector<double> hdg(O + V);
   MPQC_HIP_CHECK(hipMemcpyAsync(hdg.data(), dg, (O + V) * sizeof(double), hipMemcpyDeviceToHost, stream_));
   MPQC_HIP_CHECK(hipStreamSynchronize(stream_));
+  // singles' diagonal: FAB(a,a) - FIJ(i,i), plus with eom_cis_diagonal the two-electron
+  // part of the CIS singlet diagonal, 2 (ia|ia) - (ii|aa) = 2 Gijab(i,i,a,a) - Giajb(i,a,i,a)
+  std::vector<double> hd1((size_t)n1);
+  for (long a = 0; a < V; ++a)
+    for (long i = 0; i < O; ++i) hd1[a * O + i] = hdg[O + a] - hdg[i];
+  double* d1 = nullptr;
+  if (eom_cis_diagonal_) {
+    std::vector<double> g1((size_t)n1), g2((size_t)n1);
+    for (long i = 0; i < O; ++i) {
+      MPQC_HIP_CHECK(hipMemcpy2DAsync(g1.data() + i * V, sizeof(double), Gijab_ + (i * O + i) * V * V,
+                                      (V + 1) * sizeof(double), sizeof(double), V, hipMemcpyDeviceToHost, stream_));
+      MPQC_HIP_CHECK(hipMemcpy2DAsync(g2.data() + i * V, sizeof(double), Giajb_ + i * V * O * V + i * V,
+                                      (O * V + 1) * sizeof(double), sizeof(double), V, hipMemcpyDeviceToHost,
+                                      stream_));
+    }
+    MPQC_HIP_CHECK(hipStreamSynchronize(stream_));
+    for (long a = 0; a < V; ++a)
+      for (long i = 0; i < O; ++i) hd1[a * O + i] += 2.0 * g1[i * V + a] - g2[i * V + a];
+    d1 = adj_hist(111, n1);
+    MPQC_HIP_CHECK(hipMemcpyAsync(d1, hd1.data(), n1 * sizeof(double), hipMemcpyHostToDevice, stream_));
+  }
 
   eom_products_ = 0;
   eom_product_seconds_ = 0.0;
@@ -302,18 +325,17 @@ int CCSDContext::eom_ee(int n_roots, int max_iter, double convergence, int verbo
     }
   };
 
-  // initial vectors: unit singles on the smallest FAB(a,a) - FIJ(i,i)
+  // initial vectors: unit singles on the smallest singles' diagonal
   std::vector<double*> guess;
   {
-    std::vector<std::pair<double, long>> d1;
-    for (long a = 0; a < V; ++a)
-      for (long i = 0; i < O; ++i) d1.push_back({hdg[O + a] - hdg[i], a * O + i});
-    std::stable_sort(d1.begin(), d1.end(),
+    std::vector<std::pair<double, long>> order1;
+    for (long e = 0; e < n1; ++e) order1.push_back({hd1[e], e});
+    std::stable_sort(order1.begin(), order1.end(),
                      [](const std::pair<double, long>& x, const std::pair<double, long>& y) { return x.first < y.first; });
     for (int g = 0; g < n_guess; ++g) {
       double* x = pool.get();
       fill(x, n, 0.0, stream_);
-      hipLaunchKernelGGL(unit_vector_kernel, dim3(1), dim3(1), 0, stream_, x, d1[g].second);
+      hipLaunchKernelGGL(unit_vector_kernel, dim3(1), dim3(1), 0, stream_, x, order1[g].second);
       check_launch("unit_vector");
       guess.push_back(x);
     }
@@ -393,7 +415,7 @@ int CCSDContext::eom_ee(int n_roots, int max_iter, double convergence, int verbo
       done[r] = eom_skip_converged_ && norms[r] <= 10.0 * convergence && std::fabs(eig[r] - E[r]) <= convergence;
     for (int r = 0; r < nt; ++r) {
       double* p = pool.get();
-      hipLaunchKernelGGL(ee_precondition_kernel, dim3(grid), dim3(kEomBlock), 0, stream_, p, R[r], dg, dg + O,
+      hipLaunchKernelGGL(ee_precondition_kernel, dim3(grid), dim3(kEomBlock), 0, stream_, p, R[r], dg, dg + O, d1,
                          (int)O, (int)V, E[r], n1, n);
       check_launch("ee_precondition");
       pool.put(R[r]);
