@@ -233,6 +233,8 @@ void CCSDContext::set_option(const std::string& key, double value) {
     const int t = (int)value;
     if (t != value || t < 0) throw std::invalid_argument("plane_tile must be 0 or rows * 1000 + columns");
     set_plane_tile_override(t / 1000, t % 1000);
+  } else if (key == "plane_tile_order") {  // process-wide: 0 XCD ranges row-major, 1 plain, 2 XCD ranges column-major
+    set_plane_tile_order((int)value);
   } else if (key == "plane_lds_stages") {  // process-wide
     set_plane_lds_stages((int)value);
   } else if (key == "eom_max_vector") {  // eom_ccsd.h:33
