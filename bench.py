@@ -33,6 +33,7 @@ sys.path.insert(0, ROOT)
 
 # TFLOP/s dense MFMA, MI355X, of the unit each form's contractions run on
 PEAK = {"f64": 78.6, "f32": 157.3, "bf16x3": 2500.0, "f16x3": 2500.0, "bf16x6": 2500.0}
+MFMA_PRODUCTS = {"f64": 1, "f32": 1, "bf16x3": 3, "f16x3": 3, "bf16x6": 6}
 DTYPE = {"f64": "f64",
          "f32": "f32-mfma contractions / f64 update, DIIS, energy",
          "bf16x3": "bf16 hi+lo split contractions (3 products) / f64 update, DIIS, energy",
@@ -423,7 +424,10 @@ def bench_ccsd(args):
                      "peak": PEAK[head], "unit": "TFLOP/s",
                      "frac": round(pp["tflops"] / PEAK[head], 4), "traffic": traffic,
                      "traffic_source": traffic_src,
-                     "kernel": f"pp-ladder GEMM (M=v^2, N=o^2, K=v^2) avg {pp['avg_ms']:.3f} ms"},
+                     "kernel": f"pp-ladder GEMM (M=v^2, N=o^2, K=v^2) avg {pp['avg_ms']:.3f} ms",
+                     # the split forms issue MFMA_PRODUCTS matrix-core products per algorithmic FLOP
+                     "mfma_products_per_flop": MFMA_PRODUCTS.get(head, 1),
+                     "mfma_work_frac": round(pp["tflops"] * MFMA_PRODUCTS.get(head, 1) / PEAK[head], 4)},
         "forms": {p: {k: (round(x, 6) if isinstance(x, float) else x) for k, x in f.items()
                       if k != "kernels"} for p, f in forms.items()},
         "kernels": {p: f["kernels"] for p, f in forms.items()},

@@ -151,7 +151,11 @@ const char* mpqc_ccsd_last_error(const mpqc_ccsd_ctx* ctx);
  * "method" (0 standard, 1 df, 2 direct; ccsd.h:94-99), "reduced_abcd_memory" (1 default:
  * the df method builds (ab|cd) lazily from Xab instead of reading Gabcd and
  * Giabc, ccsd.h:102,292), "df_block_bytes" (device memory for the lazily built
- * (ab|cd) blocks, default 4 GiB; set before the first residual). */
+ * (ab|cd) blocks, default 4 GiB; set before the first residual).  Tuning keys that act on
+ * every context of the process (measurement aids; the defaults are the measured best):
+ * "plane_tile" (tile of the 16-bit plane contractions, rows * 1000 + columns, 0 automatic),
+ * "plane_lds_stages" (1 or 2), "plane_tile_order" (0 XCD ranges of row-major tiles, 1 plain,
+ * 2 XCD ranges of column-major tiles). */
 int mpqc_ccsd_set_option(mpqc_ccsd_ctx* ctx, const char* key, double value);
 
 int mpqc_ccsd_set_integral(mpqc_ccsd_ctx* ctx, int which, const double* data, int layout,
