@@ -148,7 +148,7 @@ const char* mpqc_ccsd_last_error(const mpqc_ccsd_ctx* ctx);
 /* Numeric options: "fov_sign" (+1 reference F_VV as written, -1 exact CCSD for
  * non-Brillouin references), "precision" (MPQC_CCSD_PREC_*), "kernel_timing"
  * (1 = time every contraction with HIP events on the context stream),
- * "method" (0 standard, 1 df, 2 direct; ccsd.h:94-99), "reduced_abcd_memory" (1 default:
+ * "method" (0 standard, 1 df, 2 direct, 3 direct_df; ccsd.h:73, 94-99), "reduced_abcd_memory" (1 default:
  * the df method builds (ab|cd) lazily from Xab instead of reading Gabcd and
  * Giabc, ccsd.h:102,292), "df_block_bytes" (device memory for the lazily built
  * (ab|cd) blocks, default 4 GiB; set before the first residual).  Tuning keys that act on
@@ -224,7 +224,9 @@ int mpqc_ccsd_set_df_basis(mpqc_ccsd_ctx* ctx, int naux, const int64_t* aux_hash
  * 313-322); switches the context to the direct residual, which needs every block but
  * Gabcd and forms U = (pq|rs) [Ca tau Ca^T] each iteration (compute_u2_u11,
  * ccsd.h:573-603; used at ccsd_r1_r2.h:116-117, 282-286).  The AO integrals stay
- * resident in HBM instead of being recomputed.  Runs in MPQC_CCSD_PREC_F64 and F32. */
+ * resident in HBM instead of being recomputed.  With a fitting basis as well
+ * (mpqc_ccsd_set_df_basis), "method" = 3 selects direct_df: the df residual with U
+ * (ccsd.h:436, 457; ccsd_r1_r2.h:509-514).  Runs in MPQC_CCSD_PREC_F64 and F32. */
 int mpqc_ccsd_set_direct_ao(mpqc_ccsd_ctx* ctx, int nbf, const double* ao_eri, const double* c_occ,
                             const double* c_vir, int memspace);
 int mpqc_ccsd_solver_init(mpqc_ccsd_ctx* ctx, int diis_start, int n_diis, double diis_damp,

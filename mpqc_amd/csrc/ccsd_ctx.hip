@@ -217,13 +217,15 @@ void CCSDContext::set_option(const std::string& key, double value) {
     lambda_ready_ = false;
   } else if (key == "kernel_timing") {
     kernel_timing_ = value != 0.0;
-  } else if (key == "method") {  // 0 standard, 1 df, 2 direct (ccsd.h:94-99)
-    if (value != 0.0 && value != 1.0 && value != 2.0)
-      throw std::invalid_argument("method must be 0 (standard), 1 (df) or 2 (direct)");
-    if (value == 1.0 && naux_ == 0) throw std::invalid_argument("method df needs set_df_basis first");
-    if (value == 2.0 && !Xao_) throw std::invalid_argument("method direct needs set_direct_ao first");
-    df_ = value == 1.0;
-    direct_ = value == 2.0;
+  } else if (key == "method") {  // 0 standard, 1 df, 2 direct, 3 direct_df (ccsd.h:73, 94-99)
+    if (value != 0.0 && value != 1.0 && value != 2.0 && value != 3.0)
+      throw std::invalid_argument("method must be 0 (standard), 1 (df), 2 (direct) or 3 (direct_df)");
+    if ((value == 1.0 || value == 3.0) && naux_ == 0)
+      throw std::invalid_argument("methods df and direct_df need set_df_basis first");
+    if ((value == 2.0 || value == 3.0) && !Xao_)
+      throw std::invalid_argument("methods direct and direct_df need set_direct_ao first");
+    df_ = value == 1.0 || value == 3.0;
+    direct_ = value == 2.0 || value == 3.0;
     prepared_ = false;
   } else if (key == "reduced_abcd_memory") {  // ccsd.h:102, used at :292
     reduced_abcd_ = value != 0.0;
@@ -788,9 +790,9 @@ double CCSDContext::scalar_from_device(const double* d) {
 void CCSDContext::prepare() {
   // standard: all nine blocks; df (ccsd.h:281-305): no Giabc / Gabcd under
   // reduced_abcd_memory, plus Xai, Xij, Xab
-  const unsigned need = direct_ ? 0x0ffu : !df_ ? 0x1ffu : reduced_abcd_ ? 0xebfu : 0xfffu;
+  const unsigned need = (direct_ && !df_) ? 0x0ffu : !df_ ? 0x1ffu : (reduced_abcd_ || direct_) ? 0xebfu : 0xfffu;
   if ((have_ints_ & need) != need)
-    throw std::logic_error(direct_ ? "method direct needs Fia, Fij, Fab, Gijab, Gijkl, Giajb, Giabc, Gijka"
+    throw std::logic_error((direct_ && !df_) ? "method direct needs Fia, Fij, Fab, Gijab, Gijkl, Giajb, Giabc, Gijka"
                            : !df_ ? "all of Fia, Fij, Fab, Gijab, Gijkl, Giajb, Giabc, Gijka, Gabcd must be set"
                            : reduced_abcd_ ? "method df needs Fia, Fij, Fab, Gijab, Gijkl, Giajb, Gijka, Xai, Xij, Xab"
                                            : "method df without reduced_abcd_memory also needs Giabc and Gabcd");
@@ -939,8 +941,10 @@ void CCSDContext::residual() {
   gemm(false, false, v * o, 1, v * o, 1.0, P6_, ov, t1_, 1, 1.0, r1_, 1, 0, 0, 0, 1, "r1:g_t1");
   // :110-114 r1 += g_iabc_bar(k,a,c,d) tau(c,d,k,i)
   if (df_) {
-    // df :177 r1 += X_ai_tau(K,c,i) Xab(K,a,c):  XabP(a,(K,c)) . Xt((K,c),i)
-    gemm(false, false, v, o, (int)(Q * V), 1.0, XabP_, Q * V, Xt_, o, 1.0, r1_, o, 0, 0, 0, 1, "r1:df_Xt_Xab");
+    if (direct_)  // direct_df: the standard r1 with U (ccsd.h:436, ccsd_r1_r2.h:116-117)
+      direct_r1();
+    else  // df :177 r1 += X_ai_tau(K,c,i) Xab(K,a,c):  XabP(a,(K,c)) . Xt((K,c),i)
+      gemm(false, false, v, o, (int)(Q * V), 1.0, XabP_, Q * V, Xt_, o, 1.0, r1_, o, 0, 0, 0, 1, "r1:df_Xt_Xab");
     // df :179 r1 -= X_ai_tau(K,a,l) Xij(K,l,i):  XtP(a,(K,l)) . Xij((K,l),i)
     gemm(false, false, v, o, (int)(Q * O), -1.0, XtP_, Q * O, Xij_, o, 1.0, r1_, o, 0, 0, 0, 1, "r1:df_Xt_Xij");
   } else if (direct_) {  // :116-117
@@ -1119,12 +1123,12 @@ void CCSDContext::residual() {
   } else {
     gemm(false, false, o * o, o * o, v * v, 1.0, Gijab_, vv, tau_, oo, 1.0, W_, oo, 0, 0, 0, 1, "r2:W_oooo");
     gemm(false, false, v * v, o * o, o * o, 1.0, tau_, oo, W_, oo, 1.0, r2_, oo, 0, 0, 0, 1, "r2:hh_ladder");
-    if (df_ && reduced_abcd_) {
-      df_pp_ladder();  // df :497-508
+    if (direct_) {
+      direct_r2();  // :282-286 (direct_df: :509-514)
       return;
     }
-    if (direct_) {
-      direct_r2();  // :282-286
+    if (df_ && reduced_abcd_) {
+      df_pp_ladder();  // df :497-508
       return;
     }
     // :275 b(a,b,i,j) = tau(c,d,i,j) Gabcd(a,b,c,d)  -- the particle-particle ladder

@@ -178,9 +178,9 @@ int CCSDContext::eom_ee(int n_roots, int max_iter, double convergence, int verbo
   if (!prepared_) prepare();
   const bool was_df = df_;
   const double was_fov = fov_sign_;
-  if (df_) {
+  if (df_) {  // (direct_df: the direct residual on the blocks the factors assemble)
     ensure_iabc_for_triples();
-    ensure_abcd_from_factors();
+    if (!direct_) ensure_abcd_from_factors();
     df_ = false;
     fov_sign_ = -1.0;
     prepared_ = false;

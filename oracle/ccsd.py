@@ -59,7 +59,7 @@ class Integrals:
     Gijab: np.ndarray
     Gijkl: np.ndarray
     Giajb: np.ndarray
-    Giabc: np.ndarray
+    Giabc: Optional[np.ndarray]
     Gijka: np.ndarray
     Gabcd: Optional[np.ndarray]
     # effective one-body Hamiltonian, written by cs_ccsd_r1 (ccsd_r1_r2.h:90,96)

@@ -12,9 +12,10 @@ Restates, term by term and in the reference's evaluation order
                            Xai, Xij, Xab; filled at ccsd.h:281-305, X = (K|G|p q)[inv_sqr],
                            ccsd.h:481-493)
 * ``cs_ccsd_r1_df``        ccsd_r1_r2.h:136-184
-* ``cs_ccsd_r2_df``        ccsd_r1_r2.h:309-520 with ``u`` unset: the Gabcd branch
+* ``cs_ccsd_r2_df``        ccsd_r1_r2.h:309-520: with ``u`` unset the Gabcd branch
                            (:418-430) when Gabcd is given, otherwise the lazy
-                           density-fitted ladder of reduced_abcd_memory (:497-508)
+                           density-fitted ladder of reduced_abcd_memory (:497-508);
+                           with ``u`` (method direct_df) the U terms (:509-514)
 * ``df_abcd``              direct_task_integrals.h:255-300 with integral_builder.h:228-325
                            (Physical notation: g(a,b,c,d) = sum_K bra(K,a,c) ket(K,b,d))
 
@@ -60,6 +61,9 @@ class DFIntegrals:
     Gabcd: Optional[np.ndarray] = None
     FIJ: Optional[np.ndarray] = None
     FAB: Optional[np.ndarray] = None
+    # active occupied / virtual MO coefficients of method direct_df (ccsd.h:313-322)
+    Ci: Optional[np.ndarray] = None
+    Ca: Optional[np.ndarray] = None
 
     @property
     def o(self) -> int:
@@ -104,9 +108,9 @@ def cs_ccsd_r1_df(t1, t2, tau, ints: DFIntegrals):
     return r1
 
 
-def cs_ccsd_r2_df(t1, t2, tau, ints: DFIntegrals):
-    """ccsd_r1_r2.h:309-520 (``u`` unset).  Requires ints.FIJ / ints.FAB from
-    cs_ccsd_r1_df."""
+def cs_ccsd_r2_df(t1, t2, tau, ints: DFIntegrals, u=None):
+    """ccsd_r1_r2.h:309-520.  Requires ints.FIJ / ints.FAB from the r1 of the
+    method.  ``u`` (U(p,r,i,j), AO indices p, r) replaces the ladder block."""
     G = ints
     Xai, Xij, Xab = G.Xai, G.Xij, G.Xab
     # :317-318
@@ -148,7 +152,12 @@ def cs_ccsd_r2_df(t1, t2, tau, ints: DFIntegrals):
     a_klij = W_oooo(G.Gijkl, G.Gijka, G.Gijab, tau, t1)
     r2 = r2 + _e("klij,abkl->abij", a_klij, tau)
     # particle-particle ladder
-    if G.Gabcd is not None:
+    if u is not None:
+        # :509-514 (as ccsd_r1_r2.h:282-286)
+        b_abij = (_e("prij,rb,pa->abij", u, G.Ca, G.Ca)
+                  - _e("rk,bk,prij,pa->abij", G.Ci, t1, u, G.Ca)
+                  - _e("prij,pk,rb,ak->abij", u, G.Ci, G.Ca, t1))
+    elif G.Gabcd is not None:
         # :418-430
         b_abij = _e("cdij,abcd->abij", tau, G.Gabcd)
         tmp = _e("kacd,cdij->kaij", G.Giabc, tau)

@@ -56,3 +56,49 @@ def test_direct_residual_equals_standard_at_consistent_integrals(idx):
     # planted mistake: (pr|qs) in place of (pq|rs)
     bad = DR.direct_residual(scf.eri.transpose(0, 2, 1, 3))(t1, t2, tau, _direct_ints(scf, inp))
     assert np.abs(bad[1] - r2s).max() / np.abs(r2s).max() > 1e-4
+
+
+def _exact_factors(scf, inp):
+    """Three-index factors X(K,p,q) that reproduce the active-MO integrals exactly:
+    the eigen-factorisation of the (pq|rs) supermatrix."""
+    from oracle import ccsd_df as D
+    nfz = inp.n_frozen
+    C = scf.C[:, nfz:]
+    mo = np.einsum("mnls,mp,nq,lr,st->pqrt", scf.eri, C, C, C, C, optimize=True)
+    n = C.shape[1]
+    w, U = np.linalg.eigh(mo.reshape(n * n, n * n))
+    keep = w > 1e-12 * w.max()
+    X = (U[:, keep] * np.sqrt(w[keep])).T.reshape(-1, n, n)
+    o = inp.o
+    Xai, Xij, Xab = X[:, o:, :o].copy(), X[:, :o, :o].copy(), X[:, o:, o:].copy()
+    g = D.four_index_from_factors(Xai, Xij, Xab)
+    return Xai, Xij, Xab, g
+
+
+def test_direct_df_equals_standard_with_exact_factors():
+    """direct_df (the df residual with U) at factors that reproduce the integrals
+    exactly is the standard residual; a transposed AO index in U is caught."""
+    from oracle import ccsd_df as D
+    scf, inp = _inputs(FIX["cases"][0])
+    Xai, Xij, Xab, g = _exact_factors(scf, inp)
+    for k in ("Gijab", "Gijkl", "Giajb", "Gijka"):
+        assert np.abs(g[k] - inp.blocks[k]).max() < 1e-12
+    t1, t2 = _random_amplitudes(inp.o, inp.v, seed=5)
+    tau = O.make_tau(t1, t2)
+    std = O.Integrals(**inp.blocks)
+    r1s = O.cs_ccsd_r1(t1, t2, tau, std)
+    r2s = O.cs_ccsd_r2(t1, t2, tau, std, fov_sign=-1.0)   # the df residual's g_ac convention
+    b = inp.blocks
+    nfz = inp.n_frozen
+
+    def dints():
+        return D.DFIntegrals(Fia=b["Fia"], Fij=b["Fij"], Fab=b["Fab"], Gijab=g["Gijab"], Gijkl=g["Gijkl"],
+                             Giajb=g["Giajb"], Gijka=g["Gijka"], Xai=Xai, Xij=Xij, Xab=Xab,
+                             Ci=scf.C[:, nfz:scf.ndocc].copy(), Ca=scf.C[:, scf.ndocc:].copy())
+    r1d, r2d = DR.direct_df_residual(scf.eri)(t1, t2, tau, dints())
+    e1 = np.abs(r1d - r1s).max() / np.abs(r1s).max()
+    e2 = np.abs(r2d - r2s).max() / np.abs(r2s).max()
+    print(f"direct_df vs standard: r1 {e1:.2e} r2 {e2:.2e}")
+    assert e1 < 1e-11 and e2 < 1e-11
+    bad = DR.direct_df_residual(scf.eri.transpose(0, 2, 1, 3))(t1, t2, tau, dints())
+    assert np.abs(bad[1] - r2s).max() / np.abs(r2s).max() > 1e-4
