@@ -12,3 +12,18 @@ if HERE not in sys.path:
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")
     config.addinivalue_line("markers", "slow: long-running CPU test")
+
+
+def pytest_sessionstart(session):
+    # torch's HIP runtime must initialise before the library's (two runtimes in one
+    # process): tests that hand torch device buffers to the library create torch
+    # tensors after other tests have loaded it
+    markexpr = getattr(session.config.option, "markexpr", "") or ""
+    if "not gpu" in markexpr:
+        return
+    try:
+        import torch
+    except ImportError:
+        return
+    if torch.cuda.is_available():
+        torch.zeros(1, device="cuda")
