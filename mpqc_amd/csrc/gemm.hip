@@ -364,6 +364,31 @@ __global__ void splitk_reduce_kernel(const double* __restrict__ W, double* __res
   }
 }
 
+// Many partials over a small output (nsplit >= kDeepReduceSplits, M*N*batch below
+// kWideReduceMax): a wave owns one output, lane l adds the partials s = l, l + 64, ...
+// (a dependent chain of nsplit / 64 loads instead of nsplit / 4), and the lanes
+// combine by a fixed butterfly (every lane ends with the same sum).
+constexpr int kDeepReduceSplits = 64;
+__global__ void splitk_reduce_deep_kernel(const double* __restrict__ W, double* __restrict__ C, long ldc,
+                                          long sC, int M, int N, int nsplit, int batch0, int nb, double beta) {
+  const long MN = (long)M * N;
+  const int lane = threadIdx.x & 63;
+  const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (wave >= MN * nb) return;
+  const int bz = (int)(wave / MN);
+  const long e = wave - (long)bz * MN;
+  const double* w = W + (long)bz * nsplit * MN + e;
+  double acc = 0.0;
+  for (int s = lane; s < nsplit; s += 64) acc += w[(long)s * MN];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+  if (lane == 0) {
+    const int m = (int)(e / N), n = (int)(e - (long)m * N);
+    double* c = C + (long)(batch0 + bz) * sC + (long)m * ldc + n;
+    *c = (beta == 0.0) ? acc : acc + beta * (*c);
+  }
+}
+
 // Same sum for small outputs (M*N*batch below kWideReduceMax): a wave owns 16
 // consecutive outputs and its four 16-lane groups add every fourth partial
 // (coalesced 128-byte rows), then the groups are combined in a fixed order.
@@ -628,7 +653,12 @@ void splitk_reduce(const double* W, double* C, long ldc, long sC, int M, int N, 
                    int nb, double beta, hipStream_t stream) {
   const long MN = (long)M * N;
   if (nb > 65535) throw std::invalid_argument("splitk_reduce: too many batches");
-  if (MN * nb <= kWideReduceMax) {
+  if (MN * nb <= kWideReduceMax && nsplit >= kDeepReduceSplits) {
+    const long waves = MN * nb;
+    hipLaunchKernelGGL(splitk_reduce_deep_kernel, dim3((unsigned)ceil_div(waves, 4)), dim3(256), 0, stream, W, C,
+                       ldc, sC, M, N, nsplit, batch0, nb, beta);
+    check_launch("splitk_reduce_deep");
+  } else if (MN * nb <= kWideReduceMax) {
     const long waves = ceil_div(MN, 16) * nb;
     hipLaunchKernelGGL(splitk_reduce_wide_kernel, dim3((unsigned)ceil_div(waves, 4)), dim3(256), 0,
                        stream, W, C, ldc, sC, M, N, nsplit, batch0, nb, beta);
