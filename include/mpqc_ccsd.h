@@ -214,7 +214,8 @@ int mpqc_ccsd_transform_df(mpqc_ccsd_ctx* ctx, int nbf, const double* ao_three_c
  * switches the context to the "df" residual.  Then MPQC_INT_XAI/XIJ/XAB are set
  * with mpqc_ccsd_set_integral (the fitting index uses these hashmarks in
  * MPQC_LAYOUT_TA_TILES buffers); Giabc and Gabcd are needed only with
- * reduced_abcd_memory = 0.  The df residual runs in MPQC_CCSD_PREC_F64 and F32. */
+ * reduced_abcd_memory = 0.  The df residual runs in every precision form; the 16-bit
+ * split forms need reduced_abcd_memory (the (ab|cd) blocks go through planes). */
 int mpqc_ccsd_set_df_basis(mpqc_ccsd_ctx* ctx, int naux, const int64_t* aux_hashmarks,
                            int n_aux_tiles);
 /* The AO integral-direct method ("direct", ccsd.h:73): the AO integrals (mu nu|la si)

@@ -755,8 +755,11 @@ void CCSDContext::prepare_planes() {
   }
   const std::array<long, 4> e_ijab{O, O, V, V};
   const std::array<int, 4> I{0, 1, 2, 3};
-  split(Gabcd_P_, {V, V, V, V}, {{Gabcd_, {V, V, V, V}, I, 1.0}});
-  split(Giabc_P_, {O, V, V, V}, {{Giabc_, {O, V, V, V}, I, 1.0}});
+  // (the df method has no Gabcd or Giabc: its ladder and Giabc-type terms come from the factors)
+  if (!df_) {
+    split(Gabcd_P_, {V, V, V, V}, {{Gabcd_, {V, V, V, V}, I, 1.0}});
+    split(Giabc_P_, {O, V, V, V}, {{Giabc_, {O, V, V, V}, I, 1.0}});
+  }
   split(Gijab_P_, {O, O, V, V}, {{Gijab_, e_ijab, I, 1.0}});
   // B^T operands of the J/K GEMMs: GA^T(k,c,l,d) = Gijab(k,l,c,d); GB^T = 2 Gijab(k,l,c,d) -
   // Gijab(k,l,d,c); GC^T(k,c,l,d) = Gijab(k,l,d,c)
@@ -767,13 +770,13 @@ void CCSDContext::prepare_planes() {
   // themselves; P8T(i,c,k,l) = P8(c,k,l,i)
   split(GbarT_P_, {V, V, O, O}, {{Gbar_, e_ijab, {2, 3, 0, 1}, 1.0}});
   split(Gbar_P_, e_ijab, {{Gbar_, e_ijab, I, 1.0}});
-  split(Gbar_akcd_P_, {V, O, V, V}, {{Gbar_akcd_, {V, O, V, V}, I, 1.0}});
+  if (!df_) split(Gbar_akcd_P_, {V, O, V, V}, {{Gbar_akcd_, {V, O, V, V}, I, 1.0}});
   split(P8T_P_, {O, V, O, O}, {{P8_, {V, O, O, O}, {3, 0, 1, 2}, 1.0}});
   // GiabcT(k,a,c,d) = Giabc(k,a,d,c), the A operand of the k-intermediate's T1 term
-  split(GiabcT_P_, {O, V, V, V}, {{Giabc_, {O, V, V, V}, {0, 1, 3, 2}, 1.0}});
+  if (!df_) split(GiabcT_P_, {O, V, V, V}, {{Giabc_, {O, V, V, V}, {0, 1, 3, 2}, 1.0}});
   // g_ki / g_ac operands (ccsd_r1_r2.h:219,223)
   split(P13_P_, {O, O, O, V}, {{P13_, {O, O, O, V}, I, 1.0}});
-  split(P14_P_, {V, V, O, V}, {{P14_, {V, V, O, V}, I, 1.0}});
+  if (!df_) split(P14_P_, {V, V, O, V}, {{P14_, {V, V, O, V}, I, 1.0}});
   MPQC_HIP_CHECK(hipStreamSynchronize(stream_));
   planes_ready_ = true;
 }
@@ -877,7 +880,6 @@ void CCSDContext::residual() {
   // amplitude layouts used by several GEMMs
   perm_add(t1T_, {1, 1, O, V}, 0.0, {{t1_, {1, 1, V, O}, {0, 1, 3, 2}, 1.0}});  // t1T(k,d) = t1(d,k)
   const bool sm = split_mode();
-  if (df_ && sm) throw std::logic_error("the df residual runs in the f64 and f32 forms");
   if (direct_ && sm) throw std::logic_error("the direct residual runs in the f64 and f32 forms");
   const long Q = naux_;
   if (df_) df_r1_intermediates();
@@ -1024,10 +1026,8 @@ void CCSDContext::residual() {
   gemm(false, false, v * v * o, o, o, 1.0, t2_abjk_, o, gki_, o, 0.0, X16_, o, 0, 0, 0, 1, "r2:t2_gki");
   perm_add(Rd_, {V, V, O, O}, 1.0, {{X16_, e_abij, {0, 1, 3, 2}, -1.0}});
   // :236 T(d,b,i,l) = 0.5 t2(d,b,i,l) + t1(d,i) t1(b,l), as TP(a,i,l,d) = T(d,a,i,l)
-  if (!sm) {
-    op_TP();
-    perm_add(t2P_, {V, O, O, V}, 0.0, {{t2_, e_abij, {0, 2, 3, 1}, 1.0}});  // t2P(a,i,l,d) = t2(a,d,i,l)
-  }
+  if (!sm || df_) op_TP();  // (the df j term reads TP in FP64)
+  if (!sm) perm_add(t2P_, {V, O, O, V}, 0.0, {{t2_, e_abij, {0, 2, 3, 1}, 1.0}});  // t2P(a,i,l,d) = t2(a,d,i,l)
   // :238-243 j(a,k,i,c) stored as J(a,i,k,c):
   //   Gijab(i,k,a,c) - Gijka(l,k,i,c) t1(a,l) + Giabc(k,a,c,d) t1(d,i)
   //   - Gijab(k,l,c,d) T(d,a,i,l) + 0.5 (2 Gijab(k,l,c,d) - Gijab(k,l,d,c)) t2(a,d,i,l)
@@ -1064,7 +1064,8 @@ void CCSDContext::residual() {
     const TProd pr = tp_product();
     split(TP_P_, e_aild, {{t2_, e_abij, {1, 2, 3, 0}, 0.5}}, &pr);
     split(t2P_P_, e_aild, {{t2_, e_abij, {0, 2, 3, 1}, 1.0}});
-    gemmP(o * v, o * v, o * v, -1.0, TP_P_, ov, GAt_P_, ov, 1.0, J_, ov, "r2:j_gT");
+    if (!df_)  // (df: the X T X form above)
+      gemmP(o * v, o * v, o * v, -1.0, TP_P_, ov, GAt_P_, ov, 1.0, J_, ov, "r2:j_gT");
     gemmP(o * v, o * v, o * v, 0.5, t2P_P_, ov, GBt_P_, ov, 1.0, J_, ov, "r2:j_gbar_t2");
     gemmP(o * v, o * v, o * v, -1.0, TP_P_, ov, GCt_P_, ov, 1.0, K_, ov, "r2:k_gT");
   } else {
@@ -1116,6 +1117,11 @@ void CCSDContext::residual() {
     // W^T(i,j,k,l) = W(k,l,i,j)
     split(WT_P_, {O, O, O, O}, {{W_, {O, O, O, O}, {2, 3, 0, 1}, 1.0}});
     gemmP(v * v, o * o, o * o, 1.0, tau_P_, oo, WT_P_, oo, 1.0, r2_, oo, "r2:hh_ladder");
+    if (df_ && reduced_abcd_) {
+      df_pp_ladder();  // df :497-508, the (ab|cd) blocks on 16-bit planes
+      return;
+    }
+    if (df_) throw std::logic_error("the split forms of the df method need reduced_abcd_memory");
     // :275 b(a,b,i,j) = tau(c,d,i,j) Gabcd(a,b,c,d)  -- the particle-particle ladder
     gemmP(v * v, o * o, v * v, 1.0, Gabcd_P_, vv, tauT_P_, vv, 1.0, r2_, oo, "r2:pp_ladder");
     // :277 tmp(k,a,i,j) = Giabc(k,a,c,d) tau(c,d,i,j)

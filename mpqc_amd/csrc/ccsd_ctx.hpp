@@ -237,6 +237,7 @@ class CCSDContext {
   Planes GbarT_P_, Gbar_P_, Gbar_akcd_P_, P8T_P_, GiabcT_P_, P13_P_, P14_P_;
   Planes tau_P_, tauT_P_, TP_P_, t2P_P_, Xjk_P_, K_P_, Y1t_P_, Y2t_P_, WT_P_, t1T_P_, gac_P_;
   Planes tau_ikcd_P_, tau_ackl_P_;
+  Planes Gp_P_;  // one (ab|cd) block of the df ladder
   void check_finite(const double* x, long n, const char* what);
   double scalar_from_device(const double* d);
 

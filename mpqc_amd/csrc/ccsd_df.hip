@@ -100,13 +100,22 @@ void CCSDContext::df_pp_ladder() {
   gemm(false, true, v, v, o, -1.0, t1_, o, Xai_, o, 1.0, Ydf_, v, 0, ov, vv, (int)Q, "r2:df_pp_Y");
   ensure_df_blocks();
   const long nbmax = df_block_;
+  const bool sm = split_mode();
+  if (sm && !Gp_P_.p[0]) Gp_P_ = alloc_planes(nbmax * V * vv);
   for (long a0 = 0; a0 < V; a0 += nbmax) {
     const long nb = std::min(nbmax, V - a0);
     gemm(true, false, (int)(nb * V), (int)vv, (int)Q, 1.0, Xab_ + a0 * V, vv, Ydf_, vv, 0.0, Gblk_, vv,
          0, 0, 0, 1, "r2:df_pp_abcd");
-    perm_add(Gp_, {nb, V, V, V}, 0.0, {{Gblk_, {nb, V, V, V}, {0, 2, 1, 3}, 1.0}});
-    gemm(false, false, (int)(nb * V), (int)oo, (int)vv, 1.0, Gp_, vv, tau_, oo, 0.0, bdf_ + a0 * V * oo, oo,
-         0, 0, 0, 1, "r2:df_pp_ladder");
+    if (sm) {
+      // the block's (a,b,c,d) layout straight into planes, contracted with the tau^T planes
+      split(Gp_P_, {nb, V, V, V}, {{Gblk_, {nb, V, V, V}, {0, 2, 1, 3}, 1.0}});
+      gemmP((int)(nb * V), (int)oo, (int)vv, 1.0, Gp_P_, vv, tauT_P_, vv, 0.0, bdf_ + a0 * V * oo, oo,
+            "r2:df_pp_ladder");
+    } else {
+      perm_add(Gp_, {nb, V, V, V}, 0.0, {{Gblk_, {nb, V, V, V}, {0, 2, 1, 3}, 1.0}});
+      gemm(false, false, (int)(nb * V), (int)oo, (int)vv, 1.0, Gp_, vv, tau_, oo, 0.0, bdf_ + a0 * V * oo, oo,
+           0, 0, 0, 1, "r2:df_pp_ladder");
+    }
   }
   perm_add(r2_, e_abij, 1.0, {{bdf_, e_abij, {0, 1, 2, 3}, 1.0}, {bdf_, e_abij, {1, 0, 3, 2}, 1.0}});
 }

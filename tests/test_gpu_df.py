@@ -42,6 +42,9 @@ def _rel(a, b):
     (2, 30, 45, 7, "ta", True, 1 << 20, "f64", 1e-13),   # 15 blocks of the lazy (ab|cd)
     (5, 19, 37, 24, "dense", False, 0, "f64", 1e-13),    # Gabcd / Giabc given
     (6, 21, 50, 5, "ta", True, 0, "f32", 2e-6),
+    (4, 13, 29, 4, "ta", True, 0, "f16x3", 2e-6),       # ladder blocks on 16-bit planes
+    (2, 30, 45, 7, "dense", True, 1 << 20, "f16x3", 2e-6),
+    (5, 19, 37, 24, "dense", True, 0, "bf16x3", 1e-3),
 ])
 def test_df_residual_matches_oracle(o, v, naux, block, layout, reduced, block_bytes, prec, tol):
     from mpqc_amd.ccsd import CCSD
@@ -85,6 +88,24 @@ def test_gpu_df_ccsd_trace_matches_reference(idx):
     assert abs(E - case["e_ccsd"]) < 2e-12
 
 
+def test_df_split_form_energy_water():
+    """DF-CCSD of the reference's water case in the f16x3 form against FP64 (1e-7 Eh,
+    the bound of the standard method's split form)."""
+    from mpqc_amd.ccsd import CCSD
+    case = FIX["df_cases"][0]
+    _, inp = _df_inputs(case)
+    E = {}
+    for prec in ("f64", "f16x3"):
+        with CCSD(inp.o, inp.v, precision=prec) as c:
+            c.set_df_basis(inp.naux)
+            c.set_integrals(inp.df_blocks())
+            c.set_orbital_energies(inp.base.eps_occ, inp.base.eps_vir)
+            E[prec] = c.compute_ccsd(target_precision=1e-10)
+    print(f"DF f16x3 {E['f16x3']:.12f} f64 {E['f64']:.12f} dE {E['f16x3'] - E['f64']:.2e}")
+    assert abs(E["f16x3"] - E["f64"]) < 1e-7
+    assert abs(E["f64"] - case["e_ccsd"]) < 2e-11
+
+
 def test_df_rejects_misuse():
     from mpqc_amd import _lib as L
     from mpqc_amd.ccsd import CCSD
@@ -94,12 +115,13 @@ def test_df_rejects_misuse():
         with pytest.raises(L.CCSDError):
             c.set_option("method", 1)                    # df without a fitting basis
     F, Xai, Xij, Xab, t1, t2 = _random_factors(3, 6, 8, seed=5)
-    ints = _df_ints(3, 6, F, Xai, Xij, Xab)
+    ints = _df_ints(3, 6, F, Xai, Xij, Xab, with_abcd=True)
     with CCSD(3, 6, precision="f16x3") as c:
         c.set_df_basis(8)
+        c.set_option("reduced_abcd_memory", 0)
         c.set_integrals(_blocks(ints))
         c.set_orbital_energies(np.diag(F)[:3], np.diag(F)[3:])
-        with pytest.raises(L.CCSDError):
+        with pytest.raises(L.CCSDError):                 # split forms: lazy (ab|cd) only
             c.compute_residuals(t1, t2)
     with CCSD(3, 6) as c:
         c.set_df_basis(8)
