@@ -1042,7 +1042,8 @@ def bench_eom(args):
         "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic molecule (seeded geometry), RHF + AO->MO on the host, converged FP64 CCSD",
         "config": {"workload": f"EOM-EE-CCSD singlets for {SYSTEMS[name]}", "o": o, "v": v,
-                   "n_roots": n_roots, "convergence": conv, "max_vector": 8, "parallelism": "single-gpu",
+                   "n_roots": n_roots, "convergence": conv, "max_vector": 8, "guess": "CIS",
+                   "parallelism": "single-gpu",
                    "E_CCSD": E, "singlets": [float(x) for x in roots], "jt_products": nprod,
                    "jt_product_ms": round(1e3 * per_product, 3), "jt_product_flops": flops_product},
         "roofline": {"bound": "mfma", "achieved": round(flops_product / per_product / 1e12, 3),

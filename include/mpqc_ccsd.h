@@ -180,10 +180,11 @@ int mpqc_ccsd_triples(mpqc_ccsd_ctx* ctx, double* e_t, double* seconds);
  * iteration count; verbose prints one line per iteration.  Options
  * "eom_max_vector" (subspace vectors per root, default 8), "eom_vector_threshold"
  * (norm below which a new vector is dropped, default 10 convergence) and
- * "eom_n_guess" (initial unit singles on the smallest FAB(a,a) - FIJ(i,i),
- * default 2 n_roots), "eom_extra_roots" (roots tracked above the requested ones, default 0),
- * "eom_cis_diagonal" (1: singles preconditioned and guessed with the CIS singlet diagonal
- * instead of FAB(a,a) - FIJ(i,i), default 0).  Runs in the f64 and f32 precision forms; the df method
+ * "eom_n_guess" (initial vectors, default 2 n_roots), "eom_extra_roots" (roots tracked above the requested ones, default 0),
+ * "eom_cis_diagonal" (1: singles preconditioned with the CIS singlet diagonal instead of
+ * FAB(a,a) - FIJ(i,i), which also orders the unit guesses; default 0), "eom_cis_guess" (1: the initial vectors are the
+ * eom_n_guess lowest CIS singlet eigenvectors, from a Davidson run on the CIS matrix built on the
+ * device, as the reference's CIS guess, eom_ccsd_impl.h:431-450; default 1; 0: unit singles).  Runs in the f64 and f32 precision forms; the df method
  * evaluates the same Jacobian on the four-index blocks its factors assemble
  * (Giabc and Gabcd are built on the device when not set). */
 int mpqc_ccsd_eom_ee(mpqc_ccsd_ctx* ctx, int n_roots, int max_iter, double convergence, int verbose,

@@ -250,6 +250,8 @@ void CCSDContext::set_option(const std::string& key, double value) {
     if (value < 0 || value > 64 || value != (int)value)
       throw std::invalid_argument("eom_extra_roots must be an integer in [0, 64]");
     eom_extra_roots_ = (int)value;
+  } else if (key == "eom_cis_guess") {
+    eom_cis_guess_ = value != 0.0;
   } else if (key == "eom_cis_diagonal") {
     eom_cis_diagonal_ = value != 0.0;
   } else if (key == "eom_skip_converged") {

@@ -93,6 +93,7 @@ class CCSDContext {
   int eom_extra_roots_ = 0;            // roots tracked above the requested ones
   bool eom_skip_converged_ = true;     // no new vectors for converged roots
   bool eom_cis_diagonal_ = false;      // singles preconditioned with the CIS singlet diagonal
+  bool eom_cis_guess_ = true;          // initial vectors from CIS (else unit singles)
   int eom_products_ = 0;               // J^T products of the last eom_ee
   double eom_product_seconds_ = 0.0;   // their device time (HIP events on the context stream)
 

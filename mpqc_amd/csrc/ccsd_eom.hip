@@ -75,6 +75,25 @@ __global__ __launch_bounds__(kEomBlock) void ee_precondition_kernel(double* __re
   }
 }
 
+// CIS singlet matrix A((a,i),(b,j)) (extents v, o, v, o): the Fock terms
+// A(ai,bi) += F(a,b) and A(ai,aj) -= F(i,j) (separate launches: the diagonal gets both)
+__global__ __launch_bounds__(kEomBlock) void cis_fock_vv_kernel(double* __restrict__ A, const double* __restrict__ Fab,
+                                                                int o, int v) {
+  const long total = (long)v * o * v;
+  for (long e = (long)blockIdx.x * kEomBlock + threadIdx.x; e < total; e += (long)gridDim.x * kEomBlock) {
+    const long b = e % v, ai = e / v, i = ai % o, a = ai / o;
+    A[((a * o + i) * v + b) * o + i] += Fab[a * v + b];
+  }
+}
+__global__ __launch_bounds__(kEomBlock) void cis_fock_oo_kernel(double* __restrict__ A, const double* __restrict__ Fij,
+                                                                int o, int v) {
+  const long total = (long)v * o * o;
+  for (long e = (long)blockIdx.x * kEomBlock + threadIdx.x; e < total; e += (long)gridDim.x * kEomBlock) {
+    const long j = e % o, ai = e / o, i = ai % o, a = ai / o;
+    A[((a * o + i) * v + a) * o + j] -= Fij[i * o + j];
+  }
+}
+
 // unit vector e_k
 __global__ void unit_vector_kernel(double* __restrict__ x, long k) { x[k] = 1.0; }
 
@@ -258,18 +277,20 @@ int CCSDContext::eom_ee(int n_roots, int max_iter, double convergence, int verbo
   SlotPool pool(2 * cap + 3 * nt + 1, n);
   double* scal = scalars_ + 16;
 
-  // out[k] = <xs[k], y>
-  auto dots = [&](const std::vector<const double*>& xs, const double* y, std::vector<double>& out) {
+  // out[k] = <xs[k], y> over len elements
+  auto dots = [&](const std::vector<const double*>& xs, const double* y, std::vector<double>& out, long len = 0) {
+    if (len == 0) len = n;
     out.assign(xs.size(), 0.0);
     for (size_t c0 = 0; c0 < xs.size(); c0 += 16) {
       const int m = (int)std::min<size_t>(16, xs.size() - c0);
-      multi_dot2(xs.data() + c0, xs.data() + c0, m, y, y, n, 0, scal, partials_, stream_);
+      multi_dot2(xs.data() + c0, xs.data() + c0, m, y, y, len, 0, scal, partials_, stream_);
       MPQC_HIP_CHECK(hipMemcpyAsync(out.data() + c0, scal, m * sizeof(double), hipMemcpyDeviceToHost, stream_));
     }
     MPQC_HIP_CHECK(hipStreamSynchronize(stream_));
   };
   // y = sum_k c_k xs[k] (y may be one of xs: every element is read before it is written)
-  auto combine = [&](double* y, const std::vector<const double*>& xs, const std::vector<double>& c) {
+  auto combine = [&](double* y, const std::vector<const double*>& xs, const std::vector<double>& c, long len = 0) {
+    if (len == 0) len = n;
     size_t k = 0;
     bool first = true;
     while (first || k < xs.size()) {
@@ -284,18 +305,22 @@ int CCSDContext::eom_ee(int n_roots, int max_iter, double convergence, int verbo
         pc.push_back(c[k]);
         ++k;
       }
-      lincomb(y, px.data(), pc.data(), (int)px.size(), n, stream_);
+      lincomb(y, px.data(), pc.data(), (int)px.size(), len, stream_);
       first = false;
     }
   };
-  auto norm = [&](const double* x) {
+  auto norm = [&](const double* x, long len = 0) {
     std::vector<double> d;
-    dots({x}, x, d);
+    dots({x}, x, d, len);
     return std::sqrt(std::max(d[0], 0.0));
   };
   // orthonormalise cand against basis and against each other, twice; vectors
   // whose norm after projection is below the threshold are dropped
-  auto gram_schmidt = [&](const std::vector<double*>& basis, std::vector<double*>& cand) {
+  auto gram_schmidt = [&](const std::vector<double*>& basis, std::vector<double*>& cand, long len = 0,
+                          SlotPool* pl = nullptr, double thresh = 0.0) {
+    if (len == 0) len = n;
+    if (!pl) pl = &pool;
+    if (thresh == 0.0) thresh = threshold;
     for (int pass = 0; pass < 2; ++pass) {
       std::vector<double*> kept;
       for (double* x : cand) {
@@ -303,31 +328,177 @@ int CCSDContext::eom_ee(int n_roots, int max_iter, double convergence, int verbo
         against.insert(against.end(), kept.begin(), kept.end());
         if (!against.empty()) {
           std::vector<double> c;
-          dots(against, x, c);
+          dots(against, x, c, len);
           for (double& e : c) e = -e;
           against.insert(against.begin(), x);
           c.insert(c.begin(), 1.0);
-          combine(x, against, c);
+          combine(x, against, c, len);
         }
-        const double nr = norm(x);
+        const double nr = norm(x, len);
         if (!std::isfinite(nr)) throw std::range_error("non-finite EOM vector");
-        if (nr < threshold) {
-          pool.put(x);
+        if (nr < thresh) {
+          pl->put(x);
           if (verbose) std::printf("  EOM: Gram-Schmidt drops a vector of norm %.3e\n", nr);
           continue;
         }
         const double* xs[1] = {x};
         const double s = 1.0 / nr;
-        lincomb(x, xs, &s, 1, n, stream_);
+        lincomb(x, xs, &s, 1, len, stream_);
         kept.push_back(x);
       }
       cand = kept;
     }
   };
 
-  // initial vectors: unit singles on the smallest singles' diagonal
+  // initial vectors: with eom_cis_guess the lowest CIS singlet vectors (as the reference,
+  // eom_ccsd_impl.h:431-450), else unit singles on the smallest singles' diagonal
   std::vector<double*> guess;
-  {
+  if (eom_cis_guess_) {
+    const long m = n1;
+    // A(ai,bj) = delta_ij F(a,b) - delta_ab F(i,j) + 2 <ij|ab> - <ia|jb>: the singles block of the
+    // Jacobian at t = 0 (symmetric)
+    double* A = nullptr;
+    if (hipMalloc(&A, (size_t)m * m * sizeof(double)) != hipSuccess) {
+      (void)hipGetLastError();
+      throw std::bad_alloc();
+    }
+    struct FreeA {
+      double* p;
+      ~FreeA() { (void)hipFree(p); }
+    } free_a{A};
+    {
+      const long e4[4] = {V, O, V, O}, eg[4] = {O, O, V, V}, ej[4] = {O, V, O, V};
+      const int pg[4] = {2, 0, 3, 1}, pj[4] = {1, 0, 3, 2};
+      TTerm ts[2];
+      ts[0].p = Gijab_;
+      permuted_strides(eg, pg, ts[0].s);
+      ts[0].alpha = 2.0;
+      ts[0].vol = n1 * n1;
+      ts[1].p = Giajb_;
+      permuted_strides(ej, pj, ts[1].s);
+      ts[1].alpha = -1.0;
+      ts[1].vol = n1 * n1;
+      tadd(A, e4, 0.0, ts, 2, stream_);
+      const int g1 = (int)std::max<long>(1, std::min<long>(ceil_div(V * O * V, kEomBlock), 16384));
+      hipLaunchKernelGGL(cis_fock_vv_kernel, dim3(g1), dim3(kEomBlock), 0, stream_, A, Fab_, (int)O, (int)V);
+      const int g2 = (int)std::max<long>(1, std::min<long>(ceil_div(V * O * O, kEomBlock), 16384));
+      hipLaunchKernelGGL(cis_fock_oo_kernel, dim3(g2), dim3(kEomBlock), 0, stream_, A, Fij_, (int)O, (int)V);
+      check_launch("cis_fock");
+    }
+    double* dA = adj_hist(112, m);
+    MPQC_HIP_CHECK(hipMemcpy2DAsync(dA, sizeof(double), A, (m + 1) * sizeof(double), sizeof(double), m,
+                                    hipMemcpyDeviceToDevice, stream_));
+    std::vector<double> hdA(m);
+    MPQC_HIP_CHECK(hipMemcpyAsync(hdA.data(), dA, m * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    MPQC_HIP_CHECK(hipStreamSynchronize(stream_));
+    // symmetric Davidson on A for the n_guess lowest roots, to residual norms of 1e-5
+    const int ng = n_guess, ccap = std::max(12 * ng, 2 * ng + 2);
+    SlotPool cpool(2 * ccap + 2 * ng + 1, m);
+    std::vector<double*> cb, ch, cnew;
+    {
+      std::vector<std::pair<double, long>> ord;
+      for (long e = 0; e < m; ++e) ord.push_back({hdA[e], e});
+      std::stable_sort(ord.begin(), ord.end(),
+                       [](const std::pair<double, long>& x, const std::pair<double, long>& y) { return x.first < y.first; });
+      for (int g = 0; g < ng; ++g) {
+        double* x = cpool.get();
+        fill(x, m, 0.0, stream_);
+        hipLaunchKernelGGL(unit_vector_kernel, dim3(1), dim3(1), 0, stream_, x, ord[g].second);
+        check_launch("unit_vector");
+        cnew.push_back(x);
+      }
+    }
+    std::vector<double> cG((size_t)ccap * ccap, 0.0);
+    std::vector<double*> cx;
+    const int cgrid = (int)std::max<long>(1, std::min<long>(ceil_div(m, kEomBlock), 16384));
+    for (int it = 0; it < 100 && !cnew.empty(); ++it) {
+      const int ns = (int)cb.size();
+      for (double* b : cnew) {
+        double* h = cpool.get();
+        mpqc_hip::gemm(make_gemm(false, false, (int)m, 1, (int)m, 1.0, A, m, b, 1, 0.0, h, 1), GemmPrecision::F64,
+                       ws_, stream_);
+        cb.push_back(b);
+        ch.push_back(h);
+      }
+      cnew.clear();
+      const int nv = (int)cb.size();
+      for (int ii = ns; ii < nv; ++ii) {
+        std::vector<double> row;
+        dots(std::vector<const double*>(ch.begin(), ch.begin() + ii + 1), cb[ii], row, m);
+        for (int j = 0; j <= ii; ++j) cG[(size_t)ii * ccap + j] = cG[(size_t)j * ccap + ii] = row[j];
+      }
+      std::vector<double> Gs((size_t)nv * nv);
+      for (int i = 0; i < nv; ++i)
+        for (int j = 0; j < nv; ++j) Gs[(size_t)i * nv + j] = cG[(size_t)i * ccap + j];
+      const std::vector<std::complex<double>> w = small_eigenvalues(nv, Gs);
+      std::vector<int> order(nv);
+      for (int k = 0; k < nv; ++k) order[k] = k;
+      std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return w[a].real() < w[b].real(); });
+      for (double* x : cx) cpool.put(x);
+      cx.clear();
+      std::vector<double*> cres;
+      double worst = 0.0;
+      for (int r = 0; r < ng && r < nv; ++r) {
+        const double theta = w[order[r]].real();
+        const std::vector<std::complex<double>> cv = small_eigenvector(nv, Gs, w[order[r]]);
+        std::vector<double> c(nv);
+        double cn = 0.0;
+        for (int j = 0; j < nv; ++j) {
+          c[j] = cv[j].real();
+          cn += c[j] * c[j];
+        }
+        cn = std::sqrt(cn);
+        for (double& e : c) e /= cn;
+        double* x = cpool.get();
+        combine(x, std::vector<const double*>(cb.begin(), cb.end()), c, m);
+        cx.push_back(x);
+        double* rr = cpool.get();
+        std::vector<const double*> hx(ch.begin(), ch.end());
+        std::vector<double> hc(c);
+        hx.push_back(x);
+        hc.push_back(-theta);
+        combine(rr, hx, hc, m);
+        const double nr = norm(rr, m);
+        worst = std::max(worst, nr);
+        double* p = cpool.get();
+        hipLaunchKernelGGL(ee_precondition_kernel, dim3(cgrid), dim3(kEomBlock), 0, stream_, p, rr, dg, dg + O, dA,
+                           (int)O, (int)V, theta, m, m);
+        check_launch("cis_precondition");
+        cpool.put(rr);
+        cres.push_back(p);
+      }
+      if (verbose) std::printf("  CIS %3d  subspace %3d  max|r| %.3e\n", it + 1, nv, worst);
+      if (worst < 1e-5) {
+        for (double* p : cres) cpool.put(p);
+        break;
+      }
+      if (nv + (int)cres.size() > ccap) {  // restart from the Ritz vectors and their corrections
+        for (double* b : cb) cpool.put(b);
+        for (double* h : ch) cpool.put(h);
+        cb.clear();
+        ch.clear();
+        std::fill(cG.begin(), cG.end(), 0.0);
+        for (double* x : cx) {
+          double* g = cpool.get();
+          MPQC_HIP_CHECK(hipMemcpyAsync(g, x, m * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+          cnew.push_back(g);
+        }
+        cnew.insert(cnew.end(), cres.begin(), cres.end());
+        gram_schmidt({}, cnew, m, &cpool, 1e-10);
+      } else {
+        cnew = cres;
+        gram_schmidt(cb, cnew, m, &cpool, 1e-10);
+      }
+    }
+    // the Ritz vectors as the singles of the first EOM vectors
+    for (double* x : cx) {
+      double* g = pool.get();
+      fill(g, n, 0.0, stream_);
+      MPQC_HIP_CHECK(hipMemcpyAsync(g, x, m * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+      guess.push_back(g);
+    }
+    gram_schmidt({}, guess);
+  } else {
     std::vector<std::pair<double, long>> order1;
     for (long e = 0; e < n1; ++e) order1.push_back({hd1[e], e});
     std::stable_sort(order1.begin(), order1.end(),

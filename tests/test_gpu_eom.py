@@ -24,20 +24,23 @@ def _converged(c, blocks, eo, ev, prec=1e-12):
     return c.compute_ccsd(target_precision=prec)
 
 
-@pytest.mark.parametrize("max_vector", [8, 3])
-def test_eom_singlets_standard_water(max_vector):
+@pytest.mark.parametrize("max_vector,cis_guess", [(8, 0), (3, 0), (8, 1)])
+def test_eom_singlets_standard_water(max_vector, cis_guess):
     from mpqc_amd.ccsd import CCSD
     case = FIX["eom_cases"][0]
     gcase = FIX["cases"][0]
     assert case["basis"] == gcase["basis"] and case["frozen_core"] == gcase["frozen_core"]
     _, inp = _inputs(gcase)
     n = len(case["singlets"])
-    print(f"water {case['basis']} o={inp.o} v={inp.v} roots={n} max_vector={max_vector}", flush=True)
+    print(f"water {case['basis']} o={inp.o} v={inp.v} roots={n} max_vector={max_vector} cis_guess={cis_guess}",
+          flush=True)
     with CCSD(inp.o, inp.v, precision="f64", occ_block_size=2, unocc_block_size=3, max_iter=100) as c:
         _converged(c, inp.blocks, inp.eps_occ, inp.eps_vir)
         c.set_option("eom_max_vector", max_vector)
+        c.set_option("eom_cis_guess", cis_guess)
         w, it = c.eom_ee(n, target_precision=1e-10, max_iter=200)
-    print("GPU", w, "iterations", it)
+        nprod, _ = c.eom_stats()
+    print("GPU", w, "iterations", it, "products", nprod)
     print("reference", case["singlets"])
     assert np.all(np.diff(w) >= 0)
     assert np.abs(w - np.array(case["singlets"])).max() < TOL_REF
@@ -68,7 +71,8 @@ def test_eom_singlets_df_water_and_context_restored():
     assert np.abs(r1).max() < 1e-9 and np.abs(r2).max() < 1e-9
 
 
-def test_eom_matches_oracle_random_hamiltonian():
+@pytest.mark.parametrize("cis_guess", [0, 1])
+def test_eom_matches_oracle_random_hamiltonian(cis_guess):
     from oracle import ccsd as O
     from oracle import eom as EO
     from mpqc_amd.ccsd import CCSD
@@ -80,6 +84,7 @@ def test_eom_matches_oracle_random_hamiltonian():
     with CCSD(no, nv, precision="f64", max_iter=200) as c:
         _converged(c, b, eo, ev, prec=1e-14)
         t1, t2 = c.get_amplitudes()
+        c.set_option("eom_cis_guess", cis_guess)
         w, it = c.eom_ee(3, target_precision=1e-10, max_iter=200)
     ref = EO.eom_ee_singlets(EO.standard_residual(O.Integrals(**b)), t1, t2, 3)
     print("GPU", w, "oracle", ref, "iterations", it)
