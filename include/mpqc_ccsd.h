@@ -151,7 +151,11 @@ const char* mpqc_ccsd_last_error(const mpqc_ccsd_ctx* ctx);
  * "method" (0 standard, 1 df, 2 direct, 3 direct_df; ccsd.h:73, 94-99), "reduced_abcd_memory" (1 default:
  * the df method builds (ab|cd) lazily from Xab instead of reading Gabcd and
  * Giabc, ccsd.h:102,292), "df_block_bytes" (device memory for the lazily built
- * (ab|cd) blocks, default 4 GiB; set before the first residual).  Tuning keys that act on
+ * (ab|cd) blocks, default 4 GiB; set before the first residual), "pp_symmetric" (1 default:
+ * the particle-particle ladder of ccsd_r1_r2.h:275 as two GEMMs over the symmetric and
+ * antisymmetric (c,d) pair combinations, rows a<=b, with half the FLOPs, exact for any
+ * amplitudes given Gabcd(b,a,d,c) = Gabcd(a,b,c,d); the adjoint's recorded residual keeps the
+ * full contraction; 0: the full contraction).  Tuning keys that act on
  * every context of the process (measurement aids; the defaults are the measured best):
  * "plane_tile" (tile of the 16-bit plane contractions, rows * 1000 + columns, 0 automatic),
  * "plane_lds_stages" (1 or 2), "plane_tile_order" (0 XCD ranges of row-major tiles, 1 plain,

@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 OBJDIR = os.path.join(HERE, "lib", "obj")
 LIBNAME = "libmpqc_ccsd.so"
-SOURCES = ["gemm.hip", "gemm_split.hip", "planes.hip", "tensor_ops.hip", "ccsd_ctx.hip", "ccsd_df.hip", "ccsd_direct.hip", "ccsd_transform.hip", "ccsd_triples.hip", "ccsd_adjoint.hip", "ccsd_eom.hip", "batch.hip", "abi.cpp"]
+SOURCES = ["gemm.hip", "gemm_split.hip", "planes.hip", "tensor_ops.hip", "ccsd_ctx.hip", "ccsd_df.hip", "ccsd_direct.hip", "ccsd_ppsym.hip", "ccsd_transform.hip", "ccsd_triples.hip", "ccsd_adjoint.hip", "ccsd_eom.hip", "batch.hip", "abi.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 CFLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

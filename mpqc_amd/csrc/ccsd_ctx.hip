@@ -250,6 +250,9 @@ void CCSDContext::set_option(const std::string& key, double value) {
     if (value < 0 || value > 64 || value != (int)value)
       throw std::invalid_argument("eom_extra_roots must be an integer in [0, 64]");
     eom_extra_roots_ = (int)value;
+  } else if (key == "pp_symmetric") {
+    pp_symmetric_ = value != 0.0;
+    planes_ready_ = false;
   } else if (key == "eom_cis_guess") {
     eom_cis_guess_ = value != 0.0;
   } else if (key == "eom_cis_diagonal") {
@@ -276,6 +279,7 @@ void CCSDContext::set_option(const std::string& key, double value) {
         for (auto& pa : plane_allocs_) hipFree(pa.first);
         plane_allocs_.clear();
         Gabcd_P_ = Planes();
+        Vp_P_ = Planes();
       }
     }
   } else {
@@ -749,6 +753,9 @@ void CCSDContext::prepare_planes() {
                              {o * v * v, o, v},     {v, v * o * o, v},     {v * v, 1, o * v},
                              {o * o, 1, o * v}};
     for (auto& sh : shapes) need = std::max(need, gemm_planes_workspace_bytes(sh[0], sh[1], sh[2], 1));
+    // the pair-symmetric ladder: v(v+1)/2 x o^2 x (v(v+1)/2 padded to 64)
+    const int nvs = v * (v + 1) / 2;
+    need = std::max(need, gemm_planes_workspace_bytes(nvs, o * o, (nvs + 63) / 64 * 64, 1));
     if (need > pws_bytes_) {
       const size_t n = (need + sizeof(double) - 1) / sizeof(double);
       pws_ = alloc(n);
@@ -759,7 +766,11 @@ void CCSDContext::prepare_planes() {
   const std::array<int, 4> I{0, 1, 2, 3};
   // (the df method has no Gabcd or Giabc: its ladder and Giabc-type terms come from the factors)
   if (!df_) {
-    split(Gabcd_P_, {V, V, V, V}, {{Gabcd_, {V, V, V, V}, I, 1.0}});
+    pp_sym_planes_ = pp_symmetric_usable();
+    if (pp_sym_planes_)
+      prepare_pp_symmetric();
+    else
+      split(Gabcd_P_, {V, V, V, V}, {{Gabcd_, {V, V, V, V}, I, 1.0}});
     split(Giabc_P_, {O, V, V, V}, {{Giabc_, {O, V, V, V}, I, 1.0}});
   }
   split(Gijab_P_, {O, O, V, V}, {{Gijab_, e_ijab, I, 1.0}});
@@ -854,6 +865,8 @@ void CCSDContext::prepare() {
   MPQC_HIP_CHECK(hipStreamSynchronize(stream_));
   prepared_ = true;
   planes_ready_ = false;
+  pp_packed_ = false;
+  pp_checked_ = false;
 }
 
 void CCSDContext::init_guess() {
@@ -1125,7 +1138,10 @@ void CCSDContext::residual() {
     }
     if (df_) throw std::logic_error("the split forms of the df method need reduced_abcd_memory");
     // :275 b(a,b,i,j) = tau(c,d,i,j) Gabcd(a,b,c,d)  -- the particle-particle ladder
-    gemmP(v * v, o * o, v * v, 1.0, Gabcd_P_, vv, tauT_P_, vv, 1.0, r2_, oo, "r2:pp_ladder");
+    if (pp_sym_planes_)
+      pp_ladder_symmetric();
+    else
+      gemmP(v * v, o * o, v * v, 1.0, Gabcd_P_, vv, tauT_P_, vv, 1.0, r2_, oo, "r2:pp_ladder");
     // :277 tmp(k,a,i,j) = Giabc(k,a,c,d) tau(c,d,i,j)
     gemmP(o * v, o * o, v * v, 1.0, Giabc_P_, vv, tauT_P_, vv, 0.0, tmp24_, oo, "r2:pp_t1_giabc_tau");
   } else {
@@ -1139,8 +1155,12 @@ void CCSDContext::residual() {
       df_pp_ladder();  // df :497-508
       return;
     }
-    // :275 b(a,b,i,j) = tau(c,d,i,j) Gabcd(a,b,c,d)  -- the particle-particle ladder
-    gemm(false, false, v * v, o * o, v * v, 1.0, Gabcd_, vv, tau_, oo, 1.0, r2_, oo, 0, 0, 0, 1, "r2:pp_ladder");
+    // :275 b(a,b,i,j) = tau(c,d,i,j) Gabcd(a,b,c,d)  -- the particle-particle ladder (the
+    // recorded residual of the adjoint keeps the full contraction)
+    if (!tape_ && pp_symmetric_usable())
+      pp_ladder_symmetric();
+    else
+      gemm(false, false, v * v, o * o, v * v, 1.0, Gabcd_, vv, tau_, oo, 1.0, r2_, oo, 0, 0, 0, 1, "r2:pp_ladder");
     // :277 tmp(k,a,i,j) = Giabc(k,a,c,d) tau(c,d,i,j)
     gemm(false, false, o * v, o * o, v * v, 1.0, Giabc_, vv, tau_, oo, 0.0, tmp24_, oo, 0, 0, 0, 1, "r2:pp_t1_giabc_tau");
   }

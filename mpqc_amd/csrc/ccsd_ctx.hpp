@@ -94,6 +94,7 @@ class CCSDContext {
   bool eom_skip_converged_ = true;     // no new vectors for converged roots
   bool eom_cis_diagonal_ = false;      // singles preconditioned with the CIS singlet diagonal
   bool eom_cis_guess_ = true;          // initial vectors from CIS (else unit singles)
+  bool pp_symmetric_ = true;           // split forms: pp ladder on (ab)/(cd) pair-symmetric blocks
   int eom_products_ = 0;               // J^T products of the last eom_ee
   double eom_product_seconds_ = 0.0;   // their device time (HIP events on the context stream)
 
@@ -168,6 +169,9 @@ class CCSDContext {
   void direct_u();   // U at the current tau (ccsd.h:581-596)
   void direct_r1();  // ccsd_r1_r2.h:116-117
   void direct_r2();  // ccsd_r1_r2.h:282-286
+  void pack_pp_symmetric();     // packed V+ / V- (FP64) of the pair-symmetric ladder
+  void prepare_pp_symmetric();  // and their planes (split forms)
+  void pp_ladder_symmetric();   // ccsd_r1_r2.h:275 on pair-symmetric blocks (split forms)
 
   // ---- op tape: integral preparation and the residual as a list of primitive
   // ops (copies, permuted linear combinations, contractions, tau/T builds).
@@ -235,6 +239,16 @@ class CCSDContext {
   size_t pws_bytes_ = 0;
   std::vector<std::pair<uint16_t*, size_t>> plane_allocs_;
   Planes Gabcd_P_, Giabc_P_, Gijab_P_, GAt_P_, GBt_P_, GCt_P_;
+  // pair-symmetric pp ladder: pair counts, padded row pitches, pair tables, packed operands
+  long nvs_ = 0, nva_ = 0, ldvs_ = 0, ldva_ = 0;
+  bool pp_packed_ = false;  // V+ / V- packed from the current Gabcd
+  bool pp_checked_ = false, pp_sym_ok_ = false;  // Gabcd's pair symmetry checked / holds
+  bool pp_sym_planes_ = false;                   // the split forms' planes are the packed ones
+  bool pp_symmetric_usable();
+  int2 *pv_sym_ = nullptr, *pv_anti_ = nullptr;
+  double *Vpk_ = nullptr, *Vmk_ = nullptr, *Spk_ = nullptr, *Apk_ = nullptr, *Ppk_ = nullptr, *Mpk_ = nullptr,
+         *tauTd_ = nullptr;
+  Planes Vp_P_, Vm_P_, Sp_P_, Ap_P_;
   Planes GbarT_P_, Gbar_P_, Gbar_akcd_P_, P8T_P_, GiabcT_P_, P13_P_, P14_P_;
   Planes tau_P_, tauT_P_, TP_P_, t2P_P_, Xjk_P_, K_P_, Y1t_P_, Y2t_P_, WT_P_, t1T_P_, gac_P_;
   Planes tau_ikcd_P_, tau_ackl_P_;

@@ -1,0 +1,279 @@
+// Particle-particle ladder of the split forms on pair-symmetric blocks.
+//
+// b(a,b,i,j) = sum_cd Gabcd(a,b,c,d) tau(c,d,i,j) (ccsd_r1_r2.h:275).  With the
+// integrals' symmetry Gabcd(b,a,d,c) = Gabcd(a,b,c,d) (both are (ac|bd)) and no
+// assumption on tau:
+//   b(a,b,i,j) = 1/2 [P(ab,ij) + s(a,b) M(ab,ij)]
+//   P(ab,ij) = sum_{c<=d} V+(ab,cd) S(ij,cd),   V+ = G(ab,cd) + G(ab,dc),
+//              S(ij,cd) = f_cd [tau(c,d,i,j) + tau(d,c,i,j)],  f_cc = 1/2, else 1
+//   M(ab,ij) = sum_{c<d} V-(ab,cd) A(ij,cd),    V- = G(ab,cd) - G(ab,dc),
+//              A(ij,cd) = tau(c,d,i,j) - tau(d,c,i,j)
+// with V+ symmetric and V- antisymmetric under a<->b, so rows a<=b (P) and a<b (M)
+// carry everything; s(a,b) = +1 for a<b, -1 for a>b.  The two GEMMs
+// (v(v+1)/2 x o^2 x v(v+1)/2 and v(v-1)/2 x o^2 x v(v-1)/2) hold half the FLOPs of
+// the full contraction (2 o^2 v^4).
+//
+// Pairs are numbered row-major over the upper triangle: (p,q), p<=q < n has index
+// p n - p(p-1)/2 + q - p; (p,q), p<q < n has index p (n-1) - p(p-1)/2 + q - p - 1.
+// The packed operands' rows are padded with zeros to a multiple of 64 columns (the
+// plane GEMM's vector path wants K and the row pitch in multiples of 8).
+#include <algorithm>
+#include <stdexcept>
+#include <vector>
+
+#include "ccsd_ctx.hpp"
+
+namespace mpqc_hip {
+
+namespace {
+
+constexpr int kSymBlock = 256;
+
+__device__ __forceinline__ long sym_pair(long p, long q, long n) { return p * n - p * (p - 1) / 2 + (q - p); }
+__device__ __forceinline__ long anti_pair(long p, long q, long n) {
+  return p * (n - 1) - p * (p - 1) / 2 + (q - p - 1);
+}
+
+// out(R, C) = G(a,b,c,d) + sign G(a,b,d,c) for row pair R = (a,b), column pair
+// C = (c,d) (tables of packed (p, q) pairs), zero in the padding columns
+__global__ __launch_bounds__(kSymBlock) void pack_vvvv_kernel(double* __restrict__ out,
+                                                              const double* __restrict__ G, long v,
+                                                              const int2* __restrict__ pairs, long npairs,
+                                                              long ld, double sign) {
+  const long total = npairs * ld;
+  for (long e = (long)blockIdx.x * kSymBlock + threadIdx.x; e < total; e += (long)gridDim.x * kSymBlock) {
+    const long R = e / ld, C = e - R * ld;
+    double x = 0.0;
+    if (C < npairs) {
+      const int2 ab = pairs[R], cd = pairs[C];
+      const long row = ((long)ab.x * v + ab.y) * v;
+      x = G[(row + cd.x) * v + cd.y] + sign * G[(row + cd.y) * v + cd.x];
+    }
+    out[e] = x;
+  }
+}
+
+// out(ij, C) = f [tauT(i,j,c,d) + sign tauT(i,j,d,c)] for column pair C = (c,d);
+// f = 1/2 on the diagonal c = d (sign +1 only), zero in the padding columns.
+// tauT(i,j,c,d) = tau(c,d,i,j).
+__global__ __launch_bounds__(kSymBlock) void pack_tau_kernel(double* __restrict__ out,
+                                                             const double* __restrict__ tauT, long oo, long v,
+                                                             const int2* __restrict__ cols, long ncols, long ld,
+                                                             double sign) {
+  const long total = oo * ld, vv = v * v;
+  for (long e = (long)blockIdx.x * kSymBlock + threadIdx.x; e < total; e += (long)gridDim.x * kSymBlock) {
+    const long R = e / ld, C = e - R * ld;
+    double x = 0.0;
+    if (C < ncols) {
+      const int2 cd = cols[C];
+      const double* row = tauT + R * vv;
+      x = row[(long)cd.x * v + cd.y] + sign * row[(long)cd.y * v + cd.x];
+      if (cd.x == cd.y) x *= 0.5;
+    }
+    out[e] = x;
+  }
+}
+
+// out(C, ij) = f [tau(c,d,i,j) + sign tau(d,c,i,j)] for column pair C = (c,d): the
+// K x N operand of the FP64 form's NN GEMM, read straight from tau(a,b,i,j)
+__global__ __launch_bounds__(kSymBlock) void pack_tau_rows_kernel(double* __restrict__ out,
+                                                                  const double* __restrict__ tau, long oo, long v,
+                                                                  const int2* __restrict__ pairs, long npairs,
+                                                                  double sign) {
+  const long total = npairs * oo;
+  for (long e = (long)blockIdx.x * kSymBlock + threadIdx.x; e < total; e += (long)gridDim.x * kSymBlock) {
+    const long C = e / oo, ij = e - C * oo;
+    const int2 cd = pairs[C];
+    double x = tau[((long)cd.x * v + cd.y) * oo + ij] + sign * tau[((long)cd.y * v + cd.x) * oo + ij];
+    if (cd.x == cd.y) x *= 0.5;
+    out[e] = x;
+  }
+}
+
+// r2(a,b,i,j) += 1/2 [P(ab,ij) + s(a,b) M(ab,ij)]
+__global__ __launch_bounds__(kSymBlock) void unpack_ladder_kernel(double* __restrict__ r2,
+                                                                  const double* __restrict__ P,
+                                                                  const double* __restrict__ M, long oo,
+                                                                  long v) {
+  const long total = v * v * oo;
+  for (long e = (long)blockIdx.x * kSymBlock + threadIdx.x; e < total; e += (long)gridDim.x * kSymBlock) {
+    const long ij = e % oo, ab = e / oo, b = ab % v, a = ab / v;
+    const long p = a < b ? a : b, q = a < b ? b : a;
+    double x = P[sym_pair(p, q, v) * oo + ij];
+    if (a != b) {
+      const double m = M[anti_pair(p, q, v) * oo + ij];
+      x += a < b ? m : -m;
+    }
+    r2[e] += 0.5 * x;
+  }
+}
+
+// per block: max |G(a,b,c,d) - G(b,a,d,c)| and max |G(a,b,c,d)| -> part[2 blk], part[2 blk + 1]
+__global__ __launch_bounds__(kSymBlock) void pair_asymmetry_kernel(double* __restrict__ part,
+                                                                   const double* __restrict__ G, long v) {
+  __shared__ double sd[kSymBlock], sg[kSymBlock];
+  const long total = v * v * v * v;
+  double md = 0.0, mg = 0.0;
+  for (long e = (long)blockIdx.x * kSymBlock + threadIdx.x; e < total; e += (long)gridDim.x * kSymBlock) {
+    const long d = e % v, t0 = e / v, c = t0 % v, t1 = t0 / v, b = t1 % v, a = t1 / v;
+    const double x = G[e];
+    md = fmax(md, fabs(x - G[((b * v + a) * v + d) * v + c]));
+    mg = fmax(mg, fabs(x));
+  }
+  sd[threadIdx.x] = md;
+  sg[threadIdx.x] = mg;
+  __syncthreads();
+  for (int w = kSymBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      sd[threadIdx.x] = fmax(sd[threadIdx.x], sd[threadIdx.x + w]);
+      sg[threadIdx.x] = fmax(sg[threadIdx.x], sg[threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = sd[0];
+    part[2 * blockIdx.x + 1] = sg[0];
+  }
+}
+
+int sym_grid(long n) { return (int)std::max<long>(1, std::min<long>(ceil_div(n, kSymBlock), 65536)); }
+
+std::vector<int2> pair_table(long n, bool strict) {
+  std::vector<int2> t;
+  for (long p = 0; p < n; ++p)
+    for (long q = strict ? p + 1 : p; q < n; ++q) t.push_back(make_int2((int)p, (int)q));
+  return t;
+}
+
+long pad64(long n) { return std::max<long>(64, (n + 63) / 64 * 64); }
+
+}  // namespace
+
+// whether Gabcd has the (ab)(cd) pair symmetry the form relies on, to 1e-12 of its
+// largest element (integrals from a transform hold it to round-off; a perturbation
+// that breaks it sends the ladder back to the full contraction)
+bool CCSDContext::pp_symmetric_usable() {
+  if (!pp_symmetric_ || !Gabcd_) return false;
+  if (pp_checked_) return pp_sym_ok_;
+  const long V = v_;
+  const int blocks = (int)std::max<long>(1, std::min<long>(ceil_div(V * V * V * V, kSymBlock), kReducePartials / 2));
+  hipLaunchKernelGGL(pair_asymmetry_kernel, dim3(blocks), dim3(kSymBlock), 0, stream_, partials_, Gabcd_, V);
+  check_launch("pair_asymmetry");
+  std::vector<double> h(2 * (size_t)blocks);
+  MPQC_HIP_CHECK(hipMemcpyAsync(h.data(), partials_, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
+  MPQC_HIP_CHECK(hipStreamSynchronize(stream_));
+  double md = 0.0, mg = 0.0;
+  for (int b = 0; b < blocks; ++b) {
+    md = std::max(md, h[2 * b]);
+    mg = std::max(mg, h[2 * b + 1]);
+  }
+  pp_sym_ok_ = md <= 1e-12 * mg;
+  pp_checked_ = true;
+  return pp_sym_ok_;
+}
+
+// pair tables, packed V+ / V- (FP64) and the per-iteration buffers
+void CCSDContext::pack_pp_symmetric() {
+  const long O = o_, V = v_, oo = O * O;
+  nvs_ = V * (V + 1) / 2;
+  nva_ = V * (V - 1) / 2;
+  ldvs_ = pad64(nvs_);
+  ldva_ = pad64(nva_);
+  if (!pv_sym_) {
+    auto upload_pairs = [&](long n, bool strict) {
+      const std::vector<int2> t = pair_table(n, strict);
+      int2* d = reinterpret_cast<int2*>(alloc(std::max<size_t>(t.size(), 1)));
+      if (!t.empty())
+        MPQC_HIP_CHECK(hipMemcpyAsync(d, t.data(), t.size() * sizeof(int2), hipMemcpyHostToDevice, stream_));
+      return d;
+    };
+    pv_sym_ = upload_pairs(V, false);
+    pv_anti_ = upload_pairs(V, true);
+    Vpk_ = alloc((size_t)nvs_ * ldvs_);
+    Vmk_ = alloc((size_t)std::max<long>(nva_, 1) * ldva_);
+    Spk_ = alloc((size_t)oo * ldvs_);
+    Apk_ = alloc((size_t)oo * ldva_);
+    Ppk_ = alloc((size_t)nvs_ * oo);
+    Mpk_ = alloc((size_t)std::max<long>(nva_, 1) * oo);
+    tauTd_ = alloc((size_t)oo * V * V);
+  }
+  hipLaunchKernelGGL(pack_vvvv_kernel, dim3(sym_grid(nvs_ * ldvs_)), dim3(kSymBlock), 0, stream_, Vpk_, Gabcd_, V,
+                     pv_sym_, nvs_, ldvs_, 1.0);
+  check_launch("pack_vvvv");
+  if (nva_ > 0) {
+    hipLaunchKernelGGL(pack_vvvv_kernel, dim3(sym_grid(nva_ * ldva_)), dim3(kSymBlock), 0, stream_, Vmk_, Gabcd_, V,
+                       pv_anti_, nva_, ldva_, -1.0);
+    check_launch("pack_vvvv");
+  }
+  pp_packed_ = true;
+}
+
+// the split forms' V+ / V- planes
+void CCSDContext::prepare_pp_symmetric() {
+  pack_pp_symmetric();
+  const long oo = (long)o_ * o_;
+  if (!Vp_P_.p[0]) {  // (re-allocated after a change of the plane kind)
+    Vp_P_ = alloc_planes(nvs_ * ldvs_);
+    Vm_P_ = alloc_planes(std::max<long>(nva_, 1) * ldva_);
+    Sp_P_ = alloc_planes(oo * ldvs_);
+    Ap_P_ = alloc_planes(oo * ldva_);
+  }
+  const std::array<int, 4> I{0, 1, 2, 3};
+  split(Vp_P_, {1, 1, nvs_, ldvs_}, {{Vpk_, {1, 1, nvs_, ldvs_}, I, 1.0}});
+  if (nva_ > 0) split(Vm_P_, {1, 1, nva_, ldva_}, {{Vmk_, {1, 1, nva_, ldva_}, I, 1.0}});
+}
+
+// :275 b(a,b,i,j) = tau(c,d,i,j) Gabcd(a,b,c,d), added to r2
+void CCSDContext::pp_ladder_symmetric() {
+  const long O = o_, V = v_, oo = O * O;
+  const std::array<long, 4> e_abij{V, V, O, O};
+  const std::array<int, 4> I{0, 1, 2, 3};
+  if (!split_mode()) {  // FP64 / FP32 forms: NN GEMMs over S(cd, ij), A(cd, ij) packed from tau
+    if (!pp_packed_) pack_pp_symmetric();
+    timed("r2:pp_sym_pack", 0.0, [&] {
+      hipLaunchKernelGGL(pack_tau_rows_kernel, dim3(sym_grid(nvs_ * oo)), dim3(kSymBlock), 0, stream_, Spk_, tau_, oo,
+                         V, pv_sym_, nvs_, 1.0);
+      check_launch("pack_tau_rows");
+      if (nva_ > 0) {
+        hipLaunchKernelGGL(pack_tau_rows_kernel, dim3(sym_grid(nva_ * oo)), dim3(kSymBlock), 0, stream_, Apk_, tau_,
+                           oo, V, pv_anti_, nva_, -1.0);
+        check_launch("pack_tau_rows");
+      }
+    });
+    gemm(false, false, (int)nvs_, (int)oo, (int)nvs_, 1.0, Vpk_, ldvs_, Spk_, oo, 0.0, Ppk_, oo, 0, 0, 0, 1,
+         "r2:pp_ladder_sym");
+    if (nva_ > 0)
+      gemm(false, false, (int)nva_, (int)oo, (int)nva_, 1.0, Vmk_, ldva_, Apk_, oo, 0.0, Mpk_, oo, 0, 0, 0, 1,
+           "r2:pp_ladder_anti");
+    timed("r2:pp_sym_unpack", 0.0, [&] {
+      hipLaunchKernelGGL(unpack_ladder_kernel, dim3(sym_grid(V * V * oo)), dim3(kSymBlock), 0, stream_, r2_, Ppk_,
+                         Mpk_, oo, V);
+      check_launch("unpack_ladder");
+    });
+    return;
+  }
+  perm_add(tauTd_, {O, O, V, V}, 0.0, {{tau_, e_abij, {2, 3, 0, 1}, 1.0}});
+  timed("r2:pp_sym_pack", 0.0, [&] {
+    hipLaunchKernelGGL(pack_tau_kernel, dim3(sym_grid(oo * ldvs_)), dim3(kSymBlock), 0, stream_, Spk_, tauTd_, oo, V,
+                       pv_sym_, nvs_, ldvs_, 1.0);
+    check_launch("pack_tau");
+    if (nva_ > 0) {
+      hipLaunchKernelGGL(pack_tau_kernel, dim3(sym_grid(oo * ldva_)), dim3(kSymBlock), 0, stream_, Apk_, tauTd_, oo,
+                         V, pv_anti_, nva_, ldva_, -1.0);
+      check_launch("pack_tau");
+    }
+  });
+  split(Sp_P_, {1, 1, oo, ldvs_}, {{Spk_, {1, 1, oo, ldvs_}, I, 1.0}});
+  gemmP((int)nvs_, (int)oo, (int)ldvs_, 1.0, Vp_P_, ldvs_, Sp_P_, ldvs_, 0.0, Ppk_, oo, "r2:pp_ladder_sym");
+  if (nva_ > 0) {
+    split(Ap_P_, {1, 1, oo, ldva_}, {{Apk_, {1, 1, oo, ldva_}, I, 1.0}});
+    gemmP((int)nva_, (int)oo, (int)ldva_, 1.0, Vm_P_, ldva_, Ap_P_, ldva_, 0.0, Mpk_, oo, "r2:pp_ladder_anti");
+  }
+  timed("r2:pp_sym_unpack", 0.0, [&] {
+    hipLaunchKernelGGL(unpack_ladder_kernel, dim3(sym_grid(V * V * oo)), dim3(kSymBlock), 0, stream_, r2_, Ppk_, Mpk_,
+                       oo, V);
+    check_launch("unpack_ladder");
+  });
+}
+
+}  // namespace mpqc_hip

@@ -87,7 +87,9 @@ def test_batched_hchains_match_single_molecule_path():
             o1, o2 = _oracle_residual(inp.blocks, *amps[k])
             print(f"H{chains[k][0]} o={inp.o} v={inp.v}: vs single {_rel(r1, s1):.2e} {_rel(r2, s2):.2e}"
                   f" vs oracle {_rel(r1, o1):.2e} {_rel(r2, o2):.2e}")
-            assert _rel(r1, s1) < 1e-13 and _rel(r2, s2) < 1e-13
+            # (the single-molecule context runs the pair-symmetric pp ladder: sums of a
+            # different order than the batched full contraction)
+            assert _rel(r1, s1) < 1e-12 and _rel(r2, s2) < 1e-12
             assert _rel(r1, o1) < 1e-12 and _rel(r2, o2) < 1e-12
         st = B.stages()
         assert sum(s["kind"] == "contraction" for s in st) > 30
