@@ -246,8 +246,7 @@ class CCSDContext {
   bool pp_sym_planes_ = false;                   // the split forms' planes are the packed ones
   bool pp_symmetric_usable();
   int2 *pv_sym_ = nullptr, *pv_anti_ = nullptr;
-  double *Vpk_ = nullptr, *Vmk_ = nullptr, *Spk_ = nullptr, *Apk_ = nullptr, *Ppk_ = nullptr, *Mpk_ = nullptr,
-         *tauTd_ = nullptr;
+  double *Vpk_ = nullptr, *Vmk_ = nullptr, *Spk_ = nullptr, *Apk_ = nullptr, *Ppk_ = nullptr, *Mpk_ = nullptr;
   Planes Vp_P_, Vm_P_, Sp_P_, Ap_P_;
   Planes GbarT_P_, Gbar_P_, Gbar_akcd_P_, P8T_P_, GiabcT_P_, P13_P_, P14_P_;
   Planes tau_P_, tauT_P_, TP_P_, t2P_P_, Xjk_P_, K_P_, Y1t_P_, Y2t_P_, WT_P_, t1T_P_, gac_P_;

@@ -53,39 +53,23 @@ __global__ __launch_bounds__(kSymBlock) void pack_vvvv_kernel(double* __restrict
   }
 }
 
-// out(ij, C) = f [tauT(i,j,c,d) + sign tauT(i,j,d,c)] for column pair C = (c,d);
-// f = 1/2 on the diagonal c = d (sign +1 only), zero in the padding columns.
-// tauT(i,j,c,d) = tau(c,d,i,j).
-__global__ __launch_bounds__(kSymBlock) void pack_tau_kernel(double* __restrict__ out,
-                                                             const double* __restrict__ tauT, long oo, long v,
-                                                             const int2* __restrict__ cols, long ncols, long ld,
-                                                             double sign) {
-  const long total = oo * ld, vv = v * v;
-  for (long e = (long)blockIdx.x * kSymBlock + threadIdx.x; e < total; e += (long)gridDim.x * kSymBlock) {
-    const long R = e / ld, C = e - R * ld;
-    double x = 0.0;
-    if (C < ncols) {
-      const int2 cd = cols[C];
-      const double* row = tauT + R * vv;
-      x = row[(long)cd.x * v + cd.y] + sign * row[(long)cd.y * v + cd.x];
-      if (cd.x == cd.y) x *= 0.5;
-    }
-    out[e] = x;
-  }
-}
-
-// out(C, ij) = f [tau(c,d,i,j) + sign tau(d,c,i,j)] for column pair C = (c,d): the
-// K x N operand of the FP64 form's NN GEMM, read straight from tau(a,b,i,j)
+// out(C, ij) = f [tau(c,d,i,j) + sign tau(d,c,i,j)] for column pair C = (c,d), read
+// straight from tau(a,b,i,j) (coalesced along ij); rows npairs..nrows-1 are zero (the
+// padding of the split forms, whose planes take the transpose).  The K x N operand of
+// the FP64 form's NN GEMM.
 __global__ __launch_bounds__(kSymBlock) void pack_tau_rows_kernel(double* __restrict__ out,
                                                                   const double* __restrict__ tau, long oo, long v,
                                                                   const int2* __restrict__ pairs, long npairs,
-                                                                  double sign) {
-  const long total = npairs * oo;
+                                                                  long nrows, double sign) {
+  const long total = nrows * oo;
   for (long e = (long)blockIdx.x * kSymBlock + threadIdx.x; e < total; e += (long)gridDim.x * kSymBlock) {
     const long C = e / oo, ij = e - C * oo;
-    const int2 cd = pairs[C];
-    double x = tau[((long)cd.x * v + cd.y) * oo + ij] + sign * tau[((long)cd.y * v + cd.x) * oo + ij];
-    if (cd.x == cd.y) x *= 0.5;
+    double x = 0.0;
+    if (C < npairs) {
+      const int2 cd = pairs[C];
+      x = tau[((long)cd.x * v + cd.y) * oo + ij] + sign * tau[((long)cd.y * v + cd.x) * oo + ij];
+      if (cd.x == cd.y) x *= 0.5;
+    }
     out[e] = x;
   }
 }
@@ -195,7 +179,6 @@ void CCSDContext::pack_pp_symmetric() {
     Apk_ = alloc((size_t)oo * ldva_);
     Ppk_ = alloc((size_t)nvs_ * oo);
     Mpk_ = alloc((size_t)std::max<long>(nva_, 1) * oo);
-    tauTd_ = alloc((size_t)oo * V * V);
   }
   hipLaunchKernelGGL(pack_vvvv_kernel, dim3(sym_grid(nvs_ * ldvs_)), dim3(kSymBlock), 0, stream_, Vpk_, Gabcd_, V,
                      pv_sym_, nvs_, ldvs_, 1.0);
@@ -226,17 +209,15 @@ void CCSDContext::prepare_pp_symmetric() {
 // :275 b(a,b,i,j) = tau(c,d,i,j) Gabcd(a,b,c,d), added to r2
 void CCSDContext::pp_ladder_symmetric() {
   const long O = o_, V = v_, oo = O * O;
-  const std::array<long, 4> e_abij{V, V, O, O};
-  const std::array<int, 4> I{0, 1, 2, 3};
   if (!split_mode()) {  // FP64 / FP32 forms: NN GEMMs over S(cd, ij), A(cd, ij) packed from tau
     if (!pp_packed_) pack_pp_symmetric();
     timed("r2:pp_sym_pack", 0.0, [&] {
       hipLaunchKernelGGL(pack_tau_rows_kernel, dim3(sym_grid(nvs_ * oo)), dim3(kSymBlock), 0, stream_, Spk_, tau_, oo,
-                         V, pv_sym_, nvs_, 1.0);
+                         V, pv_sym_, nvs_, nvs_, 1.0);
       check_launch("pack_tau_rows");
       if (nva_ > 0) {
         hipLaunchKernelGGL(pack_tau_rows_kernel, dim3(sym_grid(nva_ * oo)), dim3(kSymBlock), 0, stream_, Apk_, tau_,
-                           oo, V, pv_anti_, nva_, -1.0);
+                           oo, V, pv_anti_, nva_, nva_, -1.0);
         check_launch("pack_tau_rows");
       }
     });
@@ -252,21 +233,22 @@ void CCSDContext::pp_ladder_symmetric() {
     });
     return;
   }
-  perm_add(tauTd_, {O, O, V, V}, 0.0, {{tau_, e_abij, {2, 3, 0, 1}, 1.0}});
+  // S and A as (pair, ij) from tau, then transposed into (ij, pair) planes by the split
   timed("r2:pp_sym_pack", 0.0, [&] {
-    hipLaunchKernelGGL(pack_tau_kernel, dim3(sym_grid(oo * ldvs_)), dim3(kSymBlock), 0, stream_, Spk_, tauTd_, oo, V,
-                       pv_sym_, nvs_, ldvs_, 1.0);
-    check_launch("pack_tau");
+    hipLaunchKernelGGL(pack_tau_rows_kernel, dim3(sym_grid(ldvs_ * oo)), dim3(kSymBlock), 0, stream_, Spk_, tau_, oo,
+                       V, pv_sym_, nvs_, ldvs_, 1.0);
+    check_launch("pack_tau_rows");
     if (nva_ > 0) {
-      hipLaunchKernelGGL(pack_tau_kernel, dim3(sym_grid(oo * ldva_)), dim3(kSymBlock), 0, stream_, Apk_, tauTd_, oo,
-                         V, pv_anti_, nva_, ldva_, -1.0);
-      check_launch("pack_tau");
+      hipLaunchKernelGGL(pack_tau_rows_kernel, dim3(sym_grid(ldva_ * oo)), dim3(kSymBlock), 0, stream_, Apk_, tau_,
+                         oo, V, pv_anti_, nva_, ldva_, -1.0);
+      check_launch("pack_tau_rows");
     }
   });
-  split(Sp_P_, {1, 1, oo, ldvs_}, {{Spk_, {1, 1, oo, ldvs_}, I, 1.0}});
+  const std::array<int, 4> T{0, 1, 3, 2};
+  split(Sp_P_, {1, 1, oo, ldvs_}, {{Spk_, {1, 1, ldvs_, oo}, T, 1.0}});
   gemmP((int)nvs_, (int)oo, (int)ldvs_, 1.0, Vp_P_, ldvs_, Sp_P_, ldvs_, 0.0, Ppk_, oo, "r2:pp_ladder_sym");
   if (nva_ > 0) {
-    split(Ap_P_, {1, 1, oo, ldva_}, {{Apk_, {1, 1, oo, ldva_}, I, 1.0}});
+    split(Ap_P_, {1, 1, oo, ldva_}, {{Apk_, {1, 1, ldva_, oo}, T, 1.0}});
     gemmP((int)nva_, (int)oo, (int)ldva_, 1.0, Vm_P_, ldva_, Ap_P_, ldva_, 0.0, Mpk_, oo, "r2:pp_ladder_anti");
   }
   timed("r2:pp_sym_unpack", 0.0, [&] {
