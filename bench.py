@@ -10,7 +10,8 @@ Default workload (one "step" = one CCSD iteration over resident data):
              1 uEh of FP64.
 
 Prints ONE JSON line with the metric, the roofline of the dominant kernel (the
-pp-ladder contraction, algorithmic FLOPs 2 o^2 v^4 per launch) and the CPU
+V+ GEMM of the pair-symmetric pp-ladder contraction, 2 (v(v+1)/2)^2 o^2 FLOPs
+per launch; the full contraction's 2 o^2 v^4 where that form runs) and the CPU
 baseline (one oracle residual on the host cores).
 
 Other workloads: pp-ladder (R[a,b,i,j] += sum_cd tau[c,d,i,j] V[a,b,c,d],
@@ -50,7 +51,7 @@ def _args():
                    choices=["ccsd-h2o8", "ccsd-h2o6", "ccsd-h20", "pp-ladder", "copy",
                             "batched-hchains", "adjoint-h2o6", "df-h2o8", "transform-h2o8",
                             "triples-h2o6", "triples-h2o8", "eom-h2o6", "direct-h2o8"])
-    p.add_argument("--forms", default="f64,f16x3",
+    p.add_argument("--forms", default="f64,f16x3,bf16x3",
                    help="contraction precisions to run for the CCSD workloads (first = reference)")
     p.add_argument("--max-iter", type=int, default=40)
     p.add_argument("--copy-gib", type=float, default=4.0,

@@ -158,7 +158,8 @@ const char* mpqc_ccsd_last_error(const mpqc_ccsd_ctx* ctx);
  * full contraction; 0: the full contraction).  Tuning keys that act on
  * every context of the process (measurement aids; the defaults are the measured best):
  * "plane_tile" (tile of the 16-bit plane contractions, rows * 1000 + columns, 0 automatic),
- * "plane_lds_stages" (1 or 2), "plane_tile_order" (0 XCD ranges of row-major tiles, 1 plain,
+ * "plane_lds_stages" (1 or 2), "plane_split_cap" (most K splits of a plane contraction whose
+ * tiles do not fill the GPU, default 512; set before the context's first residual), "plane_tile_order" (0 XCD ranges of row-major tiles, 1 plain,
  * 2 XCD ranges of column-major tiles). */
 int mpqc_ccsd_set_option(mpqc_ccsd_ctx* ctx, const char* key, double value);
 

@@ -235,6 +235,9 @@ void CCSDContext::set_option(const std::string& key, double value) {
     const int t = (int)value;
     if (t != value || t < 0) throw std::invalid_argument("plane_tile must be 0 or rows * 1000 + columns");
     set_plane_tile_override(t / 1000, t % 1000);
+  } else if (key == "plane_split_cap") {  // process-wide: most K splits of a small-grid plane GEMM
+    if ((int)value != value) throw std::invalid_argument("plane_split_cap must be an integer");
+    set_plane_split_cap((int)value);
   } else if (key == "plane_tile_order") {  // process-wide: 0 XCD ranges row-major, 1 plain, 2 XCD ranges column-major
     set_plane_tile_order((int)value);
   } else if (key == "plane_lds_stages") {  // process-wide

@@ -470,6 +470,7 @@ int g_tile_override_m = 0, g_tile_override_n = 0;
 bool g_bdirect = false;  // direct B fragments (selected by measurement)
 int g_tile_order = 0;    // see PParams::order
 int g_lds_stages = 2;    // LDS stages of the 128 x 128 two-plane kernel
+int g_split_cap = 512;   // most K splits of a grid whose tiles alone do not fill the GPU
 
 PlanePlan plan_planes(int M, int N, int K, int batch, int force_bm = 0, int force_bn = 0) {
   PlanePlan pl;
@@ -494,7 +495,7 @@ PlanePlan plan_planes(int M, int N, int K, int batch, int force_bm = 0, int forc
   const long want = 1024;  // about 4 workgroups per CU
   if (tiles * batch < want && K >= 8 * kBK) {
     nsplit = (int)std::min<long>(ceil_div(want, tiles * batch), K / (4 * kBK));
-    nsplit = std::max(1, std::min(nsplit, 512));
+    nsplit = std::max(1, std::min(nsplit, g_split_cap));
   }
   // K chunks beyond kMaxChunk go to separate splits as well: streaming partial
   // sums to the workspace measured faster than the in-kernel FP64 flush, whose
@@ -630,6 +631,11 @@ void set_plane_lds_stages(int stages) {
   if (stages != 1 && stages != 2) throw std::invalid_argument("plane LDS stages must be 1 or 2");
   g_lds_stages = stages;
 }
+void set_plane_split_cap(int cap) {
+  if (cap < 1 || cap > 4096) throw std::invalid_argument("plane_split_cap must be in [1, 4096]");
+  g_split_cap = cap;
+}
+
 void set_plane_tile_order(int order) {
   if (order < 0 || order > 2) throw std::invalid_argument("unknown tile order");
   g_tile_order = order;

@@ -71,6 +71,8 @@ void set_plane_bdirect(bool on);
 // Tile order of the contraction grid: 0 XCD ranges of row-major tiles, 1 plain row-major,
 // 2 XCD ranges of column-major tiles.
 void set_plane_tile_order(int order);
+// process-wide: most K splits of a small-grid plane GEMM (default 512)
+void set_plane_split_cap(int cap);
 // Workspace (bytes) gemm_planes needs for its split-K partial sums at this shape.
 size_t gemm_planes_workspace_bytes(int M, int N, int K, int batch);
 
