@@ -769,7 +769,9 @@ void CCSDContext::prepare_planes() {
   const std::array<int, 4> I{0, 1, 2, 3};
   // (the df method has no Gabcd or Giabc: its ladder and Giabc-type terms come from the factors)
   if (!df_) {
-    pp_sym_planes_ = pp_symmetric_usable();
+    // (the packed operands' plane split indexes a (pairs x pitch) plane with 32 bits)
+    const long nvs = V * (V + 1) / 2, ldvs = (nvs + 63) / 64 * 64;
+    pp_sym_planes_ = nvs * ldvs < (1L << 31) && pp_symmetric_usable();
     if (pp_sym_planes_)
       prepare_pp_symmetric();
     else
