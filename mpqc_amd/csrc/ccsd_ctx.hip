@@ -1113,18 +1113,19 @@ void CCSDContext::residual() {
     perm_add(Y2_, {O, V, V, O}, 0.0, {{t2_, e_abij, {3, 0, 1, 2}, 1.0}});
     gemm(false, false, o * v, o * v, o * v, 1.0, K_, ov, Y2_, ov, 0.0, tmpR_, ov, 0, 0, 0, 1, "r2:crossed_ring");
   }
-  // :260 r2(a,b,i,j) = r2(a,b,i,j) + r2(b,a,j,i); :262 r2 += Gijab(i,j,a,b)
+  // :260 r2(a,b,i,j) = r2(a,b,i,j) + r2(b,a,j,i); :262 r2 += Gijab(i,j,a,b).  The ring
+  // terms come in pairs under (a,i) <-> (b,j), so in two passes: the half-sum
+  // H = Rd + RR(a,i,b,j) - 0.5 tmpR(a,i,b,j) - tmpR(b,i,a,j) (in Rd), then
+  // r2 = H + H(b,a,j,i) + Gijab: 9 tensor reads and writes in place of 12
   {
     const std::array<long, 4> e_aibj{V, O, V, O};
+    perm_add(Rd_, {V, V, O, O}, 1.0,
+             {{RR_, e_aibj, {0, 2, 1, 3}, 1.0},
+              {tmpR_, e_aibj, {0, 2, 1, 3}, -0.5},
+              {tmpR_, e_aibj, {2, 0, 1, 3}, -1.0}});
     perm_add(r2_, {V, V, O, O}, 0.0,
              {{Rd_, e_abij, I, 1.0},
               {Rd_, e_abij, {1, 0, 3, 2}, 1.0},
-              {RR_, e_aibj, {0, 2, 1, 3}, 1.0},
-              {RR_, e_aibj, {2, 0, 3, 1}, 1.0},
-              {tmpR_, e_aibj, {0, 2, 1, 3}, -0.5},
-              {tmpR_, e_aibj, {2, 0, 1, 3}, -1.0},
-              {tmpR_, e_aibj, {2, 0, 3, 1}, -0.5},
-              {tmpR_, e_aibj, {0, 2, 3, 1}, -1.0},
               {g_abij_, e_abij, I, 1.0}});  // Gijab(i,j,a,b) in the output's order (prepare())
   }
   // :266-269 a_klij = W_oooo (ccsd_intermediates.h:182-193); r2 += a_klij(k,l,i,j) tau(a,b,k,l)
