@@ -154,8 +154,9 @@ const char* mpqc_ccsd_last_error(const mpqc_ccsd_ctx* ctx);
  * (ab|cd) blocks, default 4 GiB; set before the first residual), "pp_symmetric" (1 default:
  * the particle-particle ladder of ccsd_r1_r2.h:275 as two GEMMs over the symmetric and
  * antisymmetric (c,d) pair combinations, rows a<=b, with half the FLOPs, exact for any
- * amplitudes given Gabcd(b,a,d,c) = Gabcd(a,b,c,d); the adjoint's recorded residual keeps the
- * full contraction; 0: the full contraction).  Tuning keys that act on
+ * amplitudes given Gabcd(b,a,d,c) = Gabcd(a,b,c,d), and W_oooo's Gijab tau term likewise given
+ * Gijab(l,k,d,c) = Gijab(k,l,c,d); blocks without the symmetry and the adjoint's recorded
+ * residual keep the full contractions; 0: the full contractions).  Tuning keys that act on
  * every context of the process (measurement aids; the defaults are the measured best):
  * "plane_tile" (tile of the 16-bit plane contractions, rows * 1000 + columns, 0 automatic),
  * "plane_lds_stages" (1 or 2), "plane_split_cap" (most K splits of a plane contraction whose

@@ -768,6 +768,7 @@ void CCSDContext::prepare_planes() {
   const std::array<long, 4> e_ijab{O, O, V, V};
   const std::array<int, 4> I{0, 1, 2, 3};
   // (the df method has no Gabcd or Giabc: its ladder and Giabc-type terms come from the factors)
+  pp_sym_planes_ = false;
   if (!df_) {
     // (the packed operands' plane split indexes a (pairs x pitch) plane with 32 bits)
     const long nvs = V * (V + 1) / 2, ldvs = (nvs + 63) / 64 * 64;
@@ -1133,8 +1134,15 @@ void CCSDContext::residual() {
   perm_add(W_, {O, O, O, O}, 0.0,
            {{Gijkl_, {O, O, O, O}, I, 1.0}, {Y23_, {O, O, O, O}, I, 1.0},
             {Y23_, {O, O, O, O}, {1, 0, 3, 2}, 1.0}});
+  // the pair-symmetric forms of W_oooo's G tau term and of the pp ladder share S and A
+  const bool psym = sm ? (pp_sym_planes_ && !df_)
+                       : (!tape_ && !direct_ && !(df_ && reduced_abcd_) && pp_symmetric_usable());
+  if (psym) pack_tau_pairs();
   if (sm) {
-    gemmP(o * o, o * o, v * v, 1.0, Gijab_P_, vv, tauT_P_, vv, 1.0, W_, oo, "r2:W_oooo");
+    if (psym && w_sym_ok_)
+      w_oooo_symmetric();
+    else
+      gemmP(o * o, o * o, v * v, 1.0, Gijab_P_, vv, tauT_P_, vv, 1.0, W_, oo, "r2:W_oooo");
     // W^T(i,j,k,l) = W(k,l,i,j)
     split(WT_P_, {O, O, O, O}, {{W_, {O, O, O, O}, {2, 3, 0, 1}, 1.0}});
     gemmP(v * v, o * o, o * o, 1.0, tau_P_, oo, WT_P_, oo, 1.0, r2_, oo, "r2:hh_ladder");
@@ -1144,14 +1152,17 @@ void CCSDContext::residual() {
     }
     if (df_) throw std::logic_error("the split forms of the df method need reduced_abcd_memory");
     // :275 b(a,b,i,j) = tau(c,d,i,j) Gabcd(a,b,c,d)  -- the particle-particle ladder
-    if (pp_sym_planes_)
+    if (psym)
       pp_ladder_symmetric();
     else
       gemmP(v * v, o * o, v * v, 1.0, Gabcd_P_, vv, tauT_P_, vv, 1.0, r2_, oo, "r2:pp_ladder");
     // :277 tmp(k,a,i,j) = Giabc(k,a,c,d) tau(c,d,i,j)
     gemmP(o * v, o * o, v * v, 1.0, Giabc_P_, vv, tauT_P_, vv, 0.0, tmp24_, oo, "r2:pp_t1_giabc_tau");
   } else {
-    gemm(false, false, o * o, o * o, v * v, 1.0, Gijab_, vv, tau_, oo, 1.0, W_, oo, 0, 0, 0, 1, "r2:W_oooo");
+    if (psym && w_sym_ok_)
+      w_oooo_symmetric();
+    else
+      gemm(false, false, o * o, o * o, v * v, 1.0, Gijab_, vv, tau_, oo, 1.0, W_, oo, 0, 0, 0, 1, "r2:W_oooo");
     gemm(false, false, v * v, o * o, o * o, 1.0, tau_, oo, W_, oo, 1.0, r2_, oo, 0, 0, 0, 1, "r2:hh_ladder");
     if (direct_) {
       direct_r2();  // :282-286 (direct_df: :509-514)
@@ -1163,7 +1174,7 @@ void CCSDContext::residual() {
     }
     // :275 b(a,b,i,j) = tau(c,d,i,j) Gabcd(a,b,c,d)  -- the particle-particle ladder (the
     // recorded residual of the adjoint keeps the full contraction)
-    if (!tape_ && pp_symmetric_usable())
+    if (psym)
       pp_ladder_symmetric();
     else
       gemm(false, false, v * v, o * o, v * v, 1.0, Gabcd_, vv, tau_, oo, 1.0, r2_, oo, 0, 0, 0, 1, "r2:pp_ladder");

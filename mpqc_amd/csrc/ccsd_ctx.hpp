@@ -171,7 +171,12 @@ class CCSDContext {
   void direct_r2();  // ccsd_r1_r2.h:282-286
   void pack_pp_symmetric();     // packed V+ / V- (FP64) of the pair-symmetric ladder
   void prepare_pp_symmetric();  // and their planes (split forms)
-  void pp_ladder_symmetric();   // ccsd_r1_r2.h:275 on pair-symmetric blocks (split forms)
+  void pack_tau_pairs();        // S, A from tau for the pair-symmetric products
+  void pair_products(const double* Pk, long lp, const Planes& Pp, const double* Mk, long lm, const Planes& Mp,
+                     long nrs, long nra, long n, double* out, double* P, double* M, const char* tag_s,
+                     const char* tag_a, const char* tag_u);
+  void pp_ladder_symmetric();   // ccsd_r1_r2.h:275 on pair-symmetric blocks
+  void w_oooo_symmetric();      // ccsd_intermediates.h:182-193, its Gijab tau term likewise
 
   // ---- op tape: integral preparation and the residual as a list of primitive
   // ops (copies, permuted linear combinations, contractions, tau/T builds).
@@ -240,12 +245,15 @@ class CCSDContext {
   std::vector<std::pair<uint16_t*, size_t>> plane_allocs_;
   Planes Gabcd_P_, Giabc_P_, Gijab_P_, GAt_P_, GBt_P_, GCt_P_;
   // pair-symmetric pp ladder: pair counts, padded row pitches, pair tables, packed operands
-  long nvs_ = 0, nva_ = 0, ldvs_ = 0, ldva_ = 0;
+  long nvs_ = 0, nva_ = 0, nos_ = 0, noa_ = 0, ldvs_ = 0, ldva_ = 0;
   bool pp_packed_ = false;  // V+ / V- packed from the current Gabcd
   bool pp_checked_ = false, pp_sym_ok_ = false;  // Gabcd's pair symmetry checked / holds
   bool pp_sym_planes_ = false;                   // the split forms' planes are the packed ones
   bool pp_symmetric_usable();
-  int2 *pv_sym_ = nullptr, *pv_anti_ = nullptr;
+  int2 *pv_sym_ = nullptr, *pv_anti_ = nullptr, *po_sym_ = nullptr, *po_anti_ = nullptr;
+  double *Gpk_ = nullptr, *Gmk_ = nullptr, *WPk_ = nullptr, *WMk_ = nullptr;  // W_oooo's G+/- and products
+  Planes Gp_P_sym_, Gm_P_sym_;
+  bool w_sym_ok_ = false;  // Gijab(l,k,d,c) = Gijab(k,l,c,d) holds (checked with Gabcd's symmetry)
   double *Vpk_ = nullptr, *Vmk_ = nullptr, *Spk_ = nullptr, *Apk_ = nullptr, *Ppk_ = nullptr, *Mpk_ = nullptr;
   Planes Vp_P_, Vm_P_, Sp_P_, Ap_P_;
   Planes GbarT_P_, Gbar_P_, Gbar_akcd_P_, P8T_P_, GiabcT_P_, P13_P_, P14_P_;

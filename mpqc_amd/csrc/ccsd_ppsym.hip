@@ -34,20 +34,22 @@ __device__ __forceinline__ long anti_pair(long p, long q, long n) {
   return p * (n - 1) - p * (p - 1) / 2 + (q - p - 1);
 }
 
-// out(R, C) = G(a,b,c,d) + sign G(a,b,d,c) for row pair R = (a,b), column pair
-// C = (c,d) (tables of packed (p, q) pairs), zero in the padding columns
-__global__ __launch_bounds__(kSymBlock) void pack_vvvv_kernel(double* __restrict__ out,
-                                                              const double* __restrict__ G, long v,
-                                                              const int2* __restrict__ pairs, long npairs,
-                                                              long ld, double sign) {
-  const long total = npairs * ld;
+// out(R, C) = G(a,b,c,d) + sign G(a,b,d,c) for row pair R = (a,b) (a, b < p) and
+// column pair C = (c,d) (c, d < q) of G with extents (p, p, q, q) (tables of packed
+// (x, y) pairs), zero in the padding columns
+__global__ __launch_bounds__(kSymBlock) void pack_pairs_kernel(double* __restrict__ out,
+                                                               const double* __restrict__ G, long p, long q,
+                                                               const int2* __restrict__ rows, long nrows,
+                                                               const int2* __restrict__ cols, long ncols, long ld,
+                                                               double sign) {
+  const long total = nrows * ld;
   for (long e = (long)blockIdx.x * kSymBlock + threadIdx.x; e < total; e += (long)gridDim.x * kSymBlock) {
     const long R = e / ld, C = e - R * ld;
     double x = 0.0;
-    if (C < npairs) {
-      const int2 ab = pairs[R], cd = pairs[C];
-      const long row = ((long)ab.x * v + ab.y) * v;
-      x = G[(row + cd.x) * v + cd.y] + sign * G[(row + cd.y) * v + cd.x];
+    if (C < ncols) {
+      const int2 ab = rows[R], cd = cols[C];
+      const long row = ((long)ab.x * p + ab.y) * q;
+      x = G[(row + cd.x) * q + cd.y] + sign * G[(row + cd.y) * q + cd.x];
     }
     out[e] = x;
   }
@@ -92,16 +94,17 @@ __global__ __launch_bounds__(kSymBlock) void unpack_ladder_kernel(double* __rest
   }
 }
 
-// per block: max |G(a,b,c,d) - G(b,a,d,c)| and max |G(a,b,c,d)| -> part[2 blk], part[2 blk + 1]
+// per block, over G with extents (p, p, q, q): max |G(a,b,c,d) - G(b,a,d,c)| and
+// max |G(a,b,c,d)| -> part[2 blk], part[2 blk + 1]
 __global__ __launch_bounds__(kSymBlock) void pair_asymmetry_kernel(double* __restrict__ part,
-                                                                   const double* __restrict__ G, long v) {
+                                                                   const double* __restrict__ G, long p, long q) {
   __shared__ double sd[kSymBlock], sg[kSymBlock];
-  const long total = v * v * v * v;
+  const long total = p * p * q * q;
   double md = 0.0, mg = 0.0;
   for (long e = (long)blockIdx.x * kSymBlock + threadIdx.x; e < total; e += (long)gridDim.x * kSymBlock) {
-    const long d = e % v, t0 = e / v, c = t0 % v, t1 = t0 / v, b = t1 % v, a = t1 / v;
+    const long d = e % q, t0 = e / q, c = t0 % q, t1 = t0 / q, b = t1 % p, a = t1 / p;
     const double x = G[e];
-    md = fmax(md, fabs(x - G[((b * v + a) * v + d) * v + c]));
+    md = fmax(md, fabs(x - G[((b * p + a) * q + d) * q + c]));
     mg = fmax(mg, fabs(x));
   }
   sd[threadIdx.x] = md;
@@ -139,19 +142,24 @@ long pad64(long n) { return std::max<long>(64, (n + 63) / 64 * 64); }
 bool CCSDContext::pp_symmetric_usable() {
   if (!pp_symmetric_ || !Gabcd_) return false;
   if (pp_checked_) return pp_sym_ok_;
-  const long V = v_;
-  const int blocks = (int)std::max<long>(1, std::min<long>(ceil_div(V * V * V * V, kSymBlock), kReducePartials / 2));
-  hipLaunchKernelGGL(pair_asymmetry_kernel, dim3(blocks), dim3(kSymBlock), 0, stream_, partials_, Gabcd_, V);
-  check_launch("pair_asymmetry");
-  std::vector<double> h(2 * (size_t)blocks);
-  MPQC_HIP_CHECK(hipMemcpyAsync(h.data(), partials_, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
-  MPQC_HIP_CHECK(hipStreamSynchronize(stream_));
-  double md = 0.0, mg = 0.0;
-  for (int b = 0; b < blocks; ++b) {
-    md = std::max(md, h[2 * b]);
-    mg = std::max(mg, h[2 * b + 1]);
-  }
-  pp_sym_ok_ = md <= 1e-12 * mg;
+  // whether G with extents (p, p, q, q) holds G(b,a,d,c) = G(a,b,c,d) to 1e-12 of its largest element
+  auto symmetric = [&](const double* G, long p, long q) {
+    const int blocks = (int)std::max<long>(1, std::min<long>(ceil_div(p * p * q * q, kSymBlock), kReducePartials / 2));
+    hipLaunchKernelGGL(pair_asymmetry_kernel, dim3(blocks), dim3(kSymBlock), 0, stream_, partials_, G, p, q);
+    check_launch("pair_asymmetry");
+    std::vector<double> h(2 * (size_t)blocks);
+    MPQC_HIP_CHECK(hipMemcpyAsync(h.data(), partials_, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    MPQC_HIP_CHECK(hipStreamSynchronize(stream_));
+    double md = 0.0, mg = 0.0;
+    for (int b = 0; b < blocks; ++b) {
+      md = std::max(md, h[2 * b]);
+      mg = std::max(mg, h[2 * b + 1]);
+    }
+    return md <= 1e-12 * mg;
+  };
+  pp_sym_ok_ = symmetric(Gabcd_, v_, v_);
+  // Gijab(l,k,d,c) = Gijab(k,l,c,d): the hole-hole intermediate's G tau term in the same form
+  w_sym_ok_ = pp_sym_ok_ && symmetric(Gijab_, o_, v_);
   pp_checked_ = true;
   return pp_sym_ok_;
 }
@@ -161,6 +169,8 @@ void CCSDContext::pack_pp_symmetric() {
   const long O = o_, V = v_, oo = O * O;
   nvs_ = V * (V + 1) / 2;
   nva_ = V * (V - 1) / 2;
+  nos_ = O * (O + 1) / 2;
+  noa_ = O * (O - 1) / 2;
   ldvs_ = pad64(nvs_);
   ldva_ = pad64(nva_);
   if (!pv_sym_) {
@@ -173,20 +183,36 @@ void CCSDContext::pack_pp_symmetric() {
     };
     pv_sym_ = upload_pairs(V, false);
     pv_anti_ = upload_pairs(V, true);
+    po_sym_ = upload_pairs(O, false);
+    po_anti_ = upload_pairs(O, true);
     Vpk_ = alloc((size_t)nvs_ * ldvs_);
     Vmk_ = alloc((size_t)std::max<long>(nva_, 1) * ldva_);
     Spk_ = alloc((size_t)oo * ldvs_);
     Apk_ = alloc((size_t)oo * ldva_);
     Ppk_ = alloc((size_t)nvs_ * oo);
     Mpk_ = alloc((size_t)std::max<long>(nva_, 1) * oo);
+    Gpk_ = alloc((size_t)nos_ * ldvs_);
+    Gmk_ = alloc((size_t)std::max<long>(noa_, 1) * ldva_);
+    WPk_ = alloc((size_t)nos_ * oo);
+    WMk_ = alloc((size_t)std::max<long>(noa_, 1) * oo);
   }
-  hipLaunchKernelGGL(pack_vvvv_kernel, dim3(sym_grid(nvs_ * ldvs_)), dim3(kSymBlock), 0, stream_, Vpk_, Gabcd_, V,
-                     pv_sym_, nvs_, ldvs_, 1.0);
-  check_launch("pack_vvvv");
+  hipLaunchKernelGGL(pack_pairs_kernel, dim3(sym_grid(nvs_ * ldvs_)), dim3(kSymBlock), 0, stream_, Vpk_, Gabcd_, V, V,
+                     pv_sym_, nvs_, pv_sym_, nvs_, ldvs_, 1.0);
+  check_launch("pack_pairs");
   if (nva_ > 0) {
-    hipLaunchKernelGGL(pack_vvvv_kernel, dim3(sym_grid(nva_ * ldva_)), dim3(kSymBlock), 0, stream_, Vmk_, Gabcd_, V,
-                       pv_anti_, nva_, ldva_, -1.0);
-    check_launch("pack_vvvv");
+    hipLaunchKernelGGL(pack_pairs_kernel, dim3(sym_grid(nva_ * ldva_)), dim3(kSymBlock), 0, stream_, Vmk_, Gabcd_, V,
+                       V, pv_anti_, nva_, pv_anti_, nva_, ldva_, -1.0);
+    check_launch("pack_pairs");
+  }
+  if (w_sym_ok_) {  // G+/-(kl, cd) of Gijab, rows k<=l (k<l)
+    hipLaunchKernelGGL(pack_pairs_kernel, dim3(sym_grid(nos_ * ldvs_)), dim3(kSymBlock), 0, stream_, Gpk_, Gijab_, O,
+                       V, po_sym_, nos_, pv_sym_, nvs_, ldvs_, 1.0);
+    check_launch("pack_pairs");
+    if (noa_ > 0 && nva_ > 0) {
+      hipLaunchKernelGGL(pack_pairs_kernel, dim3(sym_grid(noa_ * ldva_)), dim3(kSymBlock), 0, stream_, Gmk_, Gijab_,
+                         O, V, po_anti_, noa_, pv_anti_, nva_, ldva_, -1.0);
+      check_launch("pack_pairs");
+    }
   }
   pp_packed_ = true;
 }
@@ -200,62 +226,75 @@ void CCSDContext::prepare_pp_symmetric() {
     Vm_P_ = alloc_planes(std::max<long>(nva_, 1) * ldva_);
     Sp_P_ = alloc_planes(oo * ldvs_);
     Ap_P_ = alloc_planes(oo * ldva_);
+    Gp_P_sym_ = alloc_planes(nos_ * ldvs_);
+    Gm_P_sym_ = alloc_planes(std::max<long>(noa_, 1) * ldva_);
   }
   const std::array<int, 4> I{0, 1, 2, 3};
   split(Vp_P_, {1, 1, nvs_, ldvs_}, {{Vpk_, {1, 1, nvs_, ldvs_}, I, 1.0}});
   if (nva_ > 0) split(Vm_P_, {1, 1, nva_, ldva_}, {{Vmk_, {1, 1, nva_, ldva_}, I, 1.0}});
+  if (w_sym_ok_) {
+    split(Gp_P_sym_, {1, 1, nos_, ldvs_}, {{Gpk_, {1, 1, nos_, ldvs_}, I, 1.0}});
+    if (noa_ > 0 && nva_ > 0) split(Gm_P_sym_, {1, 1, noa_, ldva_}, {{Gmk_, {1, 1, noa_, ldva_}, I, 1.0}});
+  }
 }
 
-// :275 b(a,b,i,j) = tau(c,d,i,j) Gabcd(a,b,c,d), added to r2
-void CCSDContext::pp_ladder_symmetric() {
+// S(pair, ij) and A(pair, ij) from tau (FP64 form: the NN GEMMs' K x N operands; split
+// forms: padded to a multiple of 64 pairs and transposed into (ij, pair) planes)
+void CCSDContext::pack_tau_pairs() {
   const long O = o_, V = v_, oo = O * O;
-  if (!split_mode()) {  // FP64 / FP32 forms: NN GEMMs over S(cd, ij), A(cd, ij) packed from tau
-    if (!pp_packed_) pack_pp_symmetric();
-    timed("r2:pp_sym_pack", 0.0, [&] {
-      hipLaunchKernelGGL(pack_tau_rows_kernel, dim3(sym_grid(nvs_ * oo)), dim3(kSymBlock), 0, stream_, Spk_, tau_, oo,
-                         V, pv_sym_, nvs_, nvs_, 1.0);
-      check_launch("pack_tau_rows");
-      if (nva_ > 0) {
-        hipLaunchKernelGGL(pack_tau_rows_kernel, dim3(sym_grid(nva_ * oo)), dim3(kSymBlock), 0, stream_, Apk_, tau_,
-                           oo, V, pv_anti_, nva_, nva_, -1.0);
-        check_launch("pack_tau_rows");
-      }
-    });
-    gemm(false, false, (int)nvs_, (int)oo, (int)nvs_, 1.0, Vpk_, ldvs_, Spk_, oo, 0.0, Ppk_, oo, 0, 0, 0, 1,
-         "r2:pp_ladder_sym");
-    if (nva_ > 0)
-      gemm(false, false, (int)nva_, (int)oo, (int)nva_, 1.0, Vmk_, ldva_, Apk_, oo, 0.0, Mpk_, oo, 0, 0, 0, 1,
-           "r2:pp_ladder_anti");
-    timed("r2:pp_sym_unpack", 0.0, [&] {
-      hipLaunchKernelGGL(unpack_ladder_kernel, dim3(sym_grid(V * V * oo)), dim3(kSymBlock), 0, stream_, r2_, Ppk_,
-                         Mpk_, oo, V);
-      check_launch("unpack_ladder");
-    });
-    return;
-  }
-  // S and A as (pair, ij) from tau, then transposed into (ij, pair) planes by the split
+  const bool sm = split_mode();
+  if (!sm && !pp_packed_) pack_pp_symmetric();
+  const long rs = sm ? ldvs_ : nvs_, ra = sm ? ldva_ : nva_;
   timed("r2:pp_sym_pack", 0.0, [&] {
-    hipLaunchKernelGGL(pack_tau_rows_kernel, dim3(sym_grid(ldvs_ * oo)), dim3(kSymBlock), 0, stream_, Spk_, tau_, oo,
-                       V, pv_sym_, nvs_, ldvs_, 1.0);
+    hipLaunchKernelGGL(pack_tau_rows_kernel, dim3(sym_grid(rs * oo)), dim3(kSymBlock), 0, stream_, Spk_, tau_, oo, V,
+                       pv_sym_, nvs_, rs, 1.0);
     check_launch("pack_tau_rows");
     if (nva_ > 0) {
-      hipLaunchKernelGGL(pack_tau_rows_kernel, dim3(sym_grid(ldva_ * oo)), dim3(kSymBlock), 0, stream_, Apk_, tau_,
-                         oo, V, pv_anti_, nva_, ldva_, -1.0);
+      hipLaunchKernelGGL(pack_tau_rows_kernel, dim3(sym_grid(ra * oo)), dim3(kSymBlock), 0, stream_, Apk_, tau_, oo,
+                         V, pv_anti_, nva_, ra, -1.0);
       check_launch("pack_tau_rows");
     }
   });
-  const std::array<int, 4> T{0, 1, 3, 2};
-  split(Sp_P_, {1, 1, oo, ldvs_}, {{Spk_, {1, 1, ldvs_, oo}, T, 1.0}});
-  gemmP((int)nvs_, (int)oo, (int)ldvs_, 1.0, Vp_P_, ldvs_, Sp_P_, ldvs_, 0.0, Ppk_, oo, "r2:pp_ladder_sym");
-  if (nva_ > 0) {
-    split(Ap_P_, {1, 1, oo, ldva_}, {{Apk_, {1, 1, ldva_, oo}, T, 1.0}});
-    gemmP((int)nva_, (int)oo, (int)ldva_, 1.0, Vm_P_, ldva_, Ap_P_, ldva_, 0.0, Mpk_, oo, "r2:pp_ladder_anti");
+  if (sm) {
+    const std::array<int, 4> T{0, 1, 3, 2};
+    split(Sp_P_, {1, 1, oo, ldvs_}, {{Spk_, {1, 1, ldvs_, oo}, T, 1.0}});
+    if (nva_ > 0) split(Ap_P_, {1, 1, oo, ldva_}, {{Apk_, {1, 1, ldva_, oo}, T, 1.0}});
   }
-  timed("r2:pp_sym_unpack", 0.0, [&] {
-    hipLaunchKernelGGL(unpack_ladder_kernel, dim3(sym_grid(V * V * oo)), dim3(kSymBlock), 0, stream_, r2_, Ppk_, Mpk_,
-                       oo, V);
+}
+
+// out(p,q,i,j) += 1/2 [P(pq,ij) + s(p,q) M(pq,ij)] from the two halves' products
+void CCSDContext::pair_products(const double* Pk, long lp, const Planes& Pp, const double* Mk, long lm,
+                                const Planes& Mp, long nrs, long nra, long n, double* out, double* P, double* M,
+                                const char* tag_s, const char* tag_a, const char* tag_u) {
+  const long oo = (long)o_ * o_;
+  if (split_mode()) {
+    gemmP((int)nrs, (int)oo, (int)ldvs_, 1.0, Pp, ldvs_, Sp_P_, ldvs_, 0.0, P, oo, tag_s);
+    if (nra > 0 && nva_ > 0)
+      gemmP((int)nra, (int)oo, (int)ldva_, 1.0, Mp, ldva_, Ap_P_, ldva_, 0.0, M, oo, tag_a);
+  } else {
+    gemm(false, false, (int)nrs, (int)oo, (int)nvs_, 1.0, Pk, lp, Spk_, oo, 0.0, P, oo, 0, 0, 0, 1, tag_s);
+    if (nra > 0 && nva_ > 0)
+      gemm(false, false, (int)nra, (int)oo, (int)nva_, 1.0, Mk, lm, Apk_, oo, 0.0, M, oo, 0, 0, 0, 1, tag_a);
+  }
+  if (nra > 0 && nva_ == 0) fill(M, nra * oo, 0.0, stream_);  // v = 1: no (c<d) pairs, M = 0
+  timed(tag_u, 0.0, [&] {
+    hipLaunchKernelGGL(unpack_ladder_kernel, dim3(sym_grid(n * n * oo)), dim3(kSymBlock), 0, stream_, out, P, M, oo,
+                       n);
     check_launch("unpack_ladder");
   });
+}
+
+// :275 b(a,b,i,j) = tau(c,d,i,j) Gabcd(a,b,c,d), added to r2 (after pack_tau_pairs)
+void CCSDContext::pp_ladder_symmetric() {
+  pair_products(Vpk_, ldvs_, Vp_P_, Vmk_, ldva_, Vm_P_, nvs_, nva_, v_, r2_, Ppk_, Mpk_, "r2:pp_ladder_sym",
+                "r2:pp_ladder_anti", "r2:pp_sym_unpack");
+}
+
+// ccsd_intermediates.h:182-193: W(k,l,i,j) += Gijab(k,l,c,d) tau(c,d,i,j) in the same
+// pair form (rows k<=l; after pack_tau_pairs)
+void CCSDContext::w_oooo_symmetric() {
+  pair_products(Gpk_, ldvs_, Gp_P_sym_, Gmk_, ldva_, Gm_P_sym_, nos_, noa_, o_, W_, WPk_, WMk_, "r2:W_oooo_sym",
+                "r2:W_oooo_anti", "r2:W_sym_unpack");
 }
 
 }  // namespace mpqc_hip

@@ -1,6 +1,7 @@
-"""The pair-symmetric particle-particle ladder on the GPU (option pp_symmetric, the
-default) against the full contraction (pp_symmetric 0) and the oracle, in the FP64
-and split forms, at amplitudes without the (ab)(ij) symmetry (ccsd_r1_r2.h:275).
+"""The pair-symmetric particle-particle ladder and hole-hole G tau term on the GPU
+(option pp_symmetric, the default) against the full contractions (pp_symmetric 0) and
+the oracle, in the FP64 and split forms, at amplitudes without the (ab)(ij) symmetry
+(ccsd_r1_r2.h:275, ccsd_intermediates.h:182-193).
 
 Bars: FP64 1e-12 relative between the two forms and against the oracle (round-off of
 sums of different order); split forms 2e-6 (f16x3) and 1e-3 (bf16x3) relative against
@@ -68,20 +69,22 @@ def test_pair_symmetric_ladder_solve_energy():
     assert abs(E[1] - E[0]) < 1e-11
 
 
+@pytest.mark.parametrize("block", ["Gabcd", "Gijab"])
 @pytest.mark.parametrize("prec,bar", [("f64", 1e-12), ("f16x3", 2e-6)])
-def test_asymmetric_gabcd_takes_the_full_contraction(prec, bar):
-    # Gabcd without the pair symmetry (a symmetry-breaking perturbation): the context
-    # detects it and runs the full contraction, so the residual still matches the oracle
+def test_asymmetric_blocks_take_the_full_contraction(prec, bar, block):
+    # Gabcd (pp ladder) or Gijab (hole-hole intermediate) without the pair symmetry (a
+    # symmetry-breaking perturbation): the context detects it and runs the full
+    # contraction, so the residual still matches the oracle
     from oracle import ccsd as O
     no, nv, f, b, t1, t2, _, _ = _case()
     rng = np.random.default_rng(9)
     b = dict(b)
-    b["Gabcd"] = b["Gabcd"] + 1e-3 * rng.normal(size=b["Gabcd"].shape)
+    b[block] = b[block] + 1e-3 * rng.normal(size=b[block].shape)
     ints = O.Integrals(**b)
     tau = O.make_tau(t1, t2)
     r1r, r2r = O.cs_ccsd_r1(t1, t2, tau, ints), O.cs_ccsd_r2(t1, t2, tau, ints)
     r1, r2 = _residuals(prec, 1, no, nv, f, b, t1, t2)
     e1 = np.abs(r1 - r1r).max() / np.abs(r1r).max()
     e2 = np.abs(r2 - r2r).max() / np.abs(r2r).max()
-    print(f"asymmetric Gabcd {prec}: r1 {e1:.2e} r2 {e2:.2e}")
+    print(f"asymmetric {block} {prec}: r1 {e1:.2e} r2 {e2:.2e}")
     assert e1 < bar and e2 < bar

@@ -14,30 +14,36 @@ def _anti_pair(p, q, n):
     return p * (n - 1) - p * (p - 1) // 2 + (q - p - 1)
 
 
+def _pairs(n, strict):
+    return [(p, q) for p in range(n) for q in range(p + 1 if strict else p, n)]
+
+
 def _pair_symmetric_ladder(G, tau):
-    v, o = G.shape[0], tau.shape[2]
-    sym = [(p, q) for p in range(v) for q in range(p, v)]
-    anti = [(p, q) for p in range(v) for q in range(p + 1, v)]
-    assert all(_sym_pair(p, q, v) == k for k, (p, q) in enumerate(sym))
-    assert all(_anti_pair(p, q, v) == k for k, (p, q) in enumerate(anti))
+    """sum_cd G(a,b,c,d) tau(c,d,i,j) for G of extents (n, n, v, v) with the pair symmetry
+    (the pp ladder: n = v; the hole-hole intermediate's Gijab term: n = o)."""
+    n, v, o = G.shape[0], G.shape[2], tau.shape[2]
+    rs, ra, cs, ca = _pairs(n, False), _pairs(n, True), _pairs(v, False), _pairs(v, True)
+    assert all(_sym_pair(p, q, n) == k for k, (p, q) in enumerate(rs))
+    assert all(_anti_pair(p, q, n) == k for k, (p, q) in enumerate(ra))
     t = tau.reshape(v, v, o * o)
-    Vp = np.array([[G[a, b, c, d] + G[a, b, d, c] for (c, d) in sym] for (a, b) in sym]).reshape(len(sym), len(sym))
-    S = np.array([(0.5 if c == d else 1.0) * (t[c, d] + t[d, c]) for (c, d) in sym]).reshape(len(sym), o * o)
+    Vp = np.array([[G[a, b, c, d] + G[a, b, d, c] for (c, d) in cs] for (a, b) in rs]).reshape(len(rs), len(cs))
+    S = np.array([(0.5 if c == d else 1.0) * (t[c, d] + t[d, c]) for (c, d) in cs]).reshape(len(cs), o * o)
     P = Vp @ S
-    if anti:
-        Vm = np.array([[G[a, b, c, d] - G[a, b, d, c] for (c, d) in anti] for (a, b) in anti])
-        A = np.array([t[c, d] - t[d, c] for (c, d) in anti])
+    M = np.zeros((len(ra), o * o))
+    if ra and ca:
+        Vm = np.array([[G[a, b, c, d] - G[a, b, d, c] for (c, d) in ca] for (a, b) in ra])
+        A = np.array([t[c, d] - t[d, c] for (c, d) in ca])
         M = Vm @ A
-    out = np.zeros((v, v, o * o))
-    for a in range(v):
-        for b in range(v):
+    out = np.zeros((n, n, o * o))
+    for a in range(n):
+        for b in range(n):
             p, q = min(a, b), max(a, b)
-            x = P[_sym_pair(p, q, v)].copy()
+            x = P[_sym_pair(p, q, n)].copy()
             if a != b:
-                m = M[_anti_pair(p, q, v)]
+                m = M[_anti_pair(p, q, n)]
                 x += m if a < b else -m
             out[a, b] = 0.5 * x
-    return out.reshape(v, v, o, o)
+    return out.reshape(n, n, o, o)
 
 
 @pytest.mark.parametrize("o,v", [(3, 5), (1, 4), (2, 1), (4, 7)])
@@ -58,3 +64,15 @@ def test_pair_symmetric_ladder_needs_the_integral_symmetry():
     tau = rng.normal(size=(4, 4, 2, 2))
     ref = np.einsum("abcd,cdij->abij", G, tau)
     assert np.abs(_pair_symmetric_ladder(G, tau) - ref).max() > 1e-3
+
+
+@pytest.mark.parametrize("o,v", [(3, 5), (4, 1), (1, 3)])
+def test_pair_symmetric_hole_hole_term_equals_full_contraction(o, v):
+    # W(k,l,i,j) += sum_cd Gijab(k,l,c,d) tau(c,d,i,j) with Gijab(l,k,d,c) = Gijab(k,l,c,d)
+    rng = np.random.default_rng(o * 7 + v)
+    G = rng.normal(size=(o, o, v, v))
+    G = G + G.transpose(1, 0, 3, 2)
+    tau = rng.normal(size=(v, v, o, o))
+    ref = np.einsum("klcd,cdij->klij", G, tau)
+    got = _pair_symmetric_ladder(G, tau)
+    assert np.abs(got - ref).max() <= 1e-13 * max(1.0, np.abs(ref).max())
