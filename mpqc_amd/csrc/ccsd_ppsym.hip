@@ -1,4 +1,5 @@
-// Particle-particle ladder of the split forms on pair-symmetric blocks.
+// Particle-particle ladder (and the hole-hole intermediate's G tau term) on
+// pair-symmetric blocks, in the FP64 and split forms.
 //
 // b(a,b,i,j) = sum_cd Gabcd(a,b,c,d) tau(c,d,i,j) (ccsd_r1_r2.h:275).  With the
 // integrals' symmetry Gabcd(b,a,d,c) = Gabcd(a,b,c,d) (both are (ac|bd)) and no
@@ -11,7 +12,10 @@
 // with V+ symmetric and V- antisymmetric under a<->b, so rows a<=b (P) and a<b (M)
 // carry everything; s(a,b) = +1 for a<b, -1 for a>b.  The two GEMMs
 // (v(v+1)/2 x o^2 x v(v+1)/2 and v(v-1)/2 x o^2 x v(v-1)/2) hold half the FLOPs of
-// the full contraction (2 o^2 v^4).
+// the full contraction (2 o^2 v^4).  W(k,l,i,j) += Gijab(k,l,c,d) tau(c,d,i,j)
+// (ccsd_intermediates.h:182-193) has the same form with Gijab(l,k,d,c) = Gijab(k,l,c,d),
+// rows k<=l, on the same S and A.  The context checks each symmetry once per set of
+// integrals and keeps the full contraction where it does not hold.
 //
 // Pairs are numbered row-major over the upper triangle: (p,q), p<=q < n has index
 // p n - p(p-1)/2 + q - p; (p,q), p<q < n has index p (n-1) - p(p-1)/2 + q - p - 1.
