@@ -272,18 +272,14 @@ void CCSDContext::set_option(const std::string& key, double value) {
     if (!(value >= 1 << 20)) throw std::invalid_argument("df_block_bytes must be at least 1 MiB");
     df_block_bytes_ = value;
     df_block_ = 0;
+    release_planes();  // the df ladder's block planes are sized by the block
   } else if (key == "precision") {
     const int p = (int)value;
     if (p < 0 || p > 4 || p != value) throw std::invalid_argument("unknown precision");
     if ((GemmPrecision)p != prec_) {
       prec_ = (GemmPrecision)p;
       planes_ready_ = false;
-      if (Gabcd_P_.p[0]) {  // plane kind may change: re-allocate
-        for (auto& pa : plane_allocs_) hipFree(pa.first);
-        plane_allocs_.clear();
-        Gabcd_P_ = Planes();
-        Vp_P_ = Planes();
-      }
+      release_planes();  // the plane kind may change: every plane buffer is re-allocated
     }
   } else {
     throw std::invalid_argument("unknown option: " + key);
@@ -692,6 +688,26 @@ PlaneKind CCSDContext::plane_kind() const {
   }
 }
 
+void CCSDContext::ensure_planes(Planes& P, long n) {
+  if (P.p[0] && P.n >= n) return;
+  if (P.p[0]) throw std::logic_error("plane buffer re-sized without release_planes()");
+  P = alloc_planes(n);
+}
+
+// frees every 16-bit plane buffer and forgets every Planes handle that pointed into them
+void CCSDContext::release_planes() {
+  if (!stream_) return;
+  MPQC_HIP_CHECK(hipStreamSynchronize(stream_));
+  for (auto& pa : plane_allocs_) hipFree(pa.first);
+  plane_allocs_.clear();
+  for (Planes* P : {&Gabcd_P_, &Giabc_P_, &Gijab_P_, &GAt_P_, &GBt_P_, &GCt_P_, &Gp_P_sym_, &Gm_P_sym_, &Vp_P_,
+                    &Vm_P_, &Sp_P_, &Ap_P_, &GbarT_P_, &Gbar_P_, &Gbar_akcd_P_, &P8T_P_, &GiabcT_P_, &P13_P_,
+                    &P14_P_, &tau_P_, &tauT_P_, &TP_P_, &t2P_P_, &Xjk_P_, &K_P_, &Y1t_P_, &Y2t_P_, &WT_P_,
+                    &t1T_P_, &gac_P_, &tau_ikcd_P_, &tau_ackl_P_, &Gp_P_})
+    *P = Planes();
+  planes_ready_ = false;
+}
+
 Planes CCSDContext::alloc_planes(long n) {
   Planes P;
   P.n = n;
@@ -735,18 +751,18 @@ void CCSDContext::gemmP(int M, int N, int K, double alpha, const Planes& A, long
 void CCSDContext::prepare_planes() {
   const long O = o_, V = v_;
   const long oovv = O * O * V * V, ovvv = O * V * V * V, vvvv = V * V * V * V, oooo = O * O * O * O;
-  if (!Gabcd_P_.p[0]) {
-    Gabcd_P_ = alloc_planes(vvvv); Giabc_P_ = alloc_planes(ovvv); Gijab_P_ = alloc_planes(oovv);
-    GAt_P_ = alloc_planes(oovv); GBt_P_ = alloc_planes(oovv); GCt_P_ = alloc_planes(oovv);
-    tau_P_ = alloc_planes(oovv); tauT_P_ = alloc_planes(oovv); TP_P_ = alloc_planes(oovv);
-    t2P_P_ = alloc_planes(oovv); Xjk_P_ = alloc_planes(oovv); K_P_ = alloc_planes(oovv);
-    Y1t_P_ = alloc_planes(oovv); Y2t_P_ = alloc_planes(oovv); WT_P_ = alloc_planes(oooo);
-    GbarT_P_ = alloc_planes(oovv); Gbar_P_ = alloc_planes(oovv); Gbar_akcd_P_ = alloc_planes(ovvv);
-    tau_ikcd_P_ = alloc_planes(oovv); tau_ackl_P_ = alloc_planes(oovv);
-    P8T_P_ = alloc_planes(O * O * O * V); GiabcT_P_ = alloc_planes(ovvv);
-    t1T_P_ = alloc_planes(O * V); gac_P_ = alloc_planes(V * V);
-    P13_P_ = alloc_planes(O * O * O * V); P14_P_ = alloc_planes(ovvv);
-  }
+  // o^2 v^2 and smaller planes: every split form uses them
+  for (Planes* P : {&Gijab_P_, &GAt_P_, &GBt_P_, &GCt_P_, &tau_P_, &tauT_P_, &TP_P_, &t2P_P_, &Xjk_P_, &K_P_,
+                    &Y1t_P_, &Y2t_P_, &GbarT_P_, &Gbar_P_, &tau_ikcd_P_, &tau_ackl_P_})
+    ensure_planes(*P, oovv);
+  ensure_planes(WT_P_, oooo);
+  ensure_planes(P8T_P_, O * O * O * V);
+  ensure_planes(P13_P_, O * O * O * V);
+  ensure_planes(t1T_P_, O * V);
+  ensure_planes(gac_P_, V * V);
+  // o v^3 planes: only where Giabc is an input (the df method builds those terms from the factors)
+  if (!df_)
+    for (Planes* P : {&Giabc_P_, &GiabcT_P_, &Gbar_akcd_P_, &P14_P_}) ensure_planes(*P, ovvv);
   {  // split-K workspace for the largest plane contraction shapes
     const int o = o_, v = v_;
     size_t need = 0;
@@ -773,10 +789,13 @@ void CCSDContext::prepare_planes() {
     // (the packed operands' plane split indexes a (pairs x pitch) plane with 32 bits)
     const long nvs = V * (V + 1) / 2, ldvs = (nvs + 63) / 64 * 64;
     pp_sym_planes_ = nvs * ldvs < (1L << 31) && pp_symmetric_usable();
-    if (pp_sym_planes_)
+    if (pp_sym_planes_) {
       prepare_pp_symmetric();
-    else
+    } else {
+      // the full contraction's v^4 planes, only where the pair-symmetric form is not usable
+      ensure_planes(Gabcd_P_, vvvv);
       split(Gabcd_P_, {V, V, V, V}, {{Gabcd_, {V, V, V, V}, I, 1.0}});
+    }
     split(Giabc_P_, {O, V, V, V}, {{Giabc_, {O, V, V, V}, I, 1.0}});
   }
   split(Gijab_P_, {O, O, V, V}, {{Gijab_, e_ijab, I, 1.0}});

@@ -232,6 +232,8 @@ class CCSDContext {
   bool split_mode() const;
   PlaneKind plane_kind() const;
   Planes alloc_planes(long n);
+  void ensure_planes(Planes& P, long n);  // allocates P (n values per plane) unless it already is
+  void release_planes();
   void split(const Planes& out, std::array<long, 4> n, std::initializer_list<Src> srcs,
              const TProd* prod = nullptr, const char* tag = nullptr, int line = __builtin_LINE());
   TProd tp_product() const;  // t1(d,i) t1(a,l) over TP's (a,i,l,d) index order

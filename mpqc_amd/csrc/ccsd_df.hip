@@ -101,7 +101,7 @@ void CCSDContext::df_pp_ladder() {
   ensure_df_blocks();
   const long nbmax = df_block_;
   const bool sm = split_mode();
-  if (sm && !Gp_P_.p[0]) Gp_P_ = alloc_planes(nbmax * V * vv);
+  if (sm) ensure_planes(Gp_P_, nbmax * V * vv);
   for (long a0 = 0; a0 < V; a0 += nbmax) {
     const long nb = std::min(nbmax, V - a0);
     gemm(true, false, (int)(nb * V), (int)vv, (int)Q, 1.0, Xab_ + a0 * V, vv, Ydf_, vv, 0.0, Gblk_, vv,

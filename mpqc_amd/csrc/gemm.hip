@@ -419,8 +419,16 @@ __global__ void splitk_reduce_wide_kernel(const double* __restrict__ W, double* 
   }
 }
 
-template <int BM, int BN>
-void launch(const GemmDesc& d, GemmPrecision prec, const GemmWorkspace& ws, hipStream_t stream) {
+// Split-K plan of one GEMM: K slices per output tile, their length, and how many
+// batch entries share one launch (bounded by the workspace that holds the
+// partial sums).
+struct SplitKPlan {
+  int nsplit = 1, kchunk = 0, max_batch_per_launch = 65535;
+  size_t scratch_bytes = 0;  // workspace the launches touch
+};
+
+SplitKPlan plan_splitk(const GemmDesc& d, int BM, int BN, size_t ws_bytes) {
+  SplitKPlan pl;
   const int tiles_m = (int)ceil_div(d.M, BM), tiles_n = (int)ceil_div(d.N, BN);
   const long tiles = (long)tiles_m * tiles_n;
   // split-K: aim for ~4 workgroups per CU (256 CUs) when the output is small,
@@ -433,15 +441,39 @@ void launch(const GemmDesc& d, GemmPrecision prec, const GemmWorkspace& ws, hipS
   }
   const size_t slice = (size_t)d.M * d.N * sizeof(double);
   // the workspace must hold nsplit partial outputs of at least one batch entry
-  if (nsplit > 1 && ws.bytes / slice < (size_t)nsplit)
-    nsplit = (int)std::max<size_t>(1, ws.bytes / slice);
+  if (nsplit > 1 && ws_bytes / slice < (size_t)nsplit)
+    nsplit = (int)std::max<size_t>(1, ws_bytes / slice);
   int kchunk = (int)(ceil_div(ceil_div(d.K, nsplit), kBK) * kBK);
   if (nsplit > 1) nsplit = (int)ceil_div(d.K, kchunk);
   if (nsplit == 1) kchunk = (int)(ceil_div(d.K, kBK) * kBK);
   int max_batch_per_launch = 65535 / nsplit;
   if (nsplit > 1)  // workspace capacity limits how many batches go in one launch
-    max_batch_per_launch = (int)std::min<size_t>(max_batch_per_launch, ws.bytes / (slice * nsplit));
+    max_batch_per_launch = (int)std::min<size_t>(max_batch_per_launch, ws_bytes / (slice * nsplit));
   if (d.K == 0) kchunk = kBK;
+  pl.nsplit = nsplit;
+  pl.kchunk = kchunk;
+  pl.max_batch_per_launch = max_batch_per_launch;
+  pl.scratch_bytes = nsplit > 1 ? slice * nsplit * (size_t)std::min(d.batch, max_batch_per_launch) : 0;
+  return pl;
+}
+
+// 128-wide tiles unless 64-wide ones pad the output much less
+bool wide_tiles(const GemmDesc& d) {
+  bool wide = d.M > 64 && d.N > 64;
+  if (wide) {
+    const double a128 = (double)ceil_div(d.M, 128) * ceil_div(d.N, 128) * 128 * 128;
+    const double a64 = (double)ceil_div(d.M, 64) * ceil_div(d.N, 64) * 64 * 64;
+    wide = a128 <= 1.25 * a64;
+  }
+  return wide;
+}
+
+template <int BM, int BN>
+void launch(const GemmDesc& d, GemmPrecision prec, const GemmWorkspace& ws, hipStream_t stream) {
+  const int tiles_m = (int)ceil_div(d.M, BM), tiles_n = (int)ceil_div(d.N, BN);
+  const long tiles = (long)tiles_m * tiles_n;
+  const SplitKPlan plan = plan_splitk(d, BM, BN, ws.bytes);
+  const int nsplit = plan.nsplit, kchunk = plan.kchunk, max_batch_per_launch = plan.max_batch_per_launch;
   for (int b0 = 0; b0 < d.batch; b0 += max_batch_per_launch) {
     const int nb = std::min(max_batch_per_launch, d.batch - b0);
     KParams p;
@@ -680,17 +712,18 @@ void gemm(const GemmDesc& d, GemmPrecision prec, const GemmWorkspace& ws, hipStr
     gemm_split(d, prec, ws, stream);
     return;
   }
-  // 128-wide tiles unless 64-wide ones pad the output much less
-  bool wide = d.M > 64 && d.N > 64;
-  if (wide) {
-    const double a128 = (double)ceil_div(d.M, 128) * ceil_div(d.N, 128) * 128 * 128;
-    const double a64 = (double)ceil_div(d.M, 64) * ceil_div(d.N, 64) * 64 * 64;
-    wide = a128 <= 1.25 * a64;
-  }
-  if (wide)
+  if (wide_tiles(d))
     launch<128, 128>(d, prec, ws, stream);
   else
     launch<64, 64>(d, prec, ws, stream);
+}
+
+size_t gemm_scratch_bytes(const GemmDesc& d, GemmPrecision prec, size_t ws_bytes) {
+  if (d.M <= 0 || d.N <= 0 || d.batch <= 0 || d.K <= 0) return 0;
+  if (prec == GemmPrecision::BF16X3 || prec == GemmPrecision::F16X3 || prec == GemmPrecision::BF16X6)
+    return gemm_split_scratch_bytes(d, prec, ws_bytes);
+  const bool wide = wide_tiles(d);
+  return plan_splitk(d, wide ? 128 : 64, wide ? 128 : 64, ws_bytes).scratch_bytes;
 }
 
 }  // namespace mpqc_hip

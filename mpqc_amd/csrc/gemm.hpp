@@ -83,6 +83,11 @@ void free_grouped_gemm(GroupedGemmPlan& plan);
 // Split-precision paths (BF16X3, F16X3, BF16X6); called by gemm().
 void gemm_split(const GemmDesc& d, GemmPrecision prec, const GemmWorkspace& ws, hipStream_t stream);
 
+// Bytes of the workspace that gemm(d, prec, {ptr, ws_bytes}, ...) reads or writes
+// (split-K partial sums, and the split forms' reduction head); 0 when it uses none.
+size_t gemm_scratch_bytes(const GemmDesc& d, GemmPrecision prec, size_t ws_bytes);
+size_t gemm_split_scratch_bytes(const GemmDesc& d, GemmPrecision prec, size_t ws_bytes);
+
 // Convenience for plain (non-batched) calls.
 inline GemmDesc make_gemm(bool tA, bool tB, int M, int N, int K, double alpha, const double* A,
                           long lda, const double* B, long ldb, double beta, double* C, long ldc) {

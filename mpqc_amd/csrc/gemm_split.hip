@@ -375,22 +375,18 @@ __global__ void splitk_reduce2(const double* __restrict__ W, double* __restrict_
   }
 }
 
-template <int BM, int BN, int S>
-void launch_split(const GemmDesc& d, const GemmWorkspace& ws, hipStream_t stream) {
-  // workspace layout: [0, 2048) reduction partials, [2048, 2050) max|A|, max|B|,
-  // [4096, ...) split-K partial sums
-  constexpr long kHead = 4096;
-  if (ws.ptr == nullptr || ws.bytes < (size_t)kHead * sizeof(double))
-    throw std::invalid_argument("split GEMM needs a workspace");
-  double* partials = ws.ptr;
-  double* amax = ws.ptr + 2048;
-  double* bmax = ws.ptr + 2049;
-  double* wsplit = ws.ptr + kHead;
-  const size_t wbytes = ws.bytes - kHead * sizeof(double);
-  if (SplitTraits<S>::f16) {
-    absmax(d.A, d.transA, d.M, d.K, d.lda, d.sA, d.batch, partials, amax, stream);
-    absmax(d.B, !d.transB, d.N, d.K, d.ldb, d.sB, d.batch, partials, bmax, stream);
-  }
+// workspace layout: [0, 2048) reduction partials, [2048, 2050) max|A|, max|B|,
+// [4096, ...) split-K partial sums
+constexpr long kHead = 4096;
+
+struct SplitPlan {
+  int nsplit = 1, kchunk = 0, max_batch_per_launch = 65535;
+  size_t scratch_bytes = 0;  // workspace the launches touch (head included)
+};
+
+SplitPlan plan_split(const GemmDesc& d, int BM, int BN, size_t ws_bytes) {
+  SplitPlan pl;
+  const size_t wbytes = ws_bytes > (size_t)kHead * sizeof(double) ? ws_bytes - kHead * sizeof(double) : 0;
   const int tiles_m = (int)ceil_div(d.M, BM), tiles_n = (int)ceil_div(d.N, BN);
   const long tiles = (long)tiles_m * tiles_n;
   int nsplit = 1;
@@ -413,6 +409,30 @@ void launch_split(const GemmDesc& d, const GemmWorkspace& ws, hipStream_t stream
       max_batch_per_launch = (int)std::min<size_t>(max_batch_per_launch, cap);
     }
   }
+  pl.nsplit = nsplit;
+  pl.kchunk = kchunk;
+  pl.max_batch_per_launch = max_batch_per_launch;
+  pl.scratch_bytes = kHead * sizeof(double) +
+                     (nsplit > 1 ? slice * nsplit * (size_t)std::min(d.batch, max_batch_per_launch) : 0);
+  return pl;
+}
+
+template <int BM, int BN, int S>
+void launch_split(const GemmDesc& d, const GemmWorkspace& ws, hipStream_t stream) {
+  if (ws.ptr == nullptr || ws.bytes < (size_t)kHead * sizeof(double))
+    throw std::invalid_argument("split GEMM needs a workspace");
+  double* partials = ws.ptr;
+  double* amax = ws.ptr + 2048;
+  double* bmax = ws.ptr + 2049;
+  double* wsplit = ws.ptr + kHead;
+  if (SplitTraits<S>::f16) {
+    absmax(d.A, d.transA, d.M, d.K, d.lda, d.sA, d.batch, partials, amax, stream);
+    absmax(d.B, !d.transB, d.N, d.K, d.ldb, d.sB, d.batch, partials, bmax, stream);
+  }
+  const int tiles_n = (int)ceil_div(d.N, BN);
+  const long tiles = (long)ceil_div(d.M, BM) * tiles_n;
+  const SplitPlan plan = plan_split(d, BM, BN, ws.bytes);
+  const int nsplit = plan.nsplit, kchunk = plan.kchunk, max_batch_per_launch = plan.max_batch_per_launch;
   for (int b0 = 0; b0 < d.batch; b0 += max_batch_per_launch) {
     const int nb = std::min(max_batch_per_launch, d.batch - b0);
     SParams p;
@@ -445,6 +465,12 @@ void dispatch_split(const GemmDesc& d, const GemmWorkspace& ws, hipStream_t stre
 }
 
 }  // namespace
+
+size_t gemm_split_scratch_bytes(const GemmDesc& d, GemmPrecision prec, size_t ws_bytes) {
+  (void)prec;  // every split kind tiles the same way
+  const bool small = d.M <= 64 || d.N <= 64;
+  return plan_split(d, small ? 64 : 128, small ? 64 : 128, ws_bytes).scratch_bytes;
+}
 
 void gemm_split(const GemmDesc& d, GemmPrecision prec, const GemmWorkspace& ws, hipStream_t stream) {
   switch (prec) {

@@ -131,3 +131,28 @@ def test_df_rejects_misuse():
         c.set_orbital_energies(np.diag(F)[:3], np.diag(F)[3:])
         with pytest.raises(L.CCSDError):                 # Xab missing
             c.compute_residuals(t1, t2)
+
+
+def test_df_split_residual_after_precision_switch():
+    """A live context switched between split forms (f16x3: two planes, then bf16x3:
+    three, then back) re-allocates every plane buffer, the df ladder's block planes
+    included; each residual matches the DF oracle."""
+    from mpqc_amd.ccsd import CCSD, PRECISIONS
+    o, v, naux = 4, 13, 29
+    F, Xai, Xij, Xab, t1, t2 = _random_factors(o, v, naux, seed=o * 100 + v + naux)
+    ints = _df_ints(o, v, F, Xai, Xij, Xab)
+    tau = O.make_tau(t1, t2)
+    r1_ref = D.cs_ccsd_r1_df(t1, t2, tau, ints)
+    r2_ref = D.cs_ccsd_r2_df(t1, t2, tau, ints)
+    with CCSD(o, v, precision="f16x3") as c:
+        c.set_df_basis(naux)
+        c.set_integrals(_blocks(ints))
+        c.set_orbital_energies(np.diag(F)[:o], np.diag(F)[o:])
+        for prec in ("f16x3", "bf16x3", "f16x3", "f64"):
+            c.set_option("precision", PRECISIONS[prec])
+            r1, r2 = c.compute_residuals(t1, t2)
+            e1, e2 = _rel(r1, r1_ref), _rel(r2, r2_ref)
+            print(f"{prec}: r1 {e1:.2e} r2 {e2:.2e}", flush=True)
+            assert e1 < 1e-4 and e2 < 1e-4
+            if prec == "f64":
+                assert e1 < 1e-13 and e2 < 1e-13
