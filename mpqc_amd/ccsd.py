@@ -112,12 +112,17 @@ class CCSD:
     def __init__(self, o: int, v: int, precision: str = "f64", max_iter: int = 30,
                  occ_block_size: int = 24, unocc_block_size: int = 24, diis_start: int = 1,
                  n_diis: int = 8, diis_damp: float = 0.0, diis_ngroup: int = 1,
-                 diis_group_nstart: int = 1, fov_sign: float = 1.0):
+                 diis_group_nstart: int = 1, fov_sign: float = 1.0,
+                 occ_hashmarks: Optional[Sequence[int]] = None,
+                 vir_hashmarks: Optional[Sequence[int]] = None):
         self.lib = L.load()
         self.o, self.v = int(o), int(v)
         self.max_iter = int(max_iter)
-        self.occ_hm = compute_trange1(self.o, occ_block_size)
-        self.vir_hm = compute_trange1(self.v, unocc_block_size)
+        # a TiledRange1 given as hashmarks (e.g. uneven tiles) replaces the block size
+        self.occ_hm = (list(map(int, occ_hashmarks)) if occ_hashmarks is not None
+                       else compute_trange1(self.o, occ_block_size))
+        self.vir_hm = (list(map(int, vir_hashmarks)) if vir_hashmarks is not None
+                       else compute_trange1(self.v, unocc_block_size))
         occ = (ctypes.c_int64 * len(self.occ_hm))(*self.occ_hm)
         vir = (ctypes.c_int64 * len(self.vir_hm))(*self.vir_hm)
         ctx = ctypes.c_void_p()
@@ -198,9 +203,12 @@ class CCSD:
             self.ctx, ids[name], a.ctypes.data_as(ctypes.c_void_p),
             L.LAYOUT_TA_TILES if layout == "ta" else L.LAYOUT_DENSE, L.MEM_HOST))
 
-    def set_integral_device(self, name: str, dev_ptr: int):
+    def set_integral_device(self, name: str, dev_ptr: int, layout: str = "dense"):
+        """A block already in device memory at ``dev_ptr`` (dense, or TA tiles of
+        this context's TiledRange1s)."""
         self._check(self.lib.mpqc_ccsd_set_integral(self.ctx, L.INTEGRAL_IDS[name],
-                                                    ctypes.c_void_p(dev_ptr), L.LAYOUT_DENSE,
+                                                    ctypes.c_void_p(dev_ptr),
+                                                    L.LAYOUT_TA_TILES if layout == "ta" else L.LAYOUT_DENSE,
                                                     L.MEM_DEVICE))
 
     def set_integrals(self, ints, layout: str = "dense"):

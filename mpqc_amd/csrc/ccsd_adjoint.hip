@@ -91,7 +91,7 @@ void CCSDContext::zero_adjoints() {
 }
 
 void CCSDContext::record_residual() {
-  if (split_mode()) throw std::invalid_argument("the adjoint runs in the f64 and f32 forms");
+  if (split_mode()) throw std::logic_error("record_residual: the tape records FP64 or FP32 contractions");
   if (df_) throw std::invalid_argument("the adjoint runs on the standard method");
   res_tape_.clear();
   tape_ = &res_tape_;
@@ -304,6 +304,7 @@ void CCSDContext::energy_t_gradient(double* g1, double* g2) {
 
 void CCSDContext::jacobian_transpose(const double* y1, const double* y2, double* x1, double* x2) {
   const long n1 = (long)o_ * v_, n2 = n1 * n1;
+  TapePrecision tp(*this);
   record_residual();
   zero_adjoints();
   MPQC_HIP_CHECK(hipMemcpyAsync(adj(r1_), y1, n1 * sizeof(double), hipMemcpyDeviceToDevice, stream_));
@@ -336,6 +337,7 @@ int CCSDContext::lambda_solve(int max_iter, double target_precision, int verbose
   if (max_iter < 1) throw std::invalid_argument("max_iter must be positive");
   const long n1 = (long)o_ * v_, n2 = n1 * n1;
   lambda_ready_ = gradient_ready_ = false;
+  TapePrecision tp(*this);
   record_residual();
   if (!lam1_) {
     lam1_ = alloc(n1); lam2_ = alloc(n2); gt1_ = alloc(n1); gt2_ = alloc(n2);
@@ -463,6 +465,7 @@ int CCSDContext::lambda_solve(int max_iter, double target_precision, int verbose
 void CCSDContext::energy_gradient() {
   if (!lambda_ready_) throw std::logic_error("solve the Lambda equations first");
   const long O = o_, V = v_, n1 = O * V, n2 = n1 * n1;
+  TapePrecision tp(*this);
   zero_adjoints();
   MPQC_HIP_CHECK(hipMemcpyAsync(adj(r1_), lam1_, n1 * sizeof(double), hipMemcpyDeviceToDevice, stream_));
   MPQC_HIP_CHECK(hipMemcpyAsync(adj(r2_), lam2_, n2 * sizeof(double), hipMemcpyDeviceToDevice, stream_));

@@ -230,6 +230,19 @@ class CCSDContext {
 
   // ---- mixed precision: 16-bit planes of the operands of the large contractions
   bool split_mode() const;
+  // The op tape (adjoint, EOM) records and sweeps FP64 contractions in the
+  // split forms: the 16-bit plane contractions are not on the tape.  The
+  // guard switches the contraction precision for one public call.
+  struct TapePrecision {
+    CCSDContext& c;
+    GemmPrecision saved;
+    explicit TapePrecision(CCSDContext& x) : c(x), saved(x.prec_) {
+      if (x.split_mode()) x.prec_ = GemmPrecision::F64;
+    }
+    ~TapePrecision() { c.prec_ = saved; }
+    TapePrecision(const TapePrecision&) = delete;
+    TapePrecision& operator=(const TapePrecision&) = delete;
+  };
   PlaneKind plane_kind() const;
   Planes alloc_planes(long n);
   void ensure_planes(Planes& P, long n);  // allocates P (n values per plane) unless it already is

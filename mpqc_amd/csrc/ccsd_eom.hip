@@ -180,7 +180,7 @@ int CCSDContext::eom_ee(int n_roots, int max_iter, double convergence, int verbo
   if (max_iter < 1) throw std::invalid_argument("max_iter must be positive");
   if (!(convergence > 0.0)) throw std::invalid_argument("convergence must be positive");
   if (!energies) throw std::invalid_argument("null energies buffer");
-  if (split_mode()) throw std::invalid_argument("EOM runs in the f64 and f32 forms");
+  TapePrecision tp(*this);  // the Jacobian products are reverse sweeps of the (FP64 in split forms) tape
   const long O = o_, V = v_, n1 = O * V, n2 = n1 * n1, n = n1 + n2;
   if (n_roots > n1) throw std::invalid_argument("n_roots exceeds the number of single excitations");
   const int max_vector = std::max(eom_max_vector_, 2);

@@ -1,0 +1,124 @@
+"""The batched configuration at its full size: the 256 seeded H chains (n even in
+8..32, spacing 1.6-2.0 bohr, cc-pVDZ: o = n/2 = 4..16, v = 4.5 n = 36..144), one
+CCSD residual each (cc::compute_cs_ccsd_r1 + r2, ccsd_r1_r2.h:70-293) in grouped
+launches, in groups that fit a device-memory budget, in FP64 and FP32.
+
+Every chain's batched residual is compared with the single-molecule GPU path on
+the same integrals and amplitudes, and the chains with o <= 6 also with the
+oracle.  Integrals are seeded random blocks of each chain's shapes, generated on
+the device from a per-chain seed (the host front-end cannot produce the ~430 GB
+of MO integrals of the whole batch in a test).
+
+Bars (relative to the largest residual element): FP64 1e-12 (sums of different
+order); FP32 5e-6 between the two paths and 2e-6 against the oracle (the FP32
+contractions' error at these sizes).
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+BUDGET_GIB = 100.0
+TOL = {"f64": (1e-12, 1e-12), "f32": (5e-6, 2e-6)}   # (vs single-molecule path, vs oracle)
+
+
+def _chain_sizes():
+    from mpqc_amd.chem.molecule import h_chain_batch
+    return [(len(m.symbols) // 2, 9 * len(m.symbols) // 2) for m in h_chain_batch(count=256, seed=3)]
+
+
+def _context_doubles(o, v):
+    ov, oo, vv = o * v, o * o, v * v
+    return vv * vv + 3 * ov * vv + 35 * ov * ov + 10 * oo * ov + 3 * oo * oo + 64 * max(vv, ov, oo * oo, 4096)
+
+
+def _chain_inputs(torch, k, o, v):
+    """Seeded device blocks (Fock diagonals in the workload's ranges) and amplitudes of chain k."""
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(1000 + k)
+    shapes = {"Fia": (o, v), "Fij": (o, o), "Fab": (v, v), "Gijab": (o, o, v, v), "Gijkl": (o, o, o, o),
+              "Giajb": (o, v, o, v), "Giabc": (o, v, v, v), "Gijka": (o, o, o, v), "Gabcd": (v, v, v, v)}
+    b = {n: 0.01 * torch.randn(s, dtype=torch.float64, device="cuda", generator=gen) for n, s in shapes.items()}
+    eo = -0.3 - 1.2 * torch.rand(o, dtype=torch.float64, device="cuda", generator=gen)
+    ev = 0.1 + 2.9 * torch.rand(v, dtype=torch.float64, device="cuda", generator=gen)
+    b["Fij"] += torch.diag(eo)
+    b["Fab"] += torch.diag(ev)
+    t1 = 0.01 * torch.randn((v, o), dtype=torch.float64, device="cuda", generator=gen)
+    t2 = 0.01 * torch.randn((v, v, o, o), dtype=torch.float64, device="cuda", generator=gen)
+    return b, eo, ev, t1, t2
+
+
+def _rel(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_256_hchains_batched_vs_single_and_oracle(prec):
+    torch = pytest.importorskip("torch")
+    from mpqc_amd import _lib as L
+    from mpqc_amd.batch import CCSDBatch
+    from mpqc_amd.ccsd import CCSD
+    from oracle import ccsd as O
+    sizes = _chain_sizes()
+    assert len(sizes) == 256 and min(sizes) >= (4, 36) and max(sizes) <= (16, 144)
+    budget = BUDGET_GIB * (1 << 30) / 8
+    order = sorted(range(len(sizes)), key=lambda k: sizes[k])
+    groups, cur, mem = [], [], 0.0
+    for k in order:
+        need = _context_doubles(*sizes[k])
+        if cur and mem + need > budget:
+            groups.append(cur)
+            cur, mem = [], 0.0
+        cur.append(k)
+        mem += need
+    groups.append(cur)
+    tol_single, tol_oracle = TOL[prec]
+    worst_single = worst_oracle = 0.0
+    n_oracle = 0
+    for gi, grp in enumerate(groups):
+        print(f"{prec} group {gi}: {len(grp)} chains, o {sizes[grp[0]][0]}..{sizes[grp[-1]][0]}", flush=True)
+        with CCSDBatch([sizes[k][0] for k in grp], [sizes[k][1] for k in grp], precision=prec) as B:
+            for j, k in enumerate(grp):
+                o, v = sizes[k]
+                b, _, _, t1, t2 = _chain_inputs(torch, k, o, v)
+                for name, idx in L.INTEGRAL_IDS.items():
+                    B._check(B.lib.mpqc_ccsd_batch_set_integral(B.h, j, idx, ctypes.c_void_p(b[name].data_ptr()),
+                                                                L.LAYOUT_DENSE, L.MEM_DEVICE))
+                B._check(B.lib.mpqc_ccsd_batch_set_amplitudes(B.h, j, ctypes.c_void_p(t1.data_ptr()),
+                                                              ctypes.c_void_p(t2.data_ptr()),
+                                                              L.LAYOUT_DENSE, L.MEM_DEVICE))
+                torch.cuda.synchronize()
+                del b, t1, t2
+            B.residual()
+            for j, k in enumerate(grp):
+                o, v = sizes[k]
+                r1b, r2b = B.get_residuals(j)
+                b, eo, ev, t1, t2 = _chain_inputs(torch, k, o, v)
+                with CCSD(o, v, precision=prec) as c:
+                    for name in L.INTEGRAL_IDS:
+                        c.set_integral_device(name, b[name].data_ptr())
+                    c.set_orbital_energies(eo.cpu().numpy(), ev.cpu().numpy())
+                    r1s, r2s = c.compute_residuals(t1.cpu().numpy(), t2.cpu().numpy())
+                e = max(_rel(r1b, r1s), _rel(r2b, r2s))
+                worst_single = max(worst_single, e)
+                assert e < tol_single, (k, o, v, e)
+                if o <= 6:
+                    hb = {n: x.cpu().numpy() for n, x in b.items()}
+                    ints = O.Integrals(**hb)
+                    a1, a2 = t1.cpu().numpy(), t2.cpu().numpy()
+                    tau = O.make_tau(a1, a2)
+                    r1o = O.cs_ccsd_r1(a1, a2, tau, ints)
+                    r2o = O.cs_ccsd_r2(a1, a2, tau, ints)
+                    e = max(_rel(r1b, r1o), _rel(r2b, r2o))
+                    worst_oracle = max(worst_oracle, e)
+                    n_oracle += 1
+                    assert e < tol_oracle, (k, o, v, e)
+                del b, t1, t2
+        torch.cuda.empty_cache()
+    print(f"{prec}: 256 chains in {len(groups)} groups; max rel err vs single-molecule path {worst_single:.2e}, "
+          f"vs oracle ({n_oracle} chains with o <= 6) {worst_oracle:.2e}")
+    assert n_oracle > 0
+    if prec == "f32":
+        assert worst_oracle > 1e-12   # the reduced-precision contractions really ran
