@@ -327,6 +327,21 @@ int mpqc_ccsd_gemm(int transa, int transb, int m, int n, int k, double alpha, co
                    double beta, double* c, int64_t ldc, int64_t stride_c, int batch, int precision,
                    void* stream);
 
+/* mpqc_ccsd_gemm without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, kernels and results as mpqc_ccsd_gemm; its split-K partial
+ * sums live in a fixed 64 MiB device array of the library (one per GPU, held
+ * until the process ends) instead of a workspace the library allocates.  The
+ * split-K plan is made to fit that array; a GEMM whose plan would need more is
+ * refused with MPQC_CCSD_ERR_ARG.  Every call on one GPU shares that GPU's
+ * array: calls that overlap on one GPU (other streams, other threads) race and
+ * must be serialised by the caller. */
+int mpqc_ccsd_gemm_noalloc(int transa, int transb, int m, int n, int k, double alpha, const double* a,
+                           int64_t lda, int64_t stride_a, const double* b, int64_t ldb, int64_t stride_b,
+                           double beta, double* c, int64_t ldc, int64_t stride_c, int batch,
+                           int precision, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -33,7 +33,7 @@ EXPORTS = [
     "mpqc_ccsd_solver_init", "mpqc_ccsd_set_amplitudes", "mpqc_ccsd_get_amplitudes",
     "mpqc_ccsd_get_residuals", "mpqc_ccsd_get_dressed_fock", "mpqc_ccsd_init_guess",
     "mpqc_ccsd_residual", "mpqc_ccsd_energy_error", "mpqc_ccsd_update", "mpqc_ccsd_solve",
-    "mpqc_ccsd_get_trace", "mpqc_ccsd_pp_ladder", "mpqc_ccsd_gemm",
+    "mpqc_ccsd_get_trace", "mpqc_ccsd_pp_ladder", "mpqc_ccsd_gemm", "mpqc_ccsd_gemm_noalloc",
     "mpqc_ccsd_get_kernel_times", "mpqc_ccsd_jacobian_transpose", "mpqc_ccsd_lambda_solve",
     "mpqc_ccsd_get_lambda", "mpqc_ccsd_energy_gradient", "mpqc_ccsd_get_integral_gradient",
     "mpqc_ccsd_batch_create", "mpqc_ccsd_batch_destroy", "mpqc_ccsd_batch_last_error",
@@ -113,6 +113,7 @@ def load():
         "mpqc_ccsd_batch_time_stages": (I, [P, I]),
         "mpqc_ccsd_batch_get_stages": (I, [P, I, pI, P, P, P, P, P, P]),
         "mpqc_ccsd_gemm": (I, [I, I, I, I, I, D, P, I64, I64, P, I64, I64, D, P, I64, I64, I, I, P]),
+        "mpqc_ccsd_gemm_noalloc": (I, [I, I, I, I, I, D, P, I64, I64, P, I64, I64, D, P, I64, I64, I, I, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -119,6 +119,35 @@ mpqc_hip::GemmWorkspace workspace_for_current_device(size_t want) {
   return {g_ws.ptr[dev], g_ws.bytes[dev]};
 }
 
+// Scratch of the allocation-free GEMM entry: a fixed device array (one copy per
+// GPU, loaded with the library's code object and held until the process ends).
+// Host code never uses the array's name as an address: the device address of each
+// GPU's copy comes from hipGetSymbolAddress on that entry's first call on the
+// device and is kept per device.
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) double scratch[kNoAllocScratchBytes / sizeof(double)];
+
+struct NoAllocScratch {
+  std::mutex m;
+  double* ptr[64] = {nullptr};
+};
+NoAllocScratch g_scratch;
+
+mpqc_hip::GemmWorkspace noalloc_scratch_for_current_device() {
+  int dev = 0;
+  MPQC_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) throw std::runtime_error("device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch.m);
+  if (g_scratch.ptr[dev] == nullptr) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr)
+      throw mpqc_hip::HipError(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch.ptr[dev] = (double*)p;
+  }
+  return {g_scratch.ptr[dev], kNoAllocScratchBytes};
+}
+
 }  // namespace
 
 extern "C" {
@@ -454,6 +483,30 @@ int mpqc_ccsd_gemm(int transa, int transb, int m, int n, int k, double alpha, co
     const size_t want = std::max<size_t>((size_t)m * n * batch * 64 * sizeof(double), (size_t)64 << 20);
     auto ws = workspace_for_current_device(std::min<size_t>(want, (size_t)1 << 30));
     mpqc_hip::gemm(d, (mpqc_hip::GemmPrecision)precision, ws, (hipStream_t)stream);
+  });
+}
+
+int mpqc_ccsd_gemm_noalloc(int transa, int transb, int m, int n, int k, double alpha, const double* a,
+                           int64_t lda, int64_t stride_a, const double* b, int64_t ldb, int64_t stride_b,
+                           double beta, double* c, int64_t ldc, int64_t stride_c, int batch,
+                           int precision, void* stream) {
+  return guarded_free([&] {
+    if (m < 0 || n < 0 || k <= 0 || batch < 1) throw std::invalid_argument("invalid GEMM sizes");
+    if (!a || !b || !c) throw std::invalid_argument("null GEMM operand");
+    if (precision < 0 || precision > 4) throw std::invalid_argument("unknown precision");
+    mpqc_hip::GemmDesc d = mpqc_hip::make_gemm(transa != 0, transb != 0, m, n, k, alpha, a, lda, b,
+                                               ldb, beta, c, ldc);
+    d.sA = stride_a; d.sB = stride_b; d.sC = stride_c; d.batch = batch;
+    // the split-K plan is made for the fixed scratch (fewer K splits where the
+    // partial sums of more would not fit); a plan that still needs more is refused
+    const size_t need = mpqc_hip::gemm_scratch_bytes(d, (mpqc_hip::GemmPrecision)precision,
+                                                     kNoAllocScratchBytes);
+    if (need > kNoAllocScratchBytes)
+      throw std::invalid_argument("GEMM needs " + std::to_string(need) +
+                                  " bytes of scratch; the allocation-free entry holds at most " +
+                                  std::to_string(kNoAllocScratchBytes));
+    mpqc_hip::gemm(d, (mpqc_hip::GemmPrecision)precision, noalloc_scratch_for_current_device(),
+                   (hipStream_t)stream);
   });
 }
 
