@@ -73,7 +73,9 @@
  * the concatenation, in row-major tile-ordinal order, of row-major tiles over the
  * TiledRange1 hashmarks given at create time (occupied modes use the occupied
  * hashmarks, virtual modes the virtual ones).  MPQC_MEM_HOST / MPQC_MEM_DEVICE
- * says where a buffer lives.  Every function returns an MPQC_CCSD_* status; on
+ * says where a buffer lives; a device buffer is read or written only after the
+ * work already queued on the device (any stream) has finished, as with hipMemcpy,
+ * and the call returns once the copy is complete.  Every function returns an MPQC_CCSD_* status; on
  * failure mpqc_ccsd_last_error() describes it.  Calls on one context must be
  * serialised by the caller (one HIP stream per context).
  */

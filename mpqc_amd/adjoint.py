@@ -21,7 +21,7 @@ import numpy as np
 import torch
 
 from . import _lib as L
-from .ccsd import CCSD, Integrals
+from .ccsd import CCSD, PRECISIONS, Integrals
 
 NAMES = list(L.INTEGRAL_IDS)
 
@@ -40,8 +40,21 @@ class CCSDEnergy(torch.autograd.Function):
             c.set_integrals(Integrals(**{n: b.detach().cpu().numpy() for n, b in zip(NAMES, blocks)}))
             c.set_orbital_energies(np.asarray(eps_occ, dtype=np.float64),
                                    np.asarray(eps_vir, dtype=np.float64))
-            e = c.compute_ccsd(target_precision=opts.get("target_precision", 1e-11))
-            last_info["ccsd_iterations"] = c.n_iter
+            target = opts.get("target_precision", 1e-11)
+            refine = opts.get("refine")
+            last_info.pop("refine_iterations", None)
+            if refine and opts.get("precision", "f64") != "f64":
+                # iterative refinement: the reduced-precision form converges to `refine`,
+                # then FP64 iterations continue from its amplitudes to `target`
+                c.compute_ccsd(target_precision=refine)
+                last_info["ccsd_iterations"] = c.n_iter
+                c.set_option("precision", PRECISIONS["f64"])
+                c.set_option("solve_restart", 2)   # project t2 onto its exact symmetry first
+                e = c.compute_ccsd(target_precision=target)
+                last_info["refine_iterations"] = c.n_iter
+            else:
+                e = c.compute_ccsd(target_precision=target)
+                last_info["ccsd_iterations"] = c.n_iter
         except Exception:
             c.close()
             raise

@@ -201,7 +201,10 @@ int mpqc_ccsd_set_integral(mpqc_ccsd_ctx* ctx, int which, const double* data, in
 
 int mpqc_ccsd_set_direct_ao(mpqc_ccsd_ctx* ctx, int nbf, const double* ao_eri, const double* c_occ,
                             const double* c_vir, int memspace) {
-  return guarded(ctx, [&](mpqc_hip::CCSDContext& c) { c.set_direct_ao(nbf, ao_eri, c_occ, c_vir, memspace); });
+  return guarded(ctx, [&](mpqc_hip::CCSDContext& c) {
+    if (memspace == 1) MPQC_HIP_CHECK(hipDeviceSynchronize());  // the caller's producers of its buffers
+    c.set_direct_ao(nbf, ao_eri, c_occ, c_vir, memspace);
+  });
 }
 
 int mpqc_ccsd_set_df_basis(mpqc_ccsd_ctx* ctx, int naux, const int64_t* aux_hashmarks,
@@ -237,12 +240,16 @@ int mpqc_ccsd_get_integral(mpqc_ccsd_ctx* ctx, int which, double* out, int layou
 
 int mpqc_ccsd_transform_ao(mpqc_ccsd_ctx* ctx, int nbf, const double* ao_eri, const double* fock_ao,
                            const double* c_occ, const double* c_vir, int memspace) {
-  return guarded(ctx, [&](mpqc_hip::CCSDContext& c) { c.transform_ao(nbf, ao_eri, fock_ao, c_occ, c_vir, memspace); });
+  return guarded(ctx, [&](mpqc_hip::CCSDContext& c) {
+    if (memspace == 1) MPQC_HIP_CHECK(hipDeviceSynchronize());  // the caller's producers of its buffers
+    c.transform_ao(nbf, ao_eri, fock_ao, c_occ, c_vir, memspace);
+  });
 }
 
 int mpqc_ccsd_transform_df(mpqc_ccsd_ctx* ctx, int nbf, const double* ao_three_center, const double* metric,
                            const double* fock_ao, const double* c_occ, const double* c_vir, int memspace) {
   return guarded(ctx, [&](mpqc_hip::CCSDContext& c) {
+    if (memspace == 1) MPQC_HIP_CHECK(hipDeviceSynchronize());  // the caller's producers of its buffers
     c.transform_df(nbf, ao_three_center, metric, fock_ao, c_occ, c_vir, memspace);
   });
 }

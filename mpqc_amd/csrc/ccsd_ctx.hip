@@ -242,6 +242,18 @@ void CCSDContext::set_option(const std::string& key, double value) {
     set_plane_tile_order((int)value);
   } else if (key == "plane_lds_stages") {  // process-wide
     set_plane_lds_stages((int)value);
+  } else if (key == "solve_restart") {
+    // 1: solve() starts from the initial guess (ccsd.h:331-352); 0: from the context's
+    // amplitudes (e.g. those of a reduced-precision solve, refined in FP64); 2: from them
+    // after projecting t2 onto t2(a,b,i,j) = t2(b,a,j,i), the symmetry of the exact
+    // amplitudes for real orbitals (a reduced-precision solve leaves a small
+    // antisymmetric part, which FP64 iterations remove only slowly)
+    const int r = (int)value;
+    if (r != value || r < 0 || r > 2) throw std::invalid_argument("solve_restart must be 0, 1 or 2");
+    solve_restart_ = r;
+  } else if (key == "plane_dl") {  // process-wide: 0 deep-pipeline kernel where planned, 1 never
+    if ((int)value != value) throw std::invalid_argument("plane_dl must be an integer");
+    set_plane_dl_mode((int)value);
   } else if (key == "eom_max_vector") {  // eom_ccsd.h:33
     if (value < 2 || value > 64 || value != (int)value)
       throw std::invalid_argument("eom_max_vector must be an integer in [2, 64]");
@@ -296,6 +308,9 @@ void CCSDContext::upload(double* dst, const double* src, const std::array<int, 4
     total *= n[d];
   }
   if (memspace != 0 && memspace != 1) throw std::invalid_argument("memspace must be host or device");
+  // a caller's device buffer may still be written by work it queued on other streams
+  // (the context's stream does not wait for the null stream): let that finish first
+  if (memspace) MPQC_HIP_CHECK(hipDeviceSynchronize());
   if (layout == 0) {
     MPQC_HIP_CHECK(hipMemcpyAsync(dst, src, total * sizeof(double),
                                   memspace ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream_));
@@ -347,6 +362,7 @@ void CCSDContext::download(double* dst, const double* src, const std::array<int,
     total *= n[d];
   }
   if (memspace != 0 && memspace != 1) throw std::invalid_argument("memspace must be host or device");
+  if (memspace) MPQC_HIP_CHECK(hipDeviceSynchronize());  // the caller's pending work on its buffer
   if (layout == 0) {
     MPQC_HIP_CHECK(hipMemcpyAsync(dst, src, total * sizeof(double),
                                   memspace ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, stream_));
@@ -1172,7 +1188,7 @@ void CCSDContext::residual() {
     if (df_) throw std::logic_error("the split forms of the df method need reduced_abcd_memory");
     // :275 b(a,b,i,j) = tau(c,d,i,j) Gabcd(a,b,c,d)  -- the particle-particle ladder
     if (psym)
-      pp_ladder_symmetric();
+      pp_ladder_symmetric(r2_);
     else
       gemmP(v * v, o * o, v * v, 1.0, Gabcd_P_, vv, tauT_P_, vv, 1.0, r2_, oo, "r2:pp_ladder");
     // :277 tmp(k,a,i,j) = Giabc(k,a,c,d) tau(c,d,i,j)
@@ -1194,7 +1210,7 @@ void CCSDContext::residual() {
     // :275 b(a,b,i,j) = tau(c,d,i,j) Gabcd(a,b,c,d)  -- the particle-particle ladder (the
     // recorded residual of the adjoint keeps the full contraction)
     if (psym)
-      pp_ladder_symmetric();
+      pp_ladder_symmetric(r2_);
     else
       gemm(false, false, v * v, o * o, v * v, 1.0, Gabcd_, vv, tau_, oo, 1.0, r2_, oo, 0, 0, 0, 1, "r2:pp_ladder");
     // :277 tmp(k,a,i,j) = Giabc(k,a,c,d) tau(c,d,i,j)
@@ -1343,7 +1359,16 @@ void CCSDContext::update() {
 // CCSD::compute_ccsd solver loop (ccsd.h:344-477) with method = "standard".
 double CCSDContext::solve(int max_iter, double target_precision, int verbose, int* n_iter) {
   if (!have_eps_) throw std::logic_error("orbital energies not set");
-  init_guess();
+  if (solve_restart_ == 1) {
+    init_guess();
+  } else {
+    if (solve_restart_ == 2) {
+      const long V = v_, O = o_;
+      perm_add(Rd_, {V, V, O, O}, 0.0, {{t2_, {V, V, O, O}, {0, 1, 2, 3}, 0.5}, {t2_, {V, V, O, O}, {1, 0, 3, 2}, 0.5}});
+      copy_d2d(t2_, Rd_, V * V * O * O);
+    }
+    residual();  // the residual of the current amplitudes in the current form
+  }
   solver_init(diis_start_, n_diis_, diis_damp_, diis_ngroup_, diis_group_nstart_);
   trace_.clear();
   double E0 = 0.0, E1 = 0.0, dE = 0.0, err = 1.0;

@@ -175,7 +175,7 @@ class CCSDContext {
   void pair_products(const double* Pk, long lp, const Planes& Pp, const double* Mk, long lm, const Planes& Mp,
                      long nrs, long nra, long n, double* out, double* P, double* M, const char* tag_s,
                      const char* tag_a, const char* tag_u);
-  void pp_ladder_symmetric();   // ccsd_r1_r2.h:275 on pair-symmetric blocks
+  void pp_ladder_symmetric(double* out);  // ccsd_r1_r2.h:275 on pair-symmetric blocks, added to out
   void w_oooo_symmetric();      // ccsd_intermediates.h:182-193, its Gijab tau term likewise
 
   // ---- op tape: integral preparation and the residual as a list of primitive
@@ -330,6 +330,7 @@ class CCSDContext {
 
   // DIIS (TA::DIIS semantics, see DIISState in ccsd_ctx.hip)
   int diis_start_ = 1, n_diis_ = 8, diis_ngroup_ = 1, diis_group_nstart_ = 1;
+  int solve_restart_ = 1;  // solve() from the initial guess (0, 2: from the current amplitudes)
   double diis_damp_ = 0.0;
   int diis_iter_ = 0;
   std::deque<int> err_slots_, x_slots_;

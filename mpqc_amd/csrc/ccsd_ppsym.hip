@@ -288,9 +288,9 @@ void CCSDContext::pair_products(const double* Pk, long lp, const Planes& Pp, con
   });
 }
 
-// :275 b(a,b,i,j) = tau(c,d,i,j) Gabcd(a,b,c,d), added to r2 (after pack_tau_pairs)
-void CCSDContext::pp_ladder_symmetric() {
-  pair_products(Vpk_, ldvs_, Vp_P_, Vmk_, ldva_, Vm_P_, nvs_, nva_, v_, r2_, Ppk_, Mpk_, "r2:pp_ladder_sym",
+// :275 b(a,b,i,j) = tau(c,d,i,j) Gabcd(a,b,c,d), added to out (after pack_tau_pairs)
+void CCSDContext::pp_ladder_symmetric(double* out) {
+  pair_products(Vpk_, ldvs_, Vp_P_, Vmk_, ldva_, Vm_P_, nvs_, nva_, v_, out, Ppk_, Mpk_, "r2:pp_ladder_sym",
                 "r2:pp_ladder_anti", "r2:pp_sym_unpack");
 }
 

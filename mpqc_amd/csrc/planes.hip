@@ -1,18 +1,19 @@
 // 16-bit operand planes: the fused permute+split producer and the NT
-// contraction kernel on v_mfma_f32_16x16x32_{bf16,f16}.
+// contraction kernels on v_mfma_f32_{16x16x32,32x32x16}_{bf16,f16}.
 //
-// Kernel layout: 256 threads (2x2 wave64s), BM x BN output tile, K staged
-// through LDS in 32-wide slices (one MFMA K), double buffered.  Both operands
-// are k-contiguous in HBM and in LDS, so a slice row is copied with 16-byte
-// loads and every MFMA fragment (8 consecutive k of one row) is one 16-byte
-// LDS read.  The matrix cores accumulate in FP32 over K chunks of at most
-// kMaxChunk: longer contractions are split into chunks (split-K) whose FP64
-// partial sums are added in a fixed order by a separate pass, which bounds the
-// FP32 accumulation length without FP64 registers in the main loop (two
-// workgroups fit per CU).  A workgroup whose K range is longer than kMaxChunk
-// (not produced by the planner) would fold its FP32 sums into the FP64 output
-// after every chunk.  Workgroups are mapped to tiles XCD by XCD
-// (consecutive tiles on the same XCD share operand rows in that XCD's L2).
+// Large two-plane contractions run on the deep-pipeline kernel (256 x 256 tiles,
+// eight waves, direct global->LDS loads; described at gemm_planes_dl_kernel).  The
+// others run on the 64/128-tile kernel: 256 threads (2x2 wave64s), BM x BN output
+// tile, K staged through LDS in 32-wide slices (one MFMA K), double buffered.  Both
+// operands are k-contiguous in HBM and in LDS, so a slice row is copied with 16-byte
+// loads and every MFMA fragment (8 consecutive k of one row) is one 16-byte LDS read.
+// The matrix cores accumulate in FP32 over K chunks of at most kMaxChunk: longer
+// contractions are split into chunks (split-K) whose partial sums are added in FP64
+// in a fixed order by a separate pass, which bounds the FP32 accumulation length
+// without FP64 registers in the main loop.  A workgroup whose K range is longer than
+// kMaxChunk (not produced by the planner) would fold its FP32 sums into the FP64
+// output after every chunk.  Workgroups are mapped to tiles XCD by XCD (consecutive
+// tiles on the same XCD share operand rows in that XCD's L2).
 #include "planes.hpp"
 #include "gemm.hpp"
 #include "permute.hpp"
@@ -459,6 +460,254 @@ __global__ __launch_bounds__(kThr, (plane_occupancy<BM, BN, NBUF>())) void gemm_
   flush();  // also covers an empty K range (writes beta C or a zero partial)
 }
 
+// ---------------------------------------------------------------- deep-pipeline contraction
+// Large two-plane contractions: a 256 x 256 output tile per workgroup of eight waves
+// (two per SIMD, a 2 x 4 grid of 128 x 64 wave tiles, each 4 x 2 blocks of
+// v_mfma_f32_32x32x16), one workgroup per CU.  K advances in 32-wide steps through two
+// LDS stages of 64 KB that direct global->LDS loads (global_load_lds_dwordx4, no staging
+// registers) fill: a stage row is 64 contiguous bytes of one plane, so every 64-byte
+// request of the loads is used whole.  The next step's loads are issued when its stage
+// is free and overlap the current step's products; one barrier per step.  The tiles an
+// XCD owns are walked in groups of four tile rows, column-major inside a group, so the
+// workgroups of one XCD share A rows and B rows in its L2.  The epilogue stages each
+// wave's FP32 sums through the (then free) LDS and writes whole rows with 16-byte
+// stores: FP32 partials of a K chunk (at most kMaxChunk) for a fixed-order FP64 reduce,
+// or C = alpha / (sa sb) sum + beta C in FP64.
+constexpr int kDlTile = 256;
+constexpr int kDlBK = 32;       // K per stage: two 32x32x16 MFMA steps
+constexpr int kDlThreads = 512; // eight waves
+constexpr int kDlGroup = 4;     // tile rows per group of the tile order
+
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+typedef double f64x2_t __attribute__((ext_vector_type(2)));
+#define MPQC_LDS(p) ((__attribute__((address_space(3))) void*)(p))
+
+// zero source for the 8-wide k chunks beyond a contraction's K range
+__device__ __attribute__((aligned(16))) uint16_t g_dl_zero[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+
+// A stage row is 32 halves (four 16-byte chunks); chunk c of row r sits at chunk
+// c ^ dl_sw(r), so a fragment read (32 rows of one chunk) touches distinct 16-byte
+// bank groups in each 8-lane phase over 32 banks and each 16-lane phase over 64.
+__device__ inline int dl_sw(int r) { return ((r >> 1) ^ (r >> 2)) & 3; }
+
+template <bool F16>
+__device__ inline void mfma32(const u16x8_t& x, const u16x8_t& y, f32x16_t& c) {
+  if constexpr (F16)
+    c = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8_t, x), __builtin_bit_cast(f16x8_t, y), c,
+                                               0, 0, 0);
+  else
+    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8_t, x), __builtin_bit_cast(bf16x8_t, y), c,
+                                                0, 0, 0);
+}
+
+// s_waitcnt with only vmcnt(0) or only lgkmcnt(0) (the immediate holds vmcnt in bits 3:0
+// and 15:14, expcnt in 6:4, lgkmcnt in 11:8)
+__device__ inline void dl_wait_loads() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+__device__ inline void dl_wait_lds() { __builtin_amdgcn_s_waitcnt(0xC07F); }
+__device__ inline void dl_barrier() {
+  __builtin_amdgcn_sched_barrier(0);
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_sched_barrier(0);
+}
+
+struct DlParams {
+  const uint16_t* A[2];
+  const uint16_t* B[2];
+  const double* sa;
+  const double* sb;
+  double* C;
+  float* W;  // raw FP32 partials (nsplit > 1): [batch in launch][split][M][N]
+  long lda, ldb, ldc, sA, sB, sC;
+  int M, N, K;
+  int kchunk, nsplit, batch0, tiles_m, tiles_n, per_xcd;
+  double alpha, beta;
+};
+
+template <bool F16>
+__global__ __launch_bounds__(kDlThreads, 1) void gemm_planes_dl_kernel(DlParams p) {
+  // [stage][operand][plane][row * 32 + physical chunk * 8]
+  __shared__ __attribute__((aligned(1024))) uint16_t lds[2][2][2][kDlTile * kDlBK];
+
+  const int w = blockIdx.x;
+  const int tile = (w % kXcd) * p.per_xcd + w / kXcd;  // XCD ranges of the tile order
+  if (tile >= p.tiles_m * p.tiles_n) return;
+  const int grp = tile / (kDlGroup * p.tiles_n), first = grp * kDlGroup;
+  const int rows = min(kDlGroup, p.tiles_m - first), in = tile - grp * kDlGroup * p.tiles_n;
+  const int tm = first + in % rows, tn = in / rows;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 2, wn = wave & 3;
+  const int z = blockIdx.y;
+  const int b = p.batch0 + z / p.nsplit;
+  const int s = z - (z / p.nsplit) * p.nsplit;
+  const int m0 = tm * kDlTile, n0 = tn * kDlTile;
+  const int k_begin = s * p.kchunk;
+  const int k_end = min(p.K, k_begin + p.kchunk);
+  const int nsteps = k_end > k_begin ? (k_end - k_begin + kDlBK - 1) / kDlBK : 0;
+
+  // this lane's 16 bytes of each load: tile row wave*32 + hq*16 + lane/4, stored at
+  // physical chunk lane&3, i.e. logical chunk (lane&3) ^ dl_sw(row)
+  const uint16_t* src[2][2][2];  // [operand][plane][hq]: row start + logical chunk offset
+  int kofs[2];                   // [hq]: the chunk's first k relative to the step's k0
+#pragma unroll
+  for (int hq = 0; hq < 2; ++hq) {
+    const int r = wave * 32 + hq * 16 + (lane >> 2);
+    kofs[hq] = ((lane & 3) ^ dl_sw(r)) * 8;
+    const long ga = (long)b * p.sA + (long)min(m0 + r, p.M - 1) * p.lda + kofs[hq];
+    const long gb = (long)b * p.sB + (long)min(n0 + r, p.N - 1) * p.ldb + kofs[hq];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      src[0][q][hq] = p.A[q] + ga;
+      src[1][q][hq] = p.B[q] + gb;
+    }
+  }
+  auto issue = [&](int t, int st) {
+    const int k0 = k_begin + t * kDlBK;
+#pragma unroll
+    for (int op = 0; op < 2; ++op)
+#pragma unroll
+      for (int q = 0; q < 2; ++q)
+#pragma unroll
+        for (int hq = 0; hq < 2; ++hq) {
+          const uint16_t* g = k0 + kofs[hq] < k_end ? src[op][q][hq] + k0 : g_dl_zero;
+          __builtin_amdgcn_global_load_lds((const void*)g, MPQC_LDS(&lds[st][op][q][(wave * 32 + hq * 16) * kDlBK]),
+                                           16, 0, 0);
+        }
+  };
+
+  // fragment of MFMA step ss of a stage: row (lane & 31), logical chunk 2 ss + (lane >> 5)
+  int frag[2];
+#pragma unroll
+  for (int ss = 0; ss < 2; ++ss)
+    frag[ss] = (lane & 31) * kDlBK + (((2 * ss + (lane >> 5)) ^ dl_sw(lane & 31)) * 8);
+  u16x8_t fa[2][4][2], fb[2][2][2];  // [MFMA step][block][plane]
+  auto read_frags = [&](int st, int ss) {
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        fa[ss][i][q] = *reinterpret_cast<const u16x8_t*>(&lds[st][0][q][(wm * 128 + i * 32) * kDlBK + frag[ss]]);
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+        fb[ss][j][q] = *reinterpret_cast<const u16x8_t*>(&lds[st][1][q][(wn * 64 + j * 32) * kDlBK + frag[ss]]);
+    }
+  };
+  f32x16_t acc[4][2];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
+  // products in increasing magnitude: hi*lo, lo*hi, hi*hi
+  auto products = [&](int ss) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) mfma32<F16>(fa[ss][i][0], fb[ss][j][1], acc[i][j]);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) mfma32<F16>(fa[ss][i][1], fb[ss][j][0], acc[i][j]);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) mfma32<F16>(fa[ss][i][0], fb[ss][j][0], acc[i][j]);
+  };
+
+  if (nsteps > 0) {
+    issue(0, 0);
+    dl_wait_loads();
+    dl_barrier();
+    for (int t = 0; t < nsteps; ++t) {
+      const int st = t & 1;
+      // stage st^1 was read in step t-1, which every wave finished before the barrier
+      if (t + 1 < nsteps) issue(t + 1, st ^ 1);
+      read_frags(st, 0);
+      read_frags(st, 1);
+      products(0);
+      products(1);
+      dl_wait_loads();  // step t+1's stage has landed (this wave's share)
+      dl_wait_lds();    // this wave's reads of stage st are done
+      dl_barrier();
+    }
+  }
+
+  // epilogue: each wave stages its 128 x 64 FP32 tile through 16 KB of LDS, 64 rows at a
+  // time, and writes rows of 64 values with 16-byte accesses (4 lanes per 64 bytes)
+  const double unscale = p.alpha / ((*p.sa) * (*p.sb));
+  double* __restrict__ C = p.C + (long)b * p.sC;
+  float* __restrict__ Wz = p.W + (long)z * p.M * p.N;
+  float* stg = reinterpret_cast<float*>(&lds[0][0][0][0]) + wave * (64 * 64);
+  if (nsteps == 0) dl_barrier();  // (the K loop's last barrier otherwise)
+#pragma unroll
+  for (int i0 = 0; i0 < 4; i0 += 2) {
+#pragma unroll
+    for (int ii = 0; ii < 2; ++ii)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int r = 0; r < 16; ++r)
+          stg[(ii * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3)) * 64 + j * 32 + (lane & 31)] = acc[i0 + ii][j][r];
+    dl_wait_lds();
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int idx = q * 64 + lane, row = idx >> 4, c4 = idx & 15;
+      const f32x4_t v = *reinterpret_cast<const f32x4_t*>(&stg[row * 64 + c4 * 4]);
+      const int gm = m0 + wm * 128 + i0 * 32 + row, gn = n0 + wn * 64 + c4 * 4;
+      if (gm >= p.M) continue;
+      if (p.nsplit > 1) {
+        float* wr = Wz + (long)gm * p.N + gn;
+        if (gn + 3 < p.N && ((uintptr_t)wr & 15) == 0) {
+          *reinterpret_cast<f32x4_t*>(wr) = v;
+        } else {
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            if (gn + e < p.N) wr[e] = v[e];
+        }
+      } else {
+        double* c = C + (long)gm * p.ldc + gn;
+        double o[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) o[e] = unscale * (double)v[e];
+        if (gn + 3 < p.N && ((uintptr_t)c & 15) == 0) {
+          f64x2_t lo = {o[0], o[1]}, hi = {o[2], o[3]};
+          if (p.beta != 0.0) {
+            const f64x2_t c0 = *reinterpret_cast<const f64x2_t*>(c), c1 = *reinterpret_cast<const f64x2_t*>(c + 2);
+            lo += p.beta * c0;
+            hi += p.beta * c1;
+          }
+          *reinterpret_cast<f64x2_t*>(c) = lo;
+          *reinterpret_cast<f64x2_t*>(c + 2) = hi;
+        } else {
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            if (gn + e < p.N) c[e] = p.beta == 0.0 ? o[e] : o[e] + p.beta * c[e];
+        }
+      }
+    }
+    dl_wait_lds();
+  }
+}
+
+// C[b] = beta C[b] + sum_s alpha / (sa sb) * W[b, s]  (FP32 partials, FP64 sums in split order)
+__global__ __launch_bounds__(256) void splitk_reduce_f32_kernel(const float* __restrict__ W, double* __restrict__ C,
+                                                                long ldc, long sC, int M, int N, int nsplit,
+                                                                int batch0, double alpha, const double* sa,
+                                                                const double* sb, double beta) {
+  const long MN = (long)M * N;
+  const int bz = blockIdx.y;
+  const double u = alpha / ((*sa) * (*sb));
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < MN; e += (long)gridDim.x * blockDim.x) {
+    const float* wp = W + (long)bz * nsplit * MN + e;
+    double acc = 0.0;
+    for (int s2 = 0; s2 < nsplit; ++s2) acc += u * (double)wp[(long)s2 * MN];
+    const int m = (int)(e / N), n = (int)(e - (long)m * N);
+    double* c = C + (long)(batch0 + bz) * sC + (long)m * ldc + n;
+    *c = (beta == 0.0) ? acc : acc + beta * (*c);
+  }
+}
+
 // Launch plan: tile edge, split-K factor and K chunk of one contraction.
 struct PlanePlan {
   int bm, bn;  // tile: 64 x 64, 128 x 128, 256 x 128 or 256 x 256
@@ -471,6 +720,82 @@ bool g_bdirect = false;  // direct B fragments (selected by measurement)
 int g_tile_order = 0;    // see PParams::order
 int g_lds_stages = 2;    // LDS stages of the 128 x 128 two-plane kernel
 int g_split_cap = 512;   // most K splits of a grid whose tiles alone do not fill the GPU
+int g_dl_mode = 0;       // deep-pipeline kernel: 0 where the planner picks it, 1 never
+
+// Plan of the deep-pipeline kernel: used for two-plane contractions whose 256 x 256
+// tiles pad the output by at most 15 %.  The K split count c (chunks of at most
+// kMaxChunk, multiples of 32) minimises a time model: rounds of one workgroup per CU
+// (256 CUs) times the chunk's 32-wide steps at ~3 us each (measured), plus the FP32
+// partials' write and read (8 bytes per output and split at ~5 TB/s).
+struct DlPlan {
+  bool use = false;
+  int nsplit = 1, kchunk = 0;
+};
+
+DlPlan plan_dl(int M, int N, int K, int batch) {
+  DlPlan pl;
+  if (g_dl_mode == 1 || M < kDlTile || N < kDlTile || K < 2 * kDlBK) return pl;
+  const long tiles = (long)ceil_div(M, kDlTile) * ceil_div(N, kDlTile);
+  if ((double)tiles * kDlTile * kDlTile > 1.15 * (double)M * N) return pl;
+  const long work = tiles * batch;
+  const int n_lo = (int)ceil_div(K, kMaxChunk);
+  const int n_hi = std::max(n_lo, std::min(std::min(K / 128, n_lo + 64), g_split_cap));
+  double best = 0.0;
+  int pick = n_lo;
+  for (int c = n_lo; c <= n_hi; ++c) {
+    const long chunk = ceil_div(ceil_div(K, c), kDlBK) * kDlBK;
+    const int splits = (int)ceil_div(K, chunk);
+    if (splits != c) continue;
+    const double t = (double)ceil_div(work * c, 256) * (double)(chunk / kDlBK) * 3.0 +
+                     (c > 1 ? (double)c * M * N * batch * 8.0 / 5.0e6 : 0.0);
+    if (c == n_lo || t < best) {
+      best = t;
+      pick = c;
+    }
+  }
+  pl.kchunk = (int)(ceil_div(ceil_div(K, pick), kDlBK) * kDlBK);
+  pl.nsplit = (int)ceil_div(K, pl.kchunk);
+  pl.use = true;
+  return pl;
+}
+
+size_t dl_workspace_bytes(int M, int N, const DlPlan& pl, int batch) {
+  return pl.use && pl.nsplit > 1 ? (size_t)pl.nsplit * M * N * batch * sizeof(float) : 0;
+}
+
+template <bool F16>
+void launch_planes_dl(const PlaneGemmDesc& d, const DlPlan& pl, double* ws, size_t ws_bytes, hipStream_t stream) {
+  const int tiles_m = (int)ceil_div(d.M, kDlTile), tiles_n = (int)ceil_div(d.N, kDlTile);
+  const long tiles = (long)tiles_m * tiles_n;
+  const int per_xcd = (int)ceil_div(tiles, kXcd);
+  const size_t slice = (size_t)d.M * d.N * sizeof(float);
+  int max_batch = 65535 / pl.nsplit;
+  if (pl.nsplit > 1) max_batch = (int)std::min<size_t>(max_batch, ws_bytes / (slice * pl.nsplit));
+  if (max_batch < 1) throw std::runtime_error("gemm_planes: split-K workspace too small");
+  for (int b0 = 0; b0 < d.batch; b0 += max_batch) {
+    const int nb = std::min(max_batch, d.batch - b0);
+    DlParams p;
+    for (int q = 0; q < 2; ++q) {
+      p.A[q] = d.A.p[q];
+      p.B[q] = d.B.p[q];
+    }
+    p.sa = d.A.scale; p.sb = d.B.scale; p.C = d.C; p.W = reinterpret_cast<float*>(ws);
+    p.lda = d.lda; p.ldb = d.ldb; p.ldc = d.ldc; p.sA = d.sA; p.sB = d.sB; p.sC = d.sC;
+    p.M = d.M; p.N = d.N; p.K = d.K; p.kchunk = pl.kchunk; p.nsplit = pl.nsplit; p.batch0 = b0;
+    p.tiles_m = tiles_m; p.tiles_n = tiles_n; p.per_xcd = per_xcd;
+    p.alpha = d.alpha; p.beta = d.beta;
+    dim3 grid((unsigned)(per_xcd * kXcd), (unsigned)(nb * pl.nsplit));
+    hipLaunchKernelGGL((gemm_planes_dl_kernel<F16>), grid, dim3(kDlThreads), 0, stream, p);
+    check_launch("gemm_planes_dl");
+    if (pl.nsplit > 1) {
+      const long MN = (long)d.M * d.N;
+      dim3 g2((unsigned)std::min<long>(ceil_div(MN, 256), 4096), (unsigned)nb);
+      hipLaunchKernelGGL(splitk_reduce_f32_kernel, g2, dim3(256), 0, stream, p.W, d.C, d.ldc, d.sC, d.M, d.N,
+                         pl.nsplit, b0, d.alpha, d.A.scale, d.B.scale, d.beta);
+      check_launch("splitk_reduce_f32");
+    }
+  }
+}
 
 PlanePlan plan_planes(int M, int N, int K, int batch, int force_bm = 0, int force_bn = 0) {
   PlanePlan pl;
@@ -550,6 +875,15 @@ void launch_planes(const PlaneGemmDesc& d, const PlanePlan& pl, double* ws, size
 
 template <int NP, bool F16>
 void dispatch_planes(const PlaneGemmDesc& d, double* ws, size_t ws_bytes, hipStream_t stream) {
+  if constexpr (NP == 2) {
+    const bool aligned16 = d.K % 8 == 0 && d.lda % 8 == 0 && d.ldb % 8 == 0 && d.sA % 8 == 0 && d.sB % 8 == 0 &&
+                           !g_tile_override_m;
+    const DlPlan dl = plan_dl(d.M, d.N, d.K, d.batch);
+    if (aligned16 && dl.use && ws_bytes >= dl_workspace_bytes(d.M, d.N, dl, 1)) {
+      launch_planes_dl<F16>(d, dl, ws, ws_bytes, stream);
+      return;
+    }
+  }
   const PlanePlan pl = plan_planes(d.M, d.N, d.K, d.batch);
   if (pl.bm == 64) {
     launch_planes<64, 64, NP, F16>(d, pl, ws, ws_bytes, stream);
@@ -651,7 +985,13 @@ void set_plane_tile_override(int bm, int bn) {
 
 size_t gemm_planes_workspace_bytes(int M, int N, int K, int batch) {
   const PlanePlan pl = plan_planes(M, N, K, batch);
-  return pl.nsplit > 1 ? (size_t)pl.nsplit * M * N * batch * sizeof(double) : 0;
+  const size_t old = pl.nsplit > 1 ? (size_t)pl.nsplit * M * N * batch * sizeof(double) : 0;
+  return std::max(old, dl_workspace_bytes(M, N, plan_dl(M, N, K, batch), batch));
+}
+
+void set_plane_dl_mode(int mode) {
+  if (mode != 0 && mode != 1) throw std::invalid_argument("plane_dl must be 0 (planner) or 1 (off)");
+  g_dl_mode = mode;
 }
 
 void gemm_planes(const PlaneGemmDesc& d, double* ws, size_t ws_bytes, hipStream_t stream) {

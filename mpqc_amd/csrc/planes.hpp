@@ -73,6 +73,9 @@ void set_plane_bdirect(bool on);
 void set_plane_tile_order(int order);
 // process-wide: most K splits of a small-grid plane GEMM (default 512)
 void set_plane_split_cap(int cap);
+// process-wide: 0 lets the planner use the deep-pipeline 256 x 256 two-plane kernel for
+// large contractions, 1 keeps every contraction on the 64/128-tile kernel
+void set_plane_dl_mode(int mode);
 // Workspace (bytes) gemm_planes needs for its split-K partial sums at this shape.
 size_t gemm_planes_workspace_bytes(int M, int N, int K, int batch);
 

@@ -47,6 +47,7 @@ def _chain_inputs(torch, k, o, v):
     b["Fab"] += torch.diag(ev)
     t1 = 0.01 * torch.randn((v, o), dtype=torch.float64, device="cuda", generator=gen)
     t2 = 0.01 * torch.randn((v, v, o, o), dtype=torch.float64, device="cuda", generator=gen)
+    torch.cuda.synchronize()   # the generators' kernels have written every buffer
     return b, eo, ev, t1, t2
 
 
@@ -77,6 +78,7 @@ def test_256_hchains_batched_vs_single_and_oracle(prec):
     tol_single, tol_oracle = TOL[prec]
     worst_single = worst_oracle = 0.0
     n_oracle = 0
+    seen_o = set()
     for gi, grp in enumerate(groups):
         print(f"{prec} group {gi}: {len(grp)} chains, o {sizes[grp[0]][0]}..{sizes[grp[-1]][0]}", flush=True)
         with CCSDBatch([sizes[k][0] for k in grp], [sizes[k][1] for k in grp], precision=prec) as B:
@@ -103,22 +105,30 @@ def test_256_hchains_batched_vs_single_and_oracle(prec):
                     r1s, r2s = c.compute_residuals(t1.cpu().numpy(), t2.cpu().numpy())
                 e = max(_rel(r1b, r1s), _rel(r2b, r2s))
                 worst_single = max(worst_single, e)
-                assert e < tol_single, (k, o, v, e)
-                if o <= 6:
+                first_of_o = o not in seen_o
+                seen_o.add(o)
+                if o <= 6 or (first_of_o and o <= 9) or e >= tol_single:
                     hb = {n: x.cpu().numpy() for n, x in b.items()}
                     ints = O.Integrals(**hb)
                     a1, a2 = t1.cpu().numpy(), t2.cpu().numpy()
                     tau = O.make_tau(a1, a2)
                     r1o = O.cs_ccsd_r1(a1, a2, tau, ints)
                     r2o = O.cs_ccsd_r2(a1, a2, tau, ints)
-                    e = max(_rel(r1b, r1o), _rel(r2b, r2o))
-                    worst_oracle = max(worst_oracle, e)
+                    eb = max(_rel(r1b, r1o), _rel(r2b, r2o))
+                    es = max(_rel(r1s, r1o), _rel(r2s, r2o))
+                    worst_oracle = max(worst_oracle, eb)
                     n_oracle += 1
-                    assert e < tol_oracle, (k, o, v, e)
+                    if max(eb, es) >= tol_oracle or e >= tol_single:
+                        print(f"chain {k} (o={o}, v={v}): batched vs oracle r1 {_rel(r1b, r1o):.2e} "
+                              f"r2 {_rel(r2b, r2o):.2e}; single vs oracle r1 {_rel(r1s, r1o):.2e} "
+                              f"r2 {_rel(r2s, r2o):.2e}", flush=True)
+                    assert eb < tol_oracle, ("batched vs oracle", k, o, v, eb)
+                    assert es < tol_oracle, ("single-molecule vs oracle", k, o, v, es)
+                assert e < tol_single, ("batched vs single-molecule", k, o, v, e)
                 del b, t1, t2
         torch.cuda.empty_cache()
     print(f"{prec}: 256 chains in {len(groups)} groups; max rel err vs single-molecule path {worst_single:.2e}, "
-          f"vs oracle ({n_oracle} chains with o <= 6) {worst_oracle:.2e}")
+          f"vs oracle ({n_oracle} chains: every o <= 6, the first of each o <= 9) {worst_oracle:.2e}")
     assert n_oracle > 0
     if prec == "f32":
         assert worst_oracle > 1e-12   # the reduced-precision contractions really ran

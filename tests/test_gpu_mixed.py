@@ -80,3 +80,49 @@ def test_plane_tiles_residual(prec, tile, stages):
     e1, e2 = _residual_errors(prec, tile, stages)
     print(f"{prec} tile {tile} stages {stages}: rel err r1 {e1:.2e} r2 {e2:.2e}")
     assert e1 < RES_BOUNDS[prec] and e2 < RES_BOUNDS[prec]
+
+
+def _residuals_at(prec, no, nv, options, seed=31):
+    from oracle import ccsd as O
+    from mpqc_amd.ccsd import CCSD, Integrals
+    from spinorb_ccsd import mpqc_blocks, random_hamiltonian
+    f, g = random_hamiltonian(no, nv, seed=seed, eri_scale=0.03)
+    b = mpqc_blocks(f, g, no)
+    rng = np.random.default_rng(seed + 1)
+    t1 = rng.normal(0, 0.01, (nv, no))
+    t2 = rng.normal(0, 0.01, (nv, nv, no, no))
+    t2 = 0.5 * (t2 + t2.transpose(1, 0, 3, 2))
+    ints = O.Integrals(**b)
+    tau = O.make_tau(t1, t2)
+    ref = (O.cs_ccsd_r1(t1, t2, tau, ints), O.cs_ccsd_r2(t1, t2, tau, ints))
+    out = {}
+    for name, opts in options.items():
+        with CCSD(no, nv, precision=prec) as c:
+            try:
+                for k, v in opts.items():
+                    c.set_option(k, v)
+                c.set_integrals(Integrals(**b))
+                c.set_orbital_energies(np.diag(f)[:no], np.diag(f)[no:])
+                out[name] = c.compute_residuals(t1, t2)
+            finally:
+                for k in opts:
+                    c.set_option(k, 0)   # process-wide settings back to the planner's
+    return ref, out
+
+
+@pytest.mark.parametrize("prec", ["f16x3", "bf16x3"])
+def test_deep_pipeline_plane_kernel_residual(prec):
+    """o=10, v=76: the (ov)^3 ring contractions (M = N = K = 760) run on the 256 x 256
+    deep-pipeline kernel (a 248-row last tile, a last K step half beyond K, split-K with
+    FP32 partials); the residual against the oracle within the form's bound, and
+    against the 128 x 128 kernel (plane_dl = 1) within twice that bound."""
+    (r1r, r2r), out = _residuals_at(prec, 10, 76, {"dl": {}, "old": {"plane_dl": 1}})
+    errs = {}
+    for name, (r1, r2) in out.items():
+        errs[name] = (np.abs(r1 - r1r).max() / np.abs(r1r).max(), np.abs(r2 - r2r).max() / np.abs(r2r).max())
+        print(f"{prec} {name}: rel err r1 {errs[name][0]:.2e} r2 {errs[name][1]:.2e}")
+    for name in out:
+        assert max(errs[name]) < RES_BOUNDS[prec]
+        assert max(errs[name]) > 1e-13
+    d2 = np.abs(out["dl"][1] - out["old"][1]).max() / np.abs(r2r).max()
+    assert 0.0 < d2 < 2 * RES_BOUNDS[prec]     # different kernels really ran
