@@ -9,8 +9,10 @@ from test_golden import FIX, _inputs
 
 pytestmark = pytest.mark.gpu
 
-# |E_form - E_fp64| bounds (Eh) for a ~0.2 Eh correlation energy
-BOUNDS = {"f32": 1e-8, "f16x3": 1e-7, "bf16x6": 1e-8, "bf16x3": 5e-5}
+# |E_form - E_fp64| bounds (Eh) for a ~0.2 Eh correlation energy: about ten times the
+# measured errors (f32 7.4e-10, f16x3 1.6e-9, bf16x6 7.7e-10, bf16x3 7.1e-8), the
+# bf16x3 one at the 1 uEh target of the mixed forms
+BOUNDS = {"f32": 1e-8, "f16x3": 2e-8, "bf16x6": 1e-8, "bf16x3": 1e-6}
 
 
 @pytest.mark.parametrize("prec", ["f32", "f16x3", "bf16x6", "bf16x3"])
@@ -29,7 +31,9 @@ def test_mixed_precision_energy_h2o_pvdz(prec):
     assert abs(d) < BOUNDS[prec]
 
 
-RES_BOUNDS = {"f32": 1e-6, "f16x3": 2e-6, "bf16x6": 1e-6, "bf16x3": 1e-3}
+# residual bounds relative to the largest element, about ten times the measured errors at
+# o=6, v=33 (f32 1.0e-7, f16x3 1.0e-7, bf16x6 9.1e-8, bf16x3 7.3e-6)
+RES_BOUNDS = {"f32": 1e-6, "f16x3": 1e-6, "bf16x6": 1e-6, "bf16x3": 5e-5}
 
 
 def _residual_errors(prec, tile=0, stages=2):

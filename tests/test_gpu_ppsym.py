@@ -4,8 +4,9 @@ the oracle, in the FP64 and split forms, at amplitudes without the (ab)(ij) symm
 (ccsd_r1_r2.h:275, ccsd_intermediates.h:182-193).
 
 Bars: FP64 1e-12 relative between the two forms and against the oracle (round-off of
-sums of different order); split forms 2e-6 (f16x3) and 1e-3 (bf16x3) relative against
-the oracle, the bounds the split residuals meet at this size."""
+sums of different order); split forms 1e-6 (f16x3, measured 1.1e-7) and 5e-5 (bf16x3,
+measured 4.9e-6) relative against the oracle, about ten times the errors the split
+residuals show at this size."""
 import numpy as np
 import pytest
 
@@ -35,7 +36,7 @@ def _residuals(prec, sym, no, nv, f, b, t1, t2):
         return c.compute_residuals(t1, t2)
 
 
-@pytest.mark.parametrize("prec,bar", [("f64", 1e-12), ("f16x3", 2e-6), ("bf16x3", 1e-3)])
+@pytest.mark.parametrize("prec,bar", [("f64", 1e-12), ("f16x3", 1e-6), ("bf16x3", 5e-5)])
 def test_pair_symmetric_ladder_residual(prec, bar):
     no, nv, f, b, t1, t2, r1r, r2r = _case()
     print(f"pp_symmetric residual {prec} o={no} v={nv}", flush=True)
@@ -70,7 +71,7 @@ def test_pair_symmetric_ladder_solve_energy():
 
 
 @pytest.mark.parametrize("block", ["Gabcd", "Gijab"])
-@pytest.mark.parametrize("prec,bar", [("f64", 1e-12), ("f16x3", 2e-6)])
+@pytest.mark.parametrize("prec,bar", [("f64", 1e-12), ("f16x3", 1e-6)])
 def test_asymmetric_blocks_take_the_full_contraction(prec, bar, block):
     # Gabcd (pp ladder) or Gijab (hole-hole intermediate) without the pair symmetry (a
     # symmetry-breaking perturbation): the context detects it and runs the full

@@ -76,7 +76,7 @@ def test_gpu_df_triples_match_reference():
     assert abs(et - tcase["e_triples"]) < 2e-12
 
 
-@pytest.mark.parametrize("prec,tol", [("f16x3", 2e-6), ("bf16x3", 2e-4)])
+@pytest.mark.parametrize("prec,tol", [("f16x3", 5e-8), ("bf16x3", 1e-6)])  # measured <= 5.6e-9 / 7.7e-8
 def test_triples_split_precision_forms(prec, tol):
     """The W GEMMs on 16-bit planes: E(T) within the form's precision of the
     FP64 oracle (random amplitudes, several i blocks) and of FP64 at converged
