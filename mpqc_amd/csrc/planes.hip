@@ -690,6 +690,162 @@ __global__ __launch_bounds__(kDlThreads, 1) void gemm_planes_dl_kernel(DlParams 
   }
 }
 
+// ---------------------------------------------------------------- tall-K contraction
+// Two-plane contractions with a small output (M, N <= 160) and a long K (the r1
+// intermediates F_oo, F_vv and the Giabc / Gijka tau terms: K = o v^2 or o^2 v): a
+// workgroup owns the whole output, padded to MB x NB blocks of 32 x 32,
+// for one range of K, so the grid is the K ranges and every operand byte is read once.
+// The range's 32-wide steps go through kStages LDS stages filled by direct
+// global->LDS loads (rows beyond M or N read the zero page: no HBM traffic), up to
+// kStages - 1 of them in flight; wave w of W (four, or eight for large outputs) owns
+// output blocks w, w + W, ...  FP32 sums
+// over at most kMaxChunk values of k are folded into FP64 registers, and the range's
+// FP64 sums (times alpha / (sa sb)) are written as split-K partials for the
+// fixed-order reduce.
+template <int MB, int NB>
+struct TkShape {
+  static constexpr int TM = MB * 32, TN = NB * 32;
+  static constexpr int kRows = TM + TN;               // stage rows per plane: A rows, then B rows
+  static constexpr int kInstr = kRows / 16 * 2;       // 16-row loads per stage (both planes)
+  static constexpr int kBlocks = MB * NB;
+  // eight waves where four would hold too many accumulators (FP32 and FP64) per lane
+  static constexpr int kWaves = kBlocks > 16 && kInstr % 8 == 0 ? 8 : 4;
+  static constexpr int kPerWave = kInstr / kWaves;    // equal for every wave
+  static constexpr int kStageHalves = 2 * kRows * 32; // [plane][row][32 halves]
+  static constexpr int kStages = 4 * kStageHalves * 2 <= 112 * 1024 ? 4 : 3;
+  static constexpr int kMaxBlocksPerWave = (kBlocks + kWaves - 1) / kWaves;
+  static_assert(kInstr % kWaves == 0, "loads must split evenly over the waves");
+};
+
+// s_waitcnt vmcnt(n * L) for n = 0, 1, 2 (expcnt and lgkmcnt not waited on)
+template <int L>
+__device__ inline void tk_wait_steps(int n) {
+  static_assert(2 * L < 64, "vmcnt holds 6 bits");
+  if (n <= 0) __builtin_amdgcn_s_waitcnt(0x0F70);
+  else if (n == 1) __builtin_amdgcn_s_waitcnt(0x0F70 | (L & 15) | ((L >> 4) << 14));
+  else __builtin_amdgcn_s_waitcnt(0x0F70 | ((2 * L) & 15) | (((2 * L) >> 4) << 14));
+}
+
+template <int MB, int NB, bool F16>
+__global__ __launch_bounds__((64 * TkShape<MB, NB>::kWaves)) void gemm_planes_tallk_kernel(DlParams p) {
+  using S = TkShape<MB, NB>;
+  constexpr int ST = S::kStages;
+  __shared__ __attribute__((aligned(1024))) uint16_t lds[ST][S::kStageHalves];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int z = blockIdx.x;
+  const int b = p.batch0 + z / p.nsplit;
+  const int s = z - (z / p.nsplit) * p.nsplit;
+  const int k_begin = s * p.kchunk;
+  const int k_end = min(p.K, k_begin + p.kchunk);
+  const int nsteps = k_end > k_begin ? (k_end - k_begin + kDlBK - 1) / kDlBK : 0;
+
+  // load q = wave + kWaves i of a stage: plane q / (kRows/16), rows 16 * (q % (kRows/16)) .. +15
+  const uint16_t* src[S::kPerWave];
+  int kofs[S::kPerWave], dst[S::kPerWave];
+  bool valid[S::kPerWave];
+#pragma unroll
+  for (int i = 0; i < S::kPerWave; ++i) {
+    const int q = wave + S::kWaves * i;
+    const int plane = q / (S::kRows / 16), rg = q - plane * (S::kRows / 16);
+    const int r = rg * 16 + (lane >> 2);
+    kofs[i] = ((lane & 3) ^ dl_sw(r)) * 8;
+    dst[i] = (plane * S::kRows + rg * 16) * kDlBK;
+    if (r < S::TM) {
+      valid[i] = r < p.M;
+      src[i] = p.A[plane] + (long)b * p.sA + (long)min(r, p.M - 1) * p.lda + kofs[i];
+    } else {
+      valid[i] = r - S::TM < p.N;
+      src[i] = p.B[plane] + (long)b * p.sB + (long)min(r - S::TM, p.N - 1) * p.ldb + kofs[i];
+    }
+  }
+  auto issue = [&](int t, int st) {
+    const int k0 = k_begin + t * kDlBK;
+#pragma unroll
+    for (int i = 0; i < S::kPerWave; ++i) {
+      const uint16_t* g = valid[i] && k0 + kofs[i] < k_end ? src[i] + k0 : g_dl_zero;
+      __builtin_amdgcn_global_load_lds((const void*)g, MPQC_LDS(&lds[st][dst[i]]), 16, 0, 0);
+    }
+  };
+  int frag[2];
+#pragma unroll
+  for (int ss = 0; ss < 2; ++ss)
+    frag[ss] = (lane & 31) * kDlBK + (((2 * ss + (lane >> 5)) ^ dl_sw(lane & 31)) * 8);
+
+  f32x16_t acc[S::kMaxBlocksPerWave];
+  double acc64[S::kMaxBlocksPerWave][16];
+#pragma unroll
+  for (int c = 0; c < S::kMaxBlocksPerWave; ++c)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      acc[c][r] = 0.f;
+      acc64[c][r] = 0.0;
+    }
+  auto fold = [&]() {
+#pragma unroll
+    for (int c = 0; c < S::kMaxBlocksPerWave; ++c)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        acc64[c][r] += (double)acc[c][r];
+        acc[c][r] = 0.f;
+      }
+  };
+  auto products = [&](int st) {
+#pragma unroll
+    for (int c = 0; c < S::kMaxBlocksPerWave; ++c) {
+      const int blk = wave + S::kWaves * c;
+      if (blk < S::kBlocks) {  // (wave-uniform)
+        const int i = blk / NB, j = blk - (blk / NB) * NB;
+#pragma unroll
+        for (int ss = 0; ss < 2; ++ss) {
+          u16x8_t a[2], bb[2];
+#pragma unroll
+          for (int q = 0; q < 2; ++q) {
+            a[q] = *reinterpret_cast<const u16x8_t*>(&lds[st][(q * S::kRows + i * 32) * kDlBK + frag[ss]]);
+            bb[q] = *reinterpret_cast<const u16x8_t*>(&lds[st][(q * S::kRows + S::TM + j * 32) * kDlBK + frag[ss]]);
+          }
+          mfma32<F16>(a[0], bb[1], acc[c]);
+          mfma32<F16>(a[1], bb[0], acc[c]);
+          mfma32<F16>(a[0], bb[0], acc[c]);
+        }
+      }
+    }
+  };
+
+  if (nsteps > 0) {
+#pragma unroll
+    for (int t = 0; t < ST - 1; ++t)
+      if (t < nsteps) issue(t, t);
+    int since_fold = 0;
+    for (int t = 0; t < nsteps; ++t) {
+      // stage t has landed once at most min(ST - 2, nsteps - 1 - t) later steps' loads are out
+      tk_wait_steps<S::kPerWave>(min(ST - 2, nsteps - 1 - t));
+      dl_wait_lds();
+      dl_barrier();
+      // the stage of step t - 1 is free (every wave read it before this barrier)
+      if (t + ST - 1 < nsteps) issue(t + ST - 1, (t + ST - 1) % ST);
+      products(t % ST);
+      if (++since_fold == kMaxChunk / kDlBK) {
+        fold();
+        since_fold = 0;
+      }
+    }
+    fold();
+  }
+  const double unscale = p.alpha / ((*p.sa) * (*p.sb));
+  double* __restrict__ W = reinterpret_cast<double*>(p.W) + (long)z * p.M * p.N;
+#pragma unroll
+  for (int c = 0; c < S::kMaxBlocksPerWave; ++c) {
+    const int blk = wave + S::kWaves * c;
+    if (blk >= S::kBlocks) continue;
+    const int i = blk / NB, j = blk - (blk / NB) * NB;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int gm = i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3), gn = j * 32 + (lane & 31);
+      if (gm < p.M && gn < p.N) W[(long)gm * p.N + gn] = unscale * acc64[c][r];
+    }
+  }
+}
+
 // C[b] = beta C[b] + sum_s alpha / (sa sb) * W[b, s]  (FP32 partials, FP64 sums in split order)
 __global__ __launch_bounds__(256) void splitk_reduce_f32_kernel(const float* __restrict__ W, double* __restrict__ C,
                                                                 long ldc, long sC, int M, int N, int nsplit,
@@ -873,6 +1029,69 @@ void launch_planes(const PlaneGemmDesc& d, const PlanePlan& pl, double* ws, size
   }
 }
 
+// Plan of the tall-K kernel: two-plane contractions with M, N <= 160 and K >= 8192;
+// output blocks padded to 2 or 5 per dimension; one round of workgroups (as many as
+// the LDS lets run at once on 256 CUs), each streaming an equal range of K.
+struct TkPlan {
+  bool use = false;
+  int mb = 0, nb = 0, nsplit = 1, kchunk = 0;
+};
+
+template <int MB, int NB>
+int tk_occupancy() {
+  return (int)std::max<long>(1, (160L * 1024) / ((long)TkShape<MB, NB>::kStages * TkShape<MB, NB>::kStageHalves * 2));
+}
+
+TkPlan plan_tallk(int M, int N, int K, int batch) {
+  TkPlan pl;
+  if (g_dl_mode == 1 || M > 160 || N > 160 || K < 8192) return pl;
+  pl.mb = M <= 64 ? 2 : 5;
+  pl.nb = N <= 64 ? 2 : 5;
+  const int occ = pl.mb == 2 && pl.nb == 2 ? tk_occupancy<2, 2>() : pl.mb == 5 && pl.nb == 5 ? tk_occupancy<5, 5>()
+                                                                                              : tk_occupancy<5, 2>();
+  const long want = std::max<long>(1, 256L * occ / std::max(1, batch));
+  pl.kchunk = (int)(ceil_div(ceil_div(K, want), kDlBK) * kDlBK);
+  pl.nsplit = (int)ceil_div(K, pl.kchunk);
+  pl.use = true;
+  return pl;
+}
+
+size_t tk_workspace_bytes(int M, int N, const TkPlan& pl, int batch) {
+  return pl.use ? (size_t)pl.nsplit * M * N * batch * sizeof(double) : 0;
+}
+
+template <int MB, int NB, bool F16>
+void launch_planes_tallk(const PlaneGemmDesc& d, const TkPlan& pl, double* ws, size_t ws_bytes, hipStream_t stream) {
+  const size_t slice = (size_t)d.M * d.N * sizeof(double);
+  int max_batch = std::min<int>(65535, (int)std::min<size_t>(INT32_MAX / pl.nsplit, ws_bytes / (slice * pl.nsplit)));
+  if (max_batch < 1) throw std::runtime_error("gemm_planes: tall-K workspace too small");
+  for (int b0 = 0; b0 < d.batch; b0 += max_batch) {
+    const int nb = std::min(max_batch, d.batch - b0);
+    DlParams p;
+    for (int q = 0; q < 2; ++q) {
+      p.A[q] = d.A.p[q];
+      p.B[q] = d.B.p[q];
+    }
+    p.sa = d.A.scale; p.sb = d.B.scale; p.C = d.C; p.W = reinterpret_cast<float*>(ws);
+    p.lda = d.lda; p.ldb = d.ldb; p.ldc = d.ldc; p.sA = d.sA; p.sB = d.sB; p.sC = d.sC;
+    p.M = d.M; p.N = d.N; p.K = d.K; p.kchunk = pl.kchunk; p.nsplit = pl.nsplit; p.batch0 = b0;
+    p.tiles_m = 1; p.tiles_n = 1; p.per_xcd = 0;
+    p.alpha = d.alpha; p.beta = d.beta;
+    hipLaunchKernelGGL((gemm_planes_tallk_kernel<MB, NB, F16>), dim3((unsigned)(nb * pl.nsplit)),
+                       dim3(64 * TkShape<MB, NB>::kWaves), 0, stream, p);
+    check_launch("gemm_planes_tallk");
+    splitk_reduce(ws, d.C, d.ldc, d.sC, d.M, d.N, pl.nsplit, b0, nb, d.beta, stream);
+  }
+}
+
+template <bool F16>
+void dispatch_tallk(const PlaneGemmDesc& d, const TkPlan& pl, double* ws, size_t ws_bytes, hipStream_t stream) {
+  if (pl.mb == 2 && pl.nb == 2) launch_planes_tallk<2, 2, F16>(d, pl, ws, ws_bytes, stream);
+  else if (pl.mb == 2) launch_planes_tallk<2, 5, F16>(d, pl, ws, ws_bytes, stream);
+  else if (pl.nb == 2) launch_planes_tallk<5, 2, F16>(d, pl, ws, ws_bytes, stream);
+  else launch_planes_tallk<5, 5, F16>(d, pl, ws, ws_bytes, stream);
+}
+
 template <int NP, bool F16>
 void dispatch_planes(const PlaneGemmDesc& d, double* ws, size_t ws_bytes, hipStream_t stream) {
   if constexpr (NP == 2) {
@@ -881,6 +1100,11 @@ void dispatch_planes(const PlaneGemmDesc& d, double* ws, size_t ws_bytes, hipStr
     const DlPlan dl = plan_dl(d.M, d.N, d.K, d.batch);
     if (aligned16 && dl.use && ws_bytes >= dl_workspace_bytes(d.M, d.N, dl, 1)) {
       launch_planes_dl<F16>(d, dl, ws, ws_bytes, stream);
+      return;
+    }
+    const TkPlan tk = plan_tallk(d.M, d.N, d.K, d.batch);
+    if (aligned16 && tk.use && ws_bytes >= tk_workspace_bytes(d.M, d.N, tk, 1)) {
+      dispatch_tallk<F16>(d, tk, ws, ws_bytes, stream);
       return;
     }
   }
@@ -986,7 +1210,8 @@ void set_plane_tile_override(int bm, int bn) {
 size_t gemm_planes_workspace_bytes(int M, int N, int K, int batch) {
   const PlanePlan pl = plan_planes(M, N, K, batch);
   const size_t old = pl.nsplit > 1 ? (size_t)pl.nsplit * M * N * batch * sizeof(double) : 0;
-  return std::max(old, dl_workspace_bytes(M, N, plan_dl(M, N, K, batch), batch));
+  return std::max({old, dl_workspace_bytes(M, N, plan_dl(M, N, K, batch), batch),
+                   tk_workspace_bytes(M, N, plan_tallk(M, N, K, batch), batch)});
 }
 
 void set_plane_dl_mode(int mode) {
