@@ -60,11 +60,13 @@ inline size_t perm_lds_bytes(const PermGeom& g) {
 
 // Device: evaluates every output element of block (bx, by, bz) of the launch
 // grid and hands (linear output offset, sum) to store().  Called by every
-// thread of the workgroup (the tiled shape synchronises through LDS).
-template <class Store>
+// thread of the workgroup (the tiled shape synchronises through LDS).  SHAPE
+// selects the code compiled in: 0 linear only, 1 tiled only, 2 both (decided by
+// g.tiled); kernels launched for one shape keep the linear shape's registers low.
+template <int SHAPE = 2, class Store>
 __device__ inline void permute_block(const PermGeom& g, unsigned bx, unsigned by, unsigned bz,
                                      Store&& store) {
-  if (!g.tiled) {
+  if (SHAPE == 0 || (SHAPE == 2 && !g.tiled)) {
     // linear shape: q = 2, r1 = 1, r0 = 0; the host keeps the (q,3) plane below 2^31
     const int plane = (int)(g.nq * g.n3);
     const int e = (int)bx * kPThreads + (int)threadIdx.x;

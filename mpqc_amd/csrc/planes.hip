@@ -127,9 +127,10 @@ struct SplitOut {
   int kind;
 };
 
+template <int SHAPE>
 __global__ __launch_bounds__(kPThreads) void split_write_kernel(PermGeom g, SplitOut o) {
   const double sc = *o.scale;
-  permute_block(g, blockIdx.x, blockIdx.y, blockIdx.z, [&](long e, double v) {
+  permute_block<SHAPE>(g, blockIdx.x, blockIdx.y, blockIdx.z, [&](long e, double v) {
     const double x = v * sc;
     if (o.kind == (int)PlaneKind::F16x2) {
       const uint16_t h = f16_bits((float)x);
@@ -1180,7 +1181,10 @@ void split_to_planes(PlaneKind kind, const Planes& out, const long n[4], const T
   hipLaunchKernelGGL(scale_stage2, dim3(1), dim3(256), 0, stream, partials, sa, out.scale);
   const PermGeom g = make_perm_geom(n, terms, nterms, prod);
   SplitOut so{{out.p[0], out.p[1], out.p[2]}, out.scale, (int)kind};
-  hipLaunchKernelGGL(split_write_kernel, perm_grid(g), dim3(kPThreads), perm_lds_bytes(g), stream, g, so);
+  if (g.tiled)
+    hipLaunchKernelGGL(split_write_kernel<1>, perm_grid(g), dim3(kPThreads), perm_lds_bytes(g), stream, g, so);
+  else
+    hipLaunchKernelGGL(split_write_kernel<0>, perm_grid(g), dim3(kPThreads), 0, stream, g, so);
   check_launch("split_to_planes");
 }
 

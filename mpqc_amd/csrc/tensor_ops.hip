@@ -14,9 +14,11 @@ namespace {
 
 constexpr int kBlock = 256;
 
+template <int SHAPE>
 __global__ __launch_bounds__(kPThreads) void tadd_kernel(PermGeom g, double* __restrict__ out,
                                                        double beta) {
-  permute_block(g, blockIdx.x, blockIdx.y, blockIdx.z, [&](long o, double v) { out[o] = (beta == 0.0) ? v : beta * out[o] + v; });
+  permute_block<SHAPE>(g, blockIdx.x, blockIdx.y, blockIdx.z,
+                       [&](long o, double v) { out[o] = (beta == 0.0) ? v : beta * out[o] + v; });
 }
 
 // Element-wise kernels use 1-D grid-stride loops with 32-bit indices (the
@@ -450,7 +452,10 @@ void tadd(double* out, const long n[4], double beta, const TTerm* terms, int nte
   if (nterms > kMaxTerms) throw std::invalid_argument("tadd: too many terms");
   if (n[0] * n[1] * n[2] * n[3] == 0) return;
   const PermGeom g = make_perm_geom(n, terms, nterms, prod);
-  hipLaunchKernelGGL(tadd_kernel, perm_grid(g), dim3(kPThreads), perm_lds_bytes(g), stream, g, out, beta);
+  if (g.tiled)
+    hipLaunchKernelGGL(tadd_kernel<1>, perm_grid(g), dim3(kPThreads), perm_lds_bytes(g), stream, g, out, beta);
+  else
+    hipLaunchKernelGGL(tadd_kernel<0>, perm_grid(g), dim3(kPThreads), 0, stream, g, out, beta);
   check_launch("tadd");
 }
 
