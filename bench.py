@@ -427,7 +427,7 @@ def _timed_iterations(c, args, label):
     n_it = args.warmup + args.steps
     tot = sum(x["ms"] for x in kt.values()) / n_it
     log(f"{label}: {ms:.2f} ms/iteration; event-timed phases sum to {tot:.2f} ms/iteration")
-    for k, x in sorted(kt.items(), key=lambda kv: -kv[1]["ms"])[:40]:
+    for k, x in sorted(kt.items(), key=lambda kv: -kv[1]["ms"])[:70]:
         log(f"   {k:24s} {x['ms'] / n_it:8.3f} ms/it  {x['calls'] / n_it:5.1f} calls/it"
             + (f"  {x['flops'] / max(1e-12, x['ms'] * 1e-3) / 1e12:7.2f} TF" if x["flops"] else ""))
     return ms, kt

@@ -254,6 +254,9 @@ void CCSDContext::set_option(const std::string& key, double value) {
   } else if (key == "plane_dl") {  // process-wide: 0 deep-pipeline kernel where planned, 1 never
     if ((int)value != value) throw std::invalid_argument("plane_dl must be an integer");
     set_plane_dl_mode((int)value);
+  } else if (key == "thin_k") {  // process-wide: 0 streaming kernel for FP64 K <= 64 contractions where planned, 1 never
+    if ((int)value != value) throw std::invalid_argument("thin_k must be an integer");
+    set_thin_k_mode((int)value);
   } else if (key == "eom_max_vector") {  // eom_ccsd.h:33
     if (value < 2 || value > 64 || value != (int)value)
       throw std::invalid_argument("eom_max_vector must be an integer in [2, 64]");
@@ -1041,13 +1044,22 @@ void CCSDContext::residual() {
   gemm(true, false, v * v, o, v, 1.0, Giabc_, vv, t1_, o, 0.0, Rd_, oo, vv * V, 0, O, o, "r2:giabc_t1");
   //   second term as t1(a,k) * (Giajb(k,b,i,c) t1(c,j))
   gemm(false, false, o * o * v, o, v, 1.0, Giajb_, v, t1_, o, 0.0, Q_, o, 0, 0, 0, 1, "r2:giajb_t1");
-  gemm(false, false, v, v * o * o, o, -1.0, t1_, o, Q_, V * oo, 1.0, Rd_, V * oo, 0, 0, 0, 1, "r2:t1_giajb_t1");
   }
   // :210-211 Rd -= (Gijka(j,i,k,a) + Gijab(i,k,a,c) t1(c,j)) t1(b,k)
   gemm(false, false, o * o * v, o, v, 1.0, Gijab_, v, t1_, o, 0.0, M12_, o, 0, 0, 0, 1, "r2:gijab_t1");
-  perm_add(S_, {V, O, O, O}, 0.0,
-           {{S11_, {V, O, O, O}, I, 1.0}, {M12_, {O, O, V, O}, {2, 0, 3, 1}, 1.0}});
-  gemm(false, true, v, o * o, o, -1.0, t1_, o, S_, o, 1.0, Rd_, oo, 0, O * oo, V * oo, v, "r2:t1_S");
+  if (df_) {
+    perm_add(S_, {V, O, O, O}, 0.0,
+             {{S11_, {V, O, O, O}, I, 1.0}, {M12_, {O, O, V, O}, {2, 0, 3, 1}, 1.0}});
+    gemm(false, true, v, o * o, o, -1.0, t1_, o, S_, o, 1.0, Rd_, oo, 0, O * oo, V * oo, v, "r2:t1_S");
+  } else {
+    // Rd enters r2 only as Rd(a,b,i,j) + Rd(b,a,j,i) (:260), so the term
+    // -S(a,i,j,k) t1(b,k), S(a,i,j,k) = Gijka(j,i,k,a) + M12(i,k,a,j), may go in as its
+    // mirror -t1(a,k) S(b,j,i,k): both t1(a,k) terms in one pass over Rd, with
+    // Q(k,b,i,j) += S(b,j,i,k); then Rd -= t1(a,k) Q(k,(b,i,j))
+    perm_add(Q_, {O, V, O, O}, 1.0,
+             {{S11_, {V, O, O, O}, {3, 0, 2, 1}, 1.0}, {M12_, {O, O, V, O}, {1, 2, 3, 0}, 1.0}});
+    gemm(false, false, v, v * o * o, o, -1.0, t1_, o, Q_, V * oo, 1.0, Rd_, V * oo, 0, 0, 0, 1, "r2:t1_giajb_S");
+  }
   // :218-220 g_ki = F_OO(FIJ, Fia, g_ijka_bar, t1) = FIJ + Fia t1 + gbar(k,l,i,c) t1(c,l)
   //          (ccsd_intermediates.h:105-112)
   copy_d2d(gki_, FIJ_, oo);
@@ -1089,10 +1101,14 @@ void CCSDContext::residual() {
   //   - Gijab(k,l,c,d) T(d,a,i,l) + 0.5 (2 Gijab(k,l,c,d) - Gijab(k,l,d,c)) t2(a,d,i,l)
   // :245-248 k(k,a,i,c) stored as K(a,i,k,c):
   //   Giajb(k,a,i,c) - Gijka(k,l,i,c) t1(a,l) + Giabc(k,a,d,c) t1(d,i) - Gijab(k,l,d,c) T(d,a,i,l)
-  copy_d2d(J_, J0_, ov * ov);
-  copy_d2d(K_, K0_, ov * ov);
-  gemm(false, false, v, o * o * v, o, -1.0, t1_, o, X18_, oo * V, 1.0, J_, oo * V, 0, 0, 0, 1, "r2:j_gijka_t1");
-  gemm(false, false, v, o * o * v, o, -1.0, t1_, o, X19_, oo * V, 1.0, K_, oo * V, 0, 0, 0, 1, "r2:k_gijka_t1");
+  // (the integral terms J0, K0 are added with the Giabc t1 terms below; df: here)
+  if (df_) {
+    copy_d2d(J_, J0_, ov * ov);
+    copy_d2d(K_, K0_, ov * ov);
+  }
+  const double jk_beta = df_ ? 1.0 : 0.0;
+  gemm(false, false, v, o * o * v, o, -1.0, t1_, o, X18_, oo * V, jk_beta, J_, oo * V, 0, 0, 0, 1, "r2:j_gijka_t1");
+  gemm(false, false, v, o * o * v, o, -1.0, t1_, o, X19_, oo * V, jk_beta, K_, oo * V, 0, 0, 0, 1, "r2:k_gijka_t1");
   if (df_) {
     // df :353-354 j += X_ab_t1(K,a,i) Xai(K,c,k) - Xai(x,d,l) T(d,a,i,l) Xai(x,c,k)
     //   = U(K,(a,i)) Xia(K,(k,c)),  U = X_ab_t1 - Xia(x,(l,d)) TP((a,i),(l,d))^T
@@ -1111,8 +1127,8 @@ void CCSDContext::residual() {
     gemm(false, false, o * v * v, o, v, 1.0, Giabc_, v, t1_, o, 0.0, Zj_, o, 0, 0, 0, 1, "r2:j_giabc_t1");  // Zj(k,a,c,i)
     gemm(true, false, v, o, v, 1.0, Giabc_, v, t1_, o, 0.0, Zk_, o, vv, 0, V * O, o * v, "r2:k_giabc_t1");  // Zk(k,a,c,i)
   }
-  perm_add(J_, {V, O, O, V}, 1.0, {{Zj_, {O, V, V, O}, {1, 3, 0, 2}, 1.0}});
-  perm_add(K_, {V, O, O, V}, 1.0, {{Zk_, {O, V, V, O}, {1, 3, 0, 2}, 1.0}});
+  perm_add(J_, {V, O, O, V}, 1.0, {{Zj_, {O, V, V, O}, {1, 3, 0, 2}, 1.0}, {J0_, {V, O, O, V}, I, 1.0}});
+  perm_add(K_, {V, O, O, V}, 1.0, {{Zk_, {O, V, V, O}, {1, 3, 0, 2}, 1.0}, {K0_, {V, O, O, V}, I, 1.0}});
   }
   if (sm) {
     const std::array<long, 4> e_aild{V, O, O, V};

@@ -703,6 +703,199 @@ void splitk_reduce(const double* W, double* C, long ldc, long sC, int M, int N, 
   }
 }
 
+// ---------------------------------------------------------------- thin K
+// C = alpha A B + beta C for a short contraction (K <= 64) of a small A (M <= 256
+// rows) with a long B (K x N), no transposes: t1(a,k) X(k,(b,i,j)) of the CCSD
+// residual (ccsd_r1_r2.h:207-211, :238-248, :278-280).  The output streams
+// through HBM once (read and written, or only written when beta = 0); the
+// tiled kernels spend most of such a launch staging K through LDS.
+//
+// A lives in registers for the whole launch: one wave per 16-row block of A
+// (at least four waves), each holding the 16 x 4 fragments of its block for all K.  The workgroup
+// walks steps of 32 columns (two 16-column blocks) with stride gridDim.x: the
+// K x 32 tile of B is loaded by the whole workgroup in 16-byte pieces into
+// registers during the previous step and stored to LDS (double buffered, one
+// barrier per step); each wave reads its B fragments from LDS, reads its rows
+// of C before the products, runs 2 x K/4 v_mfma_f64_16x16x4f64 and writes its
+// rows of C.  (B fragments loaded by every wave straight from HBM, 8 bytes per
+// lane, ran at half this speed: the loads, not the stores or the products,
+// bounded that form.)
+namespace {
+
+struct ThinKParams {
+  const double* A;
+  const double* B;
+  double* C;
+  long lda, ldb, ldc, sA, sB, sC;
+  int M, N, K;
+  int nsteps;  // 32-column steps
+  double alpha, beta;
+};
+
+constexpr int kThinMaxWaves = 16;
+constexpr int kThinMinWaves = 4;  // waves past the last row block only load B
+constexpr int kThinCols = 32;  // columns per step
+
+// KS: steps of 4 along K (K <= 4 KS); VEC: B is 16-byte aligned with an even ldb
+template <int KS, bool VEC>
+__global__ __launch_bounds__(64 * kThinMaxWaves) void dgemm_thin_k_kernel(ThinKParams p) {
+  constexpr int ROWS = 4 * KS, LDS_LD = kThinCols + 2;
+  constexpr int CH = ROWS * kThinCols / 2;  // 16-byte pieces of a B tile
+  __shared__ double Bs[2][ROWS][LDS_LD];
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const long z = blockIdx.y;
+  const double* __restrict__ A = p.A + z * p.sA;
+  const double* __restrict__ B = p.B + z * p.sB;
+  double* __restrict__ C = p.C + z * p.sC;
+  // fragments of A: lane holds A(16 wave + lr, 4 s + lk)
+  double af[KS];
+  {
+    const int m = 16 * wave + lr;
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      const int k = 4 * s + lk;
+      af[s] = (m < p.M && k < p.K) ? A[(long)m * p.lda + k] : 0.0;
+    }
+  }
+  // the B tile of step st: piece q covers row q / 16, columns 2 (q % 16) + {0, 1}
+  constexpr int PER = (CH + 64 * kThinMinWaves - 1) / (64 * kThinMinWaves);  // pieces per thread, at most
+  const int per = (CH + nt - 1) / nt;
+  double2 rg[PER];
+  auto gload = [&](int st) {
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      if (j >= per) break;
+      const int q = tid + j * nt;
+      const int row = q / (kThinCols / 2), n = st * kThinCols + 2 * (q % (kThinCols / 2));
+      double2 x = {0.0, 0.0};
+      if (q < CH && st < p.nsteps && row < p.K) {
+        const double* b = B + (long)row * p.ldb + n;
+        if (VEC && n + 1 < p.N) {
+          x = *reinterpret_cast<const double2*>(b);
+        } else {
+          if (n < p.N) x.x = b[0];
+          if (n + 1 < p.N) x.y = b[1];
+        }
+      }
+      rg[j] = x;
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      if (j >= per) break;
+      const int q = tid + j * nt;
+      if (q < CH) {
+        const int row = q / (kThinCols / 2), c = 2 * (q % (kThinCols / 2));
+        Bs[buf][row][c] = rg[j].x;
+        Bs[buf][row][c + 1] = rg[j].y;
+      }
+    }
+  };
+  const bool read_c = p.beta != 0.0;
+  const int G = gridDim.x;
+  int st = blockIdx.x, buf = 0;
+  gload(st);
+  sstore(0);
+  __syncthreads();
+  for (; st < p.nsteps; st += G) {  // uniform over the workgroup
+    gload(st + G);
+    double cin[2][4];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int n = st * kThinCols + 16 * c + lr;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = 16 * wave + 4 * r + lk;
+        cin[c][r] = (read_c && n < p.N && m < p.M) ? C[(long)m * p.ldc + n] : 0.0;
+      }
+    }
+    double4_t acc[2] = {double4_t{0.0, 0.0, 0.0, 0.0}, double4_t{0.0, 0.0, 0.0, 0.0}};
+#pragma unroll
+    for (int s = 0; s < KS; ++s)
+#pragma unroll
+      for (int c = 0; c < 2; ++c)
+        acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[s], Bs[buf][4 * s + lk][16 * c + lr], acc[c], 0, 0, 0);
+    // accumulator: lane holds C(16 wave + 4 r + lk, n)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int n = st * kThinCols + 16 * c + lr;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = 16 * wave + 4 * r + lk;
+        if (n < p.N && m < p.M)
+          C[(long)m * p.ldc + n] = read_c ? fma(p.beta, cin[c][r], p.alpha * acc[c][r]) : p.alpha * acc[c][r];
+      }
+    }
+    sstore(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
+  }
+}
+
+bool thin_k_shape(const GemmDesc& d) {
+  return !d.transA && !d.transB && d.K <= 64 && d.M <= 16 * kThinMaxWaves && d.N >= 2048 && d.batch <= 65535;
+}
+
+// Resident workgroups of `waves` waves on the device (a grid-size heuristic
+// only: the kernel is correct for any grid).
+template <int KS, bool VEC>
+int thin_k_resident(int waves) {
+  static int cached[kThinMaxWaves + 1] = {0};
+  if (cached[waves] == 0) {
+    int dev = 0, cus = 0, per_cu = 0;
+    MPQC_HIP_CHECK(hipGetDevice(&dev));
+    MPQC_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    MPQC_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dgemm_thin_k_kernel<KS, VEC>, 64 * waves, 0));
+    cached[waves] = std::max(1, cus) * std::max(1, per_cu);
+  }
+  return cached[waves];
+}
+
+template <int KS, bool VEC>
+void launch_thin_k_ks(const ThinKParams& p, int waves, int batch, hipStream_t stream) {
+  const int grid = (int)std::min<long>(p.nsteps, std::max<long>(1, thin_k_resident<KS, VEC>(waves) / std::max(1, batch)));
+  hipLaunchKernelGGL((dgemm_thin_k_kernel<KS, VEC>), dim3((unsigned)grid, (unsigned)batch), dim3(64 * waves), 0,
+                     stream, p);
+  check_launch("dgemm_thin_k");
+}
+
+template <bool VEC>
+void launch_thin_k_vec(const ThinKParams& p, int waves, int batch, hipStream_t stream) {
+  switch ((p.K + 3) / 4) {
+    case 1: case 2: launch_thin_k_ks<2, VEC>(p, waves, batch, stream); break;
+    case 3: case 4: launch_thin_k_ks<4, VEC>(p, waves, batch, stream); break;
+    case 5: case 6: launch_thin_k_ks<6, VEC>(p, waves, batch, stream); break;
+    case 7: case 8: launch_thin_k_ks<8, VEC>(p, waves, batch, stream); break;
+    case 9: case 10: launch_thin_k_ks<10, VEC>(p, waves, batch, stream); break;
+    case 11: case 12: launch_thin_k_ks<12, VEC>(p, waves, batch, stream); break;
+    default: launch_thin_k_ks<16, VEC>(p, waves, batch, stream); break;
+  }
+}
+
+void launch_thin_k(const GemmDesc& d, hipStream_t stream) {
+  ThinKParams p;
+  p.A = d.A; p.B = d.B; p.C = d.C;
+  p.lda = d.lda; p.ldb = d.ldb; p.ldc = d.ldc; p.sA = d.sA; p.sB = d.sB; p.sC = d.sC;
+  p.M = d.M; p.N = d.N; p.K = d.K; p.alpha = d.alpha; p.beta = d.beta;
+  p.nsteps = (int)ceil_div(d.N, kThinCols);
+  const int waves = std::max(kThinMinWaves, (int)ceil_div(d.M, 16));
+  // 16-byte loads of B need every row start 16-byte aligned
+  const bool vec = ((uintptr_t)d.B % 16 == 0) && d.ldb % 2 == 0 && (d.batch == 1 || d.sB % 2 == 0);
+  if (vec)
+    launch_thin_k_vec<true>(p, waves, d.batch, stream);
+  else
+    launch_thin_k_vec<false>(p, waves, d.batch, stream);
+}
+
+int g_thin_k_never = 0;  // 1: the tiled kernels take the shapes thin_k_shape() admits
+
+}  // namespace
+
+void set_thin_k_mode(int mode) { g_thin_k_never = mode; }
+
 void gemm(const GemmDesc& d, GemmPrecision prec, const GemmWorkspace& ws, hipStream_t stream) {
   if (d.M <= 0 || d.N <= 0 || d.batch <= 0) return;
   if (d.K <= 0) {
@@ -710,6 +903,10 @@ void gemm(const GemmDesc& d, GemmPrecision prec, const GemmWorkspace& ws, hipStr
   }
   if (prec == GemmPrecision::BF16X3 || prec == GemmPrecision::F16X3 || prec == GemmPrecision::BF16X6) {
     gemm_split(d, prec, ws, stream);
+    return;
+  }
+  if (!g_thin_k_never && prec == GemmPrecision::F64 && thin_k_shape(d)) {
+    launch_thin_k(d, stream);
     return;
   }
   if (wide_tiles(d))
@@ -722,6 +919,7 @@ size_t gemm_scratch_bytes(const GemmDesc& d, GemmPrecision prec, size_t ws_bytes
   if (d.M <= 0 || d.N <= 0 || d.batch <= 0 || d.K <= 0) return 0;
   if (prec == GemmPrecision::BF16X3 || prec == GemmPrecision::F16X3 || prec == GemmPrecision::BF16X6)
     return gemm_split_scratch_bytes(d, prec, ws_bytes);
+  if (!g_thin_k_never && prec == GemmPrecision::F64 && thin_k_shape(d)) return 0;
   const bool wide = wide_tiles(d);
   return plan_splitk(d, wide ? 128 : 64, wide ? 128 : 64, ws_bytes).scratch_bytes;
 }

@@ -88,6 +88,11 @@ void gemm_split(const GemmDesc& d, GemmPrecision prec, const GemmWorkspace& ws, 
 size_t gemm_scratch_bytes(const GemmDesc& d, GemmPrecision prec, size_t ws_bytes);
 size_t gemm_split_scratch_bytes(const GemmDesc& d, GemmPrecision prec, size_t ws_bytes);
 
+// FP64 contractions with K <= 64, A of at most 256 rows and a long B (no
+// transposes) run on a streaming kernel that keeps A in registers; mode 1 sends
+// them to the tiled kernels instead (mode 0, the default, restores it).
+void set_thin_k_mode(int mode);
+
 // Convenience for plain (non-batched) calls.
 inline GemmDesc make_gemm(bool tA, bool tB, int M, int N, int K, double alpha, const double* A,
                           long lda, const double* B, long ldb, double beta, double* C, long ldc) {

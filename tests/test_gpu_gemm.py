@@ -60,6 +60,52 @@ def test_gemm_gemv_splitk():
     assert np.max(np.abs(dy.cpu().numpy() - ref)) < 1e-10
 
 
+@pytest.mark.parametrize("beta", [0.0, 1.0, -0.3])
+@pytest.mark.parametrize("m,n,k,pad,odd_ldb", [(152, 4800, 40, 0, 0), (1, 2048, 1, 0, 0), (17, 2049, 3, 3, 0),
+                                               (16, 3000, 4, 0, 1), (129, 2100, 63, 5, 0), (256, 2048, 64, 0, 0),
+                                               (255, 2501, 33, 1, 1), (128, 6000, 10, 0, 0), (70, 2111, 57, 2, 1)])
+def test_gemm_thin_k(m, n, k, pad, odd_ldb, beta):
+    """Short contractions (K <= 64) of a small A (M <= 256) with a long B: the
+    streaming kernel that keeps A in registers and stages B through LDS (ragged
+    M, N and K, leading dimensions above the minimum, 16-byte B loads with an
+    odd N and 8-byte ones with an odd ldb; beta = 0 must not read C)."""
+    from mpqc_amd.ccsd import gemm
+    rng = np.random.default_rng(m * 7 + n + k)
+    lda, ldc = k + pad, n + pad
+    ldb = n + 2 * pad
+    if ldb % 2 != odd_ldb:
+        ldb += 1
+    A = rng.standard_normal((m, lda))
+    B = rng.standard_normal((k, ldb))
+    C = rng.standard_normal((m, ldc))
+    if beta == 0.0:
+        C[:, :n] = np.nan
+    ref = C.copy()
+    ref[:, :n] = 0.7 * (A[:, :k] @ B[:, :n]) + (beta * C[:, :n] if beta != 0.0 else 0.0)
+    dA, dB, dC = _dev(A), _dev(B), _dev(C)
+    gemm(0, 0, m, n, k, 0.7, dA.data_ptr(), lda, dB.data_ptr(), ldb, beta, dC.data_ptr(), ldc)
+    torch.cuda.synchronize()
+    got = dC.cpu().numpy()
+    assert np.all(np.isfinite(got[:, :n]))
+    assert np.array_equal(got[:, n:], C[:, n:])       # padding columns untouched
+    assert np.max(np.abs(got - ref)) <= 1e-14 * k * 16 + 1e-13
+
+
+def test_gemm_thin_k_batched():
+    from mpqc_amd.ccsd import gemm
+    rng = np.random.default_rng(11)
+    batch, m, n, k = 3, 40, 2500, 12
+    A = rng.standard_normal((batch, m, k))
+    B = rng.standard_normal((batch, k, n))
+    C = rng.standard_normal((batch, m, n))
+    ref = -1.0 * np.einsum("bmk,bkn->bmn", A, B) + C
+    dA, dB, dC = _dev(A), _dev(B), _dev(C)
+    gemm(0, 0, m, n, k, -1.0, dA.data_ptr(), k, dB.data_ptr(), n, 1.0, dC.data_ptr(), n,
+         stride_a=m * k, stride_b=k * n, stride_c=m * n, batch=batch)
+    torch.cuda.synchronize()
+    assert np.max(np.abs(dC.cpu().numpy() - ref)) < 1e-12
+
+
 def test_gemm_rejects_bad_sizes():
     from mpqc_amd.ccsd import gemm
     from mpqc_amd._lib import CCSDError
