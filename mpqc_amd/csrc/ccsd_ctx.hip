@@ -170,8 +170,11 @@ void CCSDContext::construct(const int64_t* occ_hm, int n_occ_tiles, const int64_
   Zk_ = alloc(oovv); Xjk_ = alloc(oovv); Y1_ = alloc(oovv); Y2_ = alloc(oovv); RR_ = alloc(oovv);
   tmpR_ = alloc(oovv); Y23_ = alloc(oooo); W_ = alloc(oooo); tmp24_ = alloc(ooov);
   Y24_ = alloc(oovv);
-  // split-K workspace: enough for 64 slices of the largest small-output GEMM
-  const long ws_elems = std::max<long>(64L * std::max({vv, ov, oo * oo, 4096L}), 1L << 20);
+  // split-K workspace: enough for 64 slices of the largest small-output GEMM, and
+  // for 16 slices of an (ov) x (ov) or v^2 x o^2 output up to 512 MiB (small
+  // systems, where those outputs are too few tiles to fill the chip)
+  const long ws_elems = std::max<long>({64L * std::max({vv, ov, oo * oo, 4096L}), 1L << 20,
+                                        std::min<long>(16L * std::max(ov * ov, vv * oo), 64L << 20)});
   ws_.ptr = alloc(ws_elems);
   ws_.bytes = ws_elems * sizeof(double);
   partials_ = alloc(kReducePartials);
@@ -880,6 +883,12 @@ void CCSDContext::prepare() {
   if (have_iabc && !df_)
     perm_add(Gbar_akcd_, {V, O, V, V}, 0.0,
              {{Giabc_, e_iabc, {1, 0, 2, 3}, 2.0}, {Giabc_, e_iabc, {1, 0, 3, 2}, -1.0}});
+  // GiabcT(k,a,c,d) = Giabc(k,a,d,c): the k term's Giabc(k,a,d,c) t1(d,i) (:246) as one
+  // contraction over d in the FP64 forms
+  if (have_iabc && !df_) {
+    if (!GiabcT_) GiabcT_ = alloc(O * V * V * V);
+    perm_add(GiabcT_, e_iabc, 0.0, {{Giabc_, e_iabc, {0, 1, 3, 2}, 1.0}});
+  }
   // P8(c,k,l,i) = 2 Gijka(l,k,i,c) - Gijka(k,l,i,c)                 ccsd_r1_r2.h:120-121
   perm_add(P8_, {V, O, O, O}, 0.0,
            {{Gijka_, e_ijka, {3, 1, 0, 2}, 2.0}, {Gijka_, e_ijka, {3, 0, 1, 2}, -1.0}});
@@ -1125,7 +1134,7 @@ void CCSDContext::residual() {
     gemmP(o * v * v, o, v, 1.0, GiabcT_P_, v, t1T_P_, v, 0.0, Zk_, o, "r2:k_giabc_t1");
   } else {
     gemm(false, false, o * v * v, o, v, 1.0, Giabc_, v, t1_, o, 0.0, Zj_, o, 0, 0, 0, 1, "r2:j_giabc_t1");  // Zj(k,a,c,i)
-    gemm(true, false, v, o, v, 1.0, Giabc_, v, t1_, o, 0.0, Zk_, o, vv, 0, V * O, o * v, "r2:k_giabc_t1");  // Zk(k,a,c,i)
+    gemm(false, false, o * v * v, o, v, 1.0, GiabcT_, v, t1_, o, 0.0, Zk_, o, 0, 0, 0, 1, "r2:k_giabc_t1");  // Zk(k,a,c,i)
   }
   perm_add(J_, {V, O, O, V}, 1.0, {{Zj_, {O, V, V, O}, {1, 3, 0, 2}, 1.0}, {J0_, {V, O, O, V}, I, 1.0}});
   perm_add(K_, {V, O, O, V}, 1.0, {{Zk_, {O, V, V, O}, {1, 3, 0, 2}, 1.0}, {K0_, {V, O, O, V}, I, 1.0}});

@@ -311,6 +311,7 @@ class CCSDContext {
   // derived integral layouts (built once in prepare())
   double *f_ai_, *g_abij_, *Gbar_, *Gbar_cKLD_, *Gbar_kcld_, *P6_, *Gbar_akcd_, *P8_, *S11_,
       *P13_, *P14_, *J0_, *K0_, *X18_, *X19_, *GA_, *GB_, *GC_;
+  double* GiabcT_ = nullptr;  // Giabc(k,a,d,c) as (k,a,c,d), the FP64 forms' k term (allocated by prepare())
   // orbital energies
   double *eo_ = nullptr, *ev_ = nullptr;
   bool have_eps_ = false;

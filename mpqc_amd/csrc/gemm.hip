@@ -40,13 +40,16 @@ struct KParams {
 };
 
 constexpr int kXcdCount = 8;
+constexpr int kXcdMinTiles = 64;  // fewer output tiles: plain tile order
 
 // XCD-aware tile order: workgroups are dispatched round-robin over the 8 XCDs;
 // giving each XCD a contiguous range of row-major tiles keeps the operand rows
 // those tiles share inside one XCD's L2.  Returns -1 for padding workgroups.
+// With few tiles (per_xcd = 0) the order is plain: a launch of many K splits
+// over fewer than eight tiles would otherwise put every split on one XCD.
 __device__ inline int xcd_tile(const KParams& p) {
   const int w = blockIdx.x;
-  const int t = (w % kXcdCount) * p.per_xcd + w / kXcdCount;
+  const int t = p.per_xcd == 0 ? w : (w % kXcdCount) * p.per_xcd + w / kXcdCount;
   return t < p.tiles_m * p.tiles_n ? t : -1;
 }
 
@@ -457,13 +460,23 @@ SplitKPlan plan_splitk(const GemmDesc& d, int BM, int BN, size_t ws_bytes) {
   return pl;
 }
 
-// 128-wide tiles unless 64-wide ones pad the output much less
-bool wide_tiles(const GemmDesc& d) {
+// 128-wide tiles unless 64-wide ones pad the output much less, or the
+// 128-wide launch (after split-K) has too few workgroups to fill the chip (two
+// per CU) and the 64-wide one has more: small outputs with a short K, where
+// split-K cannot add parallelism.
+bool wide_tiles(const GemmDesc& d, size_t ws_bytes) {
   bool wide = d.M > 64 && d.N > 64;
   if (wide) {
     const double a128 = (double)ceil_div(d.M, 128) * ceil_div(d.N, 128) * 128 * 128;
     const double a64 = (double)ceil_div(d.M, 64) * ceil_div(d.N, 64) * 64 * 64;
     wide = a128 <= 1.25 * a64;
+  }
+  if (wide) {
+    auto wgs = [&](int bm) {
+      return ceil_div(d.M, bm) * ceil_div(d.N, bm) * d.batch * (long)plan_splitk(d, bm, bm, ws_bytes).nsplit;
+    };
+    const long w128 = wgs(128);
+    if (w128 < 512 && wgs(64) > w128) wide = false;
   }
   return wide;
 }
@@ -481,8 +494,8 @@ void launch(const GemmDesc& d, GemmPrecision prec, const GemmWorkspace& ws, hipS
     p.lda = d.lda; p.ldb = d.ldb; p.ldc = d.ldc; p.sA = d.sA; p.sB = d.sB; p.sC = d.sC;
     p.M = d.M; p.N = d.N; p.K = d.K; p.kchunk = kchunk; p.nsplit = nsplit; p.batch0 = b0;
     p.tiles_m = tiles_m; p.tiles_n = tiles_n; p.alpha = d.alpha; p.beta = d.beta;
-    p.per_xcd = (int)ceil_div(tiles, kXcdCount);
-    dim3 grid((unsigned)(p.per_xcd * kXcdCount), (unsigned)(nb * nsplit));
+    p.per_xcd = tiles >= kXcdMinTiles ? (int)ceil_div(tiles, kXcdCount) : 0;
+    dim3 grid((unsigned)(p.per_xcd ? p.per_xcd * kXcdCount : tiles), (unsigned)(nb * nsplit));
     if (prec == GemmPrecision::F32) {
       if (!d.transA && !d.transB) hipLaunchKernelGGL((sgemm_f32acc64_kernel<BM, BN, false, false>), grid, dim3(kThreads), 0, stream, p);
       else if (!d.transA && d.transB) hipLaunchKernelGGL((sgemm_f32acc64_kernel<BM, BN, false, true>), grid, dim3(kThreads), 0, stream, p);
@@ -909,7 +922,7 @@ void gemm(const GemmDesc& d, GemmPrecision prec, const GemmWorkspace& ws, hipStr
     launch_thin_k(d, stream);
     return;
   }
-  if (wide_tiles(d))
+  if (wide_tiles(d, ws.bytes))
     launch<128, 128>(d, prec, ws, stream);
   else
     launch<64, 64>(d, prec, ws, stream);
@@ -920,7 +933,7 @@ size_t gemm_scratch_bytes(const GemmDesc& d, GemmPrecision prec, size_t ws_bytes
   if (prec == GemmPrecision::BF16X3 || prec == GemmPrecision::F16X3 || prec == GemmPrecision::BF16X6)
     return gemm_split_scratch_bytes(d, prec, ws_bytes);
   if (!g_thin_k_never && prec == GemmPrecision::F64 && thin_k_shape(d)) return 0;
-  const bool wide = wide_tiles(d);
+  const bool wide = wide_tiles(d, ws_bytes);
   return plan_splitk(d, wide ? 128 : 64, wide ? 128 : 64, ws_bytes).scratch_bytes;
 }
 

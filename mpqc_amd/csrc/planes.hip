@@ -34,6 +34,7 @@ constexpr int kBK = 32;
 constexpr int kRow = kBK;
 __device__ inline int swz(int row, int chunk) { return (chunk ^ ((row >> 2) & 3)) * 8; }
 constexpr int kXcd = 8;
+constexpr int kXcdMinTiles = 64;  // fewer output tiles: plain tile order (K splits spread over every XCD)
 constexpr int kMaxChunk = 2048;  // longest K summed in FP32
 
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
@@ -218,7 +219,7 @@ __global__ __launch_bounds__(kThr, (plane_occupancy<BM, BN, NBUF>())) void gemm_
   // XCD-aware tile order: workgroup w runs on XCD w % 8; each XCD takes a
   // contiguous range of row-major tiles
   const int w = blockIdx.x;
-  const int tile = p.order == 1 ? w : (w % kXcd) * p.per_xcd + w / kXcd;
+  const int tile = (p.order == 1 || p.per_xcd == 0) ? w : (w % kXcd) * p.per_xcd + w / kXcd;
   if (tile >= p.tiles_m * p.tiles_n) return;
   int tm, tn;
   if (p.order == 2) {  // column-major tiles
@@ -531,7 +532,7 @@ __global__ __launch_bounds__(kDlThreads, 1) void gemm_planes_dl_kernel(DlParams 
   __shared__ __attribute__((aligned(1024))) uint16_t lds[2][2][2][kDlTile * kDlBK];
 
   const int w = blockIdx.x;
-  const int tile = (w % kXcd) * p.per_xcd + w / kXcd;  // XCD ranges of the tile order
+  const int tile = p.per_xcd == 0 ? w : (w % kXcd) * p.per_xcd + w / kXcd;  // XCD ranges of the tile order
   if (tile >= p.tiles_m * p.tiles_n) return;
   const int grp = tile / (kDlGroup * p.tiles_n), first = grp * kDlGroup;
   const int rows = min(kDlGroup, p.tiles_m - first), in = tile - grp * kDlGroup * p.tiles_n;
@@ -924,7 +925,7 @@ template <bool F16>
 void launch_planes_dl(const PlaneGemmDesc& d, const DlPlan& pl, double* ws, size_t ws_bytes, hipStream_t stream) {
   const int tiles_m = (int)ceil_div(d.M, kDlTile), tiles_n = (int)ceil_div(d.N, kDlTile);
   const long tiles = (long)tiles_m * tiles_n;
-  const int per_xcd = (int)ceil_div(tiles, kXcd);
+  const int per_xcd = tiles >= kXcdMinTiles ? (int)ceil_div(tiles, kXcd) : 0;  // 0: plain tile order
   const size_t slice = (size_t)d.M * d.N * sizeof(float);
   int max_batch = 65535 / pl.nsplit;
   if (pl.nsplit > 1) max_batch = (int)std::min<size_t>(max_batch, ws_bytes / (slice * pl.nsplit));
@@ -941,7 +942,7 @@ void launch_planes_dl(const PlaneGemmDesc& d, const DlPlan& pl, double* ws, size
     p.M = d.M; p.N = d.N; p.K = d.K; p.kchunk = pl.kchunk; p.nsplit = pl.nsplit; p.batch0 = b0;
     p.tiles_m = tiles_m; p.tiles_n = tiles_n; p.per_xcd = per_xcd;
     p.alpha = d.alpha; p.beta = d.beta;
-    dim3 grid((unsigned)(per_xcd * kXcd), (unsigned)(nb * pl.nsplit));
+    dim3 grid((unsigned)(per_xcd ? per_xcd * kXcd : tiles), (unsigned)(nb * pl.nsplit));
     hipLaunchKernelGGL((gemm_planes_dl_kernel<F16>), grid, dim3(kDlThreads), 0, stream, p);
     check_launch("gemm_planes_dl");
     if (pl.nsplit > 1) {
@@ -1003,7 +1004,7 @@ void launch_planes(const PlaneGemmDesc& d, const PlanePlan& pl, double* ws, size
   }
   const bool vec = d.K % 8 == 0 && d.lda % 8 == 0 && d.ldb % 8 == 0 && d.sA % 8 == 0 &&
                    d.sB % 8 == 0;
-  const int per_xcd = (int)ceil_div(tiles, kXcd);
+  const int per_xcd = tiles >= kXcdMinTiles ? (int)ceil_div(tiles, kXcd) : 0;  // 0: plain tile order
   for (int b0 = 0; b0 < d.batch; b0 += max_batch) {
     const int nb = std::min(max_batch, d.batch - b0);
     PParams p;
@@ -1016,7 +1017,7 @@ void launch_planes(const PlaneGemmDesc& d, const PlanePlan& pl, double* ws, size
     p.M = d.M; p.N = d.N; p.K = d.K; p.kchunk = kchunk; p.nsplit = nsplit; p.batch0 = b0;
     p.tiles_m = tiles_m; p.tiles_n = tiles_n; p.per_xcd = per_xcd; p.order = g_tile_order;
     p.alpha = d.alpha; p.beta = d.beta;
-    dim3 grid((unsigned)(per_xcd * kXcd), (unsigned)(nb * nsplit));
+    dim3 grid((unsigned)(per_xcd ? per_xcd * kXcd : tiles), (unsigned)(nb * nsplit));
     if (vec && NP == 2 && BM == 128 && BN == 128 && g_lds_stages == 1 && !g_bdirect)
       hipLaunchKernelGGL((gemm_planes_kernel<BM, BN, NP, F16, true, false, 1>), grid, dim3(kThr), 0, stream, p);
     else if (vec && NP == 2 && g_bdirect)
