@@ -341,6 +341,17 @@ class CCSDContext {
   std::vector<TraceRow> trace_;
 
   bool kernel_timing_ = false;
+  // residual replayed from a HIP graph (option "graph"): captured on the second
+  // residual() of a configuration, replayed until an option, the integrals or
+  // the method change (gen_ counts those changes)
+  int graph_mode_ = 1;
+  long gen_ = 0, res_seen_gen_ = -1, res_graph_gen_ = -1, res_failed_gen_ = -1;
+  hipGraph_t res_graph_ = nullptr;
+  hipGraphExec_t res_exec_ = nullptr;
+  bool graph_eligible() const;
+  void residual_ops();
+  bool capture_residual();
+  void drop_residual_graph();
   struct PendingEvent {
     int slot;
     hipEvent_t a, b;

@@ -435,11 +435,12 @@ SplitKPlan plan_splitk(const GemmDesc& d, int BM, int BN, size_t ws_bytes) {
   const int tiles_m = (int)ceil_div(d.M, BM), tiles_n = (int)ceil_div(d.N, BN);
   const long tiles = (long)tiles_m * tiles_n;
   // split-K: aim for ~4 workgroups per CU (256 CUs) when the output is small,
-  // with at least 4 K-slices of work per split
+  // with at least 16 K-slices (256 values of k) per split: shorter splits spend
+  // their time in the prologue, the epilogue and the partial sums
   int nsplit = 1;
   const long want = 1024;
-  if (tiles * d.batch < want && d.K >= 8 * kBK) {
-    nsplit = (int)std::min<long>(ceil_div(want, tiles * d.batch), d.K / (4 * kBK));
+  if (tiles * d.batch < want && d.K >= 32 * kBK) {
+    nsplit = (int)std::min<long>(ceil_div(want, tiles * d.batch), d.K / (16 * kBK));
     nsplit = std::max(1, std::min(nsplit, 512));
   }
   const size_t slice = (size_t)d.M * d.N * sizeof(double);

@@ -402,6 +402,11 @@ PermGeom make_perm_geom(const long n[4], const TTerm* terms, int nterms, const T
             break;
           }
   g.tiled = q >= 0;
+  if (g.tiled) {  // tiles smaller than a workgroup (short dims 3 and q): gathers instead
+    const long t3 = ceil_div(n[3], ceil_div(n[3], kPT));
+    const long tq = ceil_div(n[q], ceil_div(n[q], std::min<long>(kPT, std::max<long>(1, kPTEl / t3))));
+    if (t3 * tq < kPThreads) g.tiled = false;
+  }
   if (!g.tiled) q = 2;
   int r[2], nr = 0;
   for (int d = 0; d < 3; ++d)
