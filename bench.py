@@ -60,6 +60,8 @@ def _args():
     p.add_argument("--o", type=int, default=32)
     p.add_argument("--v", type=int, default=256)
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-small-system", action="store_true",
+                   help="skip the H20-chain FP64 iteration the (H2O)8 line also reports")
     p.add_argument("--batch-count", type=int, default=256, help="chains in the batched workload")
     p.add_argument("--batch-mem-gib", type=float, default=150.0,
                    help="HBM budget per group of chains in the batched workload")
@@ -531,7 +533,26 @@ def bench_ccsd(args):
     }
     if cpu:
         line["cpu_baseline"]["gpu_speedup"] = round(hf["tflops"] / cpu["value"], 1)
+    if args.workload == "ccsd-h2o8" and not args.no_small_system:
+        line["small_system"] = small_system_iteration(args)
     return line
+
+
+def small_system_iteration(args):
+    """The small-system configuration (H20 chain, o=10, v=90) in FP64: its
+    iteration time beside the headline, since at that size launch count and
+    small-tile efficiency, not the matrix cores, set the time."""
+    from mpqc_amd.ccsd import CCSD, Integrals
+    inp, _ = build_system("ccsd-h20")
+    c = CCSD(inp.o, inp.v, precision="f64", max_iter=args.max_iter)
+    c.set_integrals(Integrals(**inp.blocks))
+    c.set_orbital_energies(inp.eps_occ, inp.eps_vir)
+    E = c.compute_ccsd(target_precision=1e-8)
+    n = len(c.trace())
+    ms, _ = _timed_iterations(c, args, "h20 f64")
+    c.close()
+    return {"workload": SYSTEMS["ccsd-h20"], "form": "f64", "ms_per_iteration": round(ms, 3),
+            "iterations": n, "E_corr": "%.12f" % E, "residual": "HIP graph replay"}
 
 
 def bench_copy(args):
@@ -1261,7 +1282,8 @@ def main():
 # pass LINE_LIMIT bytes (the line is parsed from the tail of stdout)
 LINE_LIMIT = 4000
 CORE_KEYS = ("metric", "value", "unit", "n_gpus", "steps", "warmup", "ms_per_step", "higher_is_better",
-             "scaling", "vs_baseline", "dtype", "data", "config", "roofline", "cpu_baseline", "forms")
+             "scaling", "vs_baseline", "dtype", "data", "config", "roofline", "cpu_baseline", "forms",
+             "small_system")
 
 
 def emit(line):
