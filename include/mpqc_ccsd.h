@@ -158,12 +158,18 @@ const char* mpqc_ccsd_last_error(const mpqc_ccsd_ctx* ctx);
  * antisymmetric (c,d) pair combinations, rows a<=b, with half the FLOPs, exact for any
  * amplitudes given Gabcd(b,a,d,c) = Gabcd(a,b,c,d), and W_oooo's Gijab tau term likewise given
  * Gijab(l,k,d,c) = Gijab(k,l,c,d); blocks without the symmetry and the adjoint's recorded
- * residual keep the full contractions; 0: the full contractions).  Tuning keys that act on
+ * residual keep the full contractions; 0: the full contractions), "graph" (1 default: the FP64
+ * and FP32 residual is captured into a HIP graph on its second call and replayed; any option
+ * change or new integrals re-capture; 0: eager launches), "solve_restart" (1 default: solve()
+ * starts from the initial guess; 0: from the context's amplitudes; 2: from them projected onto
+ * t2(a,b,i,j) = t2(b,a,j,i)).  Tuning keys that act on
  * every context of the process (measurement aids; the defaults are the measured best):
  * "plane_tile" (tile of the 16-bit plane contractions, rows * 1000 + columns, 0 automatic),
  * "plane_lds_stages" (1 or 2), "plane_split_cap" (most K splits of a plane contraction whose
  * tiles do not fill the GPU, default 512; set before the context's first residual), "plane_tile_order" (0 XCD ranges of row-major tiles, 1 plain,
- * 2 XCD ranges of column-major tiles). */
+ * 2 XCD ranges of column-major tiles), "plane_dl" (0 the deep-pipeline plane kernel where planned,
+ * 1 never), "thin_k" (0 the streaming kernel for FP64 contractions with K <= 64 and at most 256
+ * rows of A, 1 never). */
 int mpqc_ccsd_set_option(mpqc_ccsd_ctx* ctx, const char* key, double value);
 
 int mpqc_ccsd_set_integral(mpqc_ccsd_ctx* ctx, int which, const double* data, int layout,
