@@ -502,6 +502,7 @@ void CCSDContext::set_amplitudes(const double* t1, const double* t2, int layout,
   upload(t1_, t1, {kNone, kNone, kVir, kOcc}, layout, memspace);
   upload(t2_, t2, {kVir, kVir, kOcc, kOcc}, layout, memspace);
   build_tau(tau_, t2_, t1_, o_, v_, stream_);
+  tau_fresh_ = true;
   MPQC_HIP_CHECK(hipStreamSynchronize(stream_));
 }
 
@@ -926,6 +927,7 @@ void CCSDContext::prepare() {
   planes_ready_ = false;
   pp_packed_ = false;
   pp_checked_ = false;
+  iabc_checked_ = false;
 }
 
 void CCSDContext::init_guess() {
@@ -938,6 +940,7 @@ void CCSDContext::init_guess() {
   MPQC_HIP_CHECK(hipMemcpyAsync(r1_, f_ai_, ov * sizeof(double), hipMemcpyDeviceToDevice, stream_));
   MPQC_HIP_CHECK(hipMemcpyAsync(r2_, g_abij_, oovv * sizeof(double), hipMemcpyDeviceToDevice, stream_));
   MPQC_HIP_CHECK(hipStreamSynchronize(stream_));
+  tau_fresh_ = true;
 }
 
 // ---------------------------------------------------------------- residual
@@ -985,6 +988,10 @@ bool CCSDContext::capture_residual() {
 
 void CCSDContext::residual() {
   if (!prepared_) prepare();
+  // tau("a,b,i,j") = t2("a,b,i,j") + t1("a,i") t1("b,j") (ccsd.h:389): the update and
+  // set_amplitudes leave it current; rebuilt here otherwise (and always on the tape)
+  if (!tau_fresh_ || tape_) op_tau();
+  if (!tape_ || tape_execute_) tau_fresh_ = true;  // (a tape recorded without executing runs nothing)
   if (graph_eligible()) {
     if (res_exec_ && res_graph_gen_ == gen_) {
       MPQC_HIP_CHECK(hipGraphLaunch(res_exec_, stream_));
@@ -1009,9 +1016,7 @@ void CCSDContext::residual_ops() {
   const std::array<long, 4> e_abij{V, V, O, O};
   const std::array<int, 4> I{0, 1, 2, 3};
 
-  // tau("a,b,i,j") = t2("a,b,i,j") + t1("a,i") t1("b,j")            ccsd.h:389
-  op_tau();
-  // amplitude layouts used by several GEMMs
+  // amplitude layouts used by several GEMMs (tau is current: residual())
   perm_add(t1T_, {1, 1, O, V}, 0.0, {{t1_, {1, 1, V, O}, {0, 1, 3, 2}, 1.0}});  // t1T(k,d) = t1(d,k)
   const bool sm = split_mode();
   if (direct_ && sm) throw std::logic_error("the direct residual runs in the f64 and f32 forms");
@@ -1026,6 +1031,8 @@ void CCSDContext::residual_ops() {
     amax_.ptr[0] = t1_;
     amax_.ptr[1] = t2_;
     amax_.ptr[2] = tau_;
+    amax_.ptr[3] = J_;  // final when the ring operands are split (max |K| then serves two splits)
+    amax_.ptr[4] = K_;
     split(tau_P_, e_abij, {{tau_, e_abij, I, 1.0}});
     // tau^T(i,j,c,d) = tau(c,d,i,j): the B^T operand of every tau contraction
     split(tauT_P_, {O, O, V, V}, {{tau_, e_abij, {2, 3, 0, 1}, 1.0}});
@@ -1104,6 +1111,18 @@ void CCSDContext::residual_ops() {
   }
 
   // ===================== r2: ccsd_r1_r2.h:198-293 =====================
+  // Giabc(i,c,a,b) = Giabc(i,b,a,c): Rd's Giabc t1 term is a permutation of Zj (not on the
+  // adjoint's tape, which keeps the reference's contractions)
+  const bool iabc_sym = !df_ && !tape_ && giabc_symmetric_usable();
+  // Zj(k,a,c,i) = Giabc(k,a,c,d) t1(d,i), the j term's Giabc t1 part (:240); in FP64 when it
+  // also stands for Rd's term (a T1 dressing term in 16-bit planes slows the split forms'
+  // convergence: (H2O)8 bf16x3 36 iterations to 1e-8 instead of 18)
+  auto j_giabc_t1 = [&] {
+    if (sm && !iabc_sym)
+      gemmP(o * v * v, o, v, 1.0, Giabc_P_, v, t1T_P_, v, 0.0, Zj_, o, "r2:j_giabc_t1");
+    else
+      gemm(false, false, o * v * v, o, v, 1.0, Giabc_, v, t1_, o, 0.0, Zj_, o, 0, 0, 0, 1, "r2:j_giabc_t1");
+  };
   if (df_) {
     // df :317 X_ab_t1(K,a,i) = Xab(K,a,b) t1(b,i)
     gemm(false, false, (int)(Q * V), o, v, 1.0, Xab_, v, t1_, o, 0.0, Xabt1_, o, 0, 0, 0, 1, "r2:df_Xab_t1");
@@ -1115,8 +1134,13 @@ void CCSDContext::residual_ops() {
     perm_add(Rd_, {V, V, O, O}, 0.0, {{Rdf_, {V, O, V, O}, {0, 2, 1, 3}, 1.0}});
   } else {
   // :207-208 Rd = (Giabc(i,c,a,b) - Giajb(k,b,i,c) t1(a,k)) t1(c,j)
-  //   first term, batched over i: Rd[(a,b), i, j] = sum_c Giabc[i][c,(a,b)] t1[c,j]
-  gemm(true, false, v * v, o, v, 1.0, Giabc_, vv, t1_, o, 0.0, Rd_, oo, vv * V, 0, O, o, "r2:giabc_t1");
+  if (iabc_sym) {
+    //   first term: Zj(i,b,a,j), added in the half-sum pass below
+    j_giabc_t1();
+  } else {
+    //   first term, batched over i: Rd[(a,b), i, j] = sum_c Giabc[i][c,(a,b)] t1[c,j]
+    gemm(true, false, v * v, o, v, 1.0, Giabc_, vv, t1_, o, 0.0, Rd_, oo, vv * V, 0, O, o, "r2:giabc_t1");
+  }
   //   second term as t1(a,k) * (Giajb(k,b,i,c) t1(c,j))
   gemm(false, false, o * o * v, o, v, 1.0, Giajb_, v, t1_, o, 0.0, Q_, o, 0, 0, 0, 1, "r2:giajb_t1");
   }
@@ -1133,7 +1157,8 @@ void CCSDContext::residual_ops() {
     // Q(k,b,i,j) += S(b,j,i,k); then Rd -= t1(a,k) Q(k,(b,i,j))
     perm_add(Q_, {O, V, O, O}, 1.0,
              {{S11_, {V, O, O, O}, {3, 0, 2, 1}, 1.0}, {M12_, {O, O, V, O}, {1, 2, 3, 0}, 1.0}});
-    gemm(false, false, v, v * o * o, o, -1.0, t1_, o, Q_, V * oo, 1.0, Rd_, V * oo, 0, 0, 0, 1, "r2:t1_giajb_S");
+    gemm(false, false, v, v * o * o, o, -1.0, t1_, o, Q_, V * oo, iabc_sym ? 0.0 : 1.0, Rd_, V * oo, 0, 0, 0, 1,
+         "r2:t1_giajb_S");
   }
   // :218-220 g_ki = F_OO(FIJ, Fia, g_ijka_bar, t1) = FIJ + Fia t1 + gbar(k,l,i,c) t1(c,l)
   //          (ccsd_intermediates.h:105-112)
@@ -1195,13 +1220,11 @@ void CCSDContext::residual_ops() {
     gemm(true, false, v * v, o * o, (int)Q, 1.0, Xab_, vv, V1_, oo, 0.0, Zk_, oo, 0, 0, 0, 1, "r2:df_k_V1_Xab");
     perm_add(K_, {V, O, O, V}, 1.0, {{Zk_, {V, V, O, O}, {0, 3, 2, 1}, 1.0}});
   } else {
-  if (sm) {
-    gemmP(o * v * v, o, v, 1.0, Giabc_P_, v, t1T_P_, v, 0.0, Zj_, o, "r2:j_giabc_t1");
+  if (!iabc_sym) j_giabc_t1();  // (computed with Rd's T1 dressing otherwise)
+  if (sm)
     gemmP(o * v * v, o, v, 1.0, GiabcT_P_, v, t1T_P_, v, 0.0, Zk_, o, "r2:k_giabc_t1");
-  } else {
-    gemm(false, false, o * v * v, o, v, 1.0, Giabc_, v, t1_, o, 0.0, Zj_, o, 0, 0, 0, 1, "r2:j_giabc_t1");  // Zj(k,a,c,i)
+  else
     gemm(false, false, o * v * v, o, v, 1.0, GiabcT_, v, t1_, o, 0.0, Zk_, o, 0, 0, 0, 1, "r2:k_giabc_t1");  // Zk(k,a,c,i)
-  }
   perm_add(J_, {V, O, O, V}, 1.0, {{Zj_, {O, V, V, O}, {1, 3, 0, 2}, 1.0}, {J0_, {V, O, O, V}, I, 1.0}});
   perm_add(K_, {V, O, O, V}, 1.0, {{Zk_, {O, V, V, O}, {1, 3, 0, 2}, 1.0}, {K0_, {V, O, O, V}, I, 1.0}});
   }
@@ -1246,10 +1269,17 @@ void CCSDContext::residual_ops() {
   // r2 = H + H(b,a,j,i) + Gijab: 9 tensor reads and writes in place of 12
   {
     const std::array<long, 4> e_aibj{V, O, V, O};
-    perm_add(Rd_, {V, V, O, O}, 1.0,
-             {{RR_, e_aibj, {0, 2, 1, 3}, 1.0},
-              {tmpR_, e_aibj, {0, 2, 1, 3}, -0.5},
-              {tmpR_, e_aibj, {2, 0, 1, 3}, -1.0}});
+    if (iabc_sym)  // with Rd's first T1 dressing term, Giabc(i,c,a,b) t1(c,j) = Zj(i,b,a,j)
+      perm_add(Rd_, {V, V, O, O}, 1.0,
+               {{RR_, e_aibj, {0, 2, 1, 3}, 1.0},
+                {tmpR_, e_aibj, {0, 2, 1, 3}, -0.5},
+                {tmpR_, e_aibj, {2, 0, 1, 3}, -1.0},
+                {Zj_, {O, V, V, O}, {2, 1, 0, 3}, 1.0}});
+    else
+      perm_add(Rd_, {V, V, O, O}, 1.0,
+               {{RR_, e_aibj, {0, 2, 1, 3}, 1.0},
+                {tmpR_, e_aibj, {0, 2, 1, 3}, -0.5},
+                {tmpR_, e_aibj, {2, 0, 1, 3}, -1.0}});
     perm_add(r2_, {V, V, O, O}, 0.0,
              {{Rd_, e_abij, I, 1.0},
               {Rd_, e_abij, {1, 0, 3, 2}, 1.0},
@@ -1444,6 +1474,7 @@ void CCSDContext::update() {
   }
   // tau("a,b,i,j") = t2 + t1 t1                                     ccsd.h:389
   timed("up:tau", 0.0, [&] { build_tau(tau_, t2_, t1_, o_, v_, stream_); });
+  tau_fresh_ = true;
   MPQC_HIP_CHECK(hipStreamSynchronize(stream_));
 }
 
@@ -1457,6 +1488,7 @@ double CCSDContext::solve(int max_iter, double target_precision, int verbose, in
       const long V = v_, O = o_;
       perm_add(Rd_, {V, V, O, O}, 0.0, {{t2_, {V, V, O, O}, {0, 1, 2, 3}, 0.5}, {t2_, {V, V, O, O}, {1, 0, 3, 2}, 0.5}});
       copy_d2d(t2_, Rd_, V * V * O * O);
+      tau_fresh_ = false;
     }
     residual();  // the residual of the current amplitudes in the current form
   }

@@ -269,6 +269,8 @@ class CCSDContext {
   double *Gpk_ = nullptr, *Gmk_ = nullptr, *WPk_ = nullptr, *WMk_ = nullptr;  // W_oooo's G+/- and products
   Planes Gp_P_sym_, Gm_P_sym_;
   bool w_sym_ok_ = false;  // Gijab(l,k,d,c) = Gijab(k,l,c,d) holds (checked with Gabcd's symmetry)
+  bool iabc_checked_ = false, iabc_sym_ok_ = false;  // Giabc(i,c,a,b) = Giabc(i,b,a,c) checked / holds
+  bool giabc_symmetric_usable();
   double *Vpk_ = nullptr, *Vmk_ = nullptr, *Spk_ = nullptr, *Apk_ = nullptr, *Ppk_ = nullptr, *Mpk_ = nullptr;
   Planes Vp_P_, Vm_P_, Sp_P_, Ap_P_;
   Planes GbarT_P_, Gbar_P_, Gbar_akcd_P_, P8T_P_, GiabcT_P_, P13_P_, P14_P_;
@@ -345,6 +347,7 @@ class CCSDContext {
   // residual() of a configuration, replayed until an option, the integrals or
   // the method change (gen_ counts those changes)
   int graph_mode_ = 1;
+  bool tau_fresh_ = false;  // tau_ holds t2 + t1 t1 of the current amplitudes
   long gen_ = 0, res_seen_gen_ = -1, res_graph_gen_ = -1, res_failed_gen_ = -1;
   hipGraph_t res_graph_ = nullptr;
   hipGraphExec_t res_exec_ = nullptr;

@@ -127,6 +127,36 @@ __global__ __launch_bounds__(kSymBlock) void pair_asymmetry_kernel(double* __res
   }
 }
 
+// per block, over G with extents (n0, n1, n2, n1): max |G(i,x,a,y) - G(i,y,a,x)| and
+// max |G(i,x,a,y)| -> part[2 blk], part[2 blk + 1]
+__global__ __launch_bounds__(kSymBlock) void swap13_asymmetry_kernel(double* __restrict__ part,
+                                                                     const double* __restrict__ G, long n0,
+                                                                     long n1, long n2) {
+  __shared__ double sd[kSymBlock], sg[kSymBlock];
+  const long total = n0 * n1 * n2 * n1;
+  double md = 0.0, mg = 0.0;
+  for (long e = (long)blockIdx.x * kSymBlock + threadIdx.x; e < total; e += (long)gridDim.x * kSymBlock) {
+    const long y = e % n1, t0 = e / n1, a = t0 % n2, t1 = t0 / n2, x = t1 % n1, i = t1 / n1;
+    const double g = G[e];
+    md = fmax(md, fabs(g - G[((i * n1 + y) * n2 + a) * n1 + x]));
+    mg = fmax(mg, fabs(g));
+  }
+  sd[threadIdx.x] = md;
+  sg[threadIdx.x] = mg;
+  __syncthreads();
+  for (int w = kSymBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      sd[threadIdx.x] = fmax(sd[threadIdx.x], sd[threadIdx.x + w]);
+      sg[threadIdx.x] = fmax(sg[threadIdx.x], sg[threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = sd[0];
+    part[2 * blockIdx.x + 1] = sg[0];
+  }
+}
+
 int sym_grid(long n) { return (int)std::max<long>(1, std::min<long>(ceil_div(n, kSymBlock), 65536)); }
 
 std::vector<int2> pair_table(long n, bool strict) {
@@ -166,6 +196,31 @@ bool CCSDContext::pp_symmetric_usable() {
   w_sym_ok_ = pp_sym_ok_ && symmetric(Gijab_, o_, v_);
   pp_checked_ = true;
   return pp_sym_ok_;
+}
+
+// whether Giabc(i,c,a,b) = Giabc(i,b,a,c) (<ic|ab> = (ia|cb) = (ia|bc) = <ib|ac> for real
+// orbitals) to 1e-12 of its largest element: then the T1 dressing term
+// Giabc(i,c,a,b) t1(c,j) of Rd (ccsd_r1_r2.h:207) is the j term's Zj(i,b,a,j) =
+// Giabc(i,b,a,d) t1(d,j) (:240), and Rd takes it from there instead of a contraction
+// that reads Giabc again
+bool CCSDContext::giabc_symmetric_usable() {
+  if (iabc_checked_) return iabc_sym_ok_;
+  const long total = (long)o_ * v_ * v_ * v_;
+  const int blocks = (int)std::max<long>(1, std::min<long>(ceil_div(total, kSymBlock), kReducePartials / 2));
+  hipLaunchKernelGGL(swap13_asymmetry_kernel, dim3(blocks), dim3(kSymBlock), 0, stream_, partials_, Giabc_,
+                     (long)o_, (long)v_, (long)v_);
+  check_launch("swap13_asymmetry");
+  std::vector<double> h(2 * (size_t)blocks);
+  MPQC_HIP_CHECK(hipMemcpyAsync(h.data(), partials_, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
+  MPQC_HIP_CHECK(hipStreamSynchronize(stream_));
+  double md = 0.0, mg = 0.0;
+  for (int b = 0; b < blocks; ++b) {
+    md = std::max(md, h[2 * b]);
+    mg = std::max(mg, h[2 * b + 1]);
+  }
+  iabc_sym_ok_ = md <= 1e-12 * mg;
+  iabc_checked_ = true;
+  return iabc_sym_ok_;
 }
 
 // pair tables, packed V+ / V- (FP64) and the per-iteration buffers

@@ -70,12 +70,13 @@ def test_pair_symmetric_ladder_solve_energy():
     assert abs(E[1] - E[0]) < 1e-11
 
 
-@pytest.mark.parametrize("block", ["Gabcd", "Gijab"])
+@pytest.mark.parametrize("block", ["Gabcd", "Gijab", "Giabc"])
 @pytest.mark.parametrize("prec,bar", [("f64", 1e-12), ("f16x3", 1e-6)])
 def test_asymmetric_blocks_take_the_full_contraction(prec, bar, block):
-    # Gabcd (pp ladder) or Gijab (hole-hole intermediate) without the pair symmetry (a
-    # symmetry-breaking perturbation): the context detects it and runs the full
-    # contraction, so the residual still matches the oracle
+    # Gabcd (pp ladder) or Gijab (hole-hole intermediate) without the pair symmetry, or
+    # Giabc without Giabc(i,c,a,b) = Giabc(i,b,a,c) (Rd's T1 dressing term taken from the j
+    # term's Giabc t1), after a symmetry-breaking perturbation: the context detects it and
+    # runs the reference's contraction, so the residual still matches the oracle
     from oracle import ccsd as O
     no, nv, f, b, t1, t2, _, _ = _case()
     rng = np.random.default_rng(9)
