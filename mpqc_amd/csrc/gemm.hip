@@ -904,7 +904,162 @@ void launch_thin_k(const GemmDesc& d, hipStream_t stream) {
     launch_thin_k_vec<false>(p, waves, d.batch, stream);
 }
 
-int g_thin_k_never = 0;  // 1: the tiled kernels take the shapes thin_k_shape() admits
+// ---------------------------------------------------------------- thin N
+// C = alpha A B + beta C for a tall A (M x K, K contiguous, K <= 256) against a
+// small B (K x N, N <= 64), no transposes: the T1 products Giabc(k,a,c,d) t1(d,i),
+// Giajb(k,b,i,c) t1(c,j), Gijab t1, Gijka t1 (ccsd_r1_r2.h:207-211, :240, :246,
+// ccsd_intermediates.h:182-193).  A streams through HBM once; the tiled kernels
+// pad N to 64 and stage A through LDS.
+//
+// B sits in LDS for the whole launch (zero-padded to 16 K x 16 N blocks).  Each
+// wave walks 16-row blocks of A: per 16-wide K chunk a lane loads 32 contiguous
+// bytes, A(m0 + l%16, 16t + 4g .. 16t + 4g + 3) with g = l/16 (a 16-row block reads
+// whole 128-byte row pieces), and issues four v_mfma_f64_16x16x4f64 per 16 columns of
+// N, the q-th taking k = 16t + 4g + q from lane group g (any assignment of the 16 k
+// of a chunk to the four MFMA k slots works when B uses the same one).
+struct ThinNParams {
+  const double* A;
+  const double* B;
+  double* C;
+  long lda, ldb, ldc;
+  int M, N, K;
+  double alpha, beta;
+};
+
+constexpr int kThinNThreads = 512;
+
+template <int NB>
+constexpr int thin_n_pitch() { return 16 * NB + 4; }  // 4 rows apart land 128 bytes apart in the banks
+
+// NB: 16-column blocks (N <= 16 NB); KC: 16-wide K chunks (K <= 16 KC); VEC: 16-byte
+// loads of A (A 16-byte aligned, lda even)
+template <int NB, int KC, bool VEC>
+__global__ __launch_bounds__(kThinNThreads) void dgemm_thin_n_kernel(ThinNParams p) {
+  constexpr int PITCH = thin_n_pitch<NB>();
+  __shared__ double Bs[16 * KC * PITCH];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lr = lane & 15, g = lane >> 4;
+  for (int e = tid; e < 16 * KC * 16 * NB; e += kThinNThreads) {
+    const int k = e / (16 * NB), n = e - k * (16 * NB);
+    Bs[k * PITCH + n] = (k < p.K && n < p.N) ? p.B[(long)k * p.ldb + n] : 0.0;
+  }
+  __syncthreads();
+  const long nblk = ((long)p.M + 15) / 16;
+  const long waves = (long)gridDim.x * (kThinNThreads / 64);
+  const bool read_c = p.beta != 0.0;
+  for (long blk = (long)blockIdx.x * (kThinNThreads / 64) + wave; blk < nblk; blk += waves) {
+    const long m = 16 * blk + lr;
+    const bool mrow = m < p.M;
+    // the B fragments are the same for every block: an opaque column offset keeps the
+    // compiler from hoisting all of them out of the loop (they would not fit in VGPRs)
+    int col = lr;
+    asm volatile("" : "+v"(col));
+    const double* __restrict__ a = p.A + (mrow ? m : (long)p.M - 1) * p.lda;
+    double av[KC][4];
+#pragma unroll
+    for (int t = 0; t < KC; ++t) {
+      const int k0 = 16 * t + 4 * g;
+      if (VEC && k0 + 3 < p.K) {
+        const double2 x = *reinterpret_cast<const double2*>(a + k0);
+        const double2 y = *reinterpret_cast<const double2*>(a + k0 + 2);
+        av[t][0] = x.x; av[t][1] = x.y; av[t][2] = y.x; av[t][3] = y.y;
+      } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) av[t][q] = k0 + q < p.K ? a[k0 + q] : 0.0;
+      }
+    }
+    double4_t acc[NB];
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) acc[nb] = double4_t{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int t = 0; t < KC; ++t)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const double* brow = Bs + (16 * t + 4 * g + q) * PITCH + col;
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb)
+          acc[nb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[t][q], brow[16 * nb], acc[nb], 0, 0, 0);
+      }
+    // accumulator: lane holds C(16 blk + 4 r + g, 16 nb + lr)
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) {
+      const int n = 16 * nb + lr;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const long mm = 16 * blk + 4 * r + g;
+        if (mm < p.M && n < p.N) {
+          double* c = p.C + mm * p.ldc + n;
+          *c = read_c ? fma(p.beta, *c, p.alpha * acc[nb][r]) : p.alpha * acc[nb][r];
+        }
+      }
+    }
+  }
+}
+
+bool thin_n_shape(const GemmDesc& d) {
+  const long npad = ceil_div(d.N, 16) * 16, kpad = ceil_div(d.K, 16) * 16;
+  return !d.transA && !d.transB && d.batch == 1 && d.N <= 64 && d.K <= 256 && d.M >= 4096 &&
+         kpad * (npad + 4) <= 10240;  // B in at most 80 KB of LDS
+}
+
+template <int NB, int KC, bool VEC>
+int thin_n_resident() {
+  static int cached = 0;
+  if (cached == 0) {
+    int dev = 0, cus = 0, per_cu = 0;
+    MPQC_HIP_CHECK(hipGetDevice(&dev));
+    MPQC_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    MPQC_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dgemm_thin_n_kernel<NB, KC, VEC>,
+                                                                 kThinNThreads, 0));
+    cached = std::max(1, cus) * std::max(1, per_cu);
+  }
+  return cached;
+}
+
+template <int NB, int KC, bool VEC>
+void launch_thin_n_t(const ThinNParams& p, hipStream_t stream) {
+  const long blocks = ceil_div(p.M, 16), waves_per_wg = kThinNThreads / 64;
+  const int grid = (int)std::min<long>(ceil_div(blocks, waves_per_wg), thin_n_resident<NB, KC, VEC>());
+  hipLaunchKernelGGL((dgemm_thin_n_kernel<NB, KC, VEC>), dim3((unsigned)grid), dim3(kThinNThreads), 0, stream, p);
+  check_launch("dgemm_thin_n");
+}
+
+template <int NB, bool VEC>
+void launch_thin_n_nb(const ThinNParams& p, hipStream_t stream) {
+  switch ((p.K + 15) / 16) {
+    case 1: case 2: launch_thin_n_t<NB, 2, VEC>(p, stream); break;
+    case 3: case 4: launch_thin_n_t<NB, 4, VEC>(p, stream); break;
+    case 5: case 6: launch_thin_n_t<NB, 6, VEC>(p, stream); break;
+    case 7: case 8: launch_thin_n_t<NB, 8, VEC>(p, stream); break;
+    case 9: case 10: launch_thin_n_t<NB, 10, VEC>(p, stream); break;
+    case 11: case 12: launch_thin_n_t<NB, 12, VEC>(p, stream); break;
+    default: launch_thin_n_t<NB, 16, VEC>(p, stream); break;
+  }
+}
+
+template <bool VEC>
+void launch_thin_n_vec(const ThinNParams& p, hipStream_t stream) {
+  switch ((p.N + 15) / 16) {
+    case 1: launch_thin_n_nb<1, VEC>(p, stream); break;
+    case 2: launch_thin_n_nb<2, VEC>(p, stream); break;
+    case 3: launch_thin_n_nb<3, VEC>(p, stream); break;
+    default: launch_thin_n_nb<4, VEC>(p, stream); break;
+  }
+}
+
+void launch_thin_n(const GemmDesc& d, hipStream_t stream) {
+  ThinNParams p;
+  p.A = d.A; p.B = d.B; p.C = d.C;
+  p.lda = d.lda; p.ldb = d.ldb; p.ldc = d.ldc;
+  p.M = d.M; p.N = d.N; p.K = d.K; p.alpha = d.alpha; p.beta = d.beta;
+  const bool vec = ((uintptr_t)d.A % 16 == 0) && d.lda % 2 == 0;
+  if (vec)
+    launch_thin_n_vec<true>(p, stream);
+  else
+    launch_thin_n_vec<false>(p, stream);
+}
+
+int g_thin_k_never = 0;  // 1: the tiled kernels take the shapes thin_k_shape() and thin_n_shape() admit
 
 }  // namespace
 
@@ -923,6 +1078,10 @@ void gemm(const GemmDesc& d, GemmPrecision prec, const GemmWorkspace& ws, hipStr
     launch_thin_k(d, stream);
     return;
   }
+  if (!g_thin_k_never && prec == GemmPrecision::F64 && thin_n_shape(d)) {
+    launch_thin_n(d, stream);
+    return;
+  }
   if (wide_tiles(d, ws.bytes))
     launch<128, 128>(d, prec, ws, stream);
   else
@@ -933,7 +1092,7 @@ size_t gemm_scratch_bytes(const GemmDesc& d, GemmPrecision prec, size_t ws_bytes
   if (d.M <= 0 || d.N <= 0 || d.batch <= 0 || d.K <= 0) return 0;
   if (prec == GemmPrecision::BF16X3 || prec == GemmPrecision::F16X3 || prec == GemmPrecision::BF16X6)
     return gemm_split_scratch_bytes(d, prec, ws_bytes);
-  if (!g_thin_k_never && prec == GemmPrecision::F64 && thin_k_shape(d)) return 0;
+  if (!g_thin_k_never && prec == GemmPrecision::F64 && (thin_k_shape(d) || thin_n_shape(d))) return 0;
   const bool wide = wide_tiles(d, ws_bytes);
   return plan_splitk(d, wide ? 128 : 64, wide ? 128 : 64, ws_bytes).scratch_bytes;
 }

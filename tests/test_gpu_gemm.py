@@ -91,6 +91,32 @@ def test_gemm_thin_k(m, n, k, pad, odd_ldb, beta):
     assert np.max(np.abs(got - ref)) <= 1e-14 * k * 16 + 1e-13
 
 
+@pytest.mark.parametrize("beta", [0.0, 1.0])
+@pytest.mark.parametrize("m,n,k,pad", [(5000, 40, 152, 0), (4096, 1, 3, 0), (4100, 17, 33, 1),
+                                       (10000, 64, 120, 0), (6001, 40, 150, 3), (4500, 48, 16, 2)])
+def test_gemm_thin_n(m, n, k, pad, beta):
+    """A tall A (M >= 4096, K <= 256) against a small B (N <= 64): the streaming kernel
+    with B in LDS (ragged M, N and K, an odd lda takes 8-byte loads, beta = 0 must not
+    read C)."""
+    from mpqc_amd.ccsd import gemm
+    rng = np.random.default_rng(m + 3 * n + k)
+    lda, ldb, ldc = k + pad, n + pad, n + 2 * pad
+    A = rng.standard_normal((m, lda))
+    B = rng.standard_normal((k, ldb))
+    C = rng.standard_normal((m, ldc))
+    if beta == 0.0:
+        C[:, :n] = np.nan
+    ref = C.copy()
+    ref[:, :n] = -0.9 * (A[:, :k] @ B[:, :n]) + (beta * C[:, :n] if beta != 0.0 else 0.0)
+    dA, dB, dC = _dev(A), _dev(B), _dev(C)
+    gemm(0, 0, m, n, k, -0.9, dA.data_ptr(), lda, dB.data_ptr(), ldb, beta, dC.data_ptr(), ldc)
+    torch.cuda.synchronize()
+    got = dC.cpu().numpy()
+    assert np.all(np.isfinite(got[:, :n]))
+    assert np.array_equal(got[:, n:], C[:, n:])
+    assert np.max(np.abs(got - ref)) <= 1e-14 * k * 16 + 1e-13
+
+
 def test_gemm_thin_k_batched():
     from mpqc_amd.ccsd import gemm
     rng = np.random.default_rng(11)
