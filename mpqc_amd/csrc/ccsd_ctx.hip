@@ -1190,9 +1190,9 @@ void CCSDContext::residual_ops() {
   } else {
     gemm(false, false, v, v * o * o, v, 1.0, gac_, v, t2_, V * oo, 1.0, Rd_, V * oo, 0, 0, 0, 1, "r2:g_ac_t2");
   }
-  perm_add(t2_abjk_, {V, V, O, O}, 0.0, {{t2_, e_abij, {0, 1, 3, 2}, 1.0}});
-  gemm(false, false, v * v * o, o, o, 1.0, t2_abjk_, o, gki_, o, 0.0, X16_, o, 0, 0, 0, 1, "r2:t2_gki");
-  perm_add(Rd_, {V, V, O, O}, 1.0, {{X16_, e_abij, {0, 1, 3, 2}, -1.0}});
+  //   - g_ki(k,i) t2(a,b,k,j): per (a,b), Rd(a,b,:,:) -= g^T t2(a,b,:,:), a batch of v^2
+  //   o x o x o products reading t2 and Rd in place (no transposed copies)
+  gemm(true, false, o, o, o, -1.0, gki_, o, t2_, o, 1.0, Rd_, o, 0, oo, oo, v * v, "r2:t2_gki");
   // :236 T(d,b,i,l) = 0.5 t2(d,b,i,l) + t1(d,i) t1(b,l), as TP(a,i,l,d) = T(d,a,i,l)
   if (!sm || df_) op_TP();  // (the df j term reads TP in FP64)
   if (!sm) perm_add(t2P_, {V, O, O, V}, 0.0, {{t2_, e_abij, {0, 2, 3, 1}, 1.0}});  // t2P(a,i,l,d) = t2(a,d,i,l)
