@@ -1059,6 +1059,132 @@ void launch_thin_n(const GemmDesc& d, hipStream_t stream) {
     launch_thin_n_vec<false>(p, stream);
 }
 
+// ---------------------------------------------------------------- many small products, one shared A
+// C_b = alpha op(A) B_b + beta C_b for b = 0 .. batch-1 with M, N, K <= 64, one A for
+// every b (sA = 0), no transposed B: the T2 g_ki term of Rd (ccsd_r1_r2.h:226), a
+// product per (a,b) pair of o x o blocks.  One wave per product: the wave holds A's
+// v_mfma_f64_16x16x4f64 fragments for the whole launch, loads the product's B
+// fragments and C rows (rows of 16 doubles), and walks the products with stride
+// (waves in the grid).  The tiled kernels give each such product a 64 x 64 workgroup
+// that stages 16-wide K slices through LDS.
+struct SmallParams {
+  const double* A;
+  const double* B;
+  double* C;
+  long lda, ldb, ldc, sB, sC;
+  int M, N, K, batch;
+  bool transA;
+  double alpha, beta;
+};
+
+constexpr int kSmallThreads = 256;
+
+// S: 16-row and 16-column blocks (M, N <= 16 S); KS: steps of 4 along K (K <= 4 KS)
+template <int S, int KS>
+__global__ __launch_bounds__(kSmallThreads) void dgemm_small_batched_kernel(SmallParams p) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  // A fragments: lane holds op(A)(16 ib + lr, 4 s + lk)
+  double af[S][KS];
+#pragma unroll
+  for (int ib = 0; ib < S; ++ib)
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      const int i = 16 * ib + lr, k = 4 * s + lk;
+      af[ib][s] = (i < p.M && k < p.K) ? (p.transA ? p.A[(long)k * p.lda + i] : p.A[(long)i * p.lda + k]) : 0.0;
+    }
+  const bool read_c = p.beta != 0.0;
+  const long nw = (long)gridDim.x * (kSmallThreads / 64);
+  for (long b = (long)blockIdx.x * (kSmallThreads / 64) + wave; b < p.batch; b += nw) {
+    const double* __restrict__ B = p.B + b * p.sB;
+    double* __restrict__ C = p.C + b * p.sC;
+    // one 16-column block of the product at a time (registers: A's fragments stay, the
+    // block's B fragments, C rows and accumulators come and go)
+#pragma unroll 1
+    for (int jb = 0; jb < S; ++jb) {
+      const int j = 16 * jb + lr;
+      double bf[KS];
+#pragma unroll
+      for (int s = 0; s < KS; ++s) {
+        const int k = 4 * s + lk;
+        bf[s] = (j < p.N && k < p.K) ? B[(long)k * p.ldb + j] : 0.0;
+      }
+      double cin[S][4];
+#pragma unroll
+      for (int ib = 0; ib < S; ++ib)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = 16 * ib + 4 * r + lk;
+          cin[ib][r] = (read_c && i < p.M && j < p.N) ? C[(long)i * p.ldc + j] : 0.0;
+        }
+      double4_t acc[S];
+#pragma unroll
+      for (int ib = 0; ib < S; ++ib) acc[ib] = double4_t{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int s = 0; s < KS; ++s)
+#pragma unroll
+        for (int ib = 0; ib < S; ++ib)
+          acc[ib] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[ib][s], bf[s], acc[ib], 0, 0, 0);
+      // accumulator: lane holds C(16 ib + 4 r + lk, 16 jb + lr)
+#pragma unroll
+      for (int ib = 0; ib < S; ++ib)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = 16 * ib + 4 * r + lk;
+          if (i < p.M && j < p.N)
+            C[(long)i * p.ldc + j] = read_c ? fma(p.beta, cin[ib][r], p.alpha * acc[ib][r]) : p.alpha * acc[ib][r];
+        }
+    }
+  }
+}
+
+bool small_batched_shape(const GemmDesc& d) {
+  return d.sA == 0 && !d.transB && d.M <= 64 && d.N <= 64 && d.K <= 64 && d.batch >= 256;
+}
+
+template <int S, int KS>
+void launch_small_batched_t(const SmallParams& p, hipStream_t stream) {
+  static int resident = 0;
+  if (resident == 0) {
+    int dev = 0, cus = 0, per_cu = 0;
+    MPQC_HIP_CHECK(hipGetDevice(&dev));
+    MPQC_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    MPQC_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dgemm_small_batched_kernel<S, KS>,
+                                                                 kSmallThreads, 0));
+    resident = std::max(1, cus) * std::max(1, per_cu);
+  }
+  const int grid = (int)std::min<long>(ceil_div(p.batch, kSmallThreads / 64), resident);
+  hipLaunchKernelGGL((dgemm_small_batched_kernel<S, KS>), dim3((unsigned)grid), dim3(kSmallThreads), 0, stream, p);
+  check_launch("dgemm_small_batched");
+}
+
+template <int S>
+void launch_small_batched_s(const SmallParams& p, hipStream_t stream) {
+  switch ((p.K + 3) / 4) {
+    case 1: case 2: launch_small_batched_t<S, 2>(p, stream); break;
+    case 3: case 4: launch_small_batched_t<S, 4>(p, stream); break;
+    case 5: case 6: launch_small_batched_t<S, 6>(p, stream); break;
+    case 7: case 8: launch_small_batched_t<S, 8>(p, stream); break;
+    case 9: case 10: launch_small_batched_t<S, 10>(p, stream); break;
+    case 11: case 12: launch_small_batched_t<S, 12>(p, stream); break;
+    default: launch_small_batched_t<S, 16>(p, stream); break;
+  }
+}
+
+void launch_small_batched(const GemmDesc& d, hipStream_t stream) {
+  SmallParams p;
+  p.A = d.A; p.B = d.B; p.C = d.C;
+  p.lda = d.lda; p.ldb = d.ldb; p.ldc = d.ldc; p.sB = d.sB; p.sC = d.sC;
+  p.M = d.M; p.N = d.N; p.K = d.K; p.batch = d.batch; p.transA = d.transA;
+  p.alpha = d.alpha; p.beta = d.beta;
+  switch ((std::max(d.M, d.N) + 15) / 16) {
+    case 1: launch_small_batched_s<1>(p, stream); break;
+    case 2: launch_small_batched_s<2>(p, stream); break;
+    case 3: launch_small_batched_s<3>(p, stream); break;
+    default: launch_small_batched_s<4>(p, stream); break;
+  }
+}
+
 int g_thin_k_never = 0;  // 1: the tiled kernels take the shapes thin_k_shape() and thin_n_shape() admit
 
 }  // namespace
@@ -1082,6 +1208,10 @@ void gemm(const GemmDesc& d, GemmPrecision prec, const GemmWorkspace& ws, hipStr
     launch_thin_n(d, stream);
     return;
   }
+  if (!g_thin_k_never && prec == GemmPrecision::F64 && small_batched_shape(d)) {
+    launch_small_batched(d, stream);
+    return;
+  }
   if (wide_tiles(d, ws.bytes))
     launch<128, 128>(d, prec, ws, stream);
   else
@@ -1092,7 +1222,9 @@ size_t gemm_scratch_bytes(const GemmDesc& d, GemmPrecision prec, size_t ws_bytes
   if (d.M <= 0 || d.N <= 0 || d.batch <= 0 || d.K <= 0) return 0;
   if (prec == GemmPrecision::BF16X3 || prec == GemmPrecision::F16X3 || prec == GemmPrecision::BF16X6)
     return gemm_split_scratch_bytes(d, prec, ws_bytes);
-  if (!g_thin_k_never && prec == GemmPrecision::F64 && (thin_k_shape(d) || thin_n_shape(d))) return 0;
+  if (!g_thin_k_never && prec == GemmPrecision::F64 &&
+      (thin_k_shape(d) || thin_n_shape(d) || small_batched_shape(d)))
+    return 0;
   const bool wide = wide_tiles(d, ws_bytes);
   return plan_splitk(d, wide ? 128 : 64, wide ? 128 : 64, ws_bytes).scratch_bytes;
 }

@@ -117,6 +117,24 @@ def test_gemm_thin_n(m, n, k, pad, beta):
     assert np.max(np.abs(got - ref)) <= 1e-14 * k * 16 + 1e-13
 
 
+@pytest.mark.parametrize("ta", [0, 1])
+@pytest.mark.parametrize("m,n,k,batch", [(40, 40, 40, 600), (17, 33, 9, 300), (64, 64, 64, 256), (5, 3, 2, 1000)])
+def test_gemm_small_batched_shared_a(ta, m, n, k, batch):
+    """Many small products against one A (stride 0): one wave per product, A's
+    fragments held for the whole launch (ragged M, N, K)."""
+    from mpqc_amd.ccsd import gemm
+    rng = np.random.default_rng(m * n + k + batch)
+    A = rng.standard_normal((k, m) if ta else (m, k))
+    B = rng.standard_normal((batch, k, n))
+    C = rng.standard_normal((batch, m, n))
+    ref = -1.0 * np.einsum("mk,bkn->bmn", A.T if ta else A, B) + 0.5 * C
+    dA, dB, dC = _dev(A), _dev(B), _dev(C)
+    gemm(ta, 0, m, n, k, -1.0, dA.data_ptr(), A.shape[1], dB.data_ptr(), n, 0.5, dC.data_ptr(), n,
+         stride_a=0, stride_b=k * n, stride_c=m * n, batch=batch)
+    torch.cuda.synchronize()
+    assert np.max(np.abs(dC.cpu().numpy() - ref)) < 1e-12
+
+
 def test_gemm_thin_k_batched():
     from mpqc_amd.ccsd import gemm
     rng = np.random.default_rng(11)
