@@ -1185,6 +1185,72 @@ void launch_small_batched(const GemmDesc& d, hipStream_t stream) {
   }
 }
 
+// ---------------------------------------------------------------- GEMV
+// y = alpha A x + beta y (N = 1, A row-major M x K with long rows): the r1 terms
+// Gbar(k,c,l,d) t1(d,l), (2 Gijab - Giajb) t1 and (2 t2 - t2) h (ccsd_r1_r2.h:100-108)
+// and the g_ac / g_ki terms (ccsd_intermediates.h:105-148).  One wave per row (rows
+// with stride: waves in the grid); lanes read 16 bytes of A and of x per step, eight
+// steps in flight, then a shuffle reduction.  HBM-bound: M K 8 bytes per launch.
+struct GemvParams {
+  const double* A;
+  const double* x;
+  double* y;
+  long lda, incx, incy;
+  int M, K;
+  double alpha, beta;
+};
+
+constexpr int kGemvThreads = 256;
+
+template <bool VEC>
+__global__ __launch_bounds__(kGemvThreads) void dgemv_kernel(GemvParams p) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const long nw = (long)gridDim.x * (kGemvThreads / 64);
+  for (long m = (long)blockIdx.x * (kGemvThreads / 64) + wave; m < p.M; m += nw) {
+    const double* __restrict__ a = p.A + m * p.lda;
+    double s0 = 0.0, s1 = 0.0;
+    if (VEC) {  // 16-byte pieces of the row (A 16-byte aligned, lda even, x contiguous and aligned)
+      const int k2 = p.K / 2;  // whole pieces
+#pragma unroll 8
+      for (int q = lane; q < k2; q += 64) {
+        const double2 av = reinterpret_cast<const double2*>(a)[q];
+        const double2 xv = reinterpret_cast<const double2*>(p.x)[q];
+        s0 = fma(av.x, xv.x, s0);
+        s1 = fma(av.y, xv.y, s1);
+      }
+      if ((p.K & 1) && lane == 0) s0 = fma(a[p.K - 1], p.x[p.K - 1], s0);
+    } else {
+#pragma unroll 8
+      for (int k = lane; k < p.K; k += 64) s0 = fma(a[k], p.x[(long)k * p.incx], s0);
+    }
+    double sum = s0 + s1;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 64);
+    if (lane == 0) {
+      double* y = p.y + m * p.incy;
+      *y = p.beta == 0.0 ? p.alpha * sum : fma(p.beta, *y, p.alpha * sum);
+    }
+  }
+}
+
+bool gemv_shape(const GemmDesc& d) {
+  return d.N == 1 && !d.transA && d.batch == 1 && d.K >= 256 && d.M >= 64;
+}
+
+void launch_gemv(const GemmDesc& d, hipStream_t stream) {
+  GemvParams p;
+  p.A = d.A; p.x = d.B; p.y = d.C;
+  p.lda = d.lda; p.incx = d.transB ? 1 : d.ldb; p.incy = d.ldc;
+  p.M = d.M; p.K = d.K; p.alpha = d.alpha; p.beta = d.beta;
+  const bool vec = ((uintptr_t)d.A % 16 == 0) && d.lda % 2 == 0 && p.incx == 1 && ((uintptr_t)d.B % 16 == 0);
+  const int grid = (int)std::min<long>(ceil_div(d.M, kGemvThreads / 64), 4096);
+  if (vec)
+    hipLaunchKernelGGL(dgemv_kernel<true>, dim3((unsigned)grid), dim3(kGemvThreads), 0, stream, p);
+  else
+    hipLaunchKernelGGL(dgemv_kernel<false>, dim3((unsigned)grid), dim3(kGemvThreads), 0, stream, p);
+  check_launch("dgemv");
+}
+
 int g_thin_k_never = 0;  // 1: the tiled kernels take the shapes thin_k_shape() and thin_n_shape() admit
 
 }  // namespace
@@ -1212,6 +1278,10 @@ void gemm(const GemmDesc& d, GemmPrecision prec, const GemmWorkspace& ws, hipStr
     launch_small_batched(d, stream);
     return;
   }
+  if (!g_thin_k_never && prec == GemmPrecision::F64 && gemv_shape(d)) {
+    launch_gemv(d, stream);
+    return;
+  }
   if (wide_tiles(d, ws.bytes))
     launch<128, 128>(d, prec, ws, stream);
   else
@@ -1223,7 +1293,7 @@ size_t gemm_scratch_bytes(const GemmDesc& d, GemmPrecision prec, size_t ws_bytes
   if (prec == GemmPrecision::BF16X3 || prec == GemmPrecision::F16X3 || prec == GemmPrecision::BF16X6)
     return gemm_split_scratch_bytes(d, prec, ws_bytes);
   if (!g_thin_k_never && prec == GemmPrecision::F64 &&
-      (thin_k_shape(d) || thin_n_shape(d) || small_batched_shape(d)))
+      (thin_k_shape(d) || thin_n_shape(d) || small_batched_shape(d) || gemv_shape(d)))
     return 0;
   const bool wide = wide_tiles(d, ws_bytes);
   return plan_splitk(d, wide ? 128 : 64, wide ? 128 : 64, ws_bytes).scratch_bytes;

@@ -135,6 +135,29 @@ def test_gemm_small_batched_shared_a(ta, m, n, k, batch):
     assert np.max(np.abs(dC.cpu().numpy() - ref)) < 1e-12
 
 
+@pytest.mark.parametrize("m,k,tb,incx,incy,beta", [(6080, 6080, 0, 1, 1, 1.0), (300, 1001, 1, 1, 1, 0.0),
+                                                   (257, 4000, 0, 3, 2, -0.5), (64, 256, 0, 1, 1, 0.0)])
+def test_gemm_gemv(m, k, tb, incx, incy, beta):
+    """N = 1 with long rows: one wave per row (16-byte pieces when x is contiguous,
+    strided x and y otherwise, odd K)."""
+    from mpqc_amd.ccsd import gemm
+    rng = np.random.default_rng(m + k)
+    A = rng.standard_normal((m, k))
+    x = rng.standard_normal(k * incx)
+    y = rng.standard_normal(m * incy)
+    if beta == 0.0:
+        y[::incy] = np.nan
+    ref = y.copy()
+    ref[::incy] = 0.8 * (A @ x[::incx]) + (beta * y[::incy] if beta != 0.0 else 0.0)
+    dA, dx, dy = _dev(A), _dev(x), _dev(y)
+    # op(B) = B: x[k] at B[k * ldb]; op(B) = B^T: x[k] at B[k]
+    gemm(0, tb, m, 1, k, 0.8, dA.data_ptr(), k, dx.data_ptr(), k if tb else incx, beta, dy.data_ptr(), incy)
+    torch.cuda.synchronize()
+    got = dy.cpu().numpy()
+    assert np.all(np.isfinite(got[::incy]))
+    assert np.max(np.abs(got - ref)) < 1e-10
+
+
 def test_gemm_thin_k_batched():
     from mpqc_amd.ccsd import gemm
     rng = np.random.default_rng(11)
