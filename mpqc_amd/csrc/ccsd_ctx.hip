@@ -1074,10 +1074,14 @@ void CCSDContext::residual_ops() {
   //          r1 += h_kc (2 t2(c,a,k,i) - t2(c,a,i,k) + t1(c,i) t1(a,k))
   copy_d2d(hkc_, Fia_, ov);
   gemm(false, false, o * v, 1, o * v, 1.0, Gbar_kcld_, ov, t1T_, 1, 1.0, hkc_, 1, 0, 0, 0, 1, "r1:F_ov");
-  perm_add(hT_, {1, 1, V, O}, 0.0, {{hkc_, {1, 1, O, V}, {0, 1, 3, 2}, 1.0}});  // hT(c,k) = h(k,c)
-  perm_add(Y5_, {V, O, V, O}, 0.0,
-           {{t2_, e_abij, {1, 3, 0, 2}, 2.0}, {t2_, e_abij, {1, 2, 0, 3}, -1.0}});
-  gemm(false, false, v * o, 1, v * o, 1.0, Y5_, ov, hT_, 1, 1.0, r1_, 1, 0, 0, 0, 1, "r1:hkc_t2bar");
+  if (!tape_ && t2bar_h_supported(o, v)) {  // one pass over t2 (the tape keeps the contraction)
+    timed("r1:hkc_t2bar", 2.0 * ov * ov, [&] { t2bar_h(r1_, t2_, hkc_, o, v, partials_, stream_); });
+  } else {
+    perm_add(hT_, {1, 1, V, O}, 0.0, {{hkc_, {1, 1, O, V}, {0, 1, 3, 2}, 1.0}});  // hT(c,k) = h(k,c)
+    perm_add(Y5_, {V, O, V, O}, 0.0,
+             {{t2_, e_abij, {1, 3, 0, 2}, 2.0}, {t2_, e_abij, {1, 2, 0, 3}, -1.0}});
+    gemm(false, false, v * o, 1, v * o, 1.0, Y5_, ov, hT_, 1, 1.0, r1_, 1, 0, 0, 0, 1, "r1:hkc_t2bar");
+  }
   gemm(false, false, o, o, v, 1.0, hkc_, v, t1_, o, 0.0, Zoo_, o, 0, 0, 0, 1, "r1:hkc_t1");
   gemm(false, false, v, o, o, 1.0, t1_, o, Zoo_, o, 1.0, r1_, o, 0, 0, 0, 1, "r1:t1_hkc_t1");
   // :107-108 r1 += (2 Gijab(k,i,c,a) - Giajb(k,a,i,c)) t1(c,k)

@@ -499,6 +499,79 @@ void dot(const double* x, const double* y, long n, double* out, double* partials
   check_launch("dot");
 }
 
+// r1(a,i) += sum_{c,k} (2 t2(c,a,k,i) - t2(c,a,i,k)) h(k,c) in one pass over t2:
+// workgroup (a, range of c) runs over the o x o blocks t2(c,a,:,:): each block is read
+// whole (coalesced) into LDS (rows padded by one double), then thread (i, kq) sums
+// over k = kq (mod Q) from LDS; its sums over kq go through LDS in a fixed order into
+// part[(chunk v + a) o + i], and a second kernel adds the chunks in order.  The next
+// block's loads are issued before the current block's sums.
+constexpr int kT2barMaxO = 64;  // the LDS tile: o x (o + 1) doubles
+__global__ __launch_bounds__(kBlock) void t2bar_h_stage1(double* __restrict__ part, const double* __restrict__ t2,
+                                                         const double* __restrict__ h, int o, int v, int cchunk) {
+  __shared__ double tile[kT2barMaxO * (kT2barMaxO + 1)];
+  __shared__ double s[kBlock];
+  constexpr int kPer = kT2barMaxO * kT2barMaxO / kBlock;  // elements of a block per thread, at most
+  const int a = blockIdx.x, chunk = blockIdx.y, t = threadIdx.x;
+  const int Q = kBlock / o, i = t % o, kq = t / o;
+  const int oo = o * o;
+  const int c0 = chunk * cchunk, c1 = min(v, (chunk + 1) * cchunk);
+  double reg[kPer];
+  auto load = [&](int c) {
+    const double* __restrict__ T = t2 + ((long)c * v + a) * oo;  // t2(c,a,:,:)
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      const int e = t + q * kBlock;
+      reg[q] = (c < c1 && e < oo) ? T[e] : 0.0;
+    }
+  };
+  double acc = 0.0;
+  load(c0);
+  for (int c = c0; c < c1; ++c) {
+    __syncthreads();  // the previous block's sums are done with the tile
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      const int e = t + q * kBlock;
+      if (e < oo) tile[(e / o) * (o + 1) + e % o] = reg[q];
+    }
+    __syncthreads();
+    load(c + 1);
+    if (kq < Q)
+      for (int k = kq; k < o; k += Q)
+        acc = fma(2.0 * tile[k * (o + 1) + i] - tile[i * (o + 1) + k], h[(long)k * v + c], acc);
+  }
+  s[t] = kq < Q ? acc : 0.0;
+  __syncthreads();
+  if (t < o) {
+    double sum = 0.0;
+    for (int q = 0; q < Q; ++q) sum += s[q * o + t];
+    part[((long)chunk * v + a) * o + t] = sum;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void t2bar_h_stage2(double* __restrict__ r1, const double* __restrict__ part,
+                                                         long ov, int nchunks) {
+  const long e = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (e >= ov) return;
+  double sum = 0.0;
+  for (int q = 0; q < nchunks; ++q) sum += part[(long)q * ov + e];
+  r1[e] += sum;
+}
+
+bool t2bar_h_supported(int o, int v) { return o >= 1 && o <= kT2barMaxO && (long)o * v <= kReducePartials; }
+
+void t2bar_h(double* r1, const double* t2, const double* h, int o, int v, double* partials, hipStream_t s) {
+  if (!t2bar_h_supported(o, v)) throw std::invalid_argument("t2bar_h: o must be <= 64 and o v <= partials");
+  const long ov = (long)o * v;
+  const int nchunks = (int)std::max<long>(1, std::min<long>({(long)v, kReducePartials / ov, 16}));
+  const int cchunk = (int)ceil_div(v, nchunks);
+  const int used = (int)ceil_div(v, cchunk);
+  hipLaunchKernelGGL(t2bar_h_stage1, dim3((unsigned)v, (unsigned)used), dim3(kBlock), 0, s, partials, t2, h, o, v,
+                     cchunk);
+  hipLaunchKernelGGL(t2bar_h_stage2, dim3((unsigned)ceil_div(ov, kBlock)), dim3(kBlock), 0, s, r1, partials, ov,
+                     used);
+  check_launch("t2bar_h");
+}
+
 void ccsd_energy(const double* f_ai, const double* r1, const double* t1, const double* g_abij,
                  const double* r2, const double* tau, int o, int v, double* out, double* partials,
                  hipStream_t s) {

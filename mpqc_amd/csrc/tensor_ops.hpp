@@ -59,6 +59,11 @@ void jacobi_update(double* t1, double* t2, const double* r1, const double* r2, c
 // sum_k x[k]*y[k] over n elements, written to out[0].
 void dot(const double* x, const double* y, long n, double* out, double* partials, hipStream_t s);
 
+// r1(a,i) += sum_{c,k} (2 t2(c,a,k,i) - t2(c,a,i,k)) h(k,c) (ccsd_r1_r2.h:100-101, h as
+// (k,c) with row length v), one pass over t2; needs o <= 64 and o v <= kReducePartials.
+bool t2bar_h_supported(int o, int v);
+void t2bar_h(double* r1, const double* t2, const double* h, int o, int v, double* partials, hipStream_t s);
+
 // Energy (ccsd.h:377-379):
 //   2 <f_ai + r1, t1> + <g_abij + r2, 2 tau(a,b,i,j) - tau(b,a,i,j)>
 void ccsd_energy(const double* f_ai, const double* r1, const double* t1, const double* g_abij,
