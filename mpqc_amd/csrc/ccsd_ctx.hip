@@ -503,6 +503,7 @@ void CCSDContext::set_amplitudes(const double* t1, const double* t2, int layout,
   upload(t2_, t2, {kVir, kVir, kOcc, kOcc}, layout, memspace);
   build_tau(tau_, t2_, t1_, o_, v_, stream_);
   tau_fresh_ = true;
+  tau_amax_fresh_ = false;
   MPQC_HIP_CHECK(hipStreamSynchronize(stream_));
 }
 
@@ -941,6 +942,7 @@ void CCSDContext::init_guess() {
   MPQC_HIP_CHECK(hipMemcpyAsync(r2_, g_abij_, oovv * sizeof(double), hipMemcpyDeviceToDevice, stream_));
   MPQC_HIP_CHECK(hipStreamSynchronize(stream_));
   tau_fresh_ = true;
+  tau_amax_fresh_ = false;
 }
 
 // ---------------------------------------------------------------- residual
@@ -990,7 +992,10 @@ void CCSDContext::residual() {
   if (!prepared_) prepare();
   // tau("a,b,i,j") = t2("a,b,i,j") + t1("a,i") t1("b,j") (ccsd.h:389): the update and
   // set_amplitudes leave it current; rebuilt here otherwise (and always on the tape)
-  if (!tau_fresh_ || tape_) op_tau();
+  if (!tau_fresh_ || tape_) {
+    op_tau();
+    tau_amax_fresh_ = false;
+  }
   if (!tape_ || tape_execute_) tau_fresh_ = true;  // (a tape recorded without executing runs nothing)
   if (graph_eligible()) {
     if (res_exec_ && res_graph_gen_ == gen_) {
@@ -1033,6 +1038,7 @@ void CCSDContext::residual_ops() {
     amax_.ptr[2] = tau_;
     amax_.ptr[3] = J_;  // final when the ring operands are split (max |K| then serves two splits)
     amax_.ptr[4] = K_;
+    if (tau_amax_fresh_) amax_.valid[1] = amax_.valid[2] = true;  // max |t2|, |tau| from the update
     split(tau_P_, e_abij, {{tau_, e_abij, I, 1.0}});
     // tau^T(i,j,c,d) = tau(c,d,i,j): the B^T operand of every tau contraction
     split(tauT_P_, {O, O, V, V}, {{tau_, e_abij, {2, 3, 0, 1}, 1.0}});
@@ -1477,7 +1483,13 @@ void CCSDContext::update() {
     });
   }
   // tau("a,b,i,j") = t2 + t1 t1                                     ccsd.h:389
-  timed("up:tau", 0.0, [&] { build_tau(tau_, t2_, t1_, o_, v_, stream_); });
+  if (prec_ == GemmPrecision::F16X3 && amax_.dev) {  // with the maxima the fp16 splits need
+    timed("up:tau", 0.0, [&] { build_tau_absmax(tau_, t2_, t1_, o_, v_, partials_, amax_.dev + 1, stream_); });
+    tau_amax_fresh_ = true;
+  } else {
+    timed("up:tau", 0.0, [&] { build_tau(tau_, t2_, t1_, o_, v_, stream_); });
+    tau_amax_fresh_ = false;
+  }
   tau_fresh_ = true;
   MPQC_HIP_CHECK(hipStreamSynchronize(stream_));
 }
@@ -1493,6 +1505,7 @@ double CCSDContext::solve(int max_iter, double target_precision, int verbose, in
       perm_add(Rd_, {V, V, O, O}, 0.0, {{t2_, {V, V, O, O}, {0, 1, 2, 3}, 0.5}, {t2_, {V, V, O, O}, {1, 0, 3, 2}, 0.5}});
       copy_d2d(t2_, Rd_, V * V * O * O);
       tau_fresh_ = false;
+      tau_amax_fresh_ = false;
     }
     residual();  // the residual of the current amplitudes in the current form
   }

@@ -348,6 +348,7 @@ class CCSDContext {
   // the method change (gen_ counts those changes)
   int graph_mode_ = 1;
   bool tau_fresh_ = false;  // tau_ holds t2 + t1 t1 of the current amplitudes
+  bool tau_amax_fresh_ = false;  // amax_.dev[1], [2] hold max |t2|, |tau| of the current amplitudes
   long gen_ = 0, res_seen_gen_ = -1, res_graph_gen_ = -1, res_failed_gen_ = -1;
   hipGraph_t res_graph_ = nullptr;
   hipGraphExec_t res_exec_ = nullptr;

@@ -37,6 +37,64 @@ __global__ __launch_bounds__(kBlock) void build_tau_kernel(double* __restrict__ 
   }
 }
 
+// tau as above, and per block max |t2| and max |tau| -> part[2 blk], part[2 blk + 1]
+__global__ __launch_bounds__(kBlock) void build_tau_absmax_kernel(double* __restrict__ tau,
+                                                                  const double* __restrict__ t2,
+                                                                  const double* __restrict__ t1, int o, int v,
+                                                                  double* __restrict__ part) {
+  __shared__ double s2[kBlock], st[kBlock];
+  const int oo = o * o, n = v * v * oo;
+  double m2 = 0.0, mt = 0.0;
+  for (int e = blockIdx.x * kBlock + threadIdx.x; e < n; e += gridDim.x * kBlock) {
+    const int ab = e / oo, ij = e - ab * oo;
+    const int a = ab / v, b = ab - a * v;
+    const int i = ij / o, j = ij - i * o;
+    const double x = t2[e];
+    const double y = x + t1[a * o + i] * t1[b * o + j];
+    tau[e] = y;
+    m2 = fmax(m2, fabs(x));
+    mt = fmax(mt, fabs(y));
+  }
+  s2[threadIdx.x] = m2;
+  st[threadIdx.x] = mt;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      s2[threadIdx.x] = fmax(s2[threadIdx.x], s2[threadIdx.x + w]);
+      st[threadIdx.x] = fmax(st[threadIdx.x], st[threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = s2[0];
+    part[2 * blockIdx.x + 1] = st[0];
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void absmax2_stage2(const double* __restrict__ part, int nb,
+                                                         double* __restrict__ out) {
+  __shared__ double s2[kBlock], st[kBlock];
+  double m2 = 0.0, mt = 0.0;
+  for (int b = threadIdx.x; b < nb; b += kBlock) {
+    m2 = fmax(m2, part[2 * b]);
+    mt = fmax(mt, part[2 * b + 1]);
+  }
+  s2[threadIdx.x] = m2;
+  st[threadIdx.x] = mt;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      s2[threadIdx.x] = fmax(s2[threadIdx.x], s2[threadIdx.x + w]);
+      st[threadIdx.x] = fmax(st[threadIdx.x], st[threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = s2[0];
+    out[1] = st[0];
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void build_TP_kernel(double* __restrict__ TP,
                                                           const double* __restrict__ t2,
                                                           const double* __restrict__ t1, int o,
@@ -475,6 +533,15 @@ void build_tau(double* tau, const double* t2, const double* t1, int o, int v, hi
   hipLaunchKernelGGL(build_tau_kernel, dim3(elem_grid((long)v * v * o * o)), dim3(kBlock), 0, s, tau,
                      t2, t1, o, v);
   check_launch("build_tau");
+}
+
+void build_tau_absmax(double* tau, const double* t2, const double* t1, int o, int v, double* partials,
+                      double* maxes, hipStream_t s) {
+  check_small(o, v);
+  const int nb = (int)std::min<long>(elem_grid((long)v * v * o * o), kReducePartials / 2);
+  hipLaunchKernelGGL(build_tau_absmax_kernel, dim3(nb), dim3(kBlock), 0, s, tau, t2, t1, o, v, partials);
+  hipLaunchKernelGGL(absmax2_stage2, dim3(1), dim3(kBlock), 0, s, partials, nb, maxes);
+  check_launch("build_tau_absmax");
 }
 
 void build_TP(double* TP, const double* t2, const double* t1, int o, int v, hipStream_t s) {

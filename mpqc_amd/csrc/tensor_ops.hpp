@@ -45,6 +45,10 @@ void permuted_strides(const long in_extents[4], const int perm[4], long out_s[4]
 
 // tau(a,b,i,j) = t2(a,b,i,j) + t1(a,i) t1(b,j)                 (ccsd.h:389)
 void build_tau(double* tau, const double* t2, const double* t1, int o, int v, hipStream_t s);
+// build_tau, and maxes[0] = max |t2|, maxes[1] = max |tau| (the fp16 plane scales of the
+// next residual's t2 and tau operands)
+void build_tau_absmax(double* tau, const double* t2, const double* t1, int o, int v, double* partials,
+                      double* maxes, hipStream_t s);
 
 // TP(a,i,l,d) = 0.5 t2(d,a,i,l) + t1(d,i) t1(a,l)  -- the "T" intermediate of
 // ccsd_r1_r2.h:236 laid out as the (ai),(ld) GEMM operand.
