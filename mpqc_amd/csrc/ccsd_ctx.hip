@@ -1406,17 +1406,19 @@ void CCSDContext::update() {
     for (int s : err_slots_) used[s] = true;
     while (used[eslot]) ++eslot;
   }
-  timed("up:store_r", 0.0, [&] {
-    MPQC_HIP_CHECK(hipMemcpyAsync(hist_e1_[eslot], r1_, n1 * sizeof(double), hipMemcpyDeviceToDevice, stream_));
-    MPQC_HIP_CHECK(hipMemcpyAsync(hist_e2_[eslot], r2_, n2 * sizeof(double), hipMemcpyDeviceToDevice, stream_));
-  });
   err_slots_.push_back(eslot);
   const int nvec = (int)err_slots_.size();
   {
+    // the new error vector r goes into its slot in the pass that forms its overlaps with
+    // the history (its own overlap read from r itself)
     std::vector<const double*> x1, x2;
-    for (int s : err_slots_) { x1.push_back(hist_e1_[s]); x2.push_back(hist_e2_[s]); }
+    for (int s : err_slots_) {
+      x1.push_back(s == eslot ? r1_ : hist_e1_[s]);
+      x2.push_back(s == eslot ? r2_ : hist_e2_[s]);
+    }
     timed("up:diis_dots", 0.0, [&] {
-      multi_dot2(x1.data(), x2.data(), nvec, r1_, r2_, n1, n2, scalars_ + 16, partials_, stream_);
+      multi_dot2(x1.data(), x2.data(), nvec, r1_, r2_, n1, n2, scalars_ + 16, partials_, stream_, hist_e1_[eslot],
+                 hist_e2_[eslot]);
     });
     double h[16];
     MPQC_HIP_CHECK(hipMemcpyAsync(h, scalars_ + 16, nvec * sizeof(double), hipMemcpyDeviceToHost, stream_));
@@ -1424,9 +1426,7 @@ void CCSDContext::update() {
     const int m = n_diis_;
     for (int i = 0; i < nvec; ++i) B_[(size_t)i * m + (nvec - 1)] = B_[(size_t)(nvec - 1) * m + i] = h[i];
   }
-  // --- Jacobi: t += r / D
-  timed("up:jacobi", 0.0, [&] { jacobi_update(t1_, t2_, r1_, r2_, eo_, ev_, o_, v_, stream_); });
-  // --- push x = t into its history
+  // --- Jacobi: t += r / D, the new t also written into its history slot (x = t)
   if ((int)x_slots_.size() == n_diis_) x_slots_.pop_front();
   int xslot = 0;
   {
@@ -1434,9 +1434,8 @@ void CCSDContext::update() {
     for (int s : x_slots_) used[s] = true;
     while (used[xslot]) ++xslot;
   }
-  timed("up:store_t", 0.0, [&] {
-    MPQC_HIP_CHECK(hipMemcpyAsync(hist_x1_[xslot], t1_, n1 * sizeof(double), hipMemcpyDeviceToDevice, stream_));
-    MPQC_HIP_CHECK(hipMemcpyAsync(hist_x2_[xslot], t2_, n2 * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+  timed("up:jacobi", 0.0, [&] {
+    jacobi_update(t1_, t2_, r1_, r2_, eo_, ev_, o_, v_, stream_, hist_x1_[xslot], hist_x2_[xslot]);
   });
   x_slots_.push_back(xslot);
   const bool do_extrap =

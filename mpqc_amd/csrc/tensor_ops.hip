@@ -114,17 +114,22 @@ __global__ __launch_bounds__(kBlock) void jacobi_kernel(double* __restrict__ t1,
                                                         const double* __restrict__ r1,
                                                         const double* __restrict__ r2,
                                                         const double* __restrict__ eo,
-                                                        const double* __restrict__ ev, int o, int v) {
+                                                        const double* __restrict__ ev, int o, int v,
+                                                        double* __restrict__ c1, double* __restrict__ c2) {
   const int oo = o * o, n = v * v * oo;
   for (int e = blockIdx.x * kBlock + threadIdx.x; e < n; e += gridDim.x * kBlock) {
     const int ab = e / oo, ij = e - ab * oo;
     const int a = ab / v, b = ab - a * v;
     const int i = ij / o, j = ij - i * o;
-    t2[e] += r2[e] / (eo[i] + eo[j] - ev[a] - ev[b]);
+    const double x = t2[e] + r2[e] / (eo[i] + eo[j] - ev[a] - ev[b]);
+    t2[e] = x;
+    if (c2) c2[e] = x;
   }
   for (int e = blockIdx.x * kBlock + threadIdx.x; e < v * o; e += gridDim.x * kBlock) {
     const int a = e / o, i = e - a * o;
-    t1[e] += r1[e] / (eo[i] - ev[a]);
+    const double x = t1[e] + r1[e] / (eo[i] - ev[a]);
+    t1[e] = x;
+    if (c1) c1[e] = x;
   }
 }
 
@@ -191,6 +196,8 @@ struct MultiDotArgs {
   const double* x2[16];
   const double* y1;
   const double* y2;
+  double* c1;  // copies of y (optional)
+  double* c2;
   long n1, n2;
   int nvec;
 };
@@ -202,12 +209,14 @@ __global__ __launch_bounds__(kBlock) void multidot_stage1(MultiDotArgs a, double
   for (int k = 0; k < 16; ++k) acc[k] = 0.0;
   for (long e = (long)blockIdx.x * kBlock + threadIdx.x; e < a.n2; e += (long)gridDim.x * kBlock) {
     const double y = a.y2[e];
+    if (a.c2) a.c2[e] = y;
 #pragma unroll
     for (int k = 0; k < 16; ++k)
       if (k < a.nvec) acc[k] += a.x2[k][e] * y;
   }
   for (long e = (long)blockIdx.x * kBlock + threadIdx.x; e < a.n1; e += (long)gridDim.x * kBlock) {
     const double y = a.y1[e];
+    if (a.c1) a.c1[e] = y;
 #pragma unroll
     for (int k = 0; k < 16; ++k)
       if (k < a.nvec) acc[k] += a.x1[k][e] * y;
@@ -552,10 +561,10 @@ void build_TP(double* TP, const double* t2, const double* t1, int o, int v, hipS
 }
 
 void jacobi_update(double* t1, double* t2, const double* r1, const double* r2, const double* eps_o,
-                   const double* eps_v, int o, int v, hipStream_t s) {
+                   const double* eps_v, int o, int v, hipStream_t s, double* copy1, double* copy2) {
   check_small(o, v);
   hipLaunchKernelGGL(jacobi_kernel, dim3(elem_grid((long)v * v * o * o)), dim3(kBlock), 0, s, t1, t2,
-                     r1, r2, eps_o, eps_v, o, v);
+                     r1, r2, eps_o, eps_v, o, v, copy1, copy2);
   check_launch("jacobi");
 }
 
@@ -651,7 +660,8 @@ void ccsd_energy(const double* f_ai, const double* r1, const double* t1, const d
 }
 
 void multi_dot2(const double* const* xs1, const double* const* xs2, int nvec, const double* y1,
-                const double* y2, long n1, long n2, double* out, double* partials, hipStream_t s) {
+                const double* y2, long n1, long n2, double* out, double* partials, hipStream_t s,
+                double* copy1, double* copy2) {
   if (nvec > 16) throw std::invalid_argument("multi_dot2: at most 16 vectors");
   MultiDotArgs a;
   for (int k = 0; k < 16; ++k) {
@@ -659,6 +669,7 @@ void multi_dot2(const double* const* xs1, const double* const* xs2, int nvec, co
     a.x2[k] = k < nvec ? xs2[k] : nullptr;
   }
   a.y1 = y1; a.y2 = y2; a.n1 = n1; a.n2 = n2; a.nvec = nvec;
+  a.c1 = copy1; a.c2 = copy2;
   const int g = red_grid(std::max(n1, n2));
   hipLaunchKernelGGL(multidot_stage1, dim3(g), dim3(kBlock), 0, s, a, partials);
   hipLaunchKernelGGL(reduce_stage2, dim3(nvec), dim3(kBlock), 0, s, partials, g, out, 1.0);

@@ -56,8 +56,10 @@ void build_TP(double* TP, const double* t2, const double* t1, int o, int v, hipS
 
 // Jacobi update (lcao/cc/solvers.h:165-258):
 //   t1 += r1 / (e_i - e_a);  t2 += r2 / (e_i + e_j - e_a - e_b)
+// (copy1, copy2: optional second destinations of the updated t1, t2)
 void jacobi_update(double* t1, double* t2, const double* r1, const double* r2, const double* eps_o,
-                   const double* eps_v, int o, int v, hipStream_t s);
+                   const double* eps_v, int o, int v, hipStream_t s, double* copy1 = nullptr,
+                   double* copy2 = nullptr);
 
 // Deterministic reductions; results land in device memory `out` (double).
 // sum_k x[k]*y[k] over n elements, written to out[0].
@@ -75,8 +77,10 @@ void ccsd_energy(const double* f_ai, const double* r1, const double* t1, const d
                  hipStream_t s);
 
 // out[k] = <x_k, y> for k < nvec, x_k = X + k*stride (two members: 1- and 2-body)
+// (copy1, copy2: optional destinations that receive y1, y2 in the same pass)
 void multi_dot2(const double* const* xs1, const double* const* xs2, int nvec, const double* y1,
-                const double* y2, long n1, long n2, double* out, double* partials, hipStream_t s);
+                const double* y2, long n1, long n2, double* out, double* partials, hipStream_t s,
+                double* copy1 = nullptr, double* copy2 = nullptr);
 
 // C_b[m][n] *= beta for b < batch, m < M, n < N (C_b = C + b*sC, row stride ldc);
 // beta = 0 stores zeros
