@@ -170,7 +170,6 @@ void CCSDContext::construct(const int64_t* occ_hm, int n_occ_tiles, const int64_
   TP_ = alloc(oovv); t2P_ = alloc(oovv); J_ = alloc(oovv); K_ = alloc(oovv); Zj_ = alloc(oovv);
   Zk_ = alloc(oovv); Xjk_ = alloc(oovv); Y1_ = alloc(oovv); Y2_ = alloc(oovv); RR_ = alloc(oovv);
   tmpR_ = alloc(oovv); Y23_ = alloc(oooo); W_ = alloc(oooo); tmp24_ = alloc(ooov);
-  Y24_ = alloc(oovv);
   // split-K workspace: enough for 64 slices of the largest small-output GEMM, and
   // for 16 slices of an (ov) x (ov) or v^2 x o^2 output up to 512 MiB (small
   // systems, where those outputs are too few tiles to fill the chip)
@@ -1273,6 +1272,18 @@ void CCSDContext::residual_ops() {
     perm_add(Y2_, {O, V, V, O}, 0.0, {{t2_, e_abij, {3, 0, 1, 2}, 1.0}});
     gemm(false, false, o * v, o * v, o * v, 1.0, K_, ov, Y2_, ov, 0.0, tmpR_, ov, 0, 0, 0, 1, "r2:crossed_ring");
   }
+  // :277-280 tmp(k,a,i,j) = Giabc(k,a,c,d) tau(c,d,i,j); b -= tmp(k,a,j,i) t1(b,k) + tmp(k,b,i,j) t1(a,k).
+  // The two terms are mirror images under (a,i) <-> (b,j), and Rd enters r2 as
+  // Rd(a,b,i,j) + Rd(b,a,j,i): Rd takes the second (the direct and reduced-memory df
+  // ladders form these terms themselves)
+  if (!direct_ && !(df_ && reduced_abcd_)) {
+    if (sm)
+      gemmP(o * v, o * o, v * v, 1.0, Giabc_P_, vv, tauT_P_, vv, 0.0, tmp24_, oo, "r2:pp_t1_giabc_tau");
+    else
+      gemm(false, false, o * v, o * o, v * v, 1.0, Giabc_, vv, tau_, oo, 0.0, tmp24_, oo, 0, 0, 0, 1,
+           "r2:pp_t1_giabc_tau");
+    gemm(false, false, v, v * o * o, o, -1.0, t1_, o, tmp24_, V * oo, 1.0, Rd_, V * oo, 0, 0, 0, 1, "r2:pp_t1_a");
+  }
   // :260 r2(a,b,i,j) = r2(a,b,i,j) + r2(b,a,j,i); :262 r2 += Gijab(i,j,a,b).  The ring
   // terms come in pairs under (a,i) <-> (b,j), so in two passes: the half-sum
   // H = Rd + RR(a,i,b,j) - 0.5 tmpR(a,i,b,j) - tmpR(b,i,a,j) (in Rd), then
@@ -1322,8 +1333,6 @@ void CCSDContext::residual_ops() {
       pp_ladder_symmetric(r2_);
     else
       gemmP(v * v, o * o, v * v, 1.0, Gabcd_P_, vv, tauT_P_, vv, 1.0, r2_, oo, "r2:pp_ladder");
-    // :277 tmp(k,a,i,j) = Giabc(k,a,c,d) tau(c,d,i,j)
-    gemmP(o * v, o * o, v * v, 1.0, Giabc_P_, vv, tauT_P_, vv, 0.0, tmp24_, oo, "r2:pp_t1_giabc_tau");
   } else {
     if (psym && w_sym_ok_)
       w_oooo_symmetric();
@@ -1344,13 +1353,8 @@ void CCSDContext::residual_ops() {
       pp_ladder_symmetric(r2_);
     else
       gemm(false, false, v * v, o * o, v * v, 1.0, Gabcd_, vv, tau_, oo, 1.0, r2_, oo, 0, 0, 0, 1, "r2:pp_ladder");
-    // :277 tmp(k,a,i,j) = Giabc(k,a,c,d) tau(c,d,i,j)
-    gemm(false, false, o * v, o * o, v * v, 1.0, Giabc_, vv, tau_, oo, 0.0, tmp24_, oo, 0, 0, 0, 1, "r2:pp_t1_giabc_tau");
   }
-  // :278-280 b -= tmp(k,a,j,i) t1(b,k) + tmp(k,b,i,j) t1(a,k)
-  gemm(true, true, v * o * o, v, o, 1.0, tmp24_, V * oo, t1_, o, 0.0, Y24_, v, 0, 0, 0, 1, "r2:pp_t1_b");  // Y24(a,j,i,b)
-  perm_add(r2_, {V, V, O, O}, 1.0, {{Y24_, {V, O, O, V}, {0, 3, 2, 1}, -1.0}});
-  gemm(false, false, v, v * o * o, o, -1.0, t1_, o, tmp24_, V * oo, 1.0, r2_, V * oo, 0, 0, 0, 1, "r2:pp_t1_a");
+  // (the ladder's T1 terms, :277-280, went into Rd above)
 }
 
 // ---------------------------------------------------------------- energy / error / update

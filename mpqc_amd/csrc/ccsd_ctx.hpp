@@ -322,7 +322,7 @@ class CCSDContext {
   // per-iteration scratch
   double *Xoo_, *Zoo_, *hkc_, *hT_, *t1T_, *tau_akld_, *tau_ilab_, *Y5_, *tau_kcdi_, *tau_ackl_,
       *Rd_, *Q_, *M12_, *S_, *gki_, *gac_, *t2_abjk_, *X16_, *TP_, *t2P_, *J_, *K_, *Zj_, *Zk_,
-      *Xjk_, *Y1_, *Y2_, *RR_, *tmpR_, *Y23_, *W_, *tmp24_, *Y24_;
+      *Xjk_, *Y1_, *Y2_, *RR_, *tmpR_, *Y23_, *W_, *tmp24_;
   GemmWorkspace ws_;
   double* partials_ = nullptr;
   double* scalars_ = nullptr;  // device scalars
