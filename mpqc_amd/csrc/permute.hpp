@@ -16,6 +16,8 @@
 //    grid.y / grid.z.
 #pragma once
 
+#include <cstdlib>
+
 #include "tensor_ops.hpp"
 
 namespace mpqc_hip {
@@ -58,11 +60,26 @@ inline size_t perm_lds_bytes(const PermGeom& g) {
   return g.tiled ? (size_t)g.t3 * (size_t)(g.tq + 1) * sizeof(double) : 0;
 }
 
+// Whether the tiled geometry suits SHAPE 3: no product term, exactly one input through
+// the LDS tile (contiguous along q), every other input contiguous along dim 3, and
+// tile-local offsets within 32 bits.
+inline bool perm_fast_tile(const PermGeom& g) {
+  if (!g.tiled || g.has_prod || __builtin_popcount(g.lds_mask) != 1) return false;
+  for (int t = 0; t < g.nterms; ++t) {
+    const PTerm& T = g.t[t];
+    const bool lds = (g.lds_mask >> t) & 1u;
+    if (lds ? T.sq != 1 : T.s3 != 1) return false;
+    if (std::abs(T.s3) * g.t3 >= (1L << 30) || std::abs(T.sq) * g.tq >= (1L << 30)) return false;
+  }
+  return std::abs(g.oq) * g.tq < (1L << 30);
+}
+
 // Device: evaluates every output element of block (bx, by, bz) of the launch
 // grid and hands (linear output offset, sum) to store().  Called by every
 // thread of the workgroup (the tiled shape synchronises through LDS).  SHAPE
 // selects the code compiled in: 0 linear only, 1 tiled only, 2 both (decided by
-// g.tiled); kernels launched for one shape keep the linear shape's registers low.
+// g.tiled), 3 the tile transpose of perm_fast_tile(); kernels launched for one shape
+// keep the other shapes' registers out.
 template <int SHAPE = 2, class Store>
 __device__ inline void permute_block(const PermGeom& g, unsigned bx, unsigned by, unsigned bz,
                                      Store&& store) {
@@ -92,6 +109,46 @@ __device__ inline void permute_block(const PermGeom& g, unsigned bx, unsigned by
   const long ir1 = by, ir0 = bz;
   const long b3 = (long)t3i * T3, bq = (long)tqi * TQ;
   constexpr int kPer = kPTEl / kPThreads;
+  if (SHAPE == 3) {
+    // perm_fast_tile(): the one transposed input goes through the tile, the others are
+    // read along dim 3 in the store phase; 32-bit offsets inside the tile and a kernel of
+    // its own keep its registers, and its occupancy, low
+    const int tl = __builtin_ctz(g.lds_mask);
+    const int nq = (int)min((long)TQ, g.nq - bq), n3 = (int)min((long)T3, g.n3 - b3);
+    {
+      const PTerm& T = g.t[tl];
+      const double* __restrict__ src = T.p + ir0 * T.sr0 + ir1 * T.sr1 + bq + b3 * T.s3;
+      const int s3 = (int)T.s3;
+#pragma unroll
+      for (int k = 0; k < kPer; ++k) {
+        const int e = (int)threadIdx.x + k * kPThreads;
+        if (e < nel) {
+          const int j3 = (int)(((float)e + 0.5f) * g.inv_tq), ql = e - j3 * TQ;
+          tile[j3 * ld + ql] = (ql < nq && j3 < n3) ? src[ql + j3 * s3] : 0.0;
+        }
+      }
+    }
+    __syncthreads();
+    const long obase = ir0 * g.or0 + ir1 * g.or1 + bq * g.oq + b3;
+    const int oq = (int)g.oq;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int e = (int)threadIdx.x + k * kPThreads;
+      if (e < nel) {
+        const int ql = (int)(((float)e + 0.5f) * g.inv_t3), j3 = e - ql * T3;
+        if (ql < nq && j3 < n3) {
+          double v = 0.0;
+          for (int t = 0; t < g.nterms; ++t) {  // in term order (fixed summation order)
+            const PTerm& T = g.t[t];
+            v += T.alpha * (t == tl ? tile[j3 * ld + ql]
+                                    : T.p[ir0 * T.sr0 + ir1 * T.sr1 + bq * T.sq + b3 + (ql * (int)T.sq + j3)]);
+          }
+          store(obase + ql * oq + j3, v);
+        }
+      }
+    }
+    return;
+  }
   double acc[kPer];
 #pragma unroll
   for (int k = 0; k < kPer; ++k) acc[k] = 0.0;

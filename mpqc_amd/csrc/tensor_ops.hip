@@ -452,8 +452,81 @@ void permuted_strides(const long in_extents[4], const int perm[4], long out_s[4]
   for (int d = 0; d < 4; ++d) out_s[d] = st[perm[d]];
 }
 
+namespace {
+PermGeom make_perm_geom_dims(const long n[4], const TTerm* terms, int nterms, const TProd* prod);
+
+// Adjacent output dims that every input also holds adjacent (stride of the outer =
+// stride of the inner times the inner extent; dims of extent 1 always qualify) merge
+// into one: e.g. tau(c,d,i,j) -> (i,j,c,d) is a plain (cd) x (ij) transpose.  Fewer,
+// longer dims give whole 64 x 32 tiles and longer contiguous runs.  Returns false when
+// nothing merges.
+bool merge_dims(const long n[4], const TTerm* terms, int nterms, const TProd* prod, long nm[4], TTerm* tm,
+                TProd* pm) {
+  long ext[4], st[kMaxTerms + 2][4];
+  const int ns = nterms + (prod ? 2 : 0);
+  for (int d = 0; d < 4; ++d) {
+    ext[d] = n[d];
+    for (int t = 0; t < nterms; ++t) st[t][d] = terms[t].s[d];
+    if (prod) {
+      st[nterms][d] = prod->sx[d];
+      st[nterms + 1][d] = prod->sy[d];
+    }
+  }
+  int nd = 4;
+  bool merged = false;
+  for (int d = 2; d >= 0; --d) {  // merge d with the (possibly merged) dim right of it
+    const int in = d + 1;
+    bool ok = true;
+    if (ext[d] > 1 && ext[in] > 1)
+      for (int t = 0; t < ns && ok; ++t) ok = st[t][d] == st[t][in] * ext[in];
+    if (!ok || ext[d] * ext[in] > (1L << 30)) continue;
+    const bool keep_inner = ext[in] > 1 || ext[d] == 1;  // stride of the merged dim
+    for (int t = 0; t < ns; ++t) st[t][d] = keep_inner ? st[t][in] : st[t][d];
+    ext[d] *= ext[in];
+    for (int e = in; e < nd - 1; ++e) {  // shift the dims right of `in` one place left
+      ext[e] = ext[e + 1];
+      for (int t = 0; t < ns; ++t) st[t][e] = st[t][e + 1];
+    }
+    --nd;
+    merged = true;
+  }
+  if (!merged) return false;
+  const int pad = 4 - nd;  // leading dims of extent 1
+  for (int d = 0; d < 4; ++d) {
+    nm[d] = d < pad ? 1 : ext[d - pad];
+    for (int t = 0; t < nterms; ++t) {
+      if (d == 0) tm[t] = terms[t];
+      tm[t].s[d] = d < pad ? 0 : st[t][d - pad];
+    }
+    if (prod) {
+      if (d == 0) *pm = *prod;
+      pm->sx[d] = d < pad ? 0 : st[nterms][d - pad];
+      pm->sy[d] = d < pad ? 0 : st[nterms + 1][d - pad];
+    }
+  }
+  return true;
+}
+}  // namespace
+
 PermGeom make_perm_geom(const long n[4], const TTerm* terms, int nterms, const TProd* prod) {
   if (nterms < 1 || nterms > kMaxTerms) throw std::invalid_argument("permutation: bad term count");
+  long nm[4];
+  TTerm tm[kMaxTerms];
+  TProd pm;
+  if (merge_dims(n, terms, nterms, prod, nm, tm, &pm)) {
+    try {
+      const PermGeom g = make_perm_geom_dims(nm, tm, nterms, prod ? &pm : nullptr);
+      (void)perm_grid(g);  // throws for a grid the launch could not take
+      return g;
+    } catch (const std::invalid_argument&) {
+      // (a merged dim too long for the grid or the linear plane: the dims as given)
+    }
+  }
+  return make_perm_geom_dims(n, terms, nterms, prod);
+}
+
+namespace {
+PermGeom make_perm_geom_dims(const long n[4], const TTerm* terms, int nterms, const TProd* prod) {
   PermGeom g{};
   long os[4];
   os[3] = 1;
@@ -511,6 +584,7 @@ PermGeom make_perm_geom(const long n[4], const TTerm* terms, int nterms, const T
   }
   return g;
 }
+}  // namespace
 
 dim3 perm_grid(const PermGeom& g) {
   check_grid_yz(g.nr1, g.nr0);
@@ -525,7 +599,10 @@ void tadd(double* out, const long n[4], double beta, const TTerm* terms, int nte
   if (n[0] * n[1] * n[2] * n[3] == 0) return;
   const PermGeom g = make_perm_geom(n, terms, nterms, prod);
   if (g.tiled)
-    hipLaunchKernelGGL(tadd_kernel<1>, perm_grid(g), dim3(kPThreads), perm_lds_bytes(g), stream, g, out, beta);
+    if (perm_fast_tile(g))
+      hipLaunchKernelGGL(tadd_kernel<3>, perm_grid(g), dim3(kPThreads), perm_lds_bytes(g), stream, g, out, beta);
+    else
+      hipLaunchKernelGGL(tadd_kernel<1>, perm_grid(g), dim3(kPThreads), perm_lds_bytes(g), stream, g, out, beta);
   else
     hipLaunchKernelGGL(tadd_kernel<0>, perm_grid(g), dim3(kPThreads), 0, stream, g, out, beta);
   check_launch("tadd");
