@@ -464,8 +464,9 @@ __global__ __launch_bounds__(kThr, (plane_occupancy<BM, BN, NBUF>())) void gemm_
 
 // ---------------------------------------------------------------- deep-pipeline contraction
 // Large two-plane contractions: a 256 x 256 output tile per workgroup of eight waves
-// (two per SIMD, a 2 x 4 grid of 128 x 64 wave tiles, each 4 x 2 blocks of
-// v_mfma_f32_32x32x16), one workgroup per CU.  K advances in 32-wide steps through two
+// (two per SIMD, a 2 x 4 grid of 128 x 64 wave tiles, each 8 x 4 blocks of
+// v_mfma_f32_16x16x32, which keep the matrix cores busier at two waves per SIMD than
+// 4 x 2 blocks of 32x32x16), one workgroup per CU.  K advances in 32-wide steps through two
 // LDS stages of 64 KB that direct global->LDS loads (global_load_lds_dwordx4, no staging
 // registers) fill: a stage row is 64 contiguous bytes of one plane, so every 64-byte
 // request of the loads is used whole.  The next step's loads are issued when its stage
@@ -476,7 +477,7 @@ __global__ __launch_bounds__(kThr, (plane_occupancy<BM, BN, NBUF>())) void gemm_
 // stores: FP32 partials of a K chunk (at most kMaxChunk) for a fixed-order FP64 reduce,
 // or C = alpha / (sa sb) sum + beta C in FP64.
 constexpr int kDlTile = 256;
-constexpr int kDlBK = 32;       // K per stage: two 32x32x16 MFMA steps
+constexpr int kDlBK = 32;       // K per stage: one 16x16x32 MFMA step
 constexpr int kDlThreads = 512; // eight waves
 constexpr int kDlGroup = 4;     // tile rows per group of the tile order
 
@@ -489,9 +490,13 @@ typedef double f64x2_t __attribute__((ext_vector_type(2)));
 __device__ __attribute__((aligned(16))) uint16_t g_dl_zero[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 
 // A stage row is 32 halves (four 16-byte chunks); chunk c of row r sits at chunk
-// c ^ dl_sw(r), so a fragment read (32 rows of one chunk) touches distinct 16-byte
-// bank groups in each 8-lane phase over 32 banks and each 16-lane phase over 64.
-__device__ inline int dl_sw(int r) { return ((r >> 1) ^ (r >> 2)) & 3; }
+// c ^ dl_sw(r).  A fragment read covers 16 rows of one chunk per 16 lanes: dl_sw is a
+// bijection of bits (1, 2) of r over rows of equal parity inside 8 rows and of bits
+// (2, 3) over rows of equal r mod 4, so the 16-byte bank groups differ in each 8-lane
+// phase over 32 banks and each 16-lane phase over 64.
+__device__ inline int dl_sw(int r) { return (((r >> 1) ^ (r >> 3)) & 1) | (((r >> 2) & 1) << 1); }
+// The same for fragments of 32 rows of one chunk per 32 lanes (v_mfma_f32_32x32x16).
+__device__ inline int dl_sw32(int r) { return ((r >> 1) ^ (r >> 2)) & 3; }
 
 template <bool F16>
 __device__ inline void mfma32(const u16x8_t& x, const u16x8_t& y, f32x16_t& c) {
@@ -577,45 +582,18 @@ __global__ __launch_bounds__(kDlThreads, 1) void gemm_planes_dl_kernel(DlParams 
         }
   };
 
-  // fragment of MFMA step ss of a stage: row (lane & 31), logical chunk 2 ss + (lane >> 5)
-  int frag[2];
+  // fragments (v_mfma_f32_16x16x32: lane -> row lane & 15, logical chunk lane >> 4): the
+  // wave's 128 x 64 tile is 8 x 4 blocks of 16 x 16, one MFMA K step per stage
+  const int lr = lane & 15, lk = lane >> 4;
+  const int frag = lr * kDlBK + ((lk ^ dl_sw(lr)) * 8);
+  u16x8_t fa[8][2], fb[4][2];  // [block][plane]
+  float4_t acc[8][4];
 #pragma unroll
-  for (int ss = 0; ss < 2; ++ss)
-    frag[ss] = (lane & 31) * kDlBK + (((2 * ss + (lane >> 5)) ^ dl_sw(lane & 31)) * 8);
-  u16x8_t fa[2][4][2], fb[2][2][2];  // [MFMA step][block][plane]
-  auto read_frags = [&](int st, int ss) {
+  for (int i = 0; i < 8; ++i)
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
+    for (int j = 0; j < 4; ++j)
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
-        fa[ss][i][q] = *reinterpret_cast<const u16x8_t*>(&lds[st][0][q][(wm * 128 + i * 32) * kDlBK + frag[ss]]);
-#pragma unroll
-      for (int j = 0; j < 2; ++j)
-        fb[ss][j][q] = *reinterpret_cast<const u16x8_t*>(&lds[st][1][q][(wn * 64 + j * 32) * kDlBK + frag[ss]]);
-    }
-  };
-  f32x16_t acc[4][2];
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-      for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
-  // products in increasing magnitude: hi*lo, lo*hi, hi*hi
-  auto products = [&](int ss) {
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int j = 0; j < 2; ++j) mfma32<F16>(fa[ss][i][0], fb[ss][j][1], acc[i][j]);
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int j = 0; j < 2; ++j) mfma32<F16>(fa[ss][i][1], fb[ss][j][0], acc[i][j]);
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int j = 0; j < 2; ++j) mfma32<F16>(fa[ss][i][0], fb[ss][j][0], acc[i][j]);
-  };
+      for (int r = 0; r < 4; ++r) acc[i][j][r] = 0.f;
 
   if (nsteps > 0) {
     issue(0, 0);
@@ -625,10 +603,31 @@ __global__ __launch_bounds__(kDlThreads, 1) void gemm_planes_dl_kernel(DlParams 
       const int st = t & 1;
       // stage st^1 was read in step t-1, which every wave finished before the barrier
       if (t + 1 < nsteps) issue(t + 1, st ^ 1);
-      read_frags(st, 0);
-      read_frags(st, 1);
-      products(0);
-      products(1);
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+          fa[i][q] = *reinterpret_cast<const u16x8_t*>(&lds[st][0][q][(wm * 128 + i * 16) * kDlBK + frag]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          fb[j][q] = *reinterpret_cast<const u16x8_t*>(&lds[st][1][q][(wn * 64 + j * 16) * kDlBK + frag]);
+      }
+      // products in increasing magnitude: hi*lo, lo*hi, hi*hi
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) mfma1<F16>(fa[i][0], fb[j][1], acc[i][j]);
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) mfma1<F16>(fa[i][1], fb[j][0], acc[i][j]);
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) mfma1<F16>(fa[i][0], fb[j][0], acc[i][j]);
+      // (keeps the waits below every product: hoisted above them, they would leave the
+      // next stage's loads less time to land)
+      __builtin_amdgcn_sched_barrier(0);
       dl_wait_loads();  // step t+1's stage has landed (this wave's share)
       dl_wait_lds();    // this wave's reads of stage st are done
       dl_barrier();
@@ -643,20 +642,19 @@ __global__ __launch_bounds__(kDlThreads, 1) void gemm_planes_dl_kernel(DlParams 
   float* stg = reinterpret_cast<float*>(&lds[0][0][0][0]) + wave * (64 * 64);
   if (nsteps == 0) dl_barrier();  // (the K loop's last barrier otherwise)
 #pragma unroll
-  for (int i0 = 0; i0 < 4; i0 += 2) {
+  for (int i0 = 0; i0 < 8; i0 += 4) {
 #pragma unroll
-    for (int ii = 0; ii < 2; ++ii)
+    for (int ii = 0; ii < 4; ++ii)
 #pragma unroll
-      for (int j = 0; j < 2; ++j)
+      for (int j = 0; j < 4; ++j)
 #pragma unroll
-        for (int r = 0; r < 16; ++r)
-          stg[(ii * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3)) * 64 + j * 32 + (lane & 31)] = acc[i0 + ii][j][r];
+        for (int r = 0; r < 4; ++r) stg[(ii * 16 + lk * 4 + r) * 64 + j * 16 + lr] = acc[i0 + ii][j][r];
     dl_wait_lds();
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
       const int idx = q * 64 + lane, row = idx >> 4, c4 = idx & 15;
       const f32x4_t v = *reinterpret_cast<const f32x4_t*>(&stg[row * 64 + c4 * 4]);
-      const int gm = m0 + wm * 128 + i0 * 32 + row, gn = n0 + wn * 64 + c4 * 4;
+      const int gm = m0 + wm * 128 + i0 * 16 + row, gn = n0 + wn * 64 + c4 * 4;
       if (gm >= p.M) continue;
       if (p.nsplit > 1) {
         float* wr = Wz + (long)gm * p.N + gn;
@@ -750,7 +748,7 @@ __global__ __launch_bounds__((64 * TkShape<MB, NB>::kWaves)) void gemm_planes_ta
     const int q = wave + S::kWaves * i;
     const int plane = q / (S::kRows / 16), rg = q - plane * (S::kRows / 16);
     const int r = rg * 16 + (lane >> 2);
-    kofs[i] = ((lane & 3) ^ dl_sw(r)) * 8;
+    kofs[i] = ((lane & 3) ^ dl_sw32(r)) * 8;
     dst[i] = (plane * S::kRows + rg * 16) * kDlBK;
     if (r < S::TM) {
       valid[i] = r < p.M;
@@ -771,7 +769,7 @@ __global__ __launch_bounds__((64 * TkShape<MB, NB>::kWaves)) void gemm_planes_ta
   int frag[2];
 #pragma unroll
   for (int ss = 0; ss < 2; ++ss)
-    frag[ss] = (lane & 31) * kDlBK + (((2 * ss + (lane >> 5)) ^ dl_sw(lane & 31)) * 8);
+    frag[ss] = (lane & 31) * kDlBK + (((2 * ss + (lane >> 5)) ^ dl_sw32(lane & 31)) * 8);
 
   f32x16_t acc[S::kMaxBlocksPerWave];
   double acc64[S::kMaxBlocksPerWave][16];
