@@ -864,6 +864,39 @@ __global__ __launch_bounds__(256) void splitk_reduce_f32_kernel(const float* __r
   }
 }
 
+// The same sums four outputs per thread: 16-byte partial loads and FP64 stores, 32-bit
+// indices (N % 4 == 0, C rows 16-byte aligned, M N / 4 < 2^31; the host checks).
+__global__ __launch_bounds__(256) void splitk_reduce_f32_vec_kernel(const float* __restrict__ W,
+                                                                    double* __restrict__ C, long ldc, long sC,
+                                                                    int M, int N, int nsplit, int batch0,
+                                                                    double alpha, const double* sa,
+                                                                    const double* sb, double beta) {
+  const long MN = (long)M * N;
+  const int n4 = N >> 2, total = (int)(MN >> 2);
+  const int bz = blockIdx.y;
+  const double u = alpha / ((*sa) * (*sb));
+  const float* __restrict__ wz = W + (long)bz * nsplit * MN;
+  double* __restrict__ cz = C + (long)(batch0 + bz) * sC;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < total; e += gridDim.x * blockDim.x) {
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int s2 = 0; s2 < nsplit; ++s2) {
+      const f32x4_t w = *reinterpret_cast<const f32x4_t*>(wz + (long)s2 * MN + 4 * (long)e);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) acc[k] += u * (double)w[k];
+    }
+    const int m = e / n4, c4 = e - m * n4;
+    f64x2_t* c = reinterpret_cast<f64x2_t*>(cz + (long)m * ldc + 4 * c4);
+    f64x2_t lo = {acc[0], acc[1]}, hi = {acc[2], acc[3]};
+    if (beta != 0.0) {
+      const f64x2_t c0 = c[0], c1 = c[1];
+      lo = f64x2_t{acc[0] + beta * c0[0], acc[1] + beta * c0[1]};
+      hi = f64x2_t{acc[2] + beta * c1[0], acc[3] + beta * c1[1]};
+    }
+    c[0] = lo;
+    c[1] = hi;
+  }
+}
+
 // Launch plan: tile edge, split-K factor and K chunk of one contraction.
 struct PlanePlan {
   int bm, bn;  // tile: 64 x 64, 128 x 128, 256 x 128 or 256 x 256
@@ -945,9 +978,17 @@ void launch_planes_dl(const PlaneGemmDesc& d, const DlPlan& pl, double* ws, size
     check_launch("gemm_planes_dl");
     if (pl.nsplit > 1) {
       const long MN = (long)d.M * d.N;
-      dim3 g2((unsigned)std::min<long>(ceil_div(MN, 256), 4096), (unsigned)nb);
-      hipLaunchKernelGGL(splitk_reduce_f32_kernel, g2, dim3(256), 0, stream, p.W, d.C, d.ldc, d.sC, d.M, d.N,
-                         pl.nsplit, b0, d.alpha, d.A.scale, d.B.scale, d.beta);
+      const bool vec = d.N % 4 == 0 && d.ldc % 2 == 0 && (d.batch == 1 || d.sC % 2 == 0) &&
+                       reinterpret_cast<uintptr_t>(d.C) % 16 == 0 && MN / 4 < (1L << 31) - 65536L * 256;
+      if (vec) {
+        dim3 g2((unsigned)std::min<long>(ceil_div(MN / 4, 256), 8192), (unsigned)nb);
+        hipLaunchKernelGGL(splitk_reduce_f32_vec_kernel, g2, dim3(256), 0, stream, p.W, d.C, d.ldc, d.sC, d.M,
+                           d.N, pl.nsplit, b0, d.alpha, d.A.scale, d.B.scale, d.beta);
+      } else {
+        dim3 g2((unsigned)std::min<long>(ceil_div(MN, 256), 4096), (unsigned)nb);
+        hipLaunchKernelGGL(splitk_reduce_f32_kernel, g2, dim3(256), 0, stream, p.W, d.C, d.ldc, d.sC, d.M, d.N,
+                           pl.nsplit, b0, d.alpha, d.A.scale, d.B.scale, d.beta);
+      }
       check_launch("splitk_reduce_f32");
     }
   }
