@@ -25,6 +25,7 @@ namespace mpqc_hip {
 constexpr int kPT = 64;          // largest tile edge of the tiled shape
 constexpr int kPTEl = 2048;      // largest tile (elements)
 constexpr int kPThreads = 256;   // threads per workgroup
+constexpr int kLinPer = 4;       // output elements per thread of the linear shape
 
 // One input with strides resolved against the launch geometry.
 struct PTerm {
@@ -84,22 +85,43 @@ template <int SHAPE = 2, class Store>
 __device__ inline void permute_block(const PermGeom& g, unsigned bx, unsigned by, unsigned bz,
                                      Store&& store) {
   if (SHAPE == 0 || (SHAPE == 2 && !g.tiled)) {
-    // linear shape: q = 2, r1 = 1, r0 = 0; the host keeps the (q,3) plane below 2^31
-    const int plane = (int)(g.nq * g.n3);
-    const int e = (int)bx * kPThreads + (int)threadIdx.x;
-    if (e >= plane) return;
+    // linear shape: q = 2, r1 = 1, r0 = 0; the host keeps the (q,3) plane below 2^31.
+    // kLinPer elements per thread, kPThreads apart (every access coalesced); all of a
+    // thread's sums are formed before its stores, so its loads are in flight together.
+    const long plane = g.nq * g.n3;
+    const long e0 = (long)bx * (kPThreads * kLinPer) + threadIdx.x;
     const long ir1 = by, ir0 = bz;
     const int n3 = (int)g.n3;
-    const int iq = e / n3, i3 = e - iq * n3;
-    double acc = 0.0;
+    int iq[kLinPer], i3[kLinPer];  // (elements past the plane read element 0)
+#pragma unroll
+    for (int j = 0; j < kLinPer; ++j) {
+      const long ej = e0 + (long)j * kPThreads;
+      const int e = ej < plane ? (int)ej : 0;
+      iq[j] = e / n3;
+      i3[j] = e - iq[j] * n3;
+    }
+    double acc[kLinPer];
+#pragma unroll
+    for (int j = 0; j < kLinPer; ++j) acc[j] = 0.0;
     for (int t = 0; t < g.nterms; ++t) {
       const PTerm& T = g.t[t];
-      acc += T.alpha * T.p[ir0 * T.sr0 + ir1 * T.sr1 + (long)iq * T.sq + (long)i3 * T.s3];
+      const double* base = T.p + ir0 * T.sr0 + ir1 * T.sr1;
+      double x[kLinPer];
+#pragma unroll
+      for (int j = 0; j < kLinPer; ++j) x[j] = base[(long)iq[j] * T.sq + (long)i3[j] * T.s3];
+#pragma unroll
+      for (int j = 0; j < kLinPer; ++j) acc[j] += T.alpha * x[j];
     }
     if (g.has_prod)
-      acc += g.palpha * g.px[ir0 * g.pxr0 + ir1 * g.pxr1 + (long)iq * g.pxq + (long)i3 * g.px3] *
-             g.py[ir0 * g.pyr0 + ir1 * g.pyr1 + (long)iq * g.pyq + (long)i3 * g.py3];
-    store(ir0 * g.or0 + ir1 * g.or1 + e, acc);
+#pragma unroll
+      for (int j = 0; j < kLinPer; ++j)
+        acc[j] += g.palpha * g.px[ir0 * g.pxr0 + ir1 * g.pxr1 + (long)iq[j] * g.pxq + (long)i3[j] * g.px3] *
+                  g.py[ir0 * g.pyr0 + ir1 * g.pyr1 + (long)iq[j] * g.pyq + (long)i3[j] * g.py3];
+#pragma unroll
+    for (int j = 0; j < kLinPer; ++j) {
+      const long ej = e0 + (long)j * kPThreads;
+      if (ej < plane) store(ir0 * g.or0 + ir1 * g.or1 + ej, acc[j]);
+    }
     return;
   }
   extern __shared__ double perm_tile[];  // T3 x (TQ + 1), perm_lds_bytes()
@@ -131,22 +153,30 @@ __device__ inline void permute_block(const PermGeom& g, unsigned bx, unsigned by
     __syncthreads();
     const long obase = ir0 * g.or0 + ir1 * g.or1 + bq * g.oq + b3;
     const int oq = (int)g.oq;
+    int ql[kPer], j3[kPer];
+    bool in[kPer];
+    double v[kPer];
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
       const int e = (int)threadIdx.x + k * kPThreads;
-      if (e < nel) {
-        const int ql = (int)(((float)e + 0.5f) * g.inv_t3), j3 = e - ql * T3;
-        if (ql < nq && j3 < n3) {
-          double v = 0.0;
-          for (int t = 0; t < g.nterms; ++t) {  // in term order (fixed summation order)
-            const PTerm& T = g.t[t];
-            v += T.alpha * (t == tl ? tile[j3 * ld + ql]
-                                    : T.p[ir0 * T.sr0 + ir1 * T.sr1 + bq * T.sq + b3 + (ql * (int)T.sq + j3)]);
-          }
-          store(obase + ql * oq + j3, v);
-        }
-      }
+      ql[k] = (int)(((float)e + 0.5f) * g.inv_t3);
+      j3[k] = e - ql[k] * T3;
+      in[k] = e < nel && ql[k] < nq && j3[k] < n3;
+      v[k] = 0.0;
     }
+    // terms outermost: a term's loads for all of this thread's elements issue together;
+    // each element still sums its terms in term order
+    for (int t = 0; t < g.nterms; ++t) {
+      const PTerm& T = g.t[t];
+      const double* src = T.p + ir0 * T.sr0 + ir1 * T.sr1 + bq * T.sq + b3;
+      const int sq = (int)T.sq;
+#pragma unroll
+      for (int k = 0; k < kPer; ++k)
+        if (in[k]) v[k] += T.alpha * (t == tl ? tile[j3[k] * ld + ql[k]] : src[ql[k] * sq + j3[k]]);
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; ++k)
+      if (in[k]) store(obase + ql[k] * oq + j3[k], v[k]);
     return;
   }
   double acc[kPer];

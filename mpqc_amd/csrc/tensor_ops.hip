@@ -227,6 +227,38 @@ __global__ __launch_bounds__(kBlock) void multidot_stage1(MultiDotArgs a, double
   }
 }
 
+// The same with the long part (x2, y2) read two elements at a time (16-byte loads;
+// every x2[k], y2 and c2 16-byte aligned, n2 even; the host checks).
+__global__ __launch_bounds__(kBlock) void multidot_vec_stage1(MultiDotArgs a, double* __restrict__ partials) {
+  typedef double d2_t __attribute__((ext_vector_type(2)));
+  double acc[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) acc[k] = 0.0;
+  const long n2h = a.n2 >> 1;
+  for (long e = (long)blockIdx.x * kBlock + threadIdx.x; e < n2h; e += (long)gridDim.x * kBlock) {
+    const d2_t y = reinterpret_cast<const d2_t*>(a.y2)[e];
+    if (a.c2) reinterpret_cast<d2_t*>(a.c2)[e] = y;
+#pragma unroll
+    for (int k = 0; k < 16; ++k)
+      if (k < a.nvec) {
+        const d2_t x = reinterpret_cast<const d2_t*>(a.x2[k])[e];
+        acc[k] += x[0] * y[0];
+        acc[k] += x[1] * y[1];
+      }
+  }
+  for (long e = (long)blockIdx.x * kBlock + threadIdx.x; e < a.n1; e += (long)gridDim.x * kBlock) {
+    const double y = a.y1[e];
+    if (a.c1) a.c1[e] = y;
+#pragma unroll
+    for (int k = 0; k < 16; ++k)
+      if (k < a.nvec) acc[k] += a.x1[k][e] * y;
+  }
+  for (int k = 0; k < a.nvec; ++k) {
+    const double r = block_sum(acc[k]);
+    if (threadIdx.x == 0) partials[(long)k * gridDim.x + blockIdx.x] = r;
+  }
+}
+
 struct LincombArgs {
   const double* x[16];
   double c[16];
@@ -588,7 +620,7 @@ PermGeom make_perm_geom_dims(const long n[4], const TTerm* terms, int nterms, co
 
 dim3 perm_grid(const PermGeom& g) {
   check_grid_yz(g.nr1, g.nr0);
-  const long x = g.tiled ? (long)g.tiles3 * g.tilesq : ceil_div(g.nq * g.n3, kPThreads);
+  const long x = g.tiled ? (long)g.tiles3 * g.tilesq : ceil_div(g.nq * g.n3, (long)kPThreads * kLinPer);
   if (x >= (1L << 31)) throw std::invalid_argument("permutation grid too large");
   return dim3((unsigned)x, (unsigned)g.nr1, (unsigned)g.nr0);
 }
@@ -747,8 +779,14 @@ void multi_dot2(const double* const* xs1, const double* const* xs2, int nvec, co
   }
   a.y1 = y1; a.y2 = y2; a.n1 = n1; a.n2 = n2; a.nvec = nvec;
   a.c1 = copy1; a.c2 = copy2;
-  const int g = red_grid(std::max(n1, n2));
-  hipLaunchKernelGGL(multidot_stage1, dim3(g), dim3(kBlock), 0, s, a, partials);
+  auto al16 = [](const void* p) { return reinterpret_cast<uintptr_t>(p) % 16 == 0; };
+  bool vec = n2 % 2 == 0 && n2 >= 2 * (long)kBlock * kRedGrid && al16(y2) && al16(copy2);
+  for (int k = 0; k < nvec; ++k) vec = vec && al16(xs2[k]);
+  const int g = red_grid(std::max(n1, vec ? n2 / 2 : n2));
+  if (vec)
+    hipLaunchKernelGGL(multidot_vec_stage1, dim3(g), dim3(kBlock), 0, s, a, partials);
+  else
+    hipLaunchKernelGGL(multidot_stage1, dim3(g), dim3(kBlock), 0, s, a, partials);
   hipLaunchKernelGGL(reduce_stage2, dim3(nvec), dim3(kBlock), 0, s, partials, g, out, 1.0);
   check_launch("multi_dot2");
 }
