@@ -490,11 +490,15 @@ typedef double f64x2_t __attribute__((ext_vector_type(2)));
 __device__ __attribute__((aligned(16))) uint16_t g_dl_zero[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 
 // A stage row is 32 halves (four 16-byte chunks); chunk c of row r sits at chunk
-// c ^ dl_sw(r).  A fragment read covers 16 rows of one chunk per 16 lanes: dl_sw is a
-// bijection of bits (1, 2) of r over rows of equal parity inside 8 rows and of bits
-// (2, 3) over rows of equal r mod 4, so the 16-byte bank groups differ in each 8-lane
-// phase over 32 banks and each 16-lane phase over 64.
-__device__ inline int dl_sw(int r) { return (((r >> 1) ^ (r >> 3)) & 1) | (((r >> 2) & 1) << 1); }
+// c ^ dl_sw(r).  A fragment read covers rows r = 0..15 (lane & 15) at logical chunk
+// lane >> 4.  dl_sw(r) = r1 ^ 3 r2 ^ r3 (bits of r) gives distinct 16-byte bank groups
+// for any 8 consecutive lanes over 32 banks, any 16 consecutive lanes over 64 banks, and
+// lanes i..i+7 together with i+32..i+39 over 64 banks.  The hardware still counts about
+// four bank-conflict cycles per fragment read with this or any other chunk swizzle; a
+// chunk-major stage (load lane l: row l & 15, chunk l >> 4, so every fragment read is one
+// linear 1 KB read) removes them but measured 4 % slower: its loads gather 16 rows per
+// four lanes.  The conflicts are hidden behind the products.
+__device__ inline int dl_sw(int r) { return (((r >> 1) ^ (r >> 3)) & 1) ^ (((r >> 2) & 1) * 3); }
 // The same for fragments of 32 rows of one chunk per 32 lanes (v_mfma_f32_32x32x16).
 __device__ inline int dl_sw32(int r) { return ((r >> 1) ^ (r >> 2)) & 3; }
 
@@ -648,12 +652,13 @@ __global__ __launch_bounds__(kDlThreads, 1) void gemm_planes_dl_kernel(DlParams 
 #pragma unroll
       for (int j = 0; j < 4; ++j)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) stg[(ii * 16 + lk * 4 + r) * 64 + j * 16 + lr] = acc[i0 + ii][j][r];
+        for (int r = 0; r < 4; ++r)  // (column ^ 16 ((row >> 2) & 3): the four lane groups hit distinct banks)
+          stg[(ii * 16 + lk * 4 + r) * 64 + ((j ^ lk) * 16 + lr)] = acc[i0 + ii][j][r];
     dl_wait_lds();
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
       const int idx = q * 64 + lane, row = idx >> 4, c4 = idx & 15;
-      const f32x4_t v = *reinterpret_cast<const f32x4_t*>(&stg[row * 64 + c4 * 4]);
+      const f32x4_t v = *reinterpret_cast<const f32x4_t*>(&stg[row * 64 + ((c4 * 4) ^ (((row >> 2) & 3) * 16))]);
       const int gm = m0 + wm * 128 + i0 * 16 + row, gn = n0 + wn * 64 + c4 * 4;
       if (gm >= p.M) continue;
       if (p.nsplit > 1) {
