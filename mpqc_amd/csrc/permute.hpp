@@ -61,26 +61,28 @@ inline size_t perm_lds_bytes(const PermGeom& g) {
   return g.tiled ? (size_t)g.t3 * (size_t)(g.tq + 1) * sizeof(double) : 0;
 }
 
-// Whether the tiled geometry suits SHAPE 3: no product term, exactly one input through
-// the LDS tile (contiguous along q), every other input contiguous along dim 3, and
-// tile-local offsets within 32 bits.
-inline bool perm_fast_tile(const PermGeom& g) {
-  if (!g.tiled || g.has_prod || __builtin_popcount(g.lds_mask) != 1) return false;
+// The fast tile shape the tiled geometry suits, or 0: every input through the LDS tile
+// contiguous along q, every other input contiguous along dim 3, and tile-local offsets
+// within 32 bits.  3: one input through the tile and no product term; 4: several
+// inputs through the tile and/or a product term (read with full offsets).
+inline int perm_fast_tile(const PermGeom& g) {
+  if (!g.tiled || g.lds_mask == 0) return 0;
   for (int t = 0; t < g.nterms; ++t) {
     const PTerm& T = g.t[t];
     const bool lds = (g.lds_mask >> t) & 1u;
-    if (lds ? T.sq != 1 : T.s3 != 1) return false;
-    if (std::abs(T.s3) * g.t3 >= (1L << 30) || std::abs(T.sq) * g.tq >= (1L << 30)) return false;
+    if (lds ? T.sq != 1 : T.s3 != 1) return 0;
+    if (std::abs(T.s3) * g.t3 >= (1L << 30) || std::abs(T.sq) * g.tq >= (1L << 30)) return 0;
   }
-  return std::abs(g.oq) * g.tq < (1L << 30);
+  if (std::abs(g.oq) * g.tq >= (1L << 30)) return 0;
+  return (__builtin_popcount(g.lds_mask) == 1 && !g.has_prod) ? 3 : 4;
 }
 
 // Device: evaluates every output element of block (bx, by, bz) of the launch
 // grid and hands (linear output offset, sum) to store().  Called by every
 // thread of the workgroup (the tiled shape synchronises through LDS).  SHAPE
 // selects the code compiled in: 0 linear only, 1 tiled only, 2 both (decided by
-// g.tiled), 3 the tile transpose of perm_fast_tile(); kernels launched for one shape
-// keep the other shapes' registers out.
+// g.tiled), 3 and 4 the fast tile shapes of perm_fast_tile(); kernels launched for one
+// shape keep the other shapes' registers out.
 template <int SHAPE = 2, class Store>
 __device__ inline void permute_block(const PermGeom& g, unsigned bx, unsigned by, unsigned bz,
                                      Store&& store) {
@@ -131,28 +133,27 @@ __device__ inline void permute_block(const PermGeom& g, unsigned bx, unsigned by
   const long ir1 = by, ir0 = bz;
   const long b3 = (long)t3i * T3, bq = (long)tqi * TQ;
   constexpr int kPer = kPTEl / kPThreads;
-  if (SHAPE == 3) {
-    // perm_fast_tile(): the one transposed input goes through the tile, the others are
-    // read along dim 3 in the store phase; 32-bit offsets inside the tile and a kernel of
-    // its own keep its registers, and its occupancy, low
-    const int tl = __builtin_ctz(g.lds_mask);
+  if (SHAPE == 3 || SHAPE == 4) {
+    // perm_fast_tile(): inputs through the tile one after another, the others read along
+    // dim 3; each output element sums its terms in term order, then the product term.
+    // 32-bit offsets inside the tile and kernels of their own keep the registers, and the
+    // occupancy, of these shapes low.
     const int nq = (int)min((long)TQ, g.nq - bq), n3 = (int)min((long)T3, g.n3 - b3);
-    {
-      const PTerm& T = g.t[tl];
-      const double* __restrict__ src = T.p + ir0 * T.sr0 + ir1 * T.sr1 + bq + b3 * T.s3;
+    auto load_tile = [&](const PTerm& T) {  // lanes along q, where T is contiguous
+      const double* src = T.p + ir0 * T.sr0 + ir1 * T.sr1 + bq + b3 * T.s3;
       const int s3 = (int)T.s3;
 #pragma unroll
       for (int k = 0; k < kPer; ++k) {
         const int e = (int)threadIdx.x + k * kPThreads;
         if (e < nel) {
-          const int j3 = (int)(((float)e + 0.5f) * g.inv_tq), ql = e - j3 * TQ;
-          tile[j3 * ld + ql] = (ql < nq && j3 < n3) ? src[ql + j3 * s3] : 0.0;
+          const int jj = (int)(((float)e + 0.5f) * g.inv_tq), qq = e - jj * TQ;
+          tile[jj * ld + qq] = (qq < nq && jj < n3) ? src[qq + jj * s3] : 0.0;
         }
       }
-    }
-    __syncthreads();
-    const long obase = ir0 * g.or0 + ir1 * g.or1 + bq * g.oq + b3;
-    const int oq = (int)g.oq;
+      __syncthreads();
+    };
+    const int tl = __builtin_ctz(g.lds_mask);  // (shape 3: the one input through the tile)
+    if (SHAPE == 3) load_tile(g.t[tl]);
     int ql[kPer], j3[kPer];
     bool in[kPer];
     double v[kPer];
@@ -164,16 +165,35 @@ __device__ inline void permute_block(const PermGeom& g, unsigned bx, unsigned by
       in[k] = e < nel && ql[k] < nq && j3[k] < n3;
       v[k] = 0.0;
     }
-    // terms outermost: a term's loads for all of this thread's elements issue together;
-    // each element still sums its terms in term order
+    // terms outermost: a term's loads for all of this thread's elements issue together
     for (int t = 0; t < g.nterms; ++t) {
       const PTerm& T = g.t[t];
-      const double* src = T.p + ir0 * T.sr0 + ir1 * T.sr1 + bq * T.sq + b3;
-      const int sq = (int)T.sq;
+      if ((g.lds_mask >> t) & 1u) {  // (uniform over the workgroup)
+        if (SHAPE == 4) {
+          if (t != tl) __syncthreads();  // every thread has read the previous tile
+          load_tile(T);
+        }
+#pragma unroll
+        for (int k = 0; k < kPer; ++k)
+          if (in[k]) v[k] += T.alpha * tile[j3[k] * ld + ql[k]];
+      } else {
+        const double* src = T.p + ir0 * T.sr0 + ir1 * T.sr1 + bq * T.sq + b3;
+        const int sq = (int)T.sq;
+#pragma unroll
+        for (int k = 0; k < kPer; ++k)
+          if (in[k]) v[k] += T.alpha * src[ql[k] * sq + j3[k]];
+      }
+    }
+    if (SHAPE == 4 && g.has_prod)
 #pragma unroll
       for (int k = 0; k < kPer; ++k)
-        if (in[k]) v[k] += T.alpha * (t == tl ? tile[j3[k] * ld + ql[k]] : src[ql[k] * sq + j3[k]]);
-    }
+        if (in[k]) {
+          const long iq = bq + ql[k], i3 = b3 + j3[k];
+          v[k] += g.palpha * g.px[ir0 * g.pxr0 + ir1 * g.pxr1 + iq * g.pxq + i3 * g.px3] *
+                  g.py[ir0 * g.pyr0 + ir1 * g.pyr1 + iq * g.pyq + i3 * g.py3];
+        }
+    const long obase = ir0 * g.or0 + ir1 * g.or1 + bq * g.oq + b3;
+    const int oq = (int)g.oq;
 #pragma unroll
     for (int k = 0; k < kPer; ++k)
       if (in[k]) store(obase + ql[k] * oq + j3[k], v[k]);

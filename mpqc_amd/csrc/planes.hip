@@ -1227,8 +1227,10 @@ void split_to_planes(PlaneKind kind, const Planes& out, const long n[4], const T
   const PermGeom g = make_perm_geom(n, terms, nterms, prod);
   SplitOut so{{out.p[0], out.p[1], out.p[2]}, out.scale, (int)kind};
   if (g.tiled)
-    if (perm_fast_tile(g))
+    if (perm_fast_tile(g) == 3)
       hipLaunchKernelGGL(split_write_kernel<3>, perm_grid(g), dim3(kPThreads), perm_lds_bytes(g), stream, g, so);
+    else if (perm_fast_tile(g) == 4)
+      hipLaunchKernelGGL(split_write_kernel<4>, perm_grid(g), dim3(kPThreads), perm_lds_bytes(g), stream, g, so);
     else
       hipLaunchKernelGGL(split_write_kernel<1>, perm_grid(g), dim3(kPThreads), perm_lds_bytes(g), stream, g, so);
   else

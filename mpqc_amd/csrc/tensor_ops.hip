@@ -631,8 +631,10 @@ void tadd(double* out, const long n[4], double beta, const TTerm* terms, int nte
   if (n[0] * n[1] * n[2] * n[3] == 0) return;
   const PermGeom g = make_perm_geom(n, terms, nterms, prod);
   if (g.tiled)
-    if (perm_fast_tile(g))
+    if (perm_fast_tile(g) == 3)
       hipLaunchKernelGGL(tadd_kernel<3>, perm_grid(g), dim3(kPThreads), perm_lds_bytes(g), stream, g, out, beta);
+    else if (perm_fast_tile(g) == 4)
+      hipLaunchKernelGGL(tadd_kernel<4>, perm_grid(g), dim3(kPThreads), perm_lds_bytes(g), stream, g, out, beta);
     else
       hipLaunchKernelGGL(tadd_kernel<1>, perm_grid(g), dim3(kPThreads), perm_lds_bytes(g), stream, g, out, beta);
   else
