@@ -208,11 +208,14 @@ SPLIT_BOUNDS = {"bf16x3": 2.0 ** -14, "f16x3": 2.0 ** -19, "bf16x6": 2.0 ** -19}
 
 @pytest.mark.parametrize("prec", ["bf16x3", "f16x3", "bf16x6"])
 @pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
-@pytest.mark.parametrize("m,n,k", [(17, 33, 5), (130, 70, 257), (300, 200, 3000), (40, 40, 20000)])
+@pytest.mark.parametrize("m,n,k", [(17, 33, 5), (130, 70, 257), (300, 200, 3000), (40, 40, 20000),
+                                   (512, 768, 3000), (768, 512, 1000)])
 def test_gemm_split_error_bound(prec, ta, tb, m, n, k):
     """Split-precision contractions: |C - A B| <= bound(prec) * |A| |B| elementwise
     (plus a tiny absolute term for FP16 subnormal parts), and the result is
-    measurably not FP64."""
+    measurably not FP64.  The two last shapes are whole 256 x 256 tiles of the
+    two-plane deep-pipeline kernel: two FP32 K chunks added in FP64 by the split-K
+    pass, and one chunk written in FP64 by the kernel itself."""
     from mpqc_amd.ccsd import gemm
     rng = np.random.default_rng(m * 7 + n + k)
     A = rng.standard_normal((k, m) if ta else (m, k)) * np.exp(rng.uniform(-3, 3, (k, m) if ta else (m, k)))
