@@ -491,14 +491,12 @@ __device__ __attribute__((aligned(16))) uint16_t g_dl_zero[8] = {0, 0, 0, 0, 0, 
 
 // A stage row is 32 halves (four 16-byte chunks); chunk c of row r sits at chunk
 // c ^ dl_sw(r).  A fragment read covers rows r = 0..15 (lane & 15) at logical chunk
-// lane >> 4.  dl_sw(r) = r1 ^ 3 r2 ^ r3 (bits of r) gives distinct 16-byte bank groups
-// for any 8 consecutive lanes over 32 banks, any 16 consecutive lanes over 64 banks, and
-// lanes i..i+7 together with i+32..i+39 over 64 banks.  The hardware still counts about
-// four bank-conflict cycles per fragment read with this or any other chunk swizzle; a
-// chunk-major stage (load lane l: row l & 15, chunk l >> 4, so every fragment read is one
-// linear 1 KB read) removes them but measured 4 % slower: its loads gather 16 rows per
-// four lanes.  The conflicts are hidden behind the products.
-__device__ inline int dl_sw(int r) { return (((r >> 1) ^ (r >> 3)) & 1) ^ (((r >> 2) & 1) * 3); }
+// lane >> 4.  dl_sw(r) = r0 | r2 << 1 (bits of r) was chosen by measurement among eight
+// maps of bits of r (standalone copy of this kernel at 6080^3, SQ_LDS_BANK_CONFLICT): the
+// only one whose fragment reads count no bank-conflict cycles (the others, including maps
+// conflict-free for any 8 or 16 consecutive lanes, count about four per read), and the
+// fastest by 1 %.
+__device__ inline int dl_sw(int r) { return (r & 1) | (((r >> 2) & 1) << 1); }
 // The same for fragments of 32 rows of one chunk per 32 lanes (v_mfma_f32_32x32x16).
 __device__ inline int dl_sw32(int r) { return ((r >> 1) ^ (r >> 2)) & 3; }
 
