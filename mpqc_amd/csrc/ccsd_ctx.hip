@@ -1362,13 +1362,10 @@ void CCSDContext::energy_error(double* E, double* err) {
   if (!prepared_) prepare();
   const long n1 = (long)o_ * v_, n2 = n1 * n1;
   // error = sqrt(|r1|^2 + |r2|^2) / (size(r1) + size(r2))   qc/cc/solvers.h:60-63
-  timed("en:norms", 0.0, [&] {
-    dot(r1_, r1_, n1, scalars_ + 0, partials_, stream_);
-    dot(r2_, r2_, n2, scalars_ + 1, partials_, stream_);
-  });
   // E = 2 <f_ai + r1, t1> + <g_abij + r2, 2 tau(a,b,i,j) - tau(b,a,i,j)>   ccsd.h:377-379
+  // (with the two norms, in one pass over r2)
   timed("en:energy", 0.0, [&] {
-    ccsd_energy(f_ai_, r1_, t1_, g_abij_, r2_, tau_, o_, v_, scalars_ + 2, partials_, stream_);
+    ccsd_energy_norms(f_ai_, r1_, t1_, g_abij_, r2_, tau_, o_, v_, scalars_, partials_, stream_);
   });
   double h[3];
   MPQC_HIP_CHECK(hipMemcpyAsync(h, scalars_, 3 * sizeof(double), hipMemcpyDeviceToHost, stream_));

@@ -191,6 +191,38 @@ __global__ __launch_bounds__(kBlock) void energy_stage1(
   if (threadIdx.x == 0) partials[blockIdx.x] = r;
 }
 
+// The same energy partials (same per-thread expression and grid, so the same bits) with
+// |r1|^2 and |r2|^2 from the same reads: partials[k * gridDim.x + block], k = 0 |r1|^2,
+// 1 |r2|^2, 2 energy.
+__global__ __launch_bounds__(kBlock) void energy_norms_stage1(
+    const double* __restrict__ f_ai, const double* __restrict__ r1, const double* __restrict__ t1,
+    const double* __restrict__ g, const double* __restrict__ r2, const double* __restrict__ tau,
+    int o, int v, double* __restrict__ partials) {
+  const int n2 = v * v * o * o;
+  const int oo = o * o;
+  double acc = 0.0, q2 = 0.0;
+  for (int e = blockIdx.x * kBlock + threadIdx.x; e < n2; e += gridDim.x * kBlock) {
+    const int ab = e / oo, ij = e - ab * oo;
+    const int a = ab / v, b = ab - a * v;
+    const double tba = tau[(b * v + a) * oo + ij];
+    const double r = r2[e];
+    acc += (g[e] + r) * (2.0 * tau[e] - tba);
+    q2 += r * r;
+  }
+  double acc1 = 0.0, q1 = 0.0;
+  for (long e = (long)blockIdx.x * kBlock + threadIdx.x; e < (long)v * o; e += (long)gridDim.x * kBlock) {
+    const double r = r1[e];
+    acc1 += (f_ai[e] + r) * t1[e];
+    q1 += r * r;
+  }
+  const double s1 = block_sum(q1), s2 = block_sum(q2), se = block_sum(acc + 2.0 * acc1);
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = s1;
+    partials[gridDim.x + blockIdx.x] = s2;
+    partials[2 * gridDim.x + blockIdx.x] = se;
+  }
+}
+
 struct MultiDotArgs {
   const double* x1[16];
   const double* x2[16];
@@ -768,6 +800,17 @@ void ccsd_energy(const double* f_ai, const double* r1, const double* t1, const d
                      partials);
   hipLaunchKernelGGL(reduce_stage2, dim3(1), dim3(kBlock), 0, s, partials, g, out, 1.0);
   check_launch("ccsd_energy");
+}
+
+void ccsd_energy_norms(const double* f_ai, const double* r1, const double* t1, const double* g_abij,
+                       const double* r2, const double* tau, int o, int v, double* out3, double* partials,
+                       hipStream_t s) {
+  if ((long)v * v * o * o >= (1L << 31)) throw std::invalid_argument("ccsd_energy: v^2 o^2 too large");
+  const int g = red_grid((long)v * v * o * o);
+  hipLaunchKernelGGL(energy_norms_stage1, dim3(g), dim3(kBlock), 0, s, f_ai, r1, t1, g_abij, r2, tau, o, v,
+                     partials);
+  hipLaunchKernelGGL(reduce_stage2, dim3(3), dim3(kBlock), 0, s, partials, g, out3, 1.0);
+  check_launch("ccsd_energy_norms");
 }
 
 void multi_dot2(const double* const* xs1, const double* const* xs2, int nvec, const double* y1,

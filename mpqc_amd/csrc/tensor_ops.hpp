@@ -75,6 +75,11 @@ void t2bar_h(double* r1, const double* t2, const double* h, int o, int v, double
 void ccsd_energy(const double* f_ai, const double* r1, const double* t1, const double* g_abij,
                  const double* r2, const double* tau, int o, int v, double* out, double* partials,
                  hipStream_t s);
+// The same energy into out3[2] with |r1|^2 into out3[0] and |r2|^2 into out3[1], from one
+// pass over the o^2 v^2 tensors (partials: at least 3 * 1024 doubles).
+void ccsd_energy_norms(const double* f_ai, const double* r1, const double* t1, const double* g_abij,
+                       const double* r2, const double* tau, int o, int v, double* out3, double* partials,
+                       hipStream_t s);
 
 // out[k] = <x_k, y> for k < nvec, x_k = X + k*stride (two members: 1- and 2-body)
 // (copy1, copy2: optional destinations that receive y1, y2 in the same pass)
