@@ -1441,6 +1441,8 @@ void CCSDContext::update() {
   x_slots_.push_back(xslot);
   const bool do_extrap =
       diis_iter_ > diis_start_ && ((diis_iter_ - diis_start_) % diis_ngroup_) < diis_group_nstart_;
+  std::vector<const double*> x1, x2;  // extrapolation: history amplitudes and their coefficients
+  std::vector<double> c;
   if (do_extrap) {
     const double zero_determinant = 1.0e-15, zero_norm = 1.0e-10;
     const int m = n_diis_;
@@ -1466,8 +1468,6 @@ void CCSDContext::update() {
     }
     if (absdet < zero_determinant) throw std::domain_error("DIIS: poorly-conditioned system");
     --nskip;
-    std::vector<const double*> x1, x2;
-    std::vector<double> c;
     int k = 0;
     for (int s : x_slots_) {
       if (k >= nskip) {
@@ -1477,19 +1477,21 @@ void CCSDContext::update() {
       }
       ++k;
     }
-    timed("up:extrapolate", 0.0, [&] {
-      lincomb(t1_, x1.data(), c.data(), (int)c.size(), n1, stream_);
-      lincomb(t2_, x2.data(), c.data(), (int)c.size(), n2, stream_);
-    });
   }
   // tau("a,b,i,j") = t2 + t1 t1                                     ccsd.h:389
-  if (prec_ == GemmPrecision::F16X3 && amax_.dev) {  // with the maxima the fp16 splits need
-    timed("up:tau", 0.0, [&] { build_tau_absmax(tau_, t2_, t1_, o_, v_, partials_, amax_.dev + 1, stream_); });
-    tau_amax_fresh_ = true;
+  // (with the maxima the fp16 splits need; after an extrapolation, in the pass that writes t2)
+  double* maxes = (prec_ == GemmPrecision::F16X3 && amax_.dev) ? amax_.dev + 1 : nullptr;
+  if (do_extrap) {
+    timed("up:extrapolate", 0.0, [&] {
+      lincomb(t1_, x1.data(), c.data(), (int)c.size(), n1, stream_);
+      lincomb_tau(t2_, tau_, t1_, x2.data(), c.data(), (int)c.size(), o_, v_, partials_, maxes, stream_);
+    });
+  } else if (maxes) {
+    timed("up:tau", 0.0, [&] { build_tau_absmax(tau_, t2_, t1_, o_, v_, partials_, maxes, stream_); });
   } else {
     timed("up:tau", 0.0, [&] { build_tau(tau_, t2_, t1_, o_, v_, stream_); });
-    tau_amax_fresh_ = false;
   }
+  tau_amax_fresh_ = maxes != nullptr;
   tau_fresh_ = true;
   MPQC_HIP_CHECK(hipStreamSynchronize(stream_));
 }

@@ -320,6 +320,47 @@ __global__ __launch_bounds__(kBlock) void lincomb_kernel(double* __restrict__ y,
   }
 }
 
+// t2 = sum_k c_k x_k (lincomb_kernel's sum) and tau = t2 + t1(a,i) t1(b,j) (build_tau's)
+// in one pass; with part, also the per-block max |t2| and max |tau| (build_tau_absmax's).
+template <bool MAXES>
+__global__ __launch_bounds__(kBlock) void lincomb_tau_kernel(double* __restrict__ t2, double* __restrict__ tau,
+                                                             const double* __restrict__ t1, LincombArgs a, int o,
+                                                             int v, double* __restrict__ part) {
+  const int oo = o * o, n = v * v * oo;
+  double m2 = 0.0, mt = 0.0;
+  for (int e = blockIdx.x * kBlock + threadIdx.x; e < n; e += gridDim.x * kBlock) {
+    double x = 0.0;
+    for (int k = 0; k < a.nvec; ++k) x += a.c[k] * a.x[k][e];
+    const int ab = e / oo, ij = e - ab * oo;
+    const int ai = ab / v, bi = ab - ai * v;
+    const int i = ij / o, j = ij - i * o;
+    const double y = x + t1[ai * o + i] * t1[bi * o + j];
+    t2[e] = x;
+    tau[e] = y;
+    if (MAXES) {
+      m2 = fmax(m2, fabs(x));
+      mt = fmax(mt, fabs(y));
+    }
+  }
+  if (MAXES) {
+    __shared__ double s2[kBlock], st[kBlock];
+    s2[threadIdx.x] = m2;
+    st[threadIdx.x] = mt;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) {
+        s2[threadIdx.x] = fmax(s2[threadIdx.x], s2[threadIdx.x + w]);
+        st[threadIdx.x] = fmax(st[threadIdx.x], st[threadIdx.x + w]);
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      part[2 * blockIdx.x] = s2[0];
+      part[2 * blockIdx.x + 1] = st[0];
+    }
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void nonfinite_stage1(const double* __restrict__ x, long n,
                                                            double* __restrict__ partials) {
   double acc = 0.0;
@@ -861,6 +902,26 @@ void lincomb(double* y, const double* const* xs, const double* c, int nvec, long
   const int g = (int)std::min<long>(ceil_div(n, kBlock), 8192);
   hipLaunchKernelGGL(lincomb_kernel, dim3(g), dim3(kBlock), 0, s, y, a, n);
   check_launch("lincomb");
+}
+
+void lincomb_tau(double* t2, double* tau, const double* t1, const double* const* xs, const double* c, int nvec,
+                 int o, int v, double* partials, double* maxes, hipStream_t s) {
+  check_small(o, v);
+  if (nvec > 16) throw std::invalid_argument("lincomb: at most 16 vectors");
+  LincombArgs a;
+  for (int k = 0; k < 16; ++k) {
+    a.x[k] = k < nvec ? xs[k] : nullptr;
+    a.c[k] = k < nvec ? c[k] : 0.0;
+  }
+  a.nvec = nvec;
+  const int nb = (int)std::min<long>(elem_grid((long)v * v * o * o), kReducePartials / 2);
+  if (maxes) {
+    hipLaunchKernelGGL(lincomb_tau_kernel<true>, dim3(nb), dim3(kBlock), 0, s, t2, tau, t1, a, o, v, partials);
+    hipLaunchKernelGGL(absmax2_stage2, dim3(1), dim3(kBlock), 0, s, partials, nb, maxes);
+  } else {
+    hipLaunchKernelGGL(lincomb_tau_kernel<false>, dim3(nb), dim3(kBlock), 0, s, t2, tau, t1, a, o, v, partials);
+  }
+  check_launch("lincomb_tau");
 }
 
 void count_nonfinite(const double* x, long n, double* out, double* partials, hipStream_t s) {

@@ -96,6 +96,11 @@ void fill(double* x, long n, double value, hipStream_t s);
 
 // y = sum_k c_k x_k, k < nvec (fixed summation order)
 void lincomb(double* y, const double* const* xs, const double* c, int nvec, long n, hipStream_t s);
+// t2 = sum_k c[k] xs[k] (as lincomb) and tau = t2 + t1(a,i) t1(b,j) (as build_tau) in one
+// pass over the o^2 v^2 elements; with maxes, also max |t2| and max |tau| into maxes[0..1]
+// (as build_tau_absmax).
+void lincomb_tau(double* t2, double* tau, const double* t1, const double* const* xs, const double* c, int nvec,
+                 int o, int v, double* partials, double* maxes, hipStream_t s);
 
 // Number of doubles of `partials` scratch the reductions above need.
 constexpr long kReducePartials = 4096 * 16;
