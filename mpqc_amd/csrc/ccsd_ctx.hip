@@ -761,7 +761,7 @@ void CCSDContext::split(const Planes& out, std::array<long, 4> n, std::initializ
 }
 
 void CCSDContext::gemmP(int M, int N, int K, double alpha, const Planes& A, long lda, const Planes& B,
-                        long ldb, double beta, double* C, long ldc, const char* tag) {
+                        long ldb, double beta, double* C, long ldc, const char* tag, int amax_slot) {
   if (M <= 0 || N <= 0 || K <= 0) throw std::logic_error("empty GEMM");
   if (lda < K || ldb < K || ldc < N) throw std::logic_error("GEMM leading dimension too small");
   if ((long)(M - 1) * lda + K - 1 >= A.n || (long)(N - 1) * ldb + K - 1 >= B.n)
@@ -772,7 +772,15 @@ void CCSDContext::gemmP(int M, int N, int K, double alpha, const Planes& A, long
   d.M = M; d.N = N; d.K = K; d.alpha = alpha; d.beta = beta;
   d.A = A; d.lda = lda; d.sA = 0; d.B = B; d.ldb = ldb; d.sB = 0;
   d.C = C; d.ldc = ldc; d.sC = 0; d.batch = 1;
-  timed(tag, 2.0 * M * (double)N * K, [&] { gemm_planes(d, pws_, pws_bytes_, stream_); });
+  const bool rec = amax_slot >= 0 && prec_ == GemmPrecision::F16X3 && amax_.dev && amax_.ptr[amax_slot] == C &&
+                   ldc == N;
+  if (rec) {
+    d.cmax = amax_.dev + amax_slot;
+    d.cmax_partials = partials_;
+  }
+  bool done = false;
+  timed(tag, 2.0 * M * (double)N * K, [&] { done = gemm_planes(d, pws_, pws_bytes_, stream_); });
+  if (rec) amax_.valid[amax_slot] = done;
 }
 
 // constant operands of the large contractions, split once per solve
@@ -1245,8 +1253,8 @@ void CCSDContext::residual_ops() {
     split(t2P_P_, e_aild, {{t2_, e_abij, {0, 2, 3, 1}, 1.0}});
     if (!df_)  // (df: the X T X form above)
       gemmP(o * v, o * v, o * v, -1.0, TP_P_, ov, GAt_P_, ov, 1.0, J_, ov, "r2:j_gT");
-    gemmP(o * v, o * v, o * v, 0.5, t2P_P_, ov, GBt_P_, ov, 1.0, J_, ov, "r2:j_gbar_t2");
-    gemmP(o * v, o * v, o * v, -1.0, TP_P_, ov, GCt_P_, ov, 1.0, K_, ov, "r2:k_gT");
+    gemmP(o * v, o * v, o * v, 0.5, t2P_P_, ov, GBt_P_, ov, 1.0, J_, ov, "r2:j_gbar_t2", 3);  // (final J)
+    gemmP(o * v, o * v, o * v, -1.0, TP_P_, ov, GCt_P_, ov, 1.0, K_, ov, "r2:k_gT", 4);        // (final K)
   } else {
     if (!df_)  // (df: the X T X form above)
       gemm(false, false, o * v, o * v, o * v, -1.0, TP_, ov, GA_, ov, 1.0, J_, ov, 0, 0, 0, 1, "r2:j_gT");

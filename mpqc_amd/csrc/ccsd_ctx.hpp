@@ -251,8 +251,10 @@ class CCSDContext {
              const TProd* prod = nullptr, const char* tag = nullptr, int line = __builtin_LINE());
   TProd tp_product() const;  // t1(d,i) t1(a,l) over TP's (a,i,l,d) index order
   AbsMaxCache amax_;         // max |t1|, |t2|, |tau| shared by one residual's splits
+  // (amax_slot >= 0: the result is the whole tensor amax_.ptr[amax_slot]; its max |C| is
+  // recorded in that slot of the cache where the contraction's split-K pass can produce it)
   void gemmP(int M, int N, int K, double alpha, const Planes& A, long lda, const Planes& B, long ldb,
-             double beta, double* C, long ldc, const char* tag);
+             double beta, double* C, long ldc, const char* tag, int amax_slot = -1);
   void prepare_planes();
   bool planes_ready_ = false;
   double* pws_ = nullptr;  // split-K workspace of the plane GEMMs

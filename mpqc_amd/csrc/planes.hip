@@ -869,11 +869,15 @@ __global__ __launch_bounds__(256) void splitk_reduce_f32_kernel(const float* __r
 
 // The same sums four outputs per thread: 16-byte partial loads and FP64 stores, 32-bit
 // indices (N % 4 == 0, C rows 16-byte aligned, M N / 4 < 2^31; the host checks).
+// With part (one batch), also max |C| of the block's outputs into part[blockIdx.x].
+template <bool MAX>
 __global__ __launch_bounds__(256) void splitk_reduce_f32_vec_kernel(const float* __restrict__ W,
                                                                     double* __restrict__ C, long ldc, long sC,
                                                                     int M, int N, int nsplit, int batch0,
                                                                     double alpha, const double* sa,
-                                                                    const double* sb, double beta) {
+                                                                    const double* sb, double beta,
+                                                                    double* __restrict__ part) {
+  double cm = 0.0;
   const long MN = (long)M * N;
   const int n4 = N >> 2, total = (int)(MN >> 2);
   const int bz = blockIdx.y;
@@ -897,7 +901,32 @@ __global__ __launch_bounds__(256) void splitk_reduce_f32_vec_kernel(const float*
     }
     c[0] = lo;
     c[1] = hi;
+    if (MAX) cm = fmax(cm, fmax(fmax(fabs(lo[0]), fabs(lo[1])), fmax(fabs(hi[0]), fabs(hi[1]))));
   }
+  if (MAX) {
+    __shared__ double sh[256];
+    sh[threadIdx.x] = cm;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+  }
+}
+
+// *out = max of part[0..n)
+__global__ __launch_bounds__(256) void max_stage2(const double* __restrict__ part, int n, double* __restrict__ out) {
+  double m = 0.0;
+  for (int b = threadIdx.x; b < n; b += 256) m = fmax(m, part[b]);
+  __shared__ double sh[256];
+  sh[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = sh[0];
 }
 
 // Launch plan: tile edge, split-K factor and K chunk of one contraction.
@@ -956,7 +985,8 @@ size_t dl_workspace_bytes(int M, int N, const DlPlan& pl, int batch) {
 }
 
 template <bool F16>
-void launch_planes_dl(const PlaneGemmDesc& d, const DlPlan& pl, double* ws, size_t ws_bytes, hipStream_t stream) {
+bool launch_planes_dl(const PlaneGemmDesc& d, const DlPlan& pl, double* ws, size_t ws_bytes, hipStream_t stream) {
+  bool cmax_done = false;
   const int tiles_m = (int)ceil_div(d.M, kDlTile), tiles_n = (int)ceil_div(d.N, kDlTile);
   const long tiles = (long)tiles_m * tiles_n;
   const int per_xcd = tiles >= kXcdMinTiles ? (int)ceil_div(tiles, kXcd) : 0;  // 0: plain tile order
@@ -985,8 +1015,15 @@ void launch_planes_dl(const PlaneGemmDesc& d, const DlPlan& pl, double* ws, size
                        reinterpret_cast<uintptr_t>(d.C) % 16 == 0 && MN / 4 < (1L << 31) - 65536L * 256;
       if (vec) {
         dim3 g2((unsigned)std::min<long>(ceil_div(MN / 4, 256), 8192), (unsigned)nb);
-        hipLaunchKernelGGL(splitk_reduce_f32_vec_kernel, g2, dim3(256), 0, stream, p.W, d.C, d.ldc, d.sC, d.M,
-                           d.N, pl.nsplit, b0, d.alpha, d.A.scale, d.B.scale, d.beta);
+        if (d.cmax && d.cmax_partials && d.batch == 1) {
+          hipLaunchKernelGGL(splitk_reduce_f32_vec_kernel<true>, g2, dim3(256), 0, stream, p.W, d.C, d.ldc, d.sC,
+                             d.M, d.N, pl.nsplit, b0, d.alpha, d.A.scale, d.B.scale, d.beta, d.cmax_partials);
+          hipLaunchKernelGGL(max_stage2, dim3(1), dim3(256), 0, stream, d.cmax_partials, (int)g2.x, d.cmax);
+          cmax_done = true;
+        } else {
+          hipLaunchKernelGGL(splitk_reduce_f32_vec_kernel<false>, g2, dim3(256), 0, stream, p.W, d.C, d.ldc, d.sC,
+                             d.M, d.N, pl.nsplit, b0, d.alpha, d.A.scale, d.B.scale, d.beta, nullptr);
+        }
       } else {
         dim3 g2((unsigned)std::min<long>(ceil_div(MN, 256), 4096), (unsigned)nb);
         hipLaunchKernelGGL(splitk_reduce_f32_kernel, g2, dim3(256), 0, stream, p.W, d.C, d.ldc, d.sC, d.M, d.N,
@@ -995,6 +1032,7 @@ void launch_planes_dl(const PlaneGemmDesc& d, const DlPlan& pl, double* ws, size
       check_launch("splitk_reduce_f32");
     }
   }
+  return cmax_done;
 }
 
 PlanePlan plan_planes(int M, int N, int K, int batch, int force_bm = 0, int force_bn = 0) {
@@ -1136,20 +1174,19 @@ void dispatch_tallk(const PlaneGemmDesc& d, const TkPlan& pl, double* ws, size_t
   else launch_planes_tallk<5, 5, F16>(d, pl, ws, ws_bytes, stream);
 }
 
+// Returns whether *d.cmax was written (only the deep-pipeline kernel's split-K pass does).
 template <int NP, bool F16>
-void dispatch_planes(const PlaneGemmDesc& d, double* ws, size_t ws_bytes, hipStream_t stream) {
+bool dispatch_planes(const PlaneGemmDesc& d, double* ws, size_t ws_bytes, hipStream_t stream) {
   if constexpr (NP == 2) {
     const bool aligned16 = d.K % 8 == 0 && d.lda % 8 == 0 && d.ldb % 8 == 0 && d.sA % 8 == 0 && d.sB % 8 == 0 &&
                            !g_tile_override_m;
     const DlPlan dl = plan_dl(d.M, d.N, d.K, d.batch);
-    if (aligned16 && dl.use && ws_bytes >= dl_workspace_bytes(d.M, d.N, dl, 1)) {
-      launch_planes_dl<F16>(d, dl, ws, ws_bytes, stream);
-      return;
-    }
+    if (aligned16 && dl.use && ws_bytes >= dl_workspace_bytes(d.M, d.N, dl, 1))
+      return launch_planes_dl<F16>(d, dl, ws, ws_bytes, stream);
     const TkPlan tk = plan_tallk(d.M, d.N, d.K, d.batch);
     if (aligned16 && tk.use && ws_bytes >= tk_workspace_bytes(d.M, d.N, tk, 1)) {
       dispatch_tallk<F16>(d, tk, ws, ws_bytes, stream);
-      return;
+      return false;
     }
   }
   const PlanePlan pl = plan_planes(d.M, d.N, d.K, d.batch);
@@ -1163,6 +1200,7 @@ void dispatch_planes(const PlaneGemmDesc& d, double* ws, size_t ws_bytes, hipStr
   } else {
     launch_planes<(NP == 3 ? 128 : 256), 128, NP, F16>(d, pl, ws, ws_bytes, stream);
   }
+  return false;
 }
 
 }  // namespace
@@ -1271,14 +1309,15 @@ void set_plane_dl_mode(int mode) {
   g_dl_mode = mode;
 }
 
-void gemm_planes(const PlaneGemmDesc& d, double* ws, size_t ws_bytes, hipStream_t stream) {
-  if (d.M <= 0 || d.N <= 0 || d.batch <= 0) return;
+bool gemm_planes(const PlaneGemmDesc& d, double* ws, size_t ws_bytes, hipStream_t stream) {
+  if (d.M <= 0 || d.N <= 0 || d.batch <= 0) return false;
   if (d.K <= 0) throw std::invalid_argument("gemm_planes: K must be positive");
   switch (d.kind) {
-    case PlaneKind::BF16x2: dispatch_planes<2, false>(d, ws, ws_bytes, stream); break;
-    case PlaneKind::F16x2: dispatch_planes<2, true>(d, ws, ws_bytes, stream); break;
-    case PlaneKind::BF16x3: dispatch_planes<3, false>(d, ws, ws_bytes, stream); break;
+    case PlaneKind::BF16x2: return dispatch_planes<2, false>(d, ws, ws_bytes, stream);
+    case PlaneKind::F16x2: return dispatch_planes<2, true>(d, ws, ws_bytes, stream);
+    case PlaneKind::BF16x3: return dispatch_planes<3, false>(d, ws, ws_bytes, stream);
   }
+  return false;
 }
 
 }  // namespace mpqc_hip

@@ -58,8 +58,13 @@ struct PlaneGemmDesc {
   double* C;
   long ldc, sC;
   int batch = 1;
+  // optional: max |C| over the M x N result into *cmax (device scalar; cmax_partials: at
+  // least 8192 doubles), where the result is written by the split-K pass
+  double* cmax = nullptr;
+  double* cmax_partials = nullptr;
 };
-void gemm_planes(const PlaneGemmDesc& d, double* ws, size_t ws_bytes, hipStream_t stream);
+// Returns whether *d.cmax was written.
+bool gemm_planes(const PlaneGemmDesc& d, double* ws, size_t ws_bytes, hipStream_t stream);
 // Forces the contraction tile (bm x bn; 0, 0 restores the planner's choice).
 // Used by measurements that compare tile shapes.
 void set_plane_tile_override(int bm, int bn);
