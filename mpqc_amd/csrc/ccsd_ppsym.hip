@@ -80,6 +80,50 @@ __global__ __launch_bounds__(kSymBlock) void pack_tau_rows_kernel(double* __rest
   }
 }
 
+// pack_tau_rows_kernel that also writes the block's max |out| into part[blockIdx.x]
+__global__ __launch_bounds__(kSymBlock) void pack_tau_rows_max_kernel(double* __restrict__ out,
+                                                                      const double* __restrict__ tau, long oo,
+                                                                      long v, const int2* __restrict__ pairs,
+                                                                      long npairs, long nrows, double sign,
+                                                                      double* __restrict__ part) {
+  const long total = nrows * oo;
+  double m = 0.0;
+  for (long e = (long)blockIdx.x * kSymBlock + threadIdx.x; e < total; e += (long)gridDim.x * kSymBlock) {
+    const long C = e / oo, ij = e - C * oo;
+    double x = 0.0;
+    if (C < npairs) {
+      const int2 cd = pairs[C];
+      x = tau[((long)cd.x * v + cd.y) * oo + ij] + sign * tau[((long)cd.y * v + cd.x) * oo + ij];
+      if (cd.x == cd.y) x *= 0.5;
+    }
+    out[e] = x;
+    m = fmax(m, fabs(x));
+  }
+  __shared__ double sh[kSymBlock];
+  sh[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = kSymBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// *out = max of part[0..n)
+__global__ __launch_bounds__(kSymBlock) void sym_max_stage2(const double* __restrict__ part, int n,
+                                                            double* __restrict__ out) {
+  double m = 0.0;
+  for (int b = threadIdx.x; b < n; b += kSymBlock) m = fmax(m, part[b]);
+  __shared__ double sh[kSymBlock];
+  sh[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = kSymBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = sh[0];
+}
+
 // r2(a,b,i,j) += 1/2 [P(ab,ij) + s(a,b) M(ab,ij)]
 __global__ __launch_bounds__(kSymBlock) void unpack_ladder_kernel(double* __restrict__ r2,
                                                                   const double* __restrict__ P,
@@ -304,16 +348,39 @@ void CCSDContext::pack_tau_pairs() {
   const bool sm = split_mode();
   if (!sm && !pp_packed_) pack_pp_symmetric();
   const long rs = sm ? ldvs_ : nvs_, ra = sm ? ldva_ : nva_;
+  // fp16 planes: the pack also yields the exact max |S| and |A| their scales need (cache
+  // slots 5 and 6), in place of a max pass of each split
+  const bool rec = sm && prec_ == GemmPrecision::F16X3 && amax_.dev;
   timed("r2:pp_sym_pack", 0.0, [&] {
-    hipLaunchKernelGGL(pack_tau_rows_kernel, dim3(sym_grid(rs * oo)), dim3(kSymBlock), 0, stream_, Spk_, tau_, oo, V,
-                       pv_sym_, nvs_, rs, 1.0);
+    if (rec) {
+      const int gs = (int)std::min<long>(ceil_div(rs * oo, kSymBlock), 4096);
+      hipLaunchKernelGGL(pack_tau_rows_max_kernel, dim3(gs), dim3(kSymBlock), 0, stream_, Spk_, tau_, oo, V, pv_sym_,
+                         nvs_, rs, 1.0, partials_);
+      hipLaunchKernelGGL(sym_max_stage2, dim3(1), dim3(kSymBlock), 0, stream_, partials_, gs, amax_.dev + 5);
+    } else {
+      hipLaunchKernelGGL(pack_tau_rows_kernel, dim3(sym_grid(rs * oo)), dim3(kSymBlock), 0, stream_, Spk_, tau_, oo,
+                         V, pv_sym_, nvs_, rs, 1.0);
+    }
     check_launch("pack_tau_rows");
     if (nva_ > 0) {
-      hipLaunchKernelGGL(pack_tau_rows_kernel, dim3(sym_grid(ra * oo)), dim3(kSymBlock), 0, stream_, Apk_, tau_, oo,
-                         V, pv_anti_, nva_, ra, -1.0);
+      if (rec) {
+        const int ga = (int)std::min<long>(ceil_div(ra * oo, kSymBlock), 4096);
+        hipLaunchKernelGGL(pack_tau_rows_max_kernel, dim3(ga), dim3(kSymBlock), 0, stream_, Apk_, tau_, oo, V,
+                           pv_anti_, nva_, ra, -1.0, partials_);
+        hipLaunchKernelGGL(sym_max_stage2, dim3(1), dim3(kSymBlock), 0, stream_, partials_, ga, amax_.dev + 6);
+      } else {
+        hipLaunchKernelGGL(pack_tau_rows_kernel, dim3(sym_grid(ra * oo)), dim3(kSymBlock), 0, stream_, Apk_, tau_,
+                           oo, V, pv_anti_, nva_, ra, -1.0);
+      }
       check_launch("pack_tau_rows");
     }
   });
+  if (rec) {
+    amax_.ptr[5] = Spk_;
+    amax_.valid[5] = true;
+    amax_.ptr[6] = nva_ > 0 ? Apk_ : nullptr;
+    amax_.valid[6] = nva_ > 0;
+  }
   if (sm) {
     const std::array<int, 4> T{0, 1, 3, 2};
     split(Sp_P_, {1, 1, oo, ldvs_}, {{Spk_, {1, 1, ldvs_, oo}, T, 1.0}});
